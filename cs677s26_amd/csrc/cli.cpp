@@ -1,0 +1,380 @@
+// cli.cpp -- cuda_hello drop-in command line (src/main.cu:2581-2615).
+//
+//   cuda_hello detect <in_model.bin> <image.pgm> <out.ppm>
+//              [scaleFactor=1.25] [minNeighbors=4] [minObjectSize=24]
+//              [maxDetections=200000] [benchmarkRuns=0]
+//   cuda_hello train  [faces_u8.bin] [numNeg] [maxWeakPerStage] [maxDetections]
+//              [minNeighbors] [scaleFactor] [maxStages] [minHitRate]
+//              [maxFalseAlarm] [minObjectSize] [stageNegCacheDir]
+//              [stageNegImagesDir] [stageNegSeed] [hardNegCandidateMultiplier]
+//              <out_model.bin>
+//
+// Same verbs, positional order, defaults, clamps and stdout tags as the
+// reference (src/main.cu:1370-1531, 2394-2579); exit status 0 or 1.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <limits>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "vj_api.h"
+
+namespace {
+
+int fail(const char* what) {
+    std::cerr << "[ERR] " << what << ": " << vj_last_error() << "\n";
+    return 1;
+}
+
+struct DevMem {
+    void* p = nullptr;
+    ~DevMem() {
+        if (p) vj_free(p);
+    }
+};
+
+struct Model {
+    int winW = 0, winH = 0;
+    std::vector<vj_feature> features;
+    std::vector<vj_stump> stumps;
+    std::vector<vj_stage> stages;
+};
+
+bool load_model(const std::string& path, Model& m) {
+    int64_t nF = 0, nS = 0, nSt = 0;
+    if (vj_model_peek(path.c_str(), &m.winW, &m.winH, &nF, &nS, &nSt) != VJ_OK) return false;
+    m.features.resize(static_cast<size_t>(nF));
+    m.stumps.resize(static_cast<size_t>(nS));
+    m.stages.resize(static_cast<size_t>(nSt));
+    return vj_model_load(path.c_str(), m.features.data(), nF, m.stumps.data(), nS, m.stages.data(), nSt) == VJ_OK;
+}
+
+// src/main.cu:2394-2579
+int run_detect(int argc, char** argv) {
+    if (argc < 4) {
+        std::cerr << "Usage: cuda_hello detect <in_model.bin> <image.pgm> <out.ppm>"
+                  << " [scaleFactor=1.25] [minNeighbors=4] [minObjectSize=24] [maxDetections=200000] [benchmarkRuns=0]\n";
+        return 1;
+    }
+    const std::string modelPath = argv[1], imagePath = argv[2], outPath = argv[3];
+    float scaleFactor = 1.25f;
+    int minNeighbors = 4, minObjectSize = 24, maxDetections = 200000, benchmarkRuns = 0;
+    try {
+        if (argc > 4) scaleFactor = std::stof(argv[4]);
+        if (argc > 5) minNeighbors = std::max(1, std::stoi(argv[5]));
+        if (argc > 6) minObjectSize = std::max(1, std::stoi(argv[6]));
+        if (argc > 7) maxDetections = std::max(100, std::stoi(argv[7]));
+        if (argc > 8) benchmarkRuns = std::max(0, std::stoi(argv[8]));
+    } catch (...) {
+        std::cerr << "[ERR] invalid numeric argument\n";
+        return 1;
+    }
+    Model m;
+    if (!load_model(modelPath, m)) {
+        std::cerr << "[ERR] loadCascadeModel failed: " << modelPath << "\n";
+        return 1;
+    }
+    int w = 0, h = 0;
+    if (vj_pgm_peek(imagePath.c_str(), &w, &h) != VJ_OK) return fail("loadPGM");
+    std::vector<uint8_t> img(static_cast<size_t>(w) * h);
+    if (vj_pgm_load(imagePath.c_str(), img.data(), img.size()) != VJ_OK) return fail("loadPGM");
+
+    vj_engine* eng = nullptr;
+    if (vj_engine_create(m.winW, m.winH, 1, std::max(m.winW, w), std::max(m.winH, h), 0, maxDetections, &eng) != VJ_OK)
+        return fail("FaceVisionEngine");
+    struct Guard {
+        vj_engine* e;
+        ~Guard() { vj_engine_destroy(e); }
+    } guard{eng};
+    if (vj_engine_upload_feature_pool(eng, m.features.data(), static_cast<int>(m.features.size()), nullptr) != VJ_OK)
+        return fail("uploadFeaturePool");
+    DevMem dStumps, dImage;
+    if (vj_malloc(&dStumps.p, m.stumps.size() * sizeof(vj_stump)) != VJ_OK ||
+        vj_malloc(&dImage.p, img.size()) != VJ_OK ||
+        vj_memcpy_htod(dStumps.p, m.stumps.data(), m.stumps.size() * sizeof(vj_stump), nullptr) != VJ_OK ||
+        vj_memcpy_htod(dImage.p, img.data(), img.size(), nullptr) != VJ_OK)
+        return fail("cudaMemcpy to device");
+    const vj_feature* dPool = nullptr;
+    int poolN = 0;
+    vj_engine_feature_pool(eng, &dPool, &poolN);
+    if (vj_engine_set_cascade(eng, dPool, poolN, static_cast<const vj_stump*>(dStumps.p),
+                              static_cast<int>(m.stumps.size()), m.stages.data(), static_cast<int>(m.stages.size()),
+                              nullptr) != VJ_OK)
+        return fail("setCascadeModel");
+
+    std::vector<vj_detection> dets(static_cast<size_t>(maxDetections));
+    int count = 0;
+    auto detect_once = [&]() {
+        return vj_engine_detect(eng, static_cast<const uint8_t*>(dImage.p), w, h, w, scaleFactor, minNeighbors,
+                                minObjectSize, maxDetections, 1, dets.data(), static_cast<int>(dets.size()), &count,
+                                nullptr);
+    };
+    if (benchmarkRuns > 0) {
+        for (int i = 0; i < 3; ++i)
+            if (detect_once() != VJ_OK) return fail("warmup detectMultiScale");
+        vj_event a = nullptr, b = nullptr;
+        vj_event_create(&a);
+        vj_event_create(&b);
+        float sum = 0.0f, mn = std::numeric_limits<float>::infinity(), mx = 0.0f;
+        for (int i = 0; i < benchmarkRuns; ++i) {
+            vj_event_record(a, nullptr);
+            if (detect_once() != VJ_OK) return fail("detectMultiScale");
+            vj_event_record(b, nullptr);
+            vj_event_synchronize(b);
+            float ms = 0.0f;
+            vj_event_elapsed_ms(a, b, &ms);
+            sum += ms;
+            mn = std::min(mn, ms);
+            mx = std::max(mx, ms);
+        }
+        vj_event_destroy(a);
+        vj_event_destroy(b);
+        const float avg = sum / static_cast<float>(benchmarkRuns);
+        std::cout << "[DETECT_BENCH] runs=" << benchmarkRuns << " detections(last)=" << count << " avg(ms): " << avg
+                  << " min(ms): " << mn << " max(ms): " << mx << " avg(fps): " << (avg > 0.0f ? 1000.0f / avg : 0.0f)
+                  << "\n";
+    } else {
+        const auto t0 = std::chrono::high_resolution_clock::now();
+        if (detect_once() != VJ_OK) return fail("detectMultiScale");
+        const auto t1 = std::chrono::high_resolution_clock::now();
+        std::cout << "[DETECT] detections=" << count << " elapsed(ms): "
+                  << std::chrono::duration_cast<std::chrono::milliseconds>(t1 - t0).count() << "\n";
+    }
+    const int shown = std::min(count, static_cast<int>(dets.size()));
+    if (vj_ppm_save_gray_boxes(outPath.c_str(), img.data(), w, h, dets.data(), shown) != VJ_OK) {
+        std::cerr << "[ERR] savePPM failed\n";
+        return 1;
+    }
+    std::cout << "[OK] Saved result to: " << outPath << "\n";
+    return 0;
+}
+
+void usage(const char* prog) {
+    std::cerr << "Usage: " << prog << " train ... <out_model.bin>\n";
+    std::cerr << "   or: " << prog << " detect <in_model.bin> <image.pgm> <out.ppm>\n";
+}
+
+}  // namespace
+
+int run_train(int argc, char** argv);
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        usage(argv[0]);
+        return 1;
+    }
+    const std::string mode = argv[1];
+    if (mode == "detect") return run_detect(argc - 1, argv + 1);
+    if (mode == "train") return run_train(argc - 1, argv + 1);
+    std::cerr << "[ERR] Unknown mode: " << mode << "\n";
+    usage(argv[0]);
+    return 1;
+}
+
+// ---------------------------------------------------------------------------
+// train (src/main.cu:1370-2392).  Positives from faces_u8.bin ([N][24][24]
+// u8); negatives per stage from <stageNegCacheDir>/stage_negatives/
+// non_faces_stageNN.bin (the files the reference's sampling script writes);
+// the boosting rounds run on the fused trainer (presort once per stage).
+// ---------------------------------------------------------------------------
+namespace {
+
+bool load_bin(const std::string& path, int px, std::vector<uint8_t>& out, int& count) {
+    std::ifstream ifs(path, std::ios::binary);
+    if (!ifs) {
+        std::cerr << "[ERR] Cannot open: " << path << "\n";
+        return false;
+    }
+    ifs.seekg(0, std::ios::end);
+    const std::streamoff sz = ifs.tellg();
+    ifs.seekg(0, std::ios::beg);
+    if (sz <= 0) {
+        std::cerr << "[ERR] Empty file: " << path << "\n";
+        return false;
+    }
+    if (sz % px != 0) {
+        std::cerr << "[ERR] File size is not divisible by samplePixels: " << path << "\n";
+        return false;
+    }
+    out.resize(static_cast<size_t>(sz));
+    if (!ifs.read(reinterpret_cast<char*>(out.data()), sz)) return false;
+    count = static_cast<int>(sz / px);
+    return true;
+}
+
+}  // namespace
+
+int run_train(int argc, char** argv) {
+    constexpr int kStages = 38, kTotal = 6061, kPos = 10000, kNeg = 10000;
+    const int modelArg = argc - 1;
+    if (modelArg < 1) {
+        std::cerr << "Usage: cuda_hello train [faces_u8.bin] [numNeg] [maxWeakPerStage] [maxDetections] [minNeighbors]"
+                  << " [scaleFactor] [maxStages] [minHitRate] [maxFalseAlarm] [minObjectSize] [stageNegCacheDir]"
+                  << " [stageNegImagesDir] [stageNegSeed] [hardNegCandidateMultiplier] <out_model.bin>\n";
+        return 1;
+    }
+    const std::string outPath = argv[modelArg];
+    const std::string posPath = modelArg > 1 ? argv[1] : "faces_u8.bin";
+    int numNeg = kNeg, maxWeak = 250, maxStages = kStages, seed = 677;
+    float minHit = 0.99f, maxFa = 0.40f;
+    std::string cacheDir;
+    try {
+        if (modelArg > 2) numNeg = std::max(1, std::stoi(argv[2]));
+        if (modelArg > 3) maxWeak = std::max(1, std::stoi(argv[3]));
+        if (modelArg > 6 && std::stof(argv[6]) <= 1.0f) {
+            std::cerr << "[ERR] scaleFactor must be > 1.0\n";
+            return 1;
+        }
+        if (modelArg > 7) maxStages = std::max(1, std::stoi(argv[7]));
+        if (modelArg > 8) {
+            minHit = std::stof(argv[8]);
+            if (minHit <= 0.0f || minHit > 1.0f) {
+                std::cerr << "[ERR] minHitRate must be in (0, 1]\n";
+                return 1;
+            }
+        }
+        if (modelArg > 9) {
+            maxFa = std::stof(argv[9]);
+            if (maxFa < 0.0f || maxFa > 1.0f) {
+                std::cerr << "[ERR] maxFalseAlarm must be in [0, 1]\n";
+                return 1;
+            }
+        }
+        if (modelArg > 11) cacheDir = argv[11];
+        if (modelArg > 13) seed = std::stoi(argv[13]);
+    } catch (...) {
+        std::cerr << "[ERR] invalid numeric argument\n";
+        return 1;
+    }
+    if (numNeg != kNeg) {
+        std::cout << "[INFO] numNeg is fixed to paper-style per-stage setting: " << kNeg << "\n";
+        numNeg = kNeg;
+    }
+    if (maxStages > kStages) {
+        std::cout << "[INFO] maxStages capped to " << kStages << "\n";
+        maxStages = kStages;
+    }
+    if (cacheDir.empty()) {
+        std::cerr << "[ERR] stageNegCacheDir is required. Dynamic negatives are mandatory.\n";
+        return 1;
+    }
+    const int px = 24 * 24;
+    std::vector<uint8_t> posRaw;
+    int posPool = 0;
+    if (!load_bin(posPath, px, posRaw, posPool)) return 1;
+    const int numPos = std::min(posPool, kPos);
+    const int n = numPos + numNeg;
+    int poolN = 0;
+    vj_haar_pool(24, 24, 1, nullptr, 0, &poolN);
+    std::vector<vj_feature> pool(static_cast<size_t>(poolN));
+    vj_haar_pool(24, 24, 1, pool.data(), poolN, &poolN);
+    std::cout << "[INFO] Feature count: " << poolN << "\n";
+
+    vj_trainer* tr = nullptr;
+    if (vj_trainer_create(24, 24, n, 1024, &tr) != VJ_OK) return fail("AdaBoostTrainer");
+    struct TGuard {
+        vj_trainer* t;
+        ~TGuard() { vj_trainer_destroy(t); }
+    } tguard{tr};
+    DevMem dPool, dTrain;
+    if (vj_malloc(&dPool.p, pool.size() * sizeof(vj_feature)) != VJ_OK ||
+        vj_memcpy_htod(dPool.p, pool.data(), pool.size() * sizeof(vj_feature), nullptr) != VJ_OK ||
+        vj_malloc(&dTrain.p, static_cast<size_t>(n) * px) != VJ_OK)
+        return fail("device buffers");
+    std::vector<int8_t> labels(static_cast<size_t>(n), -1);
+    for (int i = 0; i < numPos; ++i) labels[static_cast<size_t>(i)] = 1;
+    std::vector<uint8_t> trainRaw(static_cast<size_t>(n) * px);
+    std::vector<vj_stump> stumps;
+    std::vector<vj_stage> stages;
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    std::cout << "[TRAIN] Start cascade training: maxStages=" << maxStages << " maxWeakPerStage=" << maxWeak
+              << " totalFeatureBudget=" << kTotal << "\n";
+    for (int stage = 0; stage < maxStages; ++stage) {
+        if (static_cast<int>(stumps.size()) >= kTotal) break;
+        const int stageSeed = seed + stage * 1000003;
+        // positives: all, or a seeded subset of the pool (src/main.cu:1582-1598)
+        if (posPool <= numPos) {
+            std::memcpy(trainRaw.data(), posRaw.data(), static_cast<size_t>(numPos) * px);
+        } else {
+            std::vector<int> idx(static_cast<size_t>(posPool));
+            std::iota(idx.begin(), idx.end(), 0);
+            std::mt19937 rng(static_cast<uint32_t>(stageSeed ^ 0x7f4a7c15));
+            std::shuffle(idx.begin(), idx.end(), rng);
+            for (int i = 0; i < numPos; ++i)
+                std::memcpy(trainRaw.data() + static_cast<size_t>(i) * px,
+                            posRaw.data() + static_cast<size_t>(idx[static_cast<size_t>(i)]) * px, px);
+        }
+        char name[64];
+        std::snprintf(name, sizeof name, "/stage_negatives/non_faces_stage%02d.bin", stage);
+        std::vector<uint8_t> negRaw;
+        int negCount = 0;
+        if (!load_bin(cacheDir + name, px, negRaw, negCount) || negCount < numNeg) {
+            std::cerr << "[ERR] stage negatives missing or short for stage " << stage << "\n";
+            return 1;
+        }
+        std::memcpy(trainRaw.data() + static_cast<size_t>(numPos) * px, negRaw.data(), static_cast<size_t>(numNeg) * px);
+        if (vj_memcpy_htod(dTrain.p, trainRaw.data(), trainRaw.size(), nullptr) != VJ_OK) return fail("copy stage set");
+        if (vj_trainer_stage_begin(tr, static_cast<const uint8_t*>(dTrain.p), labels.data(), nullptr, n, numPos,
+                                   static_cast<const vj_feature*>(dPool.p), poolN, nullptr) != VJ_OK)
+            return fail("stage begin");
+        const int first = static_cast<int>(stumps.size());
+        const int limit = std::min(maxWeak, kTotal - first);
+        vj_round_config cfg{minHit, maxFa};
+        float stageThreshold = 0.0f;
+        bool satisfied = false;
+        for (int r = 0; r < limit; ++r) {
+            vj_weak wk;
+            if (vj_trainer_round(tr, &cfg, &wk, nullptr) != VJ_OK) return fail("round");
+            if (wk.feature < 0) {
+                std::cerr << "[WARN] No valid weak found at stage " << stage << " weakRound " << r << "\n";
+                break;
+            }
+            stumps.push_back({wk.feature, wk.theta, wk.left, wk.right});
+            stageThreshold = wk.stageThreshold;
+            std::cout << "[TRAIN] stage=" << stage << " weak=" << r << " feature=" << wk.feature << " err="
+                      << static_cast<float>(std::min(std::max(wk.err, 1e-6), 1.0 - 1e-6)) << " alpha(C)=" << wk.alpha
+                      << " parity=" << wk.parity << " hit=" << wk.hitRate << " fa=" << wk.falseAlarm
+                      << " negSurvive=" << wk.negSurvive << "/" << numNeg << "\n";
+            if (wk.stageSatisfied) {
+                satisfied = true;
+                break;
+            }
+        }
+        const int trees = static_cast<int>(stumps.size()) - first;
+        if (trees <= 0) break;
+        int remain = 0;
+        if (vj_trainer_stage_end(tr, stageThreshold, &remain, nullptr) != VJ_OK) return fail("stage end");
+        if (!satisfied && remain > 0) {
+            std::cerr << "[WARN] Stage " << stage << " did not meet target. Increase maxWeakPerStage.\n";
+            break;
+        }
+        stages.push_back({first, trees, stageThreshold - 1e-5f});
+        std::cout << "[TRAIN] Stage " << stage << " done: ntrees=" << trees << " threshold=" << stageThreshold
+                  << " remainNeg=" << remain << "/" << numNeg << "\n";
+        if (remain == 0) break;
+    }
+    const auto t1 = std::chrono::high_resolution_clock::now();
+    std::cout << "[TRAIN] elapsed(ms): " << std::chrono::duration_cast<std::chrono::milliseconds>(t1 - t0).count()
+              << "\n";
+    if (stumps.empty() || stages.empty()) {
+        std::cerr << "[ERR] no trained cascade stages\n";
+        return 1;
+    }
+    if (vj_model_save(outPath.c_str(), 24, 24, pool.data(), poolN, stumps.data(), static_cast<int64_t>(stumps.size()),
+                      stages.data(), static_cast<int64_t>(stages.size())) != VJ_OK) {
+        std::cerr << "[ERR] saveCascadeModel failed: " << outPath << "\n";
+        return 1;
+    }
+    std::cout << "[OK] Saved model to: " << outPath << " (features=" << poolN << ", stumps=" << stumps.size()
+              << ", stages=" << stages.size() << ")\n";
+    return 0;
+}

@@ -1,0 +1,1253 @@
+// vj_engine.hip -- FaceVisionEngine on gfx950: pyramid, integral images,
+// cascade evaluation and order-preserving compaction of detections.
+//
+// Reference: src/face_vision_engine.cu (FaceVisionEngine) and the CPU path it
+// must equal, src/main.cu:428-827.  Arithmetic follows the CPU path exactly:
+// exact u32/u64 integrals, int window sums, f64 normaliser, f32 Haar sums from
+// absolute integral values, f32 stage sums in stump order, CPU box rounding.
+// Build with -ffp-contract=off (no fused multiply-add anywhere).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vj_common.hpp"
+#include "vj_kernels.hpp"
+#include "vj_integral_dev.hpp"
+
+namespace vj {
+
+// ===========================================================================
+// Kernels
+// ===========================================================================
+
+// Pixel-centre bilinear resize of one pyramid level from its source level,
+// 4 output pixels per thread.  Same float sequence as the CPU path
+// (src/main.cu:565-581): fx = (x + 0.5) * sx - 0.5, clamped floor, lerp in x
+// then y, (u8) clamp((int)(v + 0.5), 0, 255).
+struct ResizeJob {
+    int32_t level;    // destination level
+    int32_t itemOff;  // first work item of this job
+};
+
+__global__ __launch_bounds__(256) void resize_kernel(const LevelDesc* __restrict__ L,
+                                                     const ResizeJob* __restrict__ jobs, int nJobs,
+                                                     const float* __restrict__ rs,
+                                                     const int32_t* __restrict__ src) {
+    const int item = blockIdx.x;
+    int lo = 0, hi = nJobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].itemOff <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    const int li = jobs[lo].level;
+    const LevelDesc dst = L[li];
+    const LevelDesc sl = L[src[li]];
+    const int segs = (dst.w + 1023) / 1024;
+    const int local = item - jobs[lo].itemOff;
+    const int y = local / segs;
+    const int x0 = (local - y * segs) * 1024 + threadIdx.x * 4;
+    if (y >= dst.h) return;
+    const float sx = rs[2 * li], sy = rs[2 * li + 1];
+    const float fy = (static_cast<float>(y) + 0.5f) * sy - 0.5f;
+    const int sy0 = max(0, min(static_cast<int>(floorf(fy)), sl.h - 1));
+    const int sy1 = min(sy0 + 1, sl.h - 1);
+    const float ay = fy - static_cast<float>(sy0);
+    const uint8_t* r0 = sl.pix + static_cast<size_t>(sy0) * sl.stride;
+    const uint8_t* r1 = sl.pix + static_cast<size_t>(sy1) * sl.stride;
+    uint8_t* out = const_cast<uint8_t*>(dst.pix) + static_cast<size_t>(y) * dst.stride;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int x = x0 + k;
+        if (x >= dst.w) break;
+        const float fx = (static_cast<float>(x) + 0.5f) * sx - 0.5f;
+        const int sx0 = max(0, min(static_cast<int>(floorf(fx)), sl.w - 1));
+        const int sx1 = min(sx0 + 1, sl.w - 1);
+        const float ax = fx - static_cast<float>(sx0);
+        const float p00 = static_cast<float>(r0[sx0]);
+        const float p01 = static_cast<float>(r0[sx1]);
+        const float p10 = static_cast<float>(r1[sx0]);
+        const float p11 = static_cast<float>(r1[sx1]);
+        const float top = p00 + (p01 - p00) * ax;
+        const float bot = p10 + (p11 - p10) * ax;
+        const float val = top + (bot - top) * ay;
+        const int v = static_cast<int>(val + 0.5f);
+        out[x] = static_cast<uint8_t>(v < 0 ? 0 : (v > 255 ? 255 : v));
+    }
+}
+
+// Loads pixels x..x+3 of a row (zero beyond w).
+__device__ __forceinline__ void load4(const uint8_t* row, int x, int w, uint32_t p[4]) {
+    if (x + 3 < w && ((reinterpret_cast<uintptr_t>(row + x) & 3u) == 0)) {
+        const uint32_t v = *reinterpret_cast<const uint32_t*>(row + x);
+        p[0] = v & 0xFFu;
+        p[1] = (v >> 8) & 0xFFu;
+        p[2] = (v >> 16) & 0xFFu;
+        p[3] = v >> 24;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) p[k] = (x + k < w) ? row[x + k] : 0u;
+    }
+}
+
+// Integral phase 1, one wavefront per (level, band of 32 rows, chunk of 256
+// columns): column sums over the band (pixels and squares) and each row's
+// total over the chunk.
+__global__ __launch_bounds__(64) void ii_colsum_kernel(const LevelDesc* __restrict__ L,
+                                                       const int32_t* __restrict__ itemOffs, int nLevels,
+                                                       uint32_t* __restrict__ colS, uint32_t* __restrict__ colQ,
+                                                       uint32_t* __restrict__ rowS, uint32_t* __restrict__ rowQ) {
+    const int item = blockIdx.x;
+    const int li = find_segment(itemOffs, nLevels, item);
+    const LevelDesc lv = L[li];
+    const int local = item - lv.itemOff;
+    const int b = local / lv.nChunks;
+    const int c = local - b * lv.nChunks;
+    const int lane = threadIdx.x;
+    const int x = c * kChunkCols + lane * 4;
+    const int y0 = b * kBandRows;
+    const int y1 = min(lv.h, y0 + kBandRows);
+    uint32_t cs[4] = {0, 0, 0, 0}, cq[4] = {0, 0, 0, 0};
+    for (int y = y0; y < y1; ++y) {
+        uint32_t p[4];
+        load4(lv.pix + static_cast<size_t>(y) * lv.stride, x, lv.w, p);
+        uint32_t rs = 0, rq = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            cs[k] += p[k];
+            cq[k] += p[k] * p[k];
+            rs += p[k];
+            rq += p[k] * p[k];
+        }
+        rs = wave_sum(rs);
+        rq = wave_sum(rq);
+        if (lane == 0) {
+            rowS[lv.rowOff + static_cast<int64_t>(y) * lv.nChunks + c] = rs;
+            rowQ[lv.rowOff + static_cast<int64_t>(y) * lv.nChunks + c] = rq;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (x + k < lv.w) {
+            colS[lv.colOff + static_cast<int64_t>(b) * lv.w + x + k] = cs[k];
+            colQ[lv.colOff + static_cast<int64_t>(b) * lv.w + x + k] = cq[k];
+        }
+    }
+}
+
+// Integral phase 2, one workgroup per (level, chunk of 256 columns): turns
+// the band column sums into exclusive vertical prefixes (column sums above
+// each band, in place) and records each band's total of them over the chunk.
+__global__ __launch_bounds__(256) void ii_vscan_kernel(const LevelDesc* __restrict__ L,
+                                                       const int32_t* __restrict__ chunkOffs, int nLevels,
+                                                       uint32_t* __restrict__ colS, uint32_t* __restrict__ colQ,
+                                                       uint32_t* __restrict__ totS, uint64_t* __restrict__ totQ) {
+    __shared__ uint32_t partS[256][4];
+    __shared__ uint64_t partQ[256][4];
+    const int item = blockIdx.x;
+    const int li = find_segment(chunkOffs, nLevels, item);
+    const LevelDesc lv = L[li];
+    const int c = item - lv.chunkItemOff;
+    const int x = c * kChunkCols + threadIdx.x;
+    const int wave = threadIdx.x >> 6;
+    uint32_t s = 0, q = 0;
+    for (int b = 0; b < lv.nBands; ++b) {
+        uint32_t vs = s, vq = q;
+        if (x < lv.w) {
+            const int64_t i = lv.colOff + static_cast<int64_t>(b) * lv.w + x;
+            const uint32_t a = colS[i], aq = colQ[i];
+            colS[i] = s;
+            colQ[i] = q;
+            s += a;
+            q += aq;
+        } else {
+            vs = 0;
+            vq = 0;
+        }
+        const uint32_t ws = wave_sum(vs);
+        const uint64_t wq = wave_sum(static_cast<uint64_t>(vq));
+        if ((threadIdx.x & 63) == 0) {
+            partS[b][wave] = ws;
+            partQ[b][wave] = wq;
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < lv.nBands; b += blockDim.x) {
+        totS[lv.totOff + static_cast<int64_t>(b) * lv.nChunks + c] =
+            partS[b][0] + partS[b][1] + partS[b][2] + partS[b][3];
+        totQ[lv.totOff + static_cast<int64_t>(b) * lv.nChunks + c] =
+            partQ[b][0] + partQ[b][1] + partQ[b][2] + partQ[b][3];
+    }
+}
+
+// Integral phase 3, one wavefront per (level, band, chunk): the integral at
+// the band's top edge comes from the vertical prefixes (scanned across the
+// chunk and the chunks to its left), each row adds its row prefix; writes
+// ii[y+1][x+1] as u32 sum and u64 square sum.
+__global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restrict__ L,
+                                                      const int32_t* __restrict__ itemOffs, int nLevels,
+                                                      const uint32_t* __restrict__ colS,
+                                                      const uint32_t* __restrict__ colQ,
+                                                      const uint32_t* __restrict__ totS,
+                                                      const uint64_t* __restrict__ totQ,
+                                                      const uint32_t* __restrict__ rowS,
+                                                      const uint32_t* __restrict__ rowQ,
+                                                      uint32_t* __restrict__ iiS, uint64_t* __restrict__ iiQ) {
+    const int item = blockIdx.x;
+    const int li = find_segment(itemOffs, nLevels, item);
+    const LevelDesc lv = L[li];
+    const int local = item - lv.itemOff;
+    const int b = local / lv.nChunks;
+    const int c = local - b * lv.nChunks;
+    const int lane = threadIdx.x;
+    const int x = c * kChunkCols + lane * 4;
+    const int y0 = b * kBandRows;
+    const int y1 = min(lv.h, y0 + kBandRows);
+    const int iw = lv.w + 1;
+    uint32_t* outS = iiS + lv.iiOff;
+    uint64_t* outQ = iiQ + lv.iiOff;
+
+    // carry from the chunks to the left: band totals and per-row totals
+    uint32_t ccS = 0;
+    uint64_t ccQ = 0;
+    for (int k = 0; k < c; ++k) {
+        ccS += totS[lv.totOff + static_cast<int64_t>(b) * lv.nChunks + k];
+        ccQ += totQ[lv.totOff + static_cast<int64_t>(b) * lv.nChunks + k];
+    }
+    uint32_t rcS = 0, rcQ = 0;
+    if (lane < kBandRows && y0 + lane < lv.h) {
+        const int64_t r = lv.rowOff + static_cast<int64_t>(y0 + lane) * lv.nChunks;
+        for (int k = 0; k < c; ++k) {
+            rcS += rowS[r + k];
+            rcQ += rowQ[r + k];
+        }
+    }
+
+    // integral along the band's top edge for this lane's four columns
+    uint32_t vS[4], vQ[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const bool in = x + k < lv.w;
+        vS[k] = in ? colS[lv.colOff + static_cast<int64_t>(b) * lv.w + x + k] : 0u;
+        vQ[k] = in ? colQ[lv.colOff + static_cast<int64_t>(b) * lv.w + x + k] : 0u;
+    }
+    const uint32_t tS = vS[0] + vS[1] + vS[2] + vS[3];
+    const uint64_t tQ = static_cast<uint64_t>(vQ[0]) + vQ[1] + vQ[2] + vQ[3];
+    const uint32_t eS = wave_incl_scan(tS) - tS;
+    const uint64_t eQ = wave_incl_scan(tQ) - tQ;
+    uint32_t aS[4];
+    uint64_t aQ[4];
+    {
+        uint32_t rs = ccS + eS;
+        uint64_t rq = ccQ + eQ;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            rs += vS[k];
+            rq += vQ[k];
+            aS[k] = rs;
+            aQ[k] = rq;
+        }
+    }
+    if (b == 0) {  // row 0 of the integral is zero
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (x + k < lv.w) {
+                outS[x + k + 1] = 0u;
+                outQ[x + k + 1] = 0u;
+            }
+        if (c == 0 && lane == 0) {
+            outS[0] = 0u;
+            outQ[0] = 0u;
+        }
+    }
+    for (int y = y0; y < y1; ++y) {
+        uint32_t p[4];
+        load4(lv.pix + static_cast<size_t>(y) * lv.stride, x, lv.w, p);
+        const uint32_t ls = p[0] + p[1] + p[2] + p[3];
+        const uint32_t lq = p[0] * p[0] + p[1] * p[1] + p[2] * p[2] + p[3] * p[3];
+        const uint32_t rowCarryS = __shfl(rcS, y - y0, kWave);
+        const uint32_t rowCarryQ = __shfl(rcQ, y - y0, kWave);
+        uint32_t rs = wave_incl_scan(ls) - ls + rowCarryS;
+        uint32_t rq = wave_incl_scan(lq) - lq + rowCarryQ;
+        uint32_t* rowOutS = outS + static_cast<int64_t>(y + 1) * iw;
+        uint64_t* rowOutQ = outQ + static_cast<int64_t>(y + 1) * iw;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            rs += p[k];
+            rq += p[k] * p[k];
+            aS[k] += rs;
+            aQ[k] += rq;
+            if (x + k < lv.w) {
+                rowOutS[x + k + 1] = aS[k];
+                rowOutQ[x + k + 1] = aQ[k];
+            }
+        }
+        if (c == 0 && lane == 0) {
+            rowOutS[0] = 0u;
+            rowOutQ[0] = 0u;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void batch_integral_kernel(const uint8_t* __restrict__ gray, int w, int h,
+                                                             int stride, int n, uint32_t* __restrict__ sumT,
+                                                             uint64_t* __restrict__ sqT) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) batch_integral_one(gray, w, h, stride, n, s, sumT, sqT);
+}
+
+__global__ __launch_bounds__(256) void invnorm_kernel(const uint32_t* __restrict__ sumT,
+                                                      const uint64_t* __restrict__ sqT, int iw, int ih, int n,
+                                                      float* __restrict__ inv) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) inv[s] = train_invnorm(sumT, sqT, iw, ih, n, s);
+}
+
+// Cascade evaluation, one workgroup per tile of 64 x tileRows windows of one
+// level.  The tile's integral footprint is staged in LDS as f32 (each
+// absolute u32 value converted once: the CPU path converts the same absolute
+// values, so every Haar sum rounds identically).  Windows failing the
+// variance gate never enter the live list; each stage then runs over the
+// live list only and the survivors are compacted by wavefront ballot, so
+// lanes stay busy however unevenly windows reject.  Survivors of the last
+// stage set their bit in the level's window mask and store their last stage
+// sum.  Stage rule and window normaliser: src/main.cu:752-804.
+__global__ __launch_bounds__(256) void cascade_kernel(const LevelDesc* __restrict__ L,
+                                                      const int32_t* __restrict__ lvlIdx,
+                                                      const int32_t* __restrict__ tileOffs, int nLv,
+                                                      int nTiles, int step,
+                                                      const uint32_t* __restrict__ iiS,
+                                                      const uint64_t* __restrict__ iiQ,
+                                                      const KStump* __restrict__ stumps,
+                                                      const KStage* __restrict__ stages, int nStages,
+                                                      int winW, int winH,
+                                                      uint64_t* __restrict__ words, float* __restrict__ scores) {
+    extern __shared__ float lds[];
+    __shared__ uint32_t listIdx[2][1024];
+    __shared__ float listInv[2][1024];
+    __shared__ int listCnt[2];
+    __shared__ unsigned long long maskWords[16];
+
+    // XCD-aware remap: consecutive tiles land on the same XCD (shared L2).
+    int t;
+    {
+        const int i = blockIdx.x, q = nTiles >> 3, r = nTiles & 7;
+        const int xcd = i & 7, pos = i >> 3;
+        t = (xcd < r) ? xcd * (q + 1) + pos : r * (q + 1) + (xcd - r) * q + pos;
+    }
+    const int j = find_segment(tileOffs, nLv, t);
+    const LevelDesc lv = L[lvlIdx[j]];
+    const int local = t - tileOffs[j];
+    const int ty = local / lv.tilesX;
+    const int tx = local - ty * lv.tilesX;
+    const int TH = lv.tileRows;
+    const int wx0 = tx * kTileWin, wy0 = ty * TH;
+    const int ix0 = wx0 * step, iy0 = wy0 * step;
+    const int FW = (kTileWin - 1) * step + winW + 1;
+    const int FH = (TH - 1) * step + winH + 1;
+    const int iw = lv.w + 1;
+    const uint32_t* sumL = iiS + lv.iiOff;
+    const uint64_t* sqL = iiQ + lv.iiOff;
+
+    for (int e = threadIdx.x; e < FW * FH; e += blockDim.x) {
+        const int r = e / FW, cc = e - r * FW;
+        const int gy = iy0 + r, gx = ix0 + cc;
+        float v = 0.0f;
+        if (gy <= lv.h && gx <= lv.w) v = static_cast<float>(static_cast<int32_t>(sumL[static_cast<int64_t>(gy) * iw + gx]));
+        lds[e] = v;
+    }
+    if (threadIdx.x < 16) maskWords[threadIdx.x] = 0ull;
+    if (threadIdx.x == 0) {
+        listCnt[0] = 0;
+        listCnt[1] = 0;
+    }
+    __syncthreads();
+
+    // variance gate (exact int sum, u32 square-sum delta, f64 normaliser)
+    const int nw = winW - 2, nh = winH - 2, narea = nw * nh;
+    const int64_t np0 = 1 + iw, np1 = (1 + nw) + iw;
+    const int64_t np2 = 1 + static_cast<int64_t>(iw) * (1 + nh), np3 = (1 + nw) + static_cast<int64_t>(iw) * (1 + nh);
+    const int lane = lane_id();
+    for (int base = 0; base < kTileWin * TH; base += blockDim.x) {
+        const int widx = base + threadIdx.x;
+        const int wx = wx0 + (widx & (kTileWin - 1));
+        const int wy = wy0 + (widx >> 6);
+        bool live = false;
+        float invNorm = 0.0f;
+        if (widx < kTileWin * TH && wx < lv.scanW && wy < lv.scanH) {
+            const int64_t o = static_cast<int64_t>(wy * step) * iw + wx * step;
+            const int32_t ns = static_cast<int32_t>(sumL[o + np0] - sumL[o + np1] - sumL[o + np2] + sumL[o + np3]);
+            const uint32_t nq = static_cast<uint32_t>(sqL[o + np0] - sqL[o + np1] - sqL[o + np2] + sqL[o + np3]);
+            const double normSum = static_cast<double>(ns);
+            const double nf2 = static_cast<double>(narea) * static_cast<double>(nq) - normSum * normSum;
+            if (nf2 > 0.0) {
+                invNorm = static_cast<float>(1.0 / sqrt(nf2));
+                live = static_cast<double>(narea) * static_cast<double>(invNorm) < 1e-1;
+            }
+        }
+        const uint64_t m = __ballot(live);
+        int wbase = 0;
+        if (lane == 0 && m) wbase = atomicAdd(&listCnt[0], __popcll(m));
+        wbase = __shfl(wbase, 0, kWave);
+        if (live) {
+            const int p = wbase + mask_rank(m);
+            listIdx[0][p] = static_cast<uint32_t>(widx);
+            listInv[0][p] = invNorm;
+        }
+    }
+    __syncthreads();
+
+    int cur = 0;
+    for (int si = 0; si < nStages; ++si) {
+        const int n = listCnt[cur];
+        if (n == 0) break;
+        const KStage st = stages[si];
+        const bool last = (si == nStages - 1);
+        for (int base = 0; base < n; base += blockDim.x) {
+            const int i = base + threadIdx.x;
+            bool pass = false;
+            float stageSum = 0.0f;
+            uint32_t widx = 0;
+            float invNorm = 0.0f;
+            if (i < n) {
+                widx = listIdx[cur][i];
+                invNorm = listInv[cur][i];
+                const int lx = static_cast<int>(widx & (kTileWin - 1)) * step;
+                const int ly = static_cast<int>(widx >> 6) * step;
+                const float* o = lds + ly * FW + lx;
+                for (int k = 0; k < st.ntrees; ++k) {
+                    const KStump sp = stumps[st.first + k];
+                    float raw = (((o[sp.off[0]] - o[sp.off[1]]) - o[sp.off[2]]) + o[sp.off[3]]) * sp.w[0] +
+                                (((o[sp.off[4]] - o[sp.off[5]]) - o[sp.off[6]]) + o[sp.off[7]]) * sp.w[1];
+                    if (sp.rectCount == 3)
+                        raw += (((o[sp.off[8]] - o[sp.off[9]]) - o[sp.off[10]]) + o[sp.off[11]]) * sp.w[2];
+                    stageSum += (raw * invNorm < sp.theta) ? sp.left : sp.right;
+                }
+                pass = !(stageSum < st.threshold);
+            }
+            if (last) {
+                if (pass) {
+                    const int wx = wx0 + static_cast<int>(widx & (kTileWin - 1));
+                    const int wy = wy0 + static_cast<int>(widx >> 6);
+                    const int64_t word = lv.wordOff + static_cast<int64_t>(wy) * lv.rowWords + tx;
+                    scores[word * 64 + (wx & 63)] = stageSum;
+                    atomicOr(&maskWords[widx >> 6], 1ull << (widx & 63));
+                }
+            } else {
+                const uint64_t m = __ballot(pass);
+                int wbase = 0;
+                if (lane == 0 && m) wbase = atomicAdd(&listCnt[cur ^ 1], __popcll(m));
+                wbase = __shfl(wbase, 0, kWave);
+                if (pass) {
+                    const int p = wbase + mask_rank(m);
+                    listIdx[cur ^ 1][p] = widx;
+                    listInv[cur ^ 1][p] = invNorm;
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) listCnt[cur] = 0;
+        cur ^= 1;
+        __syncthreads();
+    }
+
+    // every tile writes all of its mask words, so the mask needs no clearing
+    for (int r = threadIdx.x; r < TH; r += blockDim.x) {
+        const int wy = wy0 + r;
+        if (wy < lv.scanH) words[lv.wordOff + static_cast<int64_t>(wy) * lv.rowWords + tx] = maskWords[r];
+    }
+}
+
+// Per-image count of surviving windows (popcount over the image's words).
+__global__ __launch_bounds__(1024) void count_kernel(const int64_t* __restrict__ imgWordOff,
+                                                     const uint64_t* __restrict__ words,
+                                                     int32_t* __restrict__ counts) {
+    __shared__ int32_t part[16];
+    const int img = blockIdx.x;
+    const int64_t a = imgWordOff[img], b = imgWordOff[img + 1];
+    int32_t c = 0;
+    for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) c += __popcll(words[i]);
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t s = 0;
+        for (int k = 0; k < static_cast<int>(blockDim.x >> 6); ++k) s += part[k];
+        counts[img] = s;
+    }
+}
+
+// Order-preserving emission: image img's detections go to
+// out[sum_{j<img} min(count_j, cap) + rank] for rank < cap, in word order,
+// i.e. (level, y, x) scan order.  Box rounding: src/main.cu:806-813.
+__global__ __launch_bounds__(1024) void emit_kernel(const LevelDesc* __restrict__ L,
+                                                    const int64_t* __restrict__ lvWordOff, int nLevels,
+                                                    const int64_t* __restrict__ imgWordOff,
+                                                    const uint64_t* __restrict__ words,
+                                                    const float* __restrict__ scores,
+                                                    const int32_t* __restrict__ counts, int cap,
+                                                    int winW, int winH, vj_detection* __restrict__ out) {
+    __shared__ int32_t wavePart[16];
+    __shared__ int32_t outBase;
+    const int img = blockIdx.x;
+    if (threadIdx.x == 0) {
+        int32_t s = 0;
+        for (int k = 0; k < img; ++k) s += min(counts[k], cap);
+        outBase = s;
+    }
+    const int64_t a = imgWordOff[img], b = imgWordOff[img + 1];
+    const int64_t nWords = b - a;
+    const int64_t per = (nWords + blockDim.x - 1) / blockDim.x;
+    const int64_t my0 = a + min(nWords, per * threadIdx.x);
+    const int64_t my1 = a + min(nWords, per * (threadIdx.x + 1));
+    int32_t c = 0;
+    for (int64_t i = my0; i < my1; ++i) c += __popcll(words[i]);
+    const int32_t inc = wave_incl_scan(c);
+    if ((threadIdx.x & 63) == 63) wavePart[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    int32_t rank = inc - c;
+    for (int k = 0; k < static_cast<int>(threadIdx.x >> 6); ++k) rank += wavePart[k];
+    const int32_t obase = outBase;
+    if (rank >= cap) return;
+    int li = -1;
+    LevelDesc lv{};
+    for (int64_t i = my0; i < my1 && rank < cap; ++i) {
+        uint64_t m = words[i];
+        if (!m) continue;
+        if (li < 0 || i < lv.wordOff || i >= lv.wordOff + static_cast<int64_t>(lv.scanH) * lv.rowWords) {
+            li = find_segment64(lvWordOff, nLevels, i);
+            lv = L[li];
+        }
+        const int64_t rel = i - lv.wordOff;
+        const int wy = static_cast<int>(rel / lv.rowWords);
+        const int wxw = static_cast<int>(rel - static_cast<int64_t>(wy) * lv.rowWords);
+        while (m && rank < cap) {
+            const int bit = __builtin_ctzll(m);
+            m &= m - 1;
+            const int wx = wxw * 64 + bit;
+            vj_detection d;
+            d.x = static_cast<int>(static_cast<float>(wx * lv.step) * lv.scale + 0.5f);
+            d.y = static_cast<int>(static_cast<float>(wy * lv.step) * lv.scale + 0.5f);
+            d.w = static_cast<int>(static_cast<float>(winW) * lv.scale + 0.5f);
+            d.h = static_cast<int>(static_cast<float>(winH) * lv.scale + 0.5f);
+            d.scaleIdx = lv.scaleIdx;
+            d.score = scores[i * 64 + bit];
+            out[obase + rank] = d;
+            ++rank;
+        }
+    }
+}
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+
+struct Plan {
+    std::vector<LevelDesc> levels;
+    std::vector<float> rs;          // per level: resize scale x, y
+    std::vector<int32_t> src;       // per level: source level index (itself for level 0)
+    std::vector<int32_t> itemOffs, chunkOffs;
+    std::vector<int64_t> lvWordOff;
+    std::vector<int64_t> imgWordOff;
+    std::vector<std::vector<ResizeJob>> resizeJobs;  // per depth
+    std::vector<int32_t> resizeItems;                // per depth
+    std::vector<int32_t> stepLvl[3], stepTileOff[3];
+    int32_t stepTiles[3] = {0, 0, 0};
+    int32_t nItems = 0, nChunkItems = 0;
+    int64_t pyrBytes = 0, iiElems = 0, colElems = 0, totElems = 0, rowElems = 0, nWords = 0;
+    int64_t windows = 0, pyramidPixels = 0, integralBytes = 0, cascadeBytes = 0;
+    std::vector<int32_t> imgLevels;  // levels per image
+    std::vector<uint8_t> inPyr;      // per level: pixels live in the pyramid buffer
+    std::vector<int64_t> pyrOffs;    // pyramid byte offsets of the inPyr levels, in order
+};
+
+}  // namespace vj
+
+using namespace vj;
+
+struct vj_engine {
+    int winW = 24, winH = 24, maxSamples = 1, maxW = 1, maxH = 1, maxDet = 1;
+    bool tilted = false;
+
+    DevBuf<vj_feature> pool;
+    int poolN = 0;
+
+    // integral batch (API view)
+    DevBuf<uint32_t> bSum;
+    DevBuf<uint64_t> bSq;
+    DevBuf<float> bInv;
+    int curN = 0, curW = 0, curH = 0;
+
+    // cascade model
+    DevBuf<vj_feature> modelFeat;
+    DevBuf<KStump> kst;
+    DevBuf<KStage> kstage;
+    int nModelFeat = 0, nStumps = 0, nStages = 0;
+
+    // detect work buffers
+    DevBuf<uint8_t> pyr;
+    DevBuf<uint32_t> iiS, colS, colQ, rowS, rowQ, totS;
+    DevBuf<uint64_t> iiQ, totQ;
+    DevBuf<uint64_t> words;
+    DevBuf<float> scores;
+    DevBuf<LevelDesc> dLevels;
+    DevBuf<float> dRs;
+    DevBuf<int32_t> dMeta32;
+    DevBuf<int64_t> dMeta64;
+    DevBuf<ResizeJob> dJobs;
+    DevBuf<int32_t> dCounts;
+    DevBuf<vj_detection> dOut;
+    PinnedBuf<vj_detection> hOut;
+    PinnedBuf<int32_t> hCounts;
+    PinnedBuf<uint8_t> hStage;  // staging for plan uploads
+
+    vj_detect_stats stats{};
+};
+
+namespace vj {
+
+namespace {
+
+inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+
+// Pyramid schedule of one image, src/face_vision_engine.cu:1596-1714 (and the
+// CPU loop src/main.cu:717-747): float scale, round(W/scale), skip levels
+// whose object size is under minObjectSize, stop below the window.
+struct LevelSpec {
+    int scaleIdx, w, h, step;
+    float scale;
+};
+
+std::vector<LevelSpec> schedule(int W, int H, int winW, int winH, float sf, int minObj) {
+    std::vector<LevelSpec> out;
+    float scale = 1.0f;
+    int scaleIdx = 0;
+    for (;;) {
+        const int sw = static_cast<int>(std::round(static_cast<float>(W) / scale));
+        const int sh = static_cast<int>(std::round(static_cast<float>(H) / scale));
+        const int ow = static_cast<int>(std::round(static_cast<float>(winW) * scale));
+        const int oh = static_cast<int>(std::round(static_cast<float>(winH) * scale));
+        if (sw < winW || sh < winH) break;
+        if (ow < minObj || oh < minObj) {
+            scale *= sf;
+            ++scaleIdx;
+            continue;
+        }
+        out.push_back({scaleIdx, sw, sh, (scale >= 3.0f) ? 3 : ((scale >= 2.0f) ? 2 : 1), scale});
+        scale *= sf;
+        ++scaleIdx;
+        if (scaleIdx > 100000) break;
+    }
+    return out;
+}
+
+struct ImageIn {
+    const uint8_t* pix;
+    int w, h, stride;
+};
+
+// Builds the level table for a set of images.  Level 0 of an image is the
+// input itself (stride as given); each further level is resized from the
+// previous scanned level (or from the input when level 0 is skipped).
+// The first image whose level 0 is skipped still needs its input as a
+// source, so such images get an unscanned "source" level entry.
+int build_plan(const vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int minObj, Plan& P) {
+    P = Plan();
+    const int winW = e->winW, winH = e->winH;
+    int64_t pyrOff = 0;
+    P.imgWordOff.push_back(0);
+    int maxDepth = 0;
+    struct Pending {
+        int level, depth;
+    };
+    std::vector<Pending> resized;
+    auto push_level = [&P](const LevelDesc& d, int src, float rsx, float rsy, bool inPyr) {
+        P.levels.push_back(d);
+        P.rs.push_back(rsx);
+        P.rs.push_back(rsy);
+        const int idx = static_cast<int>(P.levels.size()) - 1;
+        P.src.push_back(src < 0 ? idx : src);
+        P.inPyr.push_back(inPyr ? 1 : 0);
+        return idx;
+    };
+    for (size_t im = 0; im < imgs.size(); ++im) {
+        const ImageIn& in = imgs[im];
+        const std::vector<LevelSpec> sch = schedule(in.w, in.h, winW, winH, sf, minObj);
+        P.imgLevels.push_back(static_cast<int32_t>(sch.size()));
+        int prev = -1;  // previous scanned level of this image (its resize source)
+        int depth = 0;
+        for (const LevelSpec& ls : sch) {
+            LevelDesc d{};
+            d.w = ls.w;
+            d.h = ls.h;
+            d.step = ls.step;
+            d.scale = ls.scale;
+            d.scaleIdx = ls.scaleIdx;
+            d.image = static_cast<int32_t>(im);
+            if (ls.scaleIdx == 0) {
+                d.pix = in.pix;
+                d.stride = in.stride;
+                prev = push_level(d, -1, 1.0f, 1.0f, false);
+                continue;
+            }
+            if (prev < 0) {
+                // level 0 was skipped: the input itself is the first source,
+                // as an entry that is resized from but never scanned (step 0)
+                LevelDesc s0{};
+                s0.pix = in.pix;
+                s0.w = in.w;
+                s0.h = in.h;
+                s0.stride = in.stride;
+                s0.image = static_cast<int32_t>(im);
+                prev = push_level(s0, -1, 1.0f, 1.0f, false);
+            }
+            const LevelDesc& sl = P.levels[static_cast<size_t>(prev)];
+            const float rsx = static_cast<float>(sl.w) / static_cast<float>(ls.w);
+            const float rsy = static_cast<float>(sl.h) / static_cast<float>(ls.h);
+            d.stride = static_cast<int32_t>(align_up(ls.w, 64));
+            d.pix = nullptr;
+            P.pyrOffs.push_back(pyrOff);
+            pyrOff += align_up(static_cast<int64_t>(d.stride) * ls.h, 256);
+            ++depth;
+            const int idx = push_level(d, prev, rsx, rsy, true);
+            resized.push_back({idx, depth});
+            maxDepth = std::max(maxDepth, depth);
+            prev = idx;
+        }
+    }
+    // per-level work layout (scanned levels only; source-only entries get empty ranges)
+    int64_t iiOff = 0, colOff = 0, totOff = 0, rowOff = 0, wordOff = 0;
+    int32_t items = 0, chunkItems = 0;
+    int32_t tiles[3] = {0, 0, 0};
+    int curImg = 0;
+    for (size_t li = 0; li < P.levels.size(); ++li) {
+        LevelDesc& d = P.levels[li];
+        while (curImg < d.image) {
+            P.imgWordOff.push_back(wordOff);
+            ++curImg;
+        }
+        d.itemOff = items;
+        d.chunkItemOff = chunkItems;
+        d.iiOff = iiOff;
+        d.colOff = colOff;
+        d.totOff = totOff;
+        d.rowOff = rowOff;
+        d.wordOff = wordOff;
+        d.tileOff = 0;
+        if (d.step == 0) {  // source-only entry
+            d.nBands = d.nChunks = 0;
+            d.scanW = d.scanH = d.rowWords = 0;
+            d.tilesX = d.tilesY = 0;
+            d.tileRows = 1;
+            P.itemOffs.push_back(items);
+            P.chunkOffs.push_back(chunkItems);
+            P.lvWordOff.push_back(wordOff);
+            continue;
+        }
+        d.nBands = (d.h + kBandRows - 1) / kBandRows;
+        d.nChunks = (d.w + kChunkCols - 1) / kChunkCols;
+        d.scanW = (d.w - e->winW + 1 + d.step - 1) / d.step;
+        d.scanH = (d.h - e->winH + 1 + d.step - 1) / d.step;
+        d.rowWords = (d.scanW + 63) / 64;
+        d.tileRows = tile_rows_for_step(d.step);
+        d.tilesX = d.rowWords;
+        d.tilesY = (d.scanH + d.tileRows - 1) / d.tileRows;
+        P.itemOffs.push_back(items);
+        P.chunkOffs.push_back(chunkItems);
+        P.lvWordOff.push_back(wordOff);
+        items += d.nBands * d.nChunks;
+        chunkItems += d.nChunks;
+        iiOff += align_up(static_cast<int64_t>(d.w + 1) * (d.h + 1), 64);
+        colOff += static_cast<int64_t>(d.nBands) * d.w;
+        totOff += static_cast<int64_t>(d.nBands) * d.nChunks;
+        rowOff += static_cast<int64_t>(d.h) * d.nChunks;
+        wordOff += static_cast<int64_t>(d.scanH) * d.rowWords;
+        const int si = d.step - 1;
+        P.stepLvl[si].push_back(static_cast<int32_t>(li));
+        P.stepTileOff[si].push_back(tiles[si]);
+        d.tileOff = tiles[si];
+        tiles[si] += d.tilesX * d.tilesY;
+        P.windows += static_cast<int64_t>(d.scanW) * d.scanH;
+        P.pyramidPixels += static_cast<int64_t>(d.w) * d.h;
+        P.integralBytes += static_cast<int64_t>(d.w) * d.h + 12ll * (d.w + 1) * (d.h + 1);
+        if (d.scaleIdx != 0) P.integralBytes += static_cast<int64_t>(d.w) * d.h;  // resize write
+        P.cascadeBytes += 12ll * (d.w + 1) * (d.h + 1);
+    }
+    while (curImg < static_cast<int>(imgs.size())) {
+        P.imgWordOff.push_back(wordOff);
+        ++curImg;
+    }
+    // resize reads: each resized level reads its source once
+    for (const Pending& r : resized) {
+        const LevelDesc& s = P.levels[static_cast<size_t>(P.src[static_cast<size_t>(r.level)])];
+        P.integralBytes += static_cast<int64_t>(s.w) * s.h;
+    }
+    P.nItems = items;
+    P.nChunkItems = chunkItems;
+    for (int s = 0; s < 3; ++s) P.stepTiles[s] = tiles[s];
+    P.pyrBytes = pyrOff;
+    P.iiElems = iiOff;
+    P.colElems = colOff;
+    P.totElems = totOff;
+    P.rowElems = rowOff;
+    P.nWords = wordOff;
+    P.resizeJobs.assign(static_cast<size_t>(maxDepth), {});
+    P.resizeItems.assign(static_cast<size_t>(maxDepth), 0);
+    for (const Pending& r : resized) {
+        const LevelDesc& d = P.levels[static_cast<size_t>(r.level)];
+        const int segs = (d.w + 1023) / 1024;
+        auto& jobs = P.resizeJobs[static_cast<size_t>(r.depth - 1)];
+        int32_t& n = P.resizeItems[static_cast<size_t>(r.depth - 1)];
+        jobs.push_back({r.level, n});
+        n += segs * d.h;
+    }
+    return VJ_OK;
+}
+
+int check_image_limits(const vj_engine* e, int w, int h) {
+    VJ_CHECK(w <= e->maxW && h <= e->maxH, "image larger than the engine's maxImageW x maxImageH");
+    VJ_CHECK(static_cast<int64_t>(w) * h * 255 < (int64_t{1} << 31),
+             "image too large for exact 32-bit integrals (w*h*255 must be < 2^31)");
+    VJ_CHECK(h <= kBandRows * 256 && w <= 8192, "image dimensions above 8192 are not supported");
+    return VJ_OK;
+}
+
+template <typename T>
+int upload(DevBuf<T>& dst, const std::vector<T>& v, PinnedBuf<uint8_t>& staging, size_t& stOff,
+           hipStream_t s) {
+    VJ_TRY(dst.ensure(v.size()));
+    if (v.empty()) return VJ_OK;
+    const size_t bytes = v.size() * sizeof(T);
+    std::memcpy(staging.get() + stOff, v.data(), bytes);
+    VJ_HIP(hipMemcpyAsync(dst.get(), staging.get() + stOff, bytes, hipMemcpyHostToDevice, s));
+    stOff += align_up(static_cast<int64_t>(bytes), 256);
+    return VJ_OK;
+}
+
+// Runs the whole detect pipeline for the images of a plan; leaves per-image
+// counts in e->dCounts and detections (capped per image) in e->dOut.
+int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
+    // buffers
+    VJ_TRY(e->pyr.ensure(static_cast<size_t>(std::max<int64_t>(P.pyrBytes, 1))));
+    {
+        size_t k = 0;
+        for (size_t li = 0; li < P.levels.size(); ++li)
+            if (P.inPyr[li]) P.levels[li].pix = e->pyr.get() + P.pyrOffs[k++];
+    }
+    VJ_TRY(e->iiS.ensure(static_cast<size_t>(P.iiElems)));
+    VJ_TRY(e->iiQ.ensure(static_cast<size_t>(P.iiElems)));
+    VJ_TRY(e->colS.ensure(static_cast<size_t>(P.colElems)));
+    VJ_TRY(e->colQ.ensure(static_cast<size_t>(P.colElems)));
+    VJ_TRY(e->totS.ensure(static_cast<size_t>(P.totElems)));
+    VJ_TRY(e->totQ.ensure(static_cast<size_t>(P.totElems)));
+    VJ_TRY(e->rowS.ensure(static_cast<size_t>(P.rowElems)));
+    VJ_TRY(e->rowQ.ensure(static_cast<size_t>(P.rowElems)));
+    VJ_TRY(e->words.ensure(static_cast<size_t>(P.nWords)));
+    VJ_TRY(e->scores.ensure(static_cast<size_t>(P.nWords) * 64));
+    const int nImages = static_cast<int>(P.imgWordOff.size()) - 1;
+    VJ_TRY(e->dCounts.ensure(static_cast<size_t>(nImages)));
+    VJ_TRY(e->dOut.ensure(static_cast<size_t>(cap) * nImages));
+
+    // plan upload (one staging buffer, several async copies)
+    std::vector<int32_t> meta32;
+    const size_t oItem = meta32.size();
+    meta32.insert(meta32.end(), P.itemOffs.begin(), P.itemOffs.end());
+    const size_t oChunk = meta32.size();
+    meta32.insert(meta32.end(), P.chunkOffs.begin(), P.chunkOffs.end());
+    const size_t oSrc = meta32.size();
+    meta32.insert(meta32.end(), P.src.begin(), P.src.end());
+    size_t oLvl[3], oTile[3];
+    for (int k = 0; k < 3; ++k) {
+        oLvl[k] = meta32.size();
+        meta32.insert(meta32.end(), P.stepLvl[k].begin(), P.stepLvl[k].end());
+        oTile[k] = meta32.size();
+        meta32.insert(meta32.end(), P.stepTileOff[k].begin(), P.stepTileOff[k].end());
+    }
+    std::vector<int64_t> meta64;
+    const size_t oLvWord = meta64.size();
+    meta64.insert(meta64.end(), P.lvWordOff.begin(), P.lvWordOff.end());
+    const size_t oImgWord = meta64.size();
+    meta64.insert(meta64.end(), P.imgWordOff.begin(), P.imgWordOff.end());
+    std::vector<ResizeJob> jobs;
+    std::vector<size_t> jobStart;
+    for (const auto& jv : P.resizeJobs) {
+        jobStart.push_back(jobs.size());
+        jobs.insert(jobs.end(), jv.begin(), jv.end());
+    }
+    const size_t stageBytes = P.levels.size() * sizeof(LevelDesc) + P.rs.size() * 4 + meta32.size() * 4 +
+                              meta64.size() * 8 + jobs.size() * sizeof(ResizeJob) + 6 * 256;
+    VJ_TRY(e->hStage.ensure(stageBytes));
+    size_t stOff = 0;
+    VJ_HIP(hipStreamSynchronize(s));  // staging buffer reuse across calls
+    VJ_TRY(upload(e->dLevels, P.levels, e->hStage, stOff, s));
+    VJ_TRY(upload(e->dRs, P.rs, e->hStage, stOff, s));
+    VJ_TRY(upload(e->dMeta32, meta32, e->hStage, stOff, s));
+    VJ_TRY(upload(e->dMeta64, meta64, e->hStage, stOff, s));
+    if (!jobs.empty()) VJ_TRY(upload(e->dJobs, jobs, e->hStage, stOff, s));
+
+    const LevelDesc* dL = e->dLevels.get();
+    const int32_t* m32 = e->dMeta32.get();
+    const int64_t* m64 = e->dMeta64.get();
+    const int nLevels = static_cast<int>(P.levels.size());
+
+    // 1. pyramid: one launch per resize depth (each depth needs the previous)
+    for (size_t dpt = 0; dpt < P.resizeJobs.size(); ++dpt) {
+        const int n = static_cast<int>(P.resizeJobs[dpt].size());
+        if (!n || !P.resizeItems[dpt]) continue;
+        hipLaunchKernelGGL(resize_kernel, dim3(P.resizeItems[dpt]), dim3(256), 0, s, dL,
+                           e->dJobs.get() + jobStart[dpt], n, e->dRs.get(), m32 + oSrc);
+        VJ_HIP(hipGetLastError());
+    }
+    // 2. integrals of every level
+    if (P.nItems > 0) {
+        hipLaunchKernelGGL(ii_colsum_kernel, dim3(P.nItems), dim3(64), 0, s, dL, m32 + oItem, nLevels,
+                           e->colS.get(), e->colQ.get(), e->rowS.get(), e->rowQ.get());
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(ii_vscan_kernel, dim3(P.nChunkItems), dim3(256), 0, s, dL, m32 + oChunk, nLevels,
+                           e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get());
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(ii_write_kernel, dim3(P.nItems), dim3(64), 0, s, dL, m32 + oItem, nLevels,
+                           e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get(), e->rowS.get(),
+                           e->rowQ.get(), e->iiS.get(), e->iiQ.get());
+        VJ_HIP(hipGetLastError());
+    }
+    // 3. cascade, one launch per window step (tile footprint differs)
+    for (int k = 0; k < 3; ++k) {
+        if (!P.stepTiles[k]) continue;
+        const int step = k + 1;
+        const int TH = tile_rows_for_step(step);
+        const int FW = (kTileWin - 1) * step + e->winW + 1;
+        const int FH = (TH - 1) * step + e->winH + 1;
+        const size_t ldsBytes = static_cast<size_t>(FW) * FH * sizeof(float);
+        hipLaunchKernelGGL(cascade_kernel, dim3(P.stepTiles[k]), dim3(256), ldsBytes, s, dL, m32 + oLvl[k],
+                           m32 + oTile[k], static_cast<int>(P.stepLvl[k].size()), P.stepTiles[k], step,
+                           e->iiS.get(), e->iiQ.get(), e->kst.get() + static_cast<size_t>(k) * e->nStumps,
+                           e->kstage.get(), e->nStages, e->winW, e->winH, e->words.get(), e->scores.get());
+        VJ_HIP(hipGetLastError());
+    }
+    // 4. counts and order-preserving emission
+    hipLaunchKernelGGL(count_kernel, dim3(nImages), dim3(1024), 0, s, m64 + oImgWord, e->words.get(),
+                       e->dCounts.get());
+    VJ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(emit_kernel, dim3(nImages), dim3(1024), 0, s, dL, m64 + oLvWord, nLevels,
+                       m64 + oImgWord, e->words.get(), e->scores.get(), e->dCounts.get(), cap, e->winW,
+                       e->winH, e->dOut.get());
+    VJ_HIP(hipGetLastError());
+    return VJ_OK;
+}
+
+}  // namespace
+}  // namespace vj
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+extern "C" int vj_engine_create(int winW, int winH, int maxSamples, int maxImageW, int maxImageH,
+                                int enableTilted, int maxDetections, vj_engine** out) {
+    // src/face_vision_engine.cu:659-661: every size must be positive.
+    VJ_CHECK(out, "vj_engine_create: null output");
+    *out = nullptr;
+    VJ_CHECK(winW >= 3 && winH >= 3 && winW <= 64 && winH <= 64, "vj_engine_create: window must be 3..64");
+    VJ_CHECK(maxSamples > 0 && maxImageW > 0 && maxImageH > 0 && maxDetections > 0,
+             "vj_engine_create: invalid constructor args");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        set_error("vj_engine_create: no HIP device");
+        return VJ_DEVICE_ERROR;
+    }
+    vj_engine* e = new (std::nothrow) vj_engine();
+    if (!e) {
+        set_error("vj_engine_create: out of host memory");
+        return VJ_OUT_OF_MEMORY;
+    }
+    e->winW = winW;
+    e->winH = winH;
+    e->maxSamples = maxSamples;
+    e->maxW = maxImageW;
+    e->maxH = maxImageH;
+    e->tilted = enableTilted != 0;
+    e->maxDet = maxDetections;
+    int st = e->hCounts.ensure(1);
+    if (st == VJ_OK) st = e->hOut.ensure(static_cast<size_t>(std::min(maxDetections, 1 << 16)));
+    if (st != VJ_OK) {
+        delete e;
+        return st;
+    }
+    *out = e;
+    return VJ_OK;
+}
+
+extern "C" void vj_engine_destroy(vj_engine* e) { delete e; }
+
+extern "C" int vj_engine_upload_feature_pool(vj_engine* e, const vj_feature* hFeatures, int n, vj_stream s) {
+    VJ_CHECK(e && hFeatures && n > 0, "uploadFeaturePool: empty pool");
+    VJ_TRY(e->pool.ensure(static_cast<size_t>(n)));
+    VJ_HIP(hipMemcpyAsync(e->pool.get(), hFeatures, sizeof(vj_feature) * static_cast<size_t>(n),
+                          hipMemcpyHostToDevice, static_cast<hipStream_t>(s)));
+    VJ_HIP(hipStreamSynchronize(static_cast<hipStream_t>(s)));
+    e->poolN = n;
+    return VJ_OK;
+}
+
+extern "C" int vj_engine_feature_pool(vj_engine* e, const vj_feature** d, int* n) {
+    VJ_CHECK(e && d && n, "vj_engine_feature_pool: null");
+    *d = e->pool.get();
+    *n = e->poolN;
+    return VJ_OK;
+}
+
+extern "C" int vj_engine_integral_batch(vj_engine* e, const uint8_t* dGray, int w, int h, int stride,
+                                        int n, int computeInvNorm, vj_stream sv) {
+    // src/face_vision_engine.cu:1064-1069
+    VJ_CHECK(e && dGray && w > 0 && h > 0 && stride >= w && n > 0, "computeIntegralBatch: invalid arguments");
+    VJ_CHECK(w <= e->maxW && h <= e->maxH && n <= e->maxSamples, "computeIntegralBatch: exceeds engine limits");
+    hipStream_t s = static_cast<hipStream_t>(sv);
+    const size_t area = static_cast<size_t>(w + 1) * (h + 1);
+    VJ_TRY(e->bSum.ensure(area * n));
+    VJ_TRY(e->bSq.ensure(area * n));
+    VJ_TRY(e->bInv.ensure(static_cast<size_t>(n)));
+    if (n == 1 && (w > 64 || h > 64)) {
+        // a single image goes through the detect integral kernels (bands of
+        // rows in parallel) straight into the view buffers
+        VJ_TRY(check_image_limits(e, w, h));
+        LevelDesc d{};
+        d.pix = dGray;
+        d.w = w;
+        d.h = h;
+        d.stride = stride;
+        d.step = 1;
+        d.nBands = (h + kBandRows - 1) / kBandRows;
+        d.nChunks = (w + kChunkCols - 1) / kChunkCols;
+        VJ_TRY(e->colS.ensure(static_cast<size_t>(d.nBands) * w));
+        VJ_TRY(e->colQ.ensure(static_cast<size_t>(d.nBands) * w));
+        VJ_TRY(e->totS.ensure(static_cast<size_t>(d.nBands) * d.nChunks));
+        VJ_TRY(e->totQ.ensure(static_cast<size_t>(d.nBands) * d.nChunks));
+        VJ_TRY(e->rowS.ensure(static_cast<size_t>(h) * d.nChunks));
+        VJ_TRY(e->rowQ.ensure(static_cast<size_t>(h) * d.nChunks));
+        VJ_TRY(e->hStage.ensure(sizeof(LevelDesc) + 4 * 256));
+        VJ_HIP(hipStreamSynchronize(s));
+        size_t stOff = 0;
+        std::vector<LevelDesc> lv{d};
+        std::vector<int32_t> offs{0, 0};
+        VJ_TRY(upload(e->dLevels, lv, e->hStage, stOff, s));
+        VJ_TRY(upload(e->dMeta32, offs, e->hStage, stOff, s));
+        const int items = d.nBands * d.nChunks;
+        hipLaunchKernelGGL(ii_colsum_kernel, dim3(items), dim3(64), 0, s, e->dLevels.get(), e->dMeta32.get(), 1,
+                           e->colS.get(), e->colQ.get(), e->rowS.get(), e->rowQ.get());
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(ii_vscan_kernel, dim3(d.nChunks), dim3(256), 0, s, e->dLevels.get(),
+                           e->dMeta32.get() + 1, 1, e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get());
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(ii_write_kernel, dim3(items), dim3(64), 0, s, e->dLevels.get(), e->dMeta32.get(), 1,
+                           e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get(), e->rowS.get(),
+                           e->rowQ.get(), e->bSum.get(), e->bSq.get());
+        VJ_HIP(hipGetLastError());
+    } else {
+        hipLaunchKernelGGL(batch_integral_kernel, dim3((n + 255) / 256), dim3(256), 0, s, dGray, w, h, stride, n,
+                           e->bSum.get(), e->bSq.get());
+        VJ_HIP(hipGetLastError());
+    }
+    if (computeInvNorm) {
+        hipLaunchKernelGGL(invnorm_kernel, dim3((n + 255) / 256), dim3(256), 0, s, e->bSum.get(), e->bSq.get(),
+                           w + 1, h + 1, n, e->bInv.get());
+        VJ_HIP(hipGetLastError());
+    }
+    e->curN = n;
+    e->curW = w;
+    e->curH = h;
+    return VJ_OK;
+}
+
+extern "C" int vj_engine_integral_view(vj_engine* e, vj_integral_view* out) {
+    VJ_CHECK(e && out, "integralSetView: null");
+    out->nSamples = e->curN;
+    out->iiWidth = e->curW + 1;
+    out->iiHeight = e->curH + 1;
+    out->iiArea = out->iiWidth * out->iiHeight;
+    out->sumT = e->bSum.get();
+    out->sqsumT = e->bSq.get();
+    out->invNorm = e->bInv.get();
+    return VJ_OK;
+}
+
+extern "C" int vj_engine_set_cascade(vj_engine* e, const vj_feature* dFeatures, int nF, const vj_stump* dStumps,
+                                     int nS, const vj_stage* hStages, int nSt, vj_stream sv) {
+    // src/face_vision_engine.cu:1182-1185
+    VJ_CHECK(e && dFeatures && dStumps && hStages && nF > 0 && nS > 0 && nSt > 0,
+             "setCascadeModel: invalid arguments");
+    hipStream_t s = static_cast<hipStream_t>(sv);
+    std::vector<vj_feature> hf(static_cast<size_t>(nF));
+    std::vector<vj_stump> hs(static_cast<size_t>(nS));
+    VJ_HIP(hipMemcpyAsync(hf.data(), dFeatures, sizeof(vj_feature) * hf.size(), hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipMemcpyAsync(hs.data(), dStumps, sizeof(vj_stump) * hs.size(), hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < nSt; ++k) {
+        VJ_CHECK(hStages[k].first >= 0 && hStages[k].ntrees >= 0 &&
+                     static_cast<int64_t>(hStages[k].first) + hStages[k].ntrees <= nS,
+                 "setCascadeModel: stage range outside the stump array");
+    }
+    const int mW = e->winW + 1;
+    const int area = mW * (e->winH + 1);
+    // LUT of corner offsets for each step's LDS pitch (the detector decodes
+    // p into (p / (winW+1), p % (winW+1)) as src/face_vision_engine.cu:360-365)
+    std::vector<KStump> ks(static_cast<size_t>(nS) * 3);
+    for (int i = 0; i < nS; ++i) {
+        const vj_stump& st = hs[static_cast<size_t>(i)];
+        VJ_CHECK(st.feature >= 0 && st.feature < nF, "setCascadeModel: stump feature index out of range");
+        const vj_feature& f = hf[static_cast<size_t>(st.feature)];
+        VJ_CHECK(f.rectCount == 2 || f.rectCount == 3, "setCascadeModel: feature rectCount must be 2 or 3");
+        for (int k = 0; k < 3; ++k) {
+            const int pitch = (kTileWin - 1) * (k + 1) + e->winW + 1;
+            KStump& o = ks[static_cast<size_t>(k) * nS + i];
+            std::memset(&o, 0, sizeof o);
+            for (int r = 0; r < 3; ++r) {
+                const int32_t p[4] = {f.r[r].p0, f.r[r].p1, f.r[r].p2, f.r[r].p3};
+                for (int c = 0; c < 4; ++c) {
+                    int v = p[c];
+                    if (r >= f.rectCount) v = 0;
+                    VJ_CHECK(v >= 0 && v < area, "setCascadeModel: feature corner outside the window");
+                    const int dy = v / mW, dx = v - dy * mW;
+                    o.off[r * 4 + c] = static_cast<uint16_t>(dy * pitch + dx);
+                }
+                o.w[r] = (r < f.rectCount) ? f.r[r].w : 0.0f;
+            }
+            o.theta = st.theta;
+            o.left = st.left;
+            o.right = st.right;
+            o.rectCount = f.rectCount;
+        }
+    }
+    std::vector<KStage> kg(static_cast<size_t>(nSt));
+    for (int k = 0; k < nSt; ++k) kg[static_cast<size_t>(k)] = {hStages[k].first, hStages[k].ntrees, hStages[k].threshold, 0};
+    VJ_TRY(e->modelFeat.ensure(static_cast<size_t>(nF)));
+    VJ_TRY(e->kst.ensure(ks.size()));
+    VJ_TRY(e->kstage.ensure(kg.size()));
+    VJ_HIP(hipMemcpyAsync(e->modelFeat.get(), dFeatures, sizeof(vj_feature) * nF, hipMemcpyDeviceToDevice, s));
+    VJ_HIP(hipMemcpyAsync(e->kst.get(), ks.data(), sizeof(KStump) * ks.size(), hipMemcpyHostToDevice, s));
+    VJ_HIP(hipMemcpyAsync(e->kstage.get(), kg.data(), sizeof(KStage) * kg.size(), hipMemcpyHostToDevice, s));
+    VJ_HIP(hipStreamSynchronize(s));
+    e->nModelFeat = nF;
+    e->nStumps = nS;
+    e->nStages = nSt;
+    return VJ_OK;
+}
+
+extern "C" int vj_engine_clear_cascade(vj_engine* e, vj_stream) {
+    VJ_CHECK(e, "clearCascadeModel: null");
+    e->modelFeat.release();
+    e->kst.release();
+    e->kstage.release();
+    e->nModelFeat = e->nStumps = e->nStages = 0;
+    return VJ_OK;
+}
+
+namespace {
+
+int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int minNeighbors, int minObj,
+                  int maxDet, int applyGrouping, vj_detection* hOut, int cap, int* hCounts, bool batch,
+                  hipStream_t s) {
+    VJ_CHECK(sf > 1.0f && minObj >= 1, "detectMultiScale: scaleFactor must be > 1 and minObjectSize >= 1");
+    VJ_CHECK(e->nStages > 0 && e->kst.get(), "detectMultiScale: no cascade model set");
+    for (const ImageIn& in : imgs) {
+        VJ_CHECK(in.pix && in.w > 0 && in.h > 0 && in.stride >= in.w, "detectMultiScale: invalid image");
+        VJ_TRY(check_image_limits(e, in.w, in.h));
+    }
+    const int capPer = std::max(1, std::min(maxDet, e->maxDet));
+    Plan P;
+    VJ_TRY(build_plan(e, imgs, sf, minObj, P));
+    const int nImages = static_cast<int>(imgs.size());
+    VJ_TRY(run_pipeline(e, P, capPer, s));
+    VJ_TRY(e->hCounts.ensure(static_cast<size_t>(nImages)));
+    VJ_HIP(hipMemcpyAsync(e->hCounts.get(), e->dCounts.get(), sizeof(int32_t) * nImages, hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipStreamSynchronize(s));
+    int64_t total = 0;
+    int64_t raw = 0;
+    for (int i = 0; i < nImages; ++i) {
+        raw += e->hCounts.get()[i];
+        total += std::min(e->hCounts.get()[i], capPer);
+    }
+    VJ_TRY(e->hOut.ensure(static_cast<size_t>(std::max<int64_t>(total, 1))));
+    if (total > 0) {
+        VJ_HIP(hipMemcpyAsync(e->hOut.get(), e->dOut.get(), sizeof(vj_detection) * total, hipMemcpyDeviceToHost, s));
+        VJ_HIP(hipStreamSynchronize(s));
+    }
+    e->stats.levels = 0;
+    for (int32_t n : P.imgLevels) e->stats.levels += n;
+    e->stats.windows = P.windows;
+    e->stats.rawDetections = raw;
+    e->stats.pyramidPixels = P.pyramidPixels;
+    e->stats.integralBytes = P.integralBytes;
+    e->stats.cascadeBytes = P.cascadeBytes + static_cast<int64_t>(e->nStumps) * 64 + 24 * total;
+
+    // per image: optional grouping, then copy out
+    int64_t inPos = 0, outPos = 0;
+    std::vector<vj_detection> grouped;
+    for (int i = 0; i < nImages; ++i) {
+        const int n = std::min(e->hCounts.get()[i], capPer);
+        const vj_detection* src = e->hOut.get() + inPos;
+        inPos += n;
+        int m = n;
+        if (applyGrouping) {
+            grouped.resize(static_cast<size_t>(std::max(n, 1)));
+            VJ_TRY(group_rectangles(src, n, minNeighbors, 0.2f, grouped.data(), &m));
+            src = grouped.data();
+        }
+        if (hCounts) hCounts[i] = m;
+        if (batch) {
+            VJ_CHECK(outPos + m <= cap, "detect_batch: output capacity too small");
+        }
+        const int64_t room = std::max<int64_t>(0, std::min<int64_t>(m, cap - outPos));
+        if (hOut && room > 0) std::memcpy(hOut + outPos, src, sizeof(vj_detection) * static_cast<size_t>(room));
+        outPos += m;
+    }
+    return VJ_OK;
+}
+
+}  // namespace
+
+extern "C" int vj_engine_detect(vj_engine* e, const uint8_t* dImage, int w, int h, int stride, float sf,
+                                int minNeighbors, int minObj, int maxDet, int applyGrouping, vj_detection* hOut,
+                                int cap, int* count, vj_stream sv) {
+    VJ_CHECK(e && count && cap >= 0, "detectMultiScale: bad arguments");
+    *count = 0;
+    try {
+        std::vector<ImageIn> imgs{{dImage, w, h, stride}};
+        int c = 0;
+        VJ_TRY(detect_images(e, imgs, sf, minNeighbors, minObj, maxDet, applyGrouping, hOut, cap, &c, false,
+                             static_cast<hipStream_t>(sv)));
+        *count = c;
+        return VJ_OK;
+    } catch (const std::exception& ex) {
+        set_error(ex.what());
+        return VJ_RUNTIME_ERROR;
+    }
+}
+
+extern "C" int vj_engine_detect_batch(vj_engine* e, int nImages, const uint8_t* const* dImages, const int* w,
+                                      const int* h, const int* stride, float sf, int minNeighbors, int minObj,
+                                      int maxDet, int applyGrouping, vj_detection* hOut, int cap, int* hCounts,
+                                      vj_stream sv) {
+    VJ_CHECK(e && nImages > 0 && dImages && w && h && stride && hCounts && cap >= 0,
+             "detect_batch: bad arguments");
+    try {
+        std::vector<ImageIn> imgs;
+        for (int i = 0; i < nImages; ++i) imgs.push_back({dImages[i], w[i], h[i], stride[i]});
+        return detect_images(e, imgs, sf, minNeighbors, minObj, maxDet, applyGrouping, hOut, cap, hCounts, true,
+                             static_cast<hipStream_t>(sv));
+    } catch (const std::exception& ex) {
+        set_error(ex.what());
+        return VJ_RUNTIME_ERROR;
+    }
+}
+
+extern "C" int vj_engine_last_stats(vj_engine* e, vj_detect_stats* out) {
+    VJ_CHECK(e && out, "vj_engine_last_stats: null");
+    *out = e->stats;
+    return VJ_OK;
+}

@@ -1,0 +1,484 @@
+// vj_host.cpp -- host-side parts of the drop-in boundary: error reporting,
+// the Haar feature pool, the .bin model format, rectangle grouping and
+// PGM/PPM I/O.  None of this runs on the GPU in the reference either.
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "vj_common.hpp"
+
+namespace vj {
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+    g_error = std::string(hipGetErrorName(e)) + " (" + hipGetErrorString(e) + ") in " + what +
+              " at " + file + ":" + std::to_string(line);
+    (void)hipGetLastError();
+    return VJ_DEVICE_ERROR;
+}
+
+// ---------------------------------------------------------------------------
+// Haar pool.  Enumeration order and rectangles follow
+// src/face_vision_engine.cu:882-1020: x, y, dx, dy loops; within each, the
+// five Basic templates (x2, y2, x3, y3, x2y2), then for Core x4, y4 and the
+// centre template, then for All the six tilted ones.  Weights -1/+2 (and +9
+// for the centre, +3 for tilted x3/y3) are the reference's.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct PoolWriter {
+    vj_feature* out;
+    int cap;
+    int n = 0;
+    int iiW;
+
+    vj_rect rect(int x, int y, int w, int h, float wt, bool tilted) const {
+        vj_rect r;
+        if (!tilted) {
+            r.p0 = x + iiW * y;
+            r.p1 = (x + w) + iiW * y;
+            r.p2 = x + iiW * (y + h);
+            r.p3 = (x + w) + iiW * (y + h);
+        } else {  // src/face_vision_engine.cu:626-631
+            r.p0 = x + iiW * y;
+            r.p1 = (x - h) + iiW * (y + h);
+            r.p2 = (x + w) + iiW * (y + w);
+            r.p3 = (x + w - h) + iiW * (y + w + h);
+        }
+        r.w = wt;
+        return r;
+    }
+    void emit(bool tilted, std::initializer_list<vj_rect> rs) {
+        if (out && n < cap) {
+            vj_feature f;
+            std::memset(&f, 0, sizeof f);
+            int i = 0;
+            for (const vj_rect& r : rs) f.r[i++] = r;
+            f.rectCount = static_cast<uint8_t>(i);
+            f.tilted = tilted ? 1 : 0;
+            out[n] = f;
+        }
+        ++n;
+    }
+};
+
+}  // namespace
+
+}  // namespace vj
+
+using namespace vj;
+
+extern "C" const char* vj_last_error(void) { return g_error.c_str(); }
+
+extern "C" int vj_haar_pool(int W, int H, int mode, vj_feature* out, int cap, int* count) {
+    VJ_CHECK(W > 0 && H > 0 && mode >= 0 && mode <= 2 && count, "vj_haar_pool: bad arguments");
+    PoolWriter pw{out, cap, 0, W + 1};
+    const bool core = mode != 0, tilt = mode == 2;
+    for (int x = 0; x < W; ++x) {
+        for (int y = 0; y < H; ++y) {
+            for (int dx = 1; dx <= W; ++dx) {
+                for (int dy = 1; dy <= H; ++dy) {
+                    if (x + 2 * dx <= W && y + dy <= H)
+                        pw.emit(false, {pw.rect(x, y, 2 * dx, dy, -1.f, false), pw.rect(x + dx, y, dx, dy, 2.f, false)});
+                    if (x + dx <= W && y + 2 * dy <= H)
+                        pw.emit(false, {pw.rect(x, y, dx, 2 * dy, -1.f, false), pw.rect(x, y + dy, dx, dy, 2.f, false)});
+                    if (x + 3 * dx <= W && y + dy <= H)
+                        pw.emit(false, {pw.rect(x, y, 3 * dx, dy, -1.f, false), pw.rect(x + dx, y, dx, dy, 2.f, false)});
+                    if (x + dx <= W && y + 3 * dy <= H)
+                        pw.emit(false, {pw.rect(x, y, dx, 3 * dy, -1.f, false), pw.rect(x, y + dy, dx, dy, 2.f, false)});
+                    if (x + 2 * dx <= W && y + 2 * dy <= H)
+                        pw.emit(false, {pw.rect(x, y, 2 * dx, 2 * dy, -1.f, false), pw.rect(x, y, dx, dy, 2.f, false),
+                                        pw.rect(x + dx, y + dy, dx, dy, 2.f, false)});
+                    if (core) {
+                        if (x + 4 * dx <= W && y + dy <= H)
+                            pw.emit(false, {pw.rect(x, y, 4 * dx, dy, -1.f, false), pw.rect(x + dx, y, 2 * dx, dy, 2.f, false)});
+                        if (x + dx <= W && y + 4 * dy <= H)
+                            pw.emit(false, {pw.rect(x, y, dx, 4 * dy, -1.f, false), pw.rect(x, y + dy, dx, 2 * dy, 2.f, false)});
+                        if (x + 3 * dx <= W && y + 3 * dy <= H)
+                            pw.emit(false, {pw.rect(x, y, 3 * dx, 3 * dy, -1.f, false), pw.rect(x + dx, y + dy, dx, dy, 9.f, false)});
+                    }
+                    if (tilt) {
+                        if (x + 2 * dx <= W && y + 2 * dx + dy <= H && x - dy >= 0)
+                            pw.emit(true, {pw.rect(x, y, 2 * dx, dy, -1.f, true), pw.rect(x, y, dx, dy, 2.f, true)});
+                        if (x + dx <= W && y + dx + 2 * dy <= H && x - 2 * dy >= 0)
+                            pw.emit(true, {pw.rect(x, y, dx, 2 * dy, -1.f, true), pw.rect(x, y, dx, dy, 2.f, true)});
+                        if (x + 3 * dx <= W && y + 3 * dx + dy <= H && x - dy >= 0)
+                            pw.emit(true, {pw.rect(x, y, 3 * dx, dy, -1.f, true), pw.rect(x + dx, y + dx, dx, dy, 3.f, true)});
+                        if (x + dx <= W && y + dx + 3 * dy <= H && x - 3 * dy >= 0)
+                            pw.emit(true, {pw.rect(x, y, dx, 3 * dy, -1.f, true), pw.rect(x - dy, y + dy, dx, dy, 3.f, true)});
+                        if (x + 4 * dx <= W && y + 4 * dx + dy <= H && x - dy >= 0)
+                            pw.emit(true, {pw.rect(x, y, 4 * dx, dy, -1.f, true), pw.rect(x + dx, y + dx, 2 * dx, dy, 2.f, true)});
+                        if (x + dx <= W && y + dx + 4 * dy <= H && x - 4 * dy >= 0)
+                            pw.emit(true, {pw.rect(x, y, dx, 4 * dy, -1.f, true), pw.rect(x - dy, y + dy, dx, 2 * dy, 2.f, true)});
+                    }
+                }
+            }
+        }
+    }
+    *count = pw.n;
+    return VJ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// .bin model: [header 48 B][HaarFeature x nF][GpuStump4 x nS][CascadeStage x nSt]
+// Header: magic "VJCBIN1\0", u32 version 1, i32 winW, i32 winH, 4 zero bytes
+// of padding, u64 featureCount, u64 stumpCount, u64 stageCount; native
+// little-endian (src/model_io.cpp:15-28, 66-167).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct ModelHeader {
+    char magic[8];
+    uint32_t version;
+    int32_t winW;
+    int32_t winH;
+    uint32_t pad;
+    uint64_t featureCount;
+    uint64_t stumpCount;
+    uint64_t stageCount;
+};
+static_assert(sizeof(ModelHeader) == 48, "the .bin header is 48 bytes");
+const char kMagic[8] = {'V', 'J', 'C', 'B', 'I', 'N', '1', '\0'};
+
+struct File {
+    FILE* f;
+    explicit File(FILE* p) : f(p) {}
+    ~File() {
+        if (f) std::fclose(f);
+    }
+};
+
+int read_header(FILE* f, ModelHeader& h, const char* path) {
+    if (std::fread(&h, sizeof h, 1, f) != 1) {
+        set_error(std::string("model file too short: ") + path);
+        return VJ_INVALID_ARG;
+    }
+    // src/model_io.cpp:148-152: magic, version, positive window, non-zero counts.
+    if (std::memcmp(h.magic, kMagic, 8) != 0 || h.version != 1u || h.winW <= 0 || h.winH <= 0 ||
+        h.featureCount == 0 || h.stumpCount == 0 || h.stageCount == 0) {
+        set_error(std::string("not a VJCBIN1 v1 model (bad magic, version, window or counts): ") + path);
+        return VJ_INVALID_ARG;
+    }
+    return VJ_OK;
+}
+
+}  // namespace
+
+extern "C" int vj_model_save(const char* path, int winW, int winH, const vj_feature* feats,
+                             int64_t nF, const vj_stump* stumps, int64_t nS, const vj_stage* stages,
+                             int64_t nSt) {
+    // src/model_io.cpp:84-86: every array must be non-empty.
+    VJ_CHECK(path && path[0] && winW > 0 && winH > 0 && feats && nF > 0 && stumps && nS > 0 &&
+                 stages && nSt > 0,
+             "vj_model_save: empty model or bad arguments");
+    File fp(std::fopen(path, "wb"));
+    VJ_CHECK(fp.f, std::string("cannot open for writing: ") + path);
+    ModelHeader h;
+    std::memset(&h, 0, sizeof h);
+    std::memcpy(h.magic, kMagic, 8);
+    h.version = 1;
+    h.winW = winW;
+    h.winH = winH;
+    h.featureCount = static_cast<uint64_t>(nF);
+    h.stumpCount = static_cast<uint64_t>(nS);
+    h.stageCount = static_cast<uint64_t>(nSt);
+    bool ok = std::fwrite(&h, sizeof h, 1, fp.f) == 1 &&
+              std::fwrite(feats, sizeof(vj_feature), static_cast<size_t>(nF), fp.f) == static_cast<size_t>(nF) &&
+              std::fwrite(stumps, sizeof(vj_stump), static_cast<size_t>(nS), fp.f) == static_cast<size_t>(nS) &&
+              std::fwrite(stages, sizeof(vj_stage), static_cast<size_t>(nSt), fp.f) == static_cast<size_t>(nSt);
+    if (!ok) {
+        set_error(std::string("short write: ") + path);
+        return VJ_RUNTIME_ERROR;
+    }
+    return VJ_OK;
+}
+
+extern "C" int vj_model_peek(const char* path, int* winW, int* winH, int64_t* nF, int64_t* nS,
+                             int64_t* nSt) {
+    VJ_CHECK(path && path[0], "vj_model_peek: empty path");
+    File fp(std::fopen(path, "rb"));
+    VJ_CHECK(fp.f, std::string("cannot open model: ") + path);
+    ModelHeader h;
+    VJ_TRY(read_header(fp.f, h, path));
+    if (winW) *winW = h.winW;
+    if (winH) *winH = h.winH;
+    if (nF) *nF = static_cast<int64_t>(h.featureCount);
+    if (nS) *nS = static_cast<int64_t>(h.stumpCount);
+    if (nSt) *nSt = static_cast<int64_t>(h.stageCount);
+    return VJ_OK;
+}
+
+extern "C" int vj_model_load(const char* path, vj_feature* feats, int64_t capF, vj_stump* stumps,
+                             int64_t capS, vj_stage* stages, int64_t capSt) {
+    VJ_CHECK(path && path[0], "vj_model_load: empty path");
+    File fp(std::fopen(path, "rb"));
+    VJ_CHECK(fp.f, std::string("cannot open model: ") + path);
+    ModelHeader h;
+    VJ_TRY(read_header(fp.f, h, path));
+    VJ_CHECK(feats && stumps && stages && static_cast<uint64_t>(capF) >= h.featureCount &&
+                 static_cast<uint64_t>(capS) >= h.stumpCount && static_cast<uint64_t>(capSt) >= h.stageCount,
+             "vj_model_load: output arrays too small");
+    bool ok = std::fread(feats, sizeof(vj_feature), h.featureCount, fp.f) == h.featureCount &&
+              std::fread(stumps, sizeof(vj_stump), h.stumpCount, fp.f) == h.stumpCount &&
+              std::fread(stages, sizeof(vj_stage), h.stageCount, fp.f) == h.stageCount;
+    VJ_CHECK(ok, std::string("model file truncated: ") + path);
+    return VJ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Rectangle grouping (host), src/face_vision_engine.cu:585-616, 1401-1539.
+// Sweep over boxes sorted by (x, index) joining similar boxes in a union-find
+// (union by rank, path compression); clusters with more than
+// max(1, minNeighbors) members become the integer-mean box with the first
+// maximal score; a second sweep drops clusters that sit inside a stronger one.
+// ---------------------------------------------------------------------------
+namespace vj {
+
+namespace {
+
+struct UnionFind {
+    std::vector<int> parent, rank;
+    explicit UnionFind(int n) : parent(n), rank(n, 0) { std::iota(parent.begin(), parent.end(), 0); }
+    int root(int x) {
+        int r = x;
+        while (parent[r] != r) r = parent[r];
+        while (parent[x] != r) {
+            int nx = parent[x];
+            parent[x] = r;
+            x = nx;
+        }
+        return r;
+    }
+    void join(int a, int b) {
+        a = root(a);
+        b = root(b);
+        if (a == b) return;
+        if (rank[a] < rank[b]) std::swap(a, b);
+        parent[b] = a;
+        if (rank[a] == rank[b]) ++rank[a];
+    }
+};
+
+inline bool similar_boxes(const vj_detection& a, const vj_detection& b, float eps) {
+    const int delta = static_cast<int>(eps * static_cast<float>(std::min(a.w, b.w) + std::min(a.h, b.h)) * 0.5f);
+    return std::abs(a.x - b.x) <= delta && std::abs(a.y - b.y) <= delta &&
+           std::abs((a.x + a.w) - (b.x + b.w)) <= delta && std::abs((a.y + a.h) - (b.y + b.h)) <= delta;
+}
+
+std::vector<int> order_by_x(const vj_detection* d, int n) {
+    std::vector<int> o(static_cast<size_t>(n));
+    std::iota(o.begin(), o.end(), 0);
+    std::sort(o.begin(), o.end(), [d](int a, int b) { return d[a].x != d[b].x ? d[a].x < d[b].x : a < b; });
+    return o;
+}
+
+}  // namespace
+
+int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps, vj_detection* out,
+                     int* count) {
+    *count = 0;
+    if (n <= 0) return VJ_OK;
+    UnionFind uf(n);
+    const std::vector<int> order = order_by_x(in, n);
+    for (int oi = 0; oi < n; ++oi) {
+        const vj_detection& a = in[order[oi]];
+        const int reach = static_cast<int>(eps * static_cast<float>(a.w + a.h) * 0.5f);
+        for (int oj = oi + 1; oj < n; ++oj) {
+            const vj_detection& b = in[order[oj]];
+            if (b.x - a.x > reach) break;
+            if (similar_boxes(a, b, eps)) uf.join(order[oi], order[oj]);
+        }
+    }
+    struct Acc {
+        int64_t x = 0, y = 0, w = 0, h = 0;
+        int count = 0;
+        float best = -std::numeric_limits<float>::infinity();
+        int scaleIdx = 0;
+    };
+    std::vector<Acc> acc(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) {
+        Acc& c = acc[static_cast<size_t>(uf.root(i))];
+        c.x += in[i].x;
+        c.y += in[i].y;
+        c.w += in[i].w;
+        c.h += in[i].h;
+        c.count += 1;
+        if (in[i].score > c.best) {
+            c.best = in[i].score;
+            c.scaleIdx = in[i].scaleIdx;
+        }
+    }
+    const int keep = std::max(1, minNeighbors);
+    std::vector<vj_detection> clusters;
+    std::vector<int> members;
+    for (int i = 0; i < n; ++i) {
+        const Acc& c = acc[static_cast<size_t>(i)];
+        if (uf.root(i) != i || c.count <= keep) continue;
+        vj_detection d;
+        d.x = static_cast<int>(c.x / c.count);
+        d.y = static_cast<int>(c.y / c.count);
+        d.w = static_cast<int>(c.w / c.count);
+        d.h = static_cast<int>(c.h / c.count);
+        d.scaleIdx = c.scaleIdx;
+        d.score = c.best;
+        clusters.push_back(d);
+        members.push_back(c.count);
+    }
+    const int nc = static_cast<int>(clusters.size());
+    int maxW = 0;
+    for (const vj_detection& d : clusters) maxW = std::max(maxW, d.w);
+    const std::vector<int> corder = order_by_x(clusters.data(), nc);
+    int no = 0;
+    for (int i = 0; i < nc; ++i) {
+        const vj_detection& r1 = clusters[static_cast<size_t>(i)];
+        const int n1 = members[static_cast<size_t>(i)];
+        const float lo = static_cast<float>(r1.x + r1.w) - (1.0f + eps) * static_cast<float>(maxW);
+        const float hi = static_cast<float>(r1.x) + eps * static_cast<float>(maxW);
+        bool inside = false;
+        for (int o : corder) {
+            const vj_detection& r2 = clusters[static_cast<size_t>(o)];
+            if (r2.x < static_cast<int>(lo)) continue;
+            if (r2.x > static_cast<int>(hi)) break;
+            if (o == i) continue;
+            const int n2 = members[static_cast<size_t>(o)];
+            const int dx = static_cast<int>(static_cast<float>(r2.w) * eps);
+            const int dy = static_cast<int>(static_cast<float>(r2.h) * eps);
+            if (r1.x >= r2.x - dx && r1.y >= r2.y - dy && r1.x + r1.w <= r2.x + r2.w + dx &&
+                r1.y + r1.h <= r2.y + r2.h + dy && (n2 > std::max(3, n1) || n1 < 3)) {
+                inside = true;
+                break;
+            }
+        }
+        if (!inside) out[no++] = r1;
+    }
+    *count = no;
+    return VJ_OK;
+}
+
+}  // namespace vj
+
+extern "C" int vj_group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps,
+                                   vj_detection* out, int* count) {
+    VJ_CHECK(count && n >= 0 && (n == 0 || (in && out)), "vj_group_rectangles: bad arguments");
+    try {
+        return group_rectangles(in, n, minNeighbors, eps, out, count);
+    } catch (const std::exception& ex) {
+        set_error(ex.what());
+        return VJ_RUNTIME_ERROR;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PGM (P5, maxVal <= 255, '#' comments) and PPM (P6) -- src/main.cu:286-410.
+// ---------------------------------------------------------------------------
+namespace {
+
+bool next_token(std::istream& is, std::string& tok) {
+    tok.clear();
+    while (is.good()) {
+        const int c = is.peek();
+        if (c == '#') {
+            std::string line;
+            std::getline(is, line);
+            continue;
+        }
+        if (c != EOF && std::isspace(static_cast<unsigned char>(c))) {
+            is.get();
+            continue;
+        }
+        break;
+    }
+    if (!is.good()) return false;
+    is >> tok;
+    return !tok.empty();
+}
+
+int pgm_open(const char* path, std::ifstream& ifs, int& w, int& h) {
+    ifs.open(path, std::ios::binary);
+    VJ_CHECK(ifs, std::string("cannot open PGM: ") + path);
+    std::string tok;
+    VJ_CHECK(next_token(ifs, tok) && tok == "P5", std::string("unsupported PGM format (need P5): ") + path);
+    int maxVal = 0;
+    try {
+        VJ_CHECK(next_token(ifs, tok), "PGM header truncated");
+        w = std::stoi(tok);
+        VJ_CHECK(next_token(ifs, tok), "PGM header truncated");
+        h = std::stoi(tok);
+        VJ_CHECK(next_token(ifs, tok), "PGM header truncated");
+        maxVal = std::stoi(tok);
+    } catch (...) {
+        set_error("invalid PGM header");
+        return VJ_INVALID_ARG;
+    }
+    VJ_CHECK(w > 0 && h > 0 && maxVal > 0 && maxVal <= 255, "invalid PGM metadata");
+    ifs.get();  // the single whitespace byte after maxVal
+    return VJ_OK;
+}
+
+}  // namespace
+
+extern "C" int vj_pgm_peek(const char* path, int* w, int* h) {
+    VJ_CHECK(path && w && h, "vj_pgm_peek: bad arguments");
+    std::ifstream ifs;
+    return pgm_open(path, ifs, *w, *h);
+}
+
+extern "C" int vj_pgm_load(const char* path, uint8_t* out, size_t cap) {
+    VJ_CHECK(path && out, "vj_pgm_load: bad arguments");
+    std::ifstream ifs;
+    int w = 0, h = 0;
+    VJ_TRY(pgm_open(path, ifs, w, h));
+    const size_t n = static_cast<size_t>(w) * static_cast<size_t>(h);
+    VJ_CHECK(cap >= n, "vj_pgm_load: output buffer too small");
+    VJ_CHECK(ifs.read(reinterpret_cast<char*>(out), static_cast<std::streamsize>(n)),
+             "failed to read PGM pixel payload");
+    return VJ_OK;
+}
+
+extern "C" int vj_ppm_save_gray_boxes(const char* path, const uint8_t* gray, int w, int h,
+                                      const vj_detection* boxes, int n) {
+    VJ_CHECK(path && gray && w > 0 && h > 0 && n >= 0, "vj_ppm_save_gray_boxes: bad arguments");
+    std::vector<uint8_t> rgb(static_cast<size_t>(w) * h * 3);
+    for (size_t i = 0; i < static_cast<size_t>(w) * h; ++i) rgb[3 * i] = rgb[3 * i + 1] = rgb[3 * i + 2] = gray[i];
+    auto put = [&](int x, int y) {
+        uint8_t* p = &rgb[(static_cast<size_t>(y) * w + x) * 3];
+        p[0] = p[1] = p[2] = 255;
+    };
+    // 2-pixel white frame per box, src/main.cu:373-410.
+    for (int b = 0; b < n; ++b) {
+        const vj_detection& d = boxes[b];
+        const int x0 = std::max(0, d.x), y0 = std::max(0, d.y);
+        const int x1 = std::min(w - 1, d.x + d.w - 1), y1 = std::min(h - 1, d.y + d.h - 1);
+        if (x0 > x1 || y0 > y1) continue;
+        for (int t = 0; t < 2; ++t) {
+            const int yt = std::min(h - 1, y0 + t), yb = std::max(0, y1 - t);
+            for (int x = x0; x <= x1; ++x) {
+                put(x, yt);
+                put(x, yb);
+            }
+            const int xl = std::min(w - 1, x0 + t), xr = std::max(0, x1 - t);
+            for (int y = y0; y <= y1; ++y) {
+                put(xl, y);
+                put(xr, y);
+            }
+        }
+    }
+    std::ofstream ofs(path, std::ios::binary);
+    VJ_CHECK(ofs, std::string("cannot write PPM: ") + path);
+    ofs << "P6\n" << w << " " << h << "\n255\n";
+    ofs.write(reinterpret_cast<const char*>(rgb.data()), static_cast<std::streamsize>(rgb.size()));
+    VJ_CHECK(static_cast<bool>(ofs), "short write of PPM");
+    return VJ_OK;
+}

@@ -1,0 +1,113 @@
+// vj_kernels.hpp -- device-side records and wavefront helpers shared by the
+// detect and training kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace vj {
+
+constexpr int kWave = 64;
+
+// Integral-image tiling: bands of 32 rows, chunks of 256 columns (one
+// wavefront, 4 columns per lane).
+constexpr int kBandRows = 32;
+constexpr int kChunkCols = 256;
+
+// Cascade tiling: 64 windows per tile row (one 64-bit mask word), and
+// kTileRows[step] window rows per tile.
+constexpr int kTileWin = 64;
+__host__ __device__ constexpr int tile_rows_for_step(int step) {
+    return step == 1 ? 16 : (step == 2 ? 8 : 6);
+}
+
+// One pyramid level of one image, as every level-parallel kernel sees it.
+struct LevelDesc {
+    const uint8_t* pix;   // level pixels
+    int64_t iiOff;        // first element of the level's (h+1) x (w+1) integral
+    int64_t colOff;       // first element of the level's [nBands][w] column sums
+    int64_t totOff;       // first element of the level's [nBands][nChunks] chunk totals
+    int64_t rowOff;       // first element of the level's [h][nChunks] row-chunk totals
+    int64_t wordOff;      // first 64-bit word of the level's window mask
+    int32_t w, h, stride; // level size and pixel row stride
+    int32_t step;         // window step (1, 2 or 3)
+    float scale;          // level scale
+    int32_t scaleIdx;     // index in the scale sequence (counts skipped scales)
+    int32_t image;        // image index within the call
+    int32_t nBands, nChunks;
+    int32_t scanW, scanH, rowWords;
+    int32_t tileRows, tilesX, tilesY;
+    int32_t itemOff;      // first (band, chunk) item
+    int32_t chunkItemOff; // first column-chunk item
+    int32_t tileOff;      // first cascade tile
+    int32_t srcLevel;     // level this one is resized from (-1: level 0 of its image)
+};
+
+// Cascade stump with its corner offsets decoded for one LDS tile pitch.
+// 64 bytes so a wavefront reads it with one scalar load.
+struct alignas(16) KStump {
+    uint16_t off[12];     // r0.p0..p3, r1.p0..p3, r2.p0..p3 as dy*pitch + dx
+    float w[3];
+    float theta, left, right;
+    int32_t rectCount;
+    int32_t pad[3];
+};
+static_assert(sizeof(KStump) == 64, "KStump is one 64-byte scalar load");
+
+struct KStage {
+    int32_t first, ntrees;
+    float threshold;
+    int32_t pad;
+};
+
+__device__ __forceinline__ int lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Number of set bits of a 64-bit wave mask strictly below this lane.
+__device__ __forceinline__ int mask_rank(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        T u = __shfl_up(v, d, kWave);
+        if (lane >= d) v += u;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
+    return v;
+}
+
+// Index of the last element of sorted offsets[0..n) that is <= key.
+__device__ __forceinline__ int find_segment(const int32_t* offsets, int n, int key) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (offsets[mid] <= key) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int find_segment64(const int64_t* offsets, int n, int64_t key) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (offsets[mid] <= key) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+}  // namespace vj

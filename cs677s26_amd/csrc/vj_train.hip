@@ -1,0 +1,1155 @@
+// vj_train.hip -- AdaBoostTrainer on gfx950: Haar responses over all
+// samples, per-feature presort, exact weighted threshold scan, arg-min,
+// weight update, normalisation and stage calibration.
+//
+// Reference: src/boosting_core.cu (AdaBoostTrainer, kernels K8-K12 and the
+// CUB segmented sort), the round kernels src/main.cu:829-1028 and the round
+// logic src/main.cu:2005-2348.
+//
+// Exactness.  Sums of weights are order-independent: a search weight is the
+// float32 rounding of the f64 weight, turned into an int64 count of 2^-62
+// units (exact for weights >= 2^-39), and prefix sums are exact integer sums.
+// Normalisation divides by the exact sum of the f64 weights (int128 in 2^-90
+// units, rounded to nearest once).  Ties therefore resolve identically on any
+// schedule: per feature smaller error, then smaller split, then parity +1
+// (src/boosting_core.cu:59-79); across features smaller error, then smaller
+// index (src/boosting_core.cu:87-99).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vj_common.hpp"
+#include "vj_kernels.hpp"
+#include "vj_integral_dev.hpp"
+
+namespace vj {
+
+constexpr int kSortThreads = 1024;
+constexpr int kSortBuckets = 4096;
+constexpr int kSortMaxPerThread = 20;              // N <= 20480
+constexpr int kSortMaxN = kSortThreads * kSortMaxPerThread;
+constexpr int kSearchThreads = 1024;               // 16 waves, one feature each at a time
+
+struct RoundBest {
+    int64_t errQ;
+    int32_t feature;
+    int32_t split;
+    int32_t parity;
+    float theta;
+    int32_t pad[2];
+};
+
+struct Calib {
+    float cut;
+    uint32_t posGe, posGt, negGe, negGt;
+    int32_t pad[3];
+};
+
+struct WaveBest {
+    int64_t errQ;
+    int32_t split;
+    int32_t parity;
+    int32_t feature;
+    int32_t pad;
+};
+
+__device__ __forceinline__ uint32_t float_key(float v) {
+    const uint32_t b = __float_as_uint(v);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__device__ __forceinline__ float key_float(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+// trunc(|w| * 2^62) with the sign of w, exactly (w finite, |w| <= 1).
+__device__ __forceinline__ int64_t fixed_weight(float w) {
+    const uint32_t b = __float_as_uint(w);
+    int e = static_cast<int>((b >> 23) & 0xFFu);
+    uint64_t m = b & 0x7FFFFFu;
+    if (e == 0) e = 1;
+    else m |= 0x800000u;
+    const int sh = min(e - 88, 39);
+    const uint64_t q = sh >= 0 ? (m << sh) : (sh > -64 ? (m >> (-sh)) : 0ull);
+    return (b >> 31) ? -static_cast<int64_t>(q) : static_cast<int64_t>(q);
+}
+
+// Integer sum of one rectangle of the transposed training integral.
+__device__ __forceinline__ int32_t rect_sum(const uint32_t* __restrict__ sumT, const vj_rect& r, int n, int s) {
+    return static_cast<int32_t>(sumT[static_cast<size_t>(r.p0) * n + s] - sumT[static_cast<size_t>(r.p1) * n + s] -
+                                sumT[static_cast<size_t>(r.p2) * n + s] + sumT[static_cast<size_t>(r.p3) * n + s]);
+}
+
+// Normalised response of feature f on sample s: w0*S0 + w1*S1 (+ w2*S2) in
+// float, times invNorm (src/main.cu:476-496).  Each rectangle sum is an exact
+// integer below 2^24, so converting it once equals the reference's
+// corner-by-corner float arithmetic.
+__device__ __forceinline__ float response(const vj_feature& f, const uint32_t* __restrict__ sumT,
+                                          const float* __restrict__ inv, int n, int s) {
+    const float s0 = static_cast<float>(rect_sum(sumT, f.r[0], n, s));
+    const float s1 = static_cast<float>(rect_sum(sumT, f.r[1], n, s));
+    float resp = f.r[0].w * s0 + f.r[1].w * s1;
+    if (f.rectCount == 3) resp += f.r[2].w * static_cast<float>(rect_sum(sumT, f.r[2], n, s));
+    return resp * inv[s];
+}
+
+// ---------------------------------------------------------------------------
+// Presort: one 1024-thread workgroup per feature.  Keys are the
+// order-preserving bit patterns of the responses (-0 before +0, as the
+// reference's radix sort orders them); ties break by sample index.  The
+// samples are bucketed by a monotone linear map of their value into 4096
+// LDS buckets, each element's final rank is its bucket start plus the count
+// of smaller (key, index) pairs in its bucket.  Output per position k: the
+// sample index, with bit 15 set when fl(v_k + 1e-7f) < v_{k+1}
+// (src/boosting_core.cu:395-405).  mode 0 computes the responses from the
+// integrals; mode 1 reads them from resp (row f of [count][n]) and writes
+// plain indices (sortSamplesPerFeature semantics).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kSortThreads) void presort_kernel(const vj_feature* __restrict__ feats, int fBegin,
+                                                               const uint32_t* __restrict__ sumT,
+                                                               const float* __restrict__ inv,
+                                                               const float* __restrict__ resp, int n,
+                                                               uint16_t* __restrict__ out, int64_t outPitch,
+                                                               int mode) {
+    extern __shared__ uint8_t smem[];
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(smem);                  // [kSortBuckets + 1]
+    uint32_t* keys = cnt + kSortBuckets + 4;                             // [n]
+    uint16_t* idx = reinterpret_cast<uint16_t*>(keys + n);               // [n]
+    __shared__ float redMin[16], redMax[16];
+
+    const int f = blockIdx.x;
+    const int tid = threadIdx.x;
+    float v[kSortMaxPerThread];
+    float vmin = INFINITY, vmax = -INFINITY;
+    if (mode == 0) {
+        const vj_feature ft = feats[fBegin + f];
+#pragma unroll
+        for (int j = 0; j < kSortMaxPerThread; ++j) {
+            const int s = tid + j * kSortThreads;
+            v[j] = (s < n) ? response(ft, sumT, inv, n, s) : 0.0f;
+        }
+    } else {
+        const float* row = resp + static_cast<size_t>(f) * n;
+#pragma unroll
+        for (int j = 0; j < kSortMaxPerThread; ++j) {
+            const int s = tid + j * kSortThreads;
+            v[j] = (s < n) ? row[s] : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kSortMaxPerThread; ++j) {
+        const int s = tid + j * kSortThreads;
+        if (s < n) {
+            vmin = fminf(vmin, v[j]);
+            vmax = fmaxf(vmax, v[j]);
+        }
+    }
+    for (int b = tid; b < kSortBuckets; b += kSortThreads) cnt[b] = 0u;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        vmin = fminf(vmin, __shfl_xor(vmin, d, kWave));
+        vmax = fmaxf(vmax, __shfl_xor(vmax, d, kWave));
+    }
+    if ((tid & 63) == 0) {
+        redMin[tid >> 6] = vmin;
+        redMax[tid >> 6] = vmax;
+    }
+    __syncthreads();
+    vmin = redMin[0];
+    vmax = redMax[0];
+    for (int k = 1; k < kSortThreads / kWave; ++k) {
+        vmin = fminf(vmin, redMin[k]);
+        vmax = fmaxf(vmax, redMax[k]);
+    }
+    const float range = vmax - vmin;
+    const float scale = (range > 0.0f && range < INFINITY) ? static_cast<float>(kSortBuckets) / range : 0.0f;
+
+    // histogram; the atomic's old value is the element's slot in its bucket
+    uint32_t bs[kSortMaxPerThread];
+#pragma unroll
+    for (int j = 0; j < kSortMaxPerThread; ++j) {
+        const int s = tid + j * kSortThreads;
+        bs[j] = 0;
+        if (s < n) {
+            int b = static_cast<int>((v[j] - vmin) * scale);
+            b = b < 0 ? 0 : (b >= kSortBuckets ? kSortBuckets - 1 : b);
+            const uint32_t slot = atomicAdd(&cnt[b], 1u);
+            bs[j] = (static_cast<uint32_t>(b) << 16) | slot;
+        }
+    }
+    __syncthreads();
+    // exclusive scan of the bucket counts (4 buckets per thread)
+    {
+        const int b0 = tid * (kSortBuckets / kSortThreads);
+        uint32_t c[kSortBuckets / kSortThreads];
+        uint32_t t = 0;
+#pragma unroll
+        for (int k = 0; k < kSortBuckets / kSortThreads; ++k) {
+            c[k] = cnt[b0 + k];
+            t += c[k];
+        }
+        uint32_t inc = wave_incl_scan(t);
+        __shared__ uint32_t waveTot[16];
+        if ((tid & 63) == 63) waveTot[tid >> 6] = inc;
+        __syncthreads();
+        uint32_t base = inc - t;
+        for (int k = 0; k < (tid >> 6); ++k) base += waveTot[k];
+#pragma unroll
+        for (int k = 0; k < kSortBuckets / kSortThreads; ++k) {
+            cnt[b0 + k] = base;
+            base += c[k];
+        }
+        if (tid == kSortThreads - 1) cnt[kSortBuckets] = base;
+    }
+    __syncthreads();
+    // scatter into bucket order
+#pragma unroll
+    for (int j = 0; j < kSortMaxPerThread; ++j) {
+        const int s = tid + j * kSortThreads;
+        if (s < n) {
+            const uint32_t pos = cnt[bs[j] >> 16] + (bs[j] & 0xFFFFu);
+            keys[pos] = float_key(v[j]);
+            idx[pos] = static_cast<uint16_t>(s);
+        }
+    }
+    __syncthreads();
+    // rank within the bucket by (key, index)
+    uint32_t rank[kSortMaxPerThread];
+#pragma unroll
+    for (int j = 0; j < kSortMaxPerThread; ++j) {
+        const int s = tid + j * kSortThreads;
+        rank[j] = 0;
+        if (s < n) {
+            const uint32_t b = bs[j] >> 16;
+            const uint32_t a = cnt[b], e = cnt[b + 1];
+            const uint32_t k = float_key(v[j]);
+            uint32_t r = a;
+            for (uint32_t p = a; p < e; ++p) {
+                const uint32_t kp = keys[p];
+                r += (kp < k || (kp == k && idx[p] < static_cast<uint32_t>(s))) ? 1u : 0u;
+            }
+            rank[j] = r;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSortMaxPerThread; ++j) {
+        const int s = tid + j * kSortThreads;
+        if (s < n) {
+            keys[rank[j]] = float_key(v[j]);
+            idx[rank[j]] = static_cast<uint16_t>(s);
+        }
+    }
+    __syncthreads();
+    uint16_t* o = out + static_cast<int64_t>(f) * outPitch;
+    for (int k = tid; k < n; k += kSortThreads) {
+        uint32_t e = idx[k];
+        if (mode == 0 && k + 1 < n) {
+            const float v0 = key_float(keys[k]), v1 = key_float(keys[k + 1]);
+            if (v0 + 1e-7f < v1) e |= 0x8000u;
+        }
+        o[k] = static_cast<uint16_t>(e);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Threshold scan: every wave takes one feature at a time and walks its
+// sorted order, 4 entries per lane per step; D_k = sum_{j<=k} q(order j)
+// exactly in int64 (signed: +w positives, -w negatives, 0 inactive);
+// errN (parity +1) = totNeg + D_k, errP (parity -1) = totPos - D_k at every
+// position whose valid bit is set (src/boosting_core.cu:289-456).  The
+// search weights sit in LDS as float; each WG keeps the best of its features.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool better(int64_t e, int32_t k, int32_t p, int64_t be, int32_t bk, int32_t bp) {
+    if (e != be) return e < be;
+    if (k != bk) return k < bk;
+    return p > bp;
+}
+
+// Exact threshold scan of one feature's sorted order by one wavefront; returns
+// the feature's best (error, split, parity) in every lane.
+__device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, int n, const float* lsw,
+                                             int64_t totPos, int64_t totNeg, int lane,
+                                             int64_t& fE, int32_t& fK, int32_t& fP) {
+    int64_t carry = 0;
+    fE = INT64_MAX;
+    fK = -1;
+    fP = +1;
+    for (int base = 0; base < n; base += 256) {
+        const int k0 = base + lane * 4;
+        uint64_t e4 = 0;
+        if (k0 < n) e4 = *reinterpret_cast<const uint64_t*>(row + k0);  // rows padded to 256 entries
+        int64_t q[4];
+        int64_t c = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t e = static_cast<uint32_t>(e4 >> (16 * j)) & 0xFFFFu;
+            q[j] = (k0 + j < n) ? fixed_weight(lsw[e & 0x7FFFu]) : 0;
+            c += q[j];
+        }
+        const int64_t inc = wave_incl_scan(c);
+        int64_t D = carry + inc - c;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            D += q[j];
+            const uint32_t e = static_cast<uint32_t>(e4 >> (16 * j)) & 0xFFFFu;
+            if ((e & 0x8000u) && k0 + j < n) {
+                const int32_t k = k0 + j;
+                const int64_t errP = totPos - D;
+                const int64_t errN = totNeg + D;
+                if (better(errP, k, -1, fE, fK, fP)) {
+                    fE = errP;
+                    fK = k;
+                    fP = -1;
+                }
+                if (better(errN, k, +1, fE, fK, fP)) {
+                    fE = errN;
+                    fK = k;
+                    fP = +1;
+                }
+            }
+        }
+        carry += __shfl(inc, 63, kWave);
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        const int64_t oE = __shfl_xor(fE, d, kWave);
+        const int32_t oK = __shfl_xor(fK, d, kWave);
+        const int32_t oP = __shfl_xor(fP, d, kWave);
+        if (better(oE, oK, oP, fE, fK, fP)) {
+            fE = oE;
+            fK = oK;
+            fP = oP;
+        }
+    }
+}
+
+// Loads the float search weights into LDS and returns the exact totals.
+__device__ __forceinline__ void load_weights(const float* __restrict__ sw32, int n, float* lsw, int64_t* totP,
+                                             int64_t* totN, int64_t& totPos, int64_t& totNeg) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int64_t tp = 0, tn = 0;
+    for (int s = tid; s < n; s += blockDim.x) {
+        const float w = sw32[s];
+        lsw[s] = w;
+        const int64_t q = fixed_weight(w);
+        if (q > 0) tp += q;
+        else tn -= q;
+    }
+    tp = wave_sum(tp);
+    tn = wave_sum(tn);
+    if (lane == 0) {
+        totP[wave] = tp;
+        totN[wave] = tn;
+    }
+    __syncthreads();
+    totPos = 0;
+    totNeg = 0;
+    for (int k = 0; k < static_cast<int>(blockDim.x >> 6); ++k) {
+        totPos += totP[k];
+        totNeg += totN[k];
+    }
+}
+
+__global__ __launch_bounds__(kSearchThreads) void search_kernel(const uint16_t* __restrict__ ord, int64_t pitch,
+                                                               int n, int nFeatures,
+                                                               const float* __restrict__ sw32,
+                                                               const uint8_t* __restrict__ used,
+                                                               WaveBest* __restrict__ blockBest) {
+    extern __shared__ float lsw[];  // [n]
+    __shared__ int64_t totP[16], totN[16];
+    __shared__ WaveBest wb[16];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    int64_t totPos, totNeg;
+    load_weights(sw32, n, lsw, totP, totN, totPos, totNeg);
+    int64_t bE = INT64_MAX;
+    int32_t bK = -1, bP = +1, bF = INT_MAX;
+    const int nWavesTotal = gridDim.x * (blockDim.x >> 6);
+    for (int f = blockIdx.x * (blockDim.x >> 6) + wave; f < nFeatures; f += nWavesTotal) {
+        if (used && used[f]) continue;
+        int64_t fE;
+        int32_t fK, fP;
+        scan_feature(ord + static_cast<int64_t>(f) * pitch, n, lsw, totPos, totNeg, lane, fE, fK, fP);
+        if (fK >= 0 && (fE < bE || (fE == bE && f < bF))) {
+            bE = fE;
+            bK = fK;
+            bP = fP;
+            bF = f;
+        }
+    }
+    if (lane == 0) wb[wave] = {bE, bK, bP, bF, 0};
+    __syncthreads();
+    if (tid == 0) {
+        WaveBest r = wb[0];
+        for (int k = 1; k < 16; ++k) {
+            const WaveBest& o = wb[k];
+            if (o.split >= 0 && (r.split < 0 || o.errQ < r.errQ || (o.errQ == r.errQ && o.feature < r.feature))) r = o;
+        }
+        blockBest[blockIdx.x] = r;
+    }
+}
+
+// Round arg-min over the workgroup bests; then the threshold of the winner
+// from the two samples around its split, theta = 0.5f*(v_k + v_k+1).
+__global__ __launch_bounds__(256) void reduce_best_kernel(const WaveBest* __restrict__ bb, int nb,
+                                                          const uint16_t* __restrict__ ord, int64_t pitch,
+                                                          const vj_feature* __restrict__ feats,
+                                                          const uint32_t* __restrict__ sumT,
+                                                          const float* __restrict__ inv, int n,
+                                                          RoundBest* __restrict__ out) {
+    __shared__ WaveBest sb[256];
+    WaveBest r{INT64_MAX, -1, +1, INT_MAX, 0};
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        const WaveBest o = bb[i];
+        if (o.split >= 0 && (r.split < 0 || o.errQ < r.errQ || (o.errQ == r.errQ && o.feature < r.feature))) r = o;
+    }
+    sb[threadIdx.x] = r;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            const WaveBest o = sb[threadIdx.x + st];
+            WaveBest& a = sb[threadIdx.x];
+            if (o.split >= 0 && (a.split < 0 || o.errQ < a.errQ || (o.errQ == a.errQ && o.feature < a.feature))) a = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const WaveBest b = sb[0];
+        RoundBest rb{};
+        rb.errQ = b.errQ;
+        rb.feature = b.split >= 0 ? b.feature : -1;
+        rb.split = b.split;
+        rb.parity = b.parity;
+        rb.theta = 0.0f;
+        if (b.split >= 0 && b.split + 1 < n) {
+            const uint16_t* row = ord + static_cast<int64_t>(b.feature) * pitch;
+            const int s0 = row[b.split] & 0x7FFF, s1 = row[b.split + 1] & 0x7FFF;
+            const vj_feature f = feats[b.feature];
+            const float v0 = response(f, sumT, inv, n, s0);
+            const float v1 = response(f, sumT, inv, n, s1);
+            rb.theta = 0.5f * (v0 + v1);
+        }
+        *out = rb;
+    }
+}
+
+// Stage-start integrals and normalisers of the n 24x24 patches.
+__global__ __launch_bounds__(256) void stage_integral_kernel(const uint8_t* __restrict__ patches, int w, int h, int n,
+                                                             uint32_t* __restrict__ sumT, uint64_t* __restrict__ sqT,
+                                                             float* __restrict__ inv) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    batch_integral_one(patches, w, h, w, n, s, sumT, sqT);
+    inv[s] = train_invnorm(sumT, sqT, w + 1, h + 1, n, s);
+}
+
+// Weak update (src/main.cu:829-883): strong += (resp < theta) ? L : R for
+// every sample; active misclassified samples get w *= factor, factor =
+// exp(C) computed on the host in f64.
+__global__ __launch_bounds__(256) void apply_kernel(const vj_feature* __restrict__ feats, int feature,
+                                                    const uint32_t* __restrict__ sumT,
+                                                    const float* __restrict__ inv, int n,
+                                                    const int8_t* __restrict__ label,
+                                                    const uint8_t* __restrict__ active,
+                                                    double* __restrict__ w, float* __restrict__ strong,
+                                                    float theta, int parity, float left, float right,
+                                                    double factor, uint8_t* __restrict__ used) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) used[feature] = 1;
+    if (s >= n) return;
+    const vj_feature f = feats[feature];
+    const float r = response(f, sumT, inv, n, s);
+    const bool lt = r < theta;
+    strong[s] += lt ? left : right;
+    if (!active[s]) return;
+    const int pred = (parity > 0) ? (lt ? -1 : +1) : (lt ? +1 : -1);
+    if (pred != static_cast<int>(label[s])) w[s] *= factor;
+}
+
+// Exact fixed-point value of a non-negative double in 2^-90 units (int128 as
+// hi:lo), truncated like a C cast of ldexp(w, 90).
+__device__ __forceinline__ void fixed90(double w, uint64_t& lo, uint64_t& hi) {
+    lo = hi = 0;
+    const uint64_t b = static_cast<uint64_t>(__double_as_longlong(w));
+    int e = static_cast<int>((b >> 52) & 0x7FFu);
+    uint64_t m = b & 0xFFFFFFFFFFFFFull;
+    if (e == 0) e = 1;
+    else m |= 0x10000000000000ull;
+    const int sh = e - 985;  // value = m * 2^(e-1075) * 2^90
+    if (sh >= 0) {
+        if (sh < 64) {
+            lo = m << sh;
+            hi = sh ? (m >> (64 - sh)) : 0;
+        } else if (sh < 128) {
+            hi = m << (sh - 64);
+        }
+    } else if (sh > -64) {
+        lo = m >> (-sh);
+    }
+}
+
+// Nearest double (ties to even) to the non-negative int128 hi:lo.
+__device__ __forceinline__ double u128_to_double(uint64_t lo, uint64_t hi) {
+    if (!hi && !lo) return 0.0;
+    int p = hi ? 127 - __clzll(hi) : 63 - __clzll(lo);  // msb position
+    if (p <= 52) return static_cast<double>(lo);
+    const int sh = p - 52;
+    uint64_t mant, rem, half;
+    // mant = v >> sh, rem = v & ((1<<sh)-1)
+    if (sh < 64) {
+        mant = (lo >> sh) | (sh ? (hi << (64 - sh)) : 0);
+        rem = sh ? (lo & ((1ull << sh) - 1)) : 0;
+        // bits of hi below sh-64 do not exist when sh < 64
+        half = 1ull << (sh - 1);
+        const bool up = rem > half || (rem == half && (mant & 1ull));
+        if (up) ++mant;
+    } else {
+        const int s2 = sh - 64;
+        mant = hi >> s2;
+        const uint64_t remHi = s2 ? (hi & ((1ull << s2) - 1)) : 0;
+        // compare remHi:lo with the half point (1 << (sh-1))
+        bool up;
+        if (s2 == 0) {
+            up = lo > (1ull << 63) || (lo == (1ull << 63) && (mant & 1ull));
+        } else {
+            const uint64_t halfHi = 1ull << (s2 - 1);
+            up = remHi > halfHi || (remHi == halfHi && (lo > 0 || (mant & 1ull)));
+        }
+        if (up) ++mant;
+    }
+    return ldexp(static_cast<double>(mant), sh);
+}
+
+// Weight normalisation (src/main.cu:1725-1740, kernels K14/K15): inactive
+// weights to 0, exact sum, divide when the sum exceeds 1e-30; also writes
+// the next round's float search weights (+w positive, -w negative, 0
+// inactive).  One workgroup.
+__global__ __launch_bounds__(1024) void normalize_kernel(double* __restrict__ w, const int8_t* __restrict__ label,
+                                                         const uint8_t* __restrict__ active, int n,
+                                                         float* __restrict__ sw32) {
+    __shared__ uint64_t sLo[16], sHi[16];
+    __shared__ double sS;
+    uint64_t lo = 0, hi = 0;
+    for (int s = threadIdx.x; s < n; s += blockDim.x) {
+        if (!active[s]) w[s] = 0.0;
+        uint64_t a, b;
+        fixed90(w[s], a, b);
+        const uint64_t nl = lo + a;
+        hi += b + (nl < lo ? 1ull : 0ull);
+        lo = nl;
+    }
+    // wave reduction of the 128-bit partial sums
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        const uint64_t ol = __shfl_xor(lo, d, kWave), oh = __shfl_xor(hi, d, kWave);
+        const uint64_t nl = lo + ol;
+        hi += oh + (nl < lo ? 1ull : 0ull);
+        lo = nl;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sLo[threadIdx.x >> 6] = lo;
+        sHi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t L = 0, H = 0;
+        for (int k = 0; k < static_cast<int>(blockDim.x >> 6); ++k) {
+            const uint64_t nl = L + sLo[k];
+            H += sHi[k] + (nl < L ? 1ull : 0ull);
+            L = nl;
+        }
+        sS = ldexp(u128_to_double(L, H), -90);
+    }
+    __syncthreads();
+    const double S = sS;
+    for (int s = threadIdx.x; s < n; s += blockDim.x) {
+        double x = w[s];
+        if (S > 1e-30) x = x / S;
+        w[s] = x;
+        const float f = static_cast<float>(x);
+        sw32[s] = active[s] ? (label[s] > 0 ? f : -f) : 0.0f;
+    }
+}
+
+// Stage weight reset (src/main.cu:908-924): positives 0.5/numPos, active
+// negatives 0.5/activeNeg, others 0; strong scores 0.
+__global__ __launch_bounds__(256) void reset_kernel(double* __restrict__ w, float* __restrict__ strong,
+                                                    const uint8_t* __restrict__ active, int n, int numPos,
+                                                    double wPos, double wNeg) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    strong[s] = 0.0f;
+    w[s] = (s < numPos) ? wPos : (active[s] ? wNeg : 0.0);
+}
+
+// Stage-threshold calibration (src/main.cu:2172-2226): the cutIdx-th
+// smallest positive strong score by radix select over the order-preserving
+// keys (the reference sorts them with a radix sort), then the four counts
+// pos >= cut, pos > cut, active neg >= cut, active neg > cut.
+__global__ __launch_bounds__(1024) void calibrate_kernel(const float* __restrict__ strong,
+                                                         const uint8_t* __restrict__ active, int n,
+                                                         int numPos, int cutIdx, Calib* __restrict__ out) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t sPrefix, sMask, sK;
+    __shared__ uint32_t red[4][16];
+    if (threadIdx.x == 0) {
+        sPrefix = 0;
+        sMask = 0;
+        sK = static_cast<uint32_t>(cutIdx);
+    }
+    for (int pass = 3; pass >= 0; --pass) {
+        const int shift = pass * 8;
+        for (int b = threadIdx.x; b < 256; b += blockDim.x) hist[b] = 0;
+        __syncthreads();
+        const uint32_t prefix = sPrefix, mask = sMask;
+        for (int i = threadIdx.x; i < numPos; i += blockDim.x) {
+            const uint32_t k = float_key(strong[i]);
+            if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t kk = sK, d = 0;
+            for (; d < 255; ++d) {
+                if (kk < hist[d]) break;
+                kk -= hist[d];
+            }
+            sK = kk;
+            sPrefix = prefix | (d << shift);
+            sMask = mask | (255u << shift);
+        }
+        __syncthreads();
+    }
+    const float cut = key_float(sPrefix);
+    uint32_t c[4] = {0, 0, 0, 0};
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const float s = strong[i];
+        const uint32_t ge = s >= cut, gt = s > cut;
+        if (i < numPos) {
+            c[0] += ge;
+            c[1] += gt;
+        } else if (active[i]) {
+            c[2] += ge;
+            c[3] += gt;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        c[k] = wave_sum(c[k]);
+        if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = c[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Calib r{};
+        r.cut = cut;
+        uint32_t t[4] = {0, 0, 0, 0};
+        for (int k = 0; k < 4; ++k)
+            for (int wv = 0; wv < static_cast<int>(blockDim.x >> 6); ++wv) t[k] += red[k][wv];
+        r.posGe = t[0];
+        r.posGt = t[1];
+        r.negGe = t[2];
+        r.negGt = t[3];
+        *out = r;
+    }
+}
+
+// Stage end (src/main.cu:960-983): negatives stay active iff strong >= threshold.
+__global__ __launch_bounds__(256) void stage_end_kernel(const float* __restrict__ strong, uint8_t* __restrict__ active,
+                                                        int n, int numPos, float threshold,
+                                                        uint32_t* __restrict__ count) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t keep = 0;
+    if (s < n && s >= numPos && active[s]) {
+        keep = strong[s] >= threshold;
+        active[s] = static_cast<uint8_t>(keep);
+    }
+    keep = wave_sum(keep);
+    if ((threadIdx.x & 63) == 0 && keep) atomicAdd(count, keep);
+}
+
+// Per-op API kernels --------------------------------------------------------
+
+// evaluateFeatureResponses (src/boosting_core.cu:221-261): out[f][s].
+__global__ __launch_bounds__(256) void responses_kernel(const vj_feature* __restrict__ feats, int begin,
+                                                        const uint32_t* __restrict__ sumT,
+                                                        const float* __restrict__ inv, int n,
+                                                        float* __restrict__ out) {
+    const int f = blockIdx.y;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const vj_feature ft = feats[begin + f];
+    out[static_cast<size_t>(f) * n + s] = response(ft, sumT, inv, n, s);
+}
+
+// Search weights from the reference's float weights, labels and active mask.
+__global__ __launch_bounds__(256) void search_weights_kernel(const float* __restrict__ w,
+                                                             const int8_t* __restrict__ label,
+                                                             const uint8_t* __restrict__ active, int n,
+                                                             float* __restrict__ sw32) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const bool a = active == nullptr || active[s] != 0;
+    sw32[s] = a ? (label[s] > 0 ? w[s] : -w[s]) : 0.0f;
+}
+
+// Marks valid splits of a plain sorted-index array from the responses.
+__global__ __launch_bounds__(256) void mark_valid_kernel(const float* __restrict__ resp,
+                                                         const uint16_t* __restrict__ sorted, int n,
+                                                         uint16_t* __restrict__ out, int64_t pitch) {
+    const int f = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float* row = resp + static_cast<size_t>(f) * n;
+    const uint16_t* sr = sorted + static_cast<size_t>(f) * n;
+    uint32_t e = sr[k];
+    if (k + 1 < n && row[sr[k]] + 1e-7f < row[sr[k + 1]]) e |= 0x8000u;
+    out[static_cast<int64_t>(f) * pitch + k] = static_cast<uint16_t>(e);
+}
+
+// FeatureBestSplit per feature from per-feature scan results.
+struct FeatureBestSplit {
+    float bestErr;
+    float bestTheta;
+    int32_t bestSplitPos;
+    int8_t bestParity, pad0, pad1, pad2;
+};
+struct TileBestCandidate {
+    float bestErr;
+    float bestTheta;
+    int32_t featureIdx;
+    int8_t bestParity, pad0, pad1, pad2;
+};
+
+__global__ __launch_bounds__(kSearchThreads) void search_each_kernel(const uint16_t* __restrict__ ord, int64_t pitch,
+                                                                    int n, int nFeatures,
+                                                                    const float* __restrict__ sw32,
+                                                                    const float* __restrict__ resp,
+                                                                    FeatureBestSplit* __restrict__ out) {
+    extern __shared__ float lsw[];
+    __shared__ int64_t totP[16], totN[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int64_t totPos, totNeg;
+    load_weights(sw32, n, lsw, totP, totN, totPos, totNeg);
+    const int nWavesTotal = gridDim.x * (blockDim.x >> 6);
+    for (int f = blockIdx.x * (blockDim.x >> 6) + wave; f < nFeatures; f += nWavesTotal) {
+        const uint16_t* row = ord + static_cast<int64_t>(f) * pitch;
+        int64_t fE;
+        int32_t fK, fP;
+        scan_feature(row, n, lsw, totPos, totNeg, lane, fE, fK, fP);
+        if (lane == 0) {
+            FeatureBestSplit r{};
+            if (fK >= 0 && fK + 1 < n) {
+                r.bestErr = static_cast<float>(ldexp(static_cast<double>(fE), -62));
+                r.bestSplitPos = fK;
+                r.bestParity = static_cast<int8_t>(fP);
+                const float* rr = resp + static_cast<size_t>(f) * n;
+                r.bestTheta = 0.5f * (rr[row[fK] & 0x7FFF] + rr[row[fK + 1] & 0x7FFF]);
+            } else {
+                r.bestErr = FLT_MAX;
+                r.bestTheta = 0.0f;
+                r.bestSplitPos = -1;
+                r.bestParity = +1;
+            }
+            out[f] = r;
+        }
+    }
+}
+
+// selectBestInTile (src/boosting_core.cu:155-192): masked arg-min, smaller
+// index on ties.
+__global__ __launch_bounds__(256) void select_kernel(const FeatureBestSplit* __restrict__ best, int begin,
+                                                     int count, const uint8_t* __restrict__ used,
+                                                     TileBestCandidate* __restrict__ out) {
+    __shared__ float sE[256];
+    __shared__ int sF[256];
+    __shared__ float sT[256];
+    __shared__ int sP[256];
+    float e = FLT_MAX, th = 0.0f;
+    int fi = INT_MAX, par = +1;
+    for (int i = threadIdx.x; i < count; i += blockDim.x) {
+        const int f = begin + i;
+        if (used && used[f]) continue;
+        const FeatureBestSplit b = best[i];
+        if (b.bestErr < e || (b.bestErr == e && f < fi)) {
+            e = b.bestErr;
+            th = b.bestTheta;
+            fi = f;
+            par = b.bestParity;
+        }
+    }
+    sE[threadIdx.x] = e;
+    sF[threadIdx.x] = fi;
+    sT[threadIdx.x] = th;
+    sP[threadIdx.x] = par;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            const int o = threadIdx.x + st;
+            if (sE[o] < sE[threadIdx.x] || (sE[o] == sE[threadIdx.x] && sF[o] < sF[threadIdx.x])) {
+                sE[threadIdx.x] = sE[o];
+                sF[threadIdx.x] = sF[o];
+                sT[threadIdx.x] = sT[o];
+                sP[threadIdx.x] = sP[o];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        TileBestCandidate r{};
+        r.bestErr = sE[0];
+        r.bestTheta = sT[0];
+        r.featureIdx = sF[0];
+        r.bestParity = static_cast<int8_t>(sP[0]);
+        *out = r;
+    }
+}
+
+}  // namespace vj
+
+using namespace vj;
+
+struct vj_trainer {
+    int winW = 24, winH = 24, maxSamples = 0, maxTile = 0;
+    // per-op binding
+    vj_integral_view ii{};
+    const int8_t* bLabel = nullptr;
+    const uint8_t* bActive = nullptr;
+    const float* bWeight = nullptr;
+    int bN = 0;
+    DevBuf<float> opSw;
+    DevBuf<uint16_t> opOrd;
+
+    // fused stage state
+    int n = 0, numPos = 0, nFeat = 0, stageNegBase = 0;
+    int64_t pitch = 0;
+    const vj_feature* feats = nullptr;
+    DevBuf<uint32_t> sumT;
+    DevBuf<uint64_t> sqT;
+    DevBuf<float> inv;
+    DevBuf<uint16_t> ord;
+    DevBuf<int8_t> label;
+    DevBuf<uint8_t> active;
+    DevBuf<double> w;
+    DevBuf<float> strong;
+    DevBuf<float> sw32;
+    DevBuf<uint8_t> used;
+    DevBuf<WaveBest> blockBest;
+    DevBuf<RoundBest> roundBest;
+    DevBuf<Calib> calib;
+    DevBuf<uint32_t> counter;
+    PinnedBuf<RoundBest> hRound;
+    PinnedBuf<Calib> hCalib;
+    PinnedBuf<uint32_t> hCounter;
+    int searchBlocks = 0;
+};
+
+namespace {
+
+int search_blocks() {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) cus = p.multiProcessorCount;
+    }
+    (void)hipGetLastError();
+    return 2 * cus;
+}
+
+size_t presort_lds(int n) { return (kSortBuckets + 4) * 4 + static_cast<size_t>(n) * 4 + static_cast<size_t>(n) * 2 + 16; }
+
+// Every corner of every feature must index the (winW+1) x (winH+1) window
+// integral; checked on the host before any kernel reads through them.
+int validate_features(const vj_feature* dFeatures, int begin, int count, int winW, int winH, hipStream_t s) {
+    std::vector<vj_feature> hf(static_cast<size_t>(count));
+    VJ_HIP(hipMemcpyAsync(hf.data(), dFeatures + begin, sizeof(vj_feature) * hf.size(), hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipStreamSynchronize(s));
+    const int area = (winW + 1) * (winH + 1);
+    for (const vj_feature& f : hf) {
+        VJ_CHECK(f.rectCount == 2 || f.rectCount == 3, "feature rectCount must be 2 or 3");
+        for (int r = 0; r < f.rectCount; ++r) {
+            const int32_t p[4] = {f.r[r].p0, f.r[r].p1, f.r[r].p2, f.r[r].p3};
+            for (int c = 0; c < 4; ++c) VJ_CHECK(p[c] >= 0 && p[c] < area, "feature corner outside the window integral");
+        }
+    }
+    return VJ_OK;
+}
+
+int set_lds_limits() {
+    static bool done = false;
+    if (done) return VJ_OK;
+    VJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(presort_kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024));
+    VJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(search_kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 2048));
+    VJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(search_each_kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 2048));
+    done = true;
+    return VJ_OK;
+}
+
+}  // namespace
+
+extern "C" int vj_trainer_create(int winW, int winH, int maxSamples, int maxTile, vj_trainer** out) {
+    // src/boosting_core.cu:468-476
+    VJ_CHECK(out, "vj_trainer_create: null output");
+    *out = nullptr;
+    VJ_CHECK(winW > 0 && winH > 0 && maxSamples > 0 && maxTile > 0, "AdaBoostTrainer: invalid maxSamples/maxFeaturesPerTile");
+    VJ_CHECK(static_cast<int64_t>(maxSamples) * maxTile <= INT_MAX,
+             "AdaBoostTrainer: maxSamples*maxFeaturesPerTile exceeds int range");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        set_error("vj_trainer_create: no HIP device");
+        return VJ_DEVICE_ERROR;
+    }
+    VJ_TRY(set_lds_limits());
+    vj_trainer* t = new (std::nothrow) vj_trainer();
+    VJ_CHECK(t, "vj_trainer_create: out of host memory");
+    t->winW = winW;
+    t->winH = winH;
+    t->maxSamples = maxSamples;
+    t->maxTile = maxTile;
+    t->searchBlocks = search_blocks();
+    int st = t->hRound.ensure(1);
+    if (st == VJ_OK) st = t->hCalib.ensure(1);
+    if (st == VJ_OK) st = t->hCounter.ensure(1);
+    if (st != VJ_OK) {
+        delete t;
+        return st;
+    }
+    *out = t;
+    return VJ_OK;
+}
+
+extern "C" void vj_trainer_destroy(vj_trainer* t) { delete t; }
+
+extern "C" int vj_trainer_bind(vj_trainer* t, const vj_integral_view* ii, const int8_t* dLabel,
+                               const uint8_t* dActive, const float* dWeight, int n) {
+    // src/boosting_core.cu:688-702
+    VJ_CHECK(t && ii && ii->sumT && ii->invNorm && dLabel && dWeight, "bindTrainingSet: null view");
+    VJ_CHECK(ii->nSamples > 0 && ii->nSamples <= t->maxSamples && ii->nSamples <= 65535,
+             "bindTrainingSet: sample count out of range");
+    VJ_CHECK(n == ii->nSamples, "bindTrainingSet: label/weight count mismatch");
+    t->ii = *ii;
+    t->bLabel = dLabel;
+    t->bActive = dActive;
+    t->bWeight = dWeight;
+    t->bN = n;
+    return VJ_OK;
+}
+
+extern "C" int vj_trainer_evaluate_responses(vj_trainer* t, const vj_feature* dFeatures, int begin, int count,
+                                             float* dOut, vj_stream sv) {
+    VJ_CHECK(t && dFeatures && t->ii.sumT && t->ii.invNorm, "evaluateFeatureResponses: not bound");
+    VJ_CHECK(begin >= 0 && count > 0 && count <= t->maxTile && dOut, "evaluateFeatureResponses: bad range");
+    VJ_CHECK(t->ii.iiWidth == t->winW + 1 && t->ii.iiHeight == t->winH + 1, "evaluateFeatureResponses: window mismatch");
+    VJ_TRY(validate_features(dFeatures, begin, count, t->winW, t->winH, static_cast<hipStream_t>(sv)));
+    const int n = t->bN;
+    hipLaunchKernelGGL(responses_kernel, dim3((n + 255) / 256, count), dim3(256), 0, static_cast<hipStream_t>(sv),
+                       dFeatures, begin, t->ii.sumT, t->ii.invNorm, n, dOut);
+    VJ_HIP(hipGetLastError());
+    return VJ_OK;
+}
+
+extern "C" int vj_trainer_sort_per_feature(vj_trainer* t, const float* dResp, int count, uint16_t* dOut,
+                                           vj_stream sv) {
+    VJ_CHECK(t && dResp && dOut && count > 0 && count <= t->maxTile, "sortSamplesPerFeature: bad arguments");
+    const int n = t->bN;
+    VJ_CHECK(n > 0 && n <= kSortMaxN, "sortSamplesPerFeature: sample count above the LDS sort limit (20480)");
+    hipLaunchKernelGGL(presort_kernel, dim3(count), dim3(kSortThreads), presort_lds(n), static_cast<hipStream_t>(sv),
+                       nullptr, 0, nullptr, nullptr, dResp, n, dOut, static_cast<int64_t>(n), 1);
+    VJ_HIP(hipGetLastError());
+    return VJ_OK;
+}
+
+extern "C" int vj_trainer_find_threshold(vj_trainer* t, const float* dResp, const uint16_t* dSorted, int count,
+                                         void* dOutBest, vj_stream sv) {
+    VJ_CHECK(t && dResp && dSorted && dOutBest && t->bLabel && t->bWeight, "evaluateAndFindThreshold: not bound");
+    VJ_CHECK(count > 0 && count <= t->maxTile, "evaluateAndFindThreshold: bad count");
+    const int n = t->bN;
+    VJ_CHECK(n > 1 && n <= 32767, "evaluateAndFindThreshold: sample count must be 2..32767");
+    hipStream_t s = static_cast<hipStream_t>(sv);
+    const int64_t pitch = (n + 255) / 256 * 256;
+    VJ_TRY(t->opSw.ensure(static_cast<size_t>(n)));
+    VJ_TRY(t->opOrd.ensure(static_cast<size_t>(pitch) * count));
+    hipLaunchKernelGGL(search_weights_kernel, dim3((n + 255) / 256), dim3(256), 0, s, t->bWeight, t->bLabel,
+                       t->bActive, n, t->opSw.get());
+    VJ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(mark_valid_kernel, dim3((n + 255) / 256, count), dim3(256), 0, s, dResp, dSorted, n,
+                       t->opOrd.get(), pitch);
+    VJ_HIP(hipGetLastError());
+    const int blocks = std::min((count + 15) / 16, t->searchBlocks);
+    hipLaunchKernelGGL(search_each_kernel, dim3(blocks), dim3(kSearchThreads), static_cast<size_t>(n) * 4, s,
+                       t->opOrd.get(), pitch, n, count, t->opSw.get(), dResp,
+                       static_cast<FeatureBestSplit*>(dOutBest));
+    VJ_HIP(hipGetLastError());
+    return VJ_OK;
+}
+
+extern "C" int vj_trainer_select_best(vj_trainer* t, const void* dBest, int begin, int count, const uint8_t* dUsed,
+                                      void* dOut, vj_stream sv) {
+    VJ_CHECK(t && dBest && dOut && begin >= 0 && count > 0 && count <= t->maxTile, "selectBestInTile: bad arguments");
+    hipLaunchKernelGGL(select_kernel, dim3(1), dim3(256), 0, static_cast<hipStream_t>(sv),
+                       static_cast<const FeatureBestSplit*>(dBest), begin, count, dUsed,
+                       static_cast<TileBestCandidate*>(dOut));
+    VJ_HIP(hipGetLastError());
+    return VJ_OK;
+}
+
+extern "C" int vj_trainer_stage_begin(vj_trainer* t, const uint8_t* dPatches, const int8_t* hLabels,
+                                      const uint8_t* hActive, int n, int numPos, const vj_feature* dFeatures,
+                                      int nFeatures, vj_stream sv) {
+    VJ_CHECK(t && dPatches && hLabels && dFeatures && nFeatures > 0, "stage_begin: bad arguments");
+    VJ_CHECK(n > 1 && n <= t->maxSamples && n <= kSortMaxN,
+             "stage_begin: sample count must be 2..min(maxSamples, 20480)");
+    VJ_CHECK(numPos > 0 && numPos < n, "stage_begin: need positives and negatives");
+    VJ_CHECK(static_cast<int64_t>(t->winW) * t->winH * 255 < (1 << 24),
+             "stage_begin: window too large for exact float rectangle sums");
+    hipStream_t s = static_cast<hipStream_t>(sv);
+    VJ_TRY(validate_features(dFeatures, 0, nFeatures, t->winW, t->winH, s));
+    const int iw = t->winW + 1, ih = t->winH + 1;
+    const size_t area = static_cast<size_t>(iw) * ih;
+    t->pitch = (n + 255) / 256 * 256;
+    VJ_TRY(t->sumT.ensure(area * n));
+    VJ_TRY(t->sqT.ensure(area * n));
+    VJ_TRY(t->inv.ensure(static_cast<size_t>(n)));
+    VJ_TRY(t->ord.ensure(static_cast<size_t>(t->pitch) * nFeatures));
+    VJ_TRY(t->label.ensure(static_cast<size_t>(n)));
+    VJ_TRY(t->active.ensure(static_cast<size_t>(n)));
+    VJ_TRY(t->w.ensure(static_cast<size_t>(n)));
+    VJ_TRY(t->strong.ensure(static_cast<size_t>(n)));
+    VJ_TRY(t->sw32.ensure(static_cast<size_t>(n)));
+    VJ_TRY(t->used.ensure(static_cast<size_t>(nFeatures)));
+    VJ_TRY(t->blockBest.ensure(static_cast<size_t>(t->searchBlocks)));
+    VJ_TRY(t->roundBest.ensure(1));
+    VJ_TRY(t->calib.ensure(1));
+    VJ_TRY(t->counter.ensure(1));
+    std::vector<uint8_t> act(static_cast<size_t>(n), 1);
+    if (hActive) std::memcpy(act.data(), hActive, static_cast<size_t>(n));
+    for (int i = 0; i < numPos; ++i) act[static_cast<size_t>(i)] = 1;
+    int activeNeg = 0;
+    for (int i = numPos; i < n; ++i) activeNeg += act[static_cast<size_t>(i)] ? 1 : 0;
+    VJ_CHECK(activeNeg > 0, "stage_begin: no active negatives");
+    VJ_HIP(hipMemcpyAsync(t->label.get(), hLabels, static_cast<size_t>(n), hipMemcpyHostToDevice, s));
+    VJ_HIP(hipMemcpyAsync(t->active.get(), act.data(), static_cast<size_t>(n), hipMemcpyHostToDevice, s));
+    VJ_HIP(hipMemsetAsync(t->used.get(), 0, static_cast<size_t>(nFeatures), s));
+    // integrals and normalisers (src/main.cu:1945)
+    hipLaunchKernelGGL(stage_integral_kernel, dim3((n + 255) / 256), dim3(256), 0, s, dPatches, t->winW,
+                       t->winH, n, t->sumT.get(), t->sqT.get(), t->inv.get());
+    VJ_HIP(hipGetLastError());
+    // sample order of every feature, once for the whole stage
+    hipLaunchKernelGGL(presort_kernel, dim3(nFeatures), dim3(kSortThreads), presort_lds(n), s, dFeatures, 0,
+                       t->sumT.get(), t->inv.get(), nullptr, n, t->ord.get(), t->pitch, 0);
+    VJ_HIP(hipGetLastError());
+    // weights 0.5/numPos and 0.5/activeNeg, normalised (src/main.cu:1955-1962)
+    hipLaunchKernelGGL(reset_kernel, dim3((n + 255) / 256), dim3(256), 0, s, t->w.get(), t->strong.get(),
+                       t->active.get(), n, numPos, 0.5 / static_cast<double>(numPos), 0.5 / static_cast<double>(activeNeg));
+    VJ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(normalize_kernel, dim3(1), dim3(1024), 0, s, t->w.get(), t->label.get(), t->active.get(), n,
+                       t->sw32.get());
+    VJ_HIP(hipGetLastError());
+    VJ_HIP(hipStreamSynchronize(s));  // hLabels / act staging must stay valid until here
+    t->n = n;
+    t->numPos = numPos;
+    t->nFeat = nFeatures;
+    t->feats = dFeatures;
+    t->stageNegBase = activeNeg;
+    return VJ_OK;
+}
+
+extern "C" int vj_trainer_round(vj_trainer* t, const vj_round_config* cfg, vj_weak* out, vj_stream sv) {
+    VJ_CHECK(t && out && t->n > 0, "round: no stage begun");
+    hipStream_t s = static_cast<hipStream_t>(sv);
+    const float minHit = cfg ? cfg->minHitRate : 0.99f;
+    const float maxFa = cfg ? cfg->maxFalseAlarm : 0.40f;
+    std::memset(out, 0, sizeof *out);
+    out->feature = -1;
+    const int blocks = std::min((t->nFeat + 15) / 16, t->searchBlocks);
+    hipLaunchKernelGGL(search_kernel, dim3(blocks), dim3(kSearchThreads), static_cast<size_t>(t->n) * 4, s,
+                       t->ord.get(), t->pitch, t->n, t->nFeat, t->sw32.get(), t->used.get(), t->blockBest.get());
+    VJ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reduce_best_kernel, dim3(1), dim3(256), 0, s, t->blockBest.get(), blocks, t->ord.get(), t->pitch,
+                       t->feats, t->sumT.get(), t->inv.get(), t->n, t->roundBest.get());
+    VJ_HIP(hipGetLastError());
+    VJ_HIP(hipMemcpyAsync(t->hRound.get(), t->roundBest.get(), sizeof(RoundBest), hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipStreamSynchronize(s));
+    const RoundBest rb = *t->hRound.get();
+    if (rb.feature < 0) return VJ_OK;  // no valid split anywhere (src/main.cu:2065-2070)
+    // host round logic (src/main.cu:2082-2103), in f64
+    const double err = std::ldexp(static_cast<double>(rb.errQ), -62);
+    const double ec = std::min(std::max(err, 1e-6), 1.0 - 1e-6);
+    const double C = std::log((1.0 - ec) / ec);
+    const double factor = std::exp(C);
+    const float fc = static_cast<float>(C);
+    out->feature = rb.feature;
+    out->theta = rb.theta;
+    out->parity = rb.parity;
+    out->split = rb.split;
+    out->err = err;
+    out->alpha = C;
+    out->left = rb.parity > 0 ? -fc : fc;
+    out->right = rb.parity > 0 ? fc : -fc;
+    hipLaunchKernelGGL(apply_kernel, dim3((t->n + 255) / 256), dim3(256), 0, s, t->feats, rb.feature, t->sumT.get(),
+                       t->inv.get(), t->n, t->label.get(), t->active.get(), t->w.get(), t->strong.get(), rb.theta,
+                       rb.parity, out->left, out->right, factor, t->used.get());
+    VJ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(normalize_kernel, dim3(1), dim3(1024), 0, s, t->w.get(), t->label.get(), t->active.get(), t->n,
+                       t->sw32.get());
+    VJ_HIP(hipGetLastError());
+    // calibration (src/main.cu:2172-2226)
+    const int minPass = std::max(1, static_cast<int>(std::ceil(minHit * static_cast<float>(t->numPos))));
+    const int cutIdx = t->numPos - minPass;
+    VJ_CHECK(cutIdx >= 0 && cutIdx < t->numPos, "round: minHitRate gives no valid cut index");
+    hipLaunchKernelGGL(calibrate_kernel, dim3(1), dim3(1024), 0, s, t->strong.get(), t->active.get(), t->n, t->numPos,
+                       cutIdx, t->calib.get());
+    VJ_HIP(hipGetLastError());
+    VJ_HIP(hipMemcpyAsync(t->hCalib.get(), t->calib.get(), sizeof(Calib), hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipStreamSynchronize(s));
+    const Calib c = *t->hCalib.get();
+    const bool strictGt = static_cast<int>(c.posGt) >= minPass;
+    out->stageThreshold = strictGt ? std::nextafter(c.cut, INFINITY) : c.cut;
+    out->posPass = static_cast<int32_t>(strictGt ? c.posGt : c.posGe);
+    out->negSurvive = static_cast<int32_t>(strictGt ? c.negGt : c.negGe);
+    out->hitRate = static_cast<float>(out->posPass) / static_cast<float>(t->numPos);
+    out->falseAlarm = static_cast<float>(out->negSurvive) / static_cast<float>(t->stageNegBase);
+    out->stageSatisfied = ((out->hitRate >= minHit && out->falseAlarm <= maxFa) || out->negSurvive == 0) ? 1 : 0;
+    return VJ_OK;
+}
+
+extern "C" int vj_trainer_stage_end(vj_trainer* t, float threshold, int* remainNeg, vj_stream sv) {
+    VJ_CHECK(t && remainNeg && t->n > 0, "stage_end: no stage begun");
+    hipStream_t s = static_cast<hipStream_t>(sv);
+    *t->hCounter.get() = 0u;
+    VJ_HIP(hipMemcpyAsync(t->counter.get(), t->hCounter.get(), sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(stage_end_kernel, dim3((t->n + 255) / 256), dim3(256), 0, s, t->strong.get(), t->active.get(),
+                       t->n, t->numPos, threshold, t->counter.get());
+    VJ_HIP(hipGetLastError());
+    VJ_HIP(hipMemcpyAsync(t->hCounter.get(), t->counter.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipStreamSynchronize(s));
+    *remainNeg = static_cast<int>(*t->hCounter.get());
+    return VJ_OK;
+}
+
+extern "C" int vj_trainer_read_state(vj_trainer* t, double* hW, float* hStrong, uint8_t* hActive, float* hInv) {
+    VJ_CHECK(t && t->n > 0, "read_state: no stage begun");
+    const size_t n = static_cast<size_t>(t->n);
+    if (hW) VJ_HIP(hipMemcpy(hW, t->w.get(), n * sizeof(double), hipMemcpyDeviceToHost));
+    if (hStrong) VJ_HIP(hipMemcpy(hStrong, t->strong.get(), n * sizeof(float), hipMemcpyDeviceToHost));
+    if (hActive) VJ_HIP(hipMemcpy(hActive, t->active.get(), n, hipMemcpyDeviceToHost));
+    if (hInv) VJ_HIP(hipMemcpy(hInv, t->inv.get(), n * sizeof(float), hipMemcpyDeviceToHost));
+    return VJ_OK;
+}
+
+extern "C" int vj_trainer_read_orders(vj_trainer* t, int begin, int count, uint16_t* hOut) {
+    VJ_CHECK(t && hOut && begin >= 0 && count > 0 && begin + count <= t->nFeat, "read_orders: bad range");
+    VJ_HIP(hipMemcpy2D(hOut, static_cast<size_t>(t->n) * 2, t->ord.get() + static_cast<size_t>(begin) * t->pitch,
+                       static_cast<size_t>(t->pitch) * 2, static_cast<size_t>(t->n) * 2, static_cast<size_t>(count),
+                       hipMemcpyDeviceToHost));
+    return VJ_OK;
+}

@@ -1,0 +1,193 @@
+"""FaceVisionEngine and ModelIO over the C ABI.
+
+Mirrors include/vj/face_vision_engine.cuh:19-196 and include/vj/model_io.hpp
+(same method names and argument meaning).  Methods that return vj::Status in
+the reference raise ``VJError`` carrying that status here; the constructor
+raises where the reference throws (src/face_vision_engine.cu:659-725).
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._native import DetectStats, IntegralView, check, lib
+from .device import DeviceArray, _s
+from .vjtypes import DETECTION, FEATURE, STAGE, STUMP
+
+BASIC, CORE, ALL = 0, 1, 2
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    if isinstance(a, int):
+        return a
+    return np.ascontiguousarray(a).ctypes.data
+
+
+class FaceVisionEngine:
+    Basic, Core, All = BASIC, CORE, ALL
+
+    def __init__(self, win=(24, 24), maxSamples=1, maxImageW=1920, maxImageH=1080, enableTilted=False,
+                 maxDetections=200000):
+        h = C.c_void_p()
+        check(lib().vj_engine_create(int(win[0]), int(win[1]), int(maxSamples), int(maxImageW), int(maxImageH),
+                                     int(bool(enableTilted)), int(maxDetections), C.byref(h)),
+              "FaceVisionEngine")
+        self.handle = h.value
+        self.win = (int(win[0]), int(win[1]))
+        self.maxDetections = int(maxDetections)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().vj_engine_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- feature pool (src/face_vision_engine.cu:877-1055) -------------------
+    @staticmethod
+    def countHaarFeatures(win=(24, 24), mode=CORE):
+        n = C.c_int()
+        check(lib().vj_haar_pool(int(win[0]), int(win[1]), int(mode), None, 0, C.byref(n)), "countHaarFeatures")
+        return n.value
+
+    @staticmethod
+    def buildHaarFeaturePool(win=(24, 24), mode=CORE):
+        n = FaceVisionEngine.countHaarFeatures(win, mode)
+        out = np.zeros(n, FEATURE)
+        m = C.c_int()
+        check(lib().vj_haar_pool(int(win[0]), int(win[1]), int(mode), out.ctypes.data, n, C.byref(m)),
+              "buildHaarFeaturePool")
+        return out
+
+    def uploadFeaturePool(self, features, stream=None):
+        f = np.ascontiguousarray(features, FEATURE)
+        check(lib().vj_engine_upload_feature_pool(self.handle, f.ctypes.data, len(f), _s(stream)),
+              "uploadFeaturePool")
+
+    def deviceFeaturePool(self):
+        p, n = C.c_void_p(), C.c_int()
+        check(lib().vj_engine_feature_pool(self.handle, C.byref(p), C.byref(n)), "deviceFeaturePool")
+        return p.value, n.value
+
+    # -- integrals (src/face_vision_engine.cu:1057-1162) ----------------------
+    def computeIntegralBatchTransposed(self, dGray, imageW, imageH, imageStride, nSamples, computeInvNorm=True,
+                                       stream=None):
+        check(lib().vj_engine_integral_batch(self.handle, _ptr(dGray), int(imageW), int(imageH), int(imageStride),
+                                             int(nSamples), int(bool(computeInvNorm)), _s(stream)),
+              "computeIntegralBatchTransposed")
+
+    def integralSetView(self):
+        v = IntegralView()
+        check(lib().vj_engine_integral_view(self.handle, C.byref(v)), "integralSetView")
+        return v
+
+    def integral_to_host(self):
+        """(sumT u32 [iiArea][n], sqsumT u64 [iiArea][n], invNorm f32 [n]) of the current set."""
+        v = self.integralSetView()
+        area, n = v.iiArea, v.nSamples
+        s = np.empty((area, n), np.uint32)
+        q = np.empty((area, n), np.uint64)
+        inv = np.empty(n, np.float32)
+        check(lib().vj_memcpy_dtoh(s.ctypes.data, v.sumT, s.nbytes, None), "copy sumT")
+        check(lib().vj_memcpy_dtoh(q.ctypes.data, v.sqsumT, q.nbytes, None), "copy sqsumT")
+        check(lib().vj_memcpy_dtoh(inv.ctypes.data, v.invNorm, inv.nbytes, None), "copy invNorm")
+        return s, q, inv
+
+    # -- cascade (src/face_vision_engine.cu:1176-1354) ------------------------
+    def setCascadeModel(self, dFeatures, featureCount, dStumps, stumpCount, stages, stream=None):
+        st = np.ascontiguousarray(stages, STAGE)
+        check(lib().vj_engine_set_cascade(self.handle, _ptr(dFeatures), int(featureCount), _ptr(dStumps),
+                                          int(stumpCount), st.ctypes.data, len(st), _s(stream)),
+              "setCascadeModel")
+
+    def clearCascadeModel(self, stream=None):
+        check(lib().vj_engine_clear_cascade(self.handle, _s(stream)), "clearCascadeModel")
+
+    def set_cascade_host(self, features, stumps, stages):
+        """Uploads host arrays and sets them as the cascade (copies are kept by the engine)."""
+        df = DeviceArray.from_host(np.ascontiguousarray(features, FEATURE))
+        ds = DeviceArray.from_host(np.ascontiguousarray(stumps, STUMP))
+        self.setCascadeModel(df, len(features), ds, len(stumps), stages)
+        df.close()
+        ds.close()
+
+    # -- detection (src/face_vision_engine.cu:1541-1747) ----------------------
+    def detectMultiScale(self, dImage, imageW, imageH, imageStride, scaleFactor=1.25, minNeighbors=4,
+                         minObjectSize=24, maxDetections=200000, applyGrouping=True, stream=None):
+        cap = max(1, min(int(maxDetections), self.maxDetections))
+        out = np.zeros(cap, DETECTION)
+        n = C.c_int()
+        check(lib().vj_engine_detect(self.handle, _ptr(dImage), int(imageW), int(imageH), int(imageStride),
+                                     float(scaleFactor), int(minNeighbors), int(minObjectSize), int(maxDetections),
+                                     int(bool(applyGrouping)), out.ctypes.data, cap, C.byref(n), _s(stream)),
+              "detectMultiScale")
+        return out[:min(n.value, cap)].copy()
+
+    def detect_batch(self, dImages, sizes, scaleFactor=1.25, minNeighbors=4, minObjectSize=24,
+                     maxDetections=200000, applyGrouping=True, cap=None, stream=None):
+        """Detections of several device images in one pipeline; list of arrays."""
+        k = len(dImages)
+        ptrs = (C.c_void_p * k)(*[_ptr(d) for d in dImages])
+        ws = (C.c_int * k)(*[int(s[0]) for s in sizes])
+        hs = (C.c_int * k)(*[int(s[1]) for s in sizes])
+        strides = (C.c_int * k)(*[int(s[2]) if len(s) > 2 else int(s[0]) for s in sizes])
+        counts = (C.c_int * k)()
+        if cap is None:
+            cap = k * max(1, min(int(maxDetections), self.maxDetections))
+        out = np.zeros(max(1, cap), DETECTION)
+        check(lib().vj_engine_detect_batch(self.handle, k, ptrs, ws, hs, strides, float(scaleFactor),
+                                           int(minNeighbors), int(minObjectSize), int(maxDetections),
+                                           int(bool(applyGrouping)), out.ctypes.data, cap, counts, _s(stream)),
+              "detect_batch")
+        res, pos = [], 0
+        for i in range(k):
+            res.append(out[pos:pos + counts[i]].copy())
+            pos += counts[i]
+        return res
+
+    def last_stats(self):
+        s = DetectStats()
+        check(lib().vj_engine_last_stats(self.handle, C.byref(s)), "last_stats")
+        return {f: getattr(s, f) for f, _ in DetectStats._fields_}
+
+
+def group_rectangles(dets, minNeighbors=4, eps=0.2):
+    """groupRectanglesCpu (src/face_vision_engine.cu:1401-1539), host C++."""
+    d = np.ascontiguousarray(dets, DETECTION)
+    out = np.zeros(max(1, len(d)), DETECTION)
+    n = C.c_int()
+    check(lib().vj_group_rectangles(d.ctypes.data if len(d) else None, len(d), int(minNeighbors), float(eps),
+                                    out.ctypes.data, C.byref(n)), "groupRectangles")
+    return out[:n.value].copy()
+
+
+# -- ModelIO (src/model_io.cpp) ---------------------------------------------
+
+def saveCascadeModel(path, win, features, stumps, stages):
+    f = np.ascontiguousarray(features, FEATURE)
+    s = np.ascontiguousarray(stumps, STUMP)
+    g = np.ascontiguousarray(stages, STAGE)
+    check(lib().vj_model_save(str(path).encode(), int(win[0]), int(win[1]), f.ctypes.data, len(f), s.ctypes.data,
+                              len(s), g.ctypes.data, len(g)), "saveCascadeModel")
+
+
+def loadCascadeModel(path):
+    """Returns (win, features, stumps, stages)."""
+    ww, wh = C.c_int(), C.c_int()
+    nf, ns, ng = C.c_int64(), C.c_int64(), C.c_int64()
+    p = str(path).encode()
+    check(lib().vj_model_peek(p, C.byref(ww), C.byref(wh), C.byref(nf), C.byref(ns), C.byref(ng)),
+          "loadCascadeModel")
+    f = np.zeros(nf.value, FEATURE)
+    s = np.zeros(ns.value, STUMP)
+    g = np.zeros(ng.value, STAGE)
+    check(lib().vj_model_load(p, f.ctypes.data, nf.value, s.ctypes.data, ns.value, g.ctypes.data, ng.value),
+          "loadCascadeModel")
+    return (ww.value, wh.value), f, s, g

@@ -1,0 +1,234 @@
+/*
+ * vj_api.h -- C ABI of the MI355X-native Viola-Jones hot path (libvj_hip.so).
+ *
+ * Plain pointers and sizes only; no C++ or HIP types cross this boundary
+ * (streams and events are opaque void* that are hipStream_t / hipEvent_t).
+ * Each entry point names the reference interface it replaces, as
+ * path:line in the reference repository (hengyizhu/CS677s26).
+ *
+ * Ownership: input pointers are borrowed for the duration of the call;
+ * handles own their device and pinned-host buffers.  No C++ exception
+ * crosses the ABI; failures return a status code and vj_last_error() gives
+ * a message (thread-local).
+ *
+ * Device pointers ("d" prefix) must come from vj_malloc (or any hipMalloc'd
+ * memory of the current device).  Host pointers ("h" prefix) are ordinary
+ * memory.
+ */
+#ifndef VJ_API_H
+#define VJ_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* vj::Status, include/vj/common_types.cuh:11-17 (same numeric values;
+ * VJ_DEVICE_ERROR is the reference's kCudaError slot). */
+enum {
+    VJ_OK = 0,
+    VJ_INVALID_ARG = 1,
+    VJ_DEVICE_ERROR = 2,
+    VJ_OUT_OF_MEMORY = 3,
+    VJ_RUNTIME_ERROR = 4
+};
+
+/* POD records, byte-identical to include/vj/common_types.cuh:26-143. */
+typedef struct vj_rect { int32_t p0, p1, p2, p3; float w; } vj_rect;               /* FastRect     */
+typedef struct vj_feature {                                                           /* HaarFeature  */
+    vj_rect r[3];
+    uint8_t rectCount, tilted;
+    uint16_t reserved;
+} vj_feature;
+typedef struct vj_stump { int32_t feature; float theta, left, right; } vj_stump;     /* GpuStump4 (x = int bits) */
+typedef struct vj_stage { int32_t first, ntrees; float threshold; } vj_stage;        /* CascadeStage */
+typedef struct vj_detection { int32_t x, y, w, h, scaleIdx; float score; } vj_detection; /* Detection */
+typedef struct vj_weak {                                                              /* WeakClassifier + round stats */
+    int32_t feature;        /* chosen feature index (-1: none)                  */
+    float theta;            /* 0.5f*(v_k + v_k+1)                                 */
+    int32_t parity;         /* +1: resp<theta votes -1; -1: resp<theta votes +1 */
+    int32_t split;          /* position k in the feature's sorted order          */
+    double err;             /* weighted error (exact sum of the search weights)  */
+    double alpha;           /* C = log((1-e')/e'), e' = clamp(err, 1e-6, 1-1e-6) */
+    float left, right;      /* leaf values stored in the stump (float)(-+C)      */
+    float stageThreshold;   /* calibrated threshold after this round             */
+    float hitRate, falseAlarm;
+    int32_t posPass, negSurvive;
+    int32_t stageSatisfied; /* hit >= minHitRate && fa <= maxFalseAlarm, or no negatives left */
+} vj_weak;
+
+typedef void* vj_stream; /* hipStream_t; NULL = the default (null) stream */
+typedef void* vj_event;  /* hipEvent_t */
+
+const char* vj_last_error(void);
+
+/* ---- device plumbing: the reference calls the CUDA runtime directly
+ * (src/main.cu:52-101 CudaBuffer; cudaMemcpy/cudaEvent* throughout). ------- */
+int vj_device_count(int* n);
+int vj_device_name(char* buf, int cap);
+int vj_malloc(void** dptr, size_t bytes);
+int vj_free(void* dptr);
+int vj_host_alloc(void** hptr, size_t bytes); /* pinned */
+int vj_host_free(void* hptr);
+int vj_memcpy_htod(void* dst, const void* src, size_t bytes, vj_stream s);
+int vj_memcpy_dtoh(void* dst, const void* src, size_t bytes, vj_stream s);
+int vj_memcpy_dtod(void* dst, const void* src, size_t bytes, vj_stream s);
+int vj_memset(void* dst, int value, size_t bytes, vj_stream s);
+int vj_stream_create(vj_stream* s);
+int vj_stream_destroy(vj_stream s);
+int vj_stream_synchronize(vj_stream s);
+int vj_device_synchronize(void);
+int vj_event_create(vj_event* e);
+int vj_event_destroy(vj_event e);
+int vj_event_record(vj_event e, vj_stream s);
+int vj_event_synchronize(vj_event e);
+int vj_event_elapsed_ms(vj_event start, vj_event stop, float* ms);
+
+/* ---- Haar pool: FaceVisionEngine::buildHaarFeaturePool / countHaarFeatures,
+ * src/face_vision_engine.cu:877-1020.  mode 0 Basic, 1 Core, 2 All.
+ * *count receives the pool size; at most cap records are written. -------- */
+int vj_haar_pool(int winW, int winH, int mode, vj_feature* hOut, int cap, int* count);
+
+/* ---- model file: ModelIO::saveCascadeModel / loadCascadeModel,
+ * src/model_io.cpp:66-167 (.bin "VJCBIN1", version 1, byte-identical). --- */
+int vj_model_save(const char* path, int winW, int winH,
+                  const vj_feature* hFeatures, int64_t nFeatures,
+                  const vj_stump* hStumps, int64_t nStumps,
+                  const vj_stage* hStages, int64_t nStages);
+int vj_model_peek(const char* path, int* winW, int* winH,
+                  int64_t* nFeatures, int64_t* nStumps, int64_t* nStages);
+int vj_model_load(const char* path, vj_feature* hFeatures, int64_t capFeatures,
+                  vj_stump* hStumps, int64_t capStumps, vj_stage* hStages, int64_t capStages);
+
+/* ---- grouping: FaceVisionEngine::groupRectanglesCpu,
+ * src/face_vision_engine.cu:1401-1539 (host).  out needs room for n. ---- */
+int vj_group_rectangles(const vj_detection* hIn, int n, int minNeighbors, float eps,
+                        vj_detection* hOut, int* count);
+
+/* ---- PGM / PPM: loadPGM / savePPM, src/main.cu:286-371 -------------------- */
+int vj_pgm_peek(const char* path, int* w, int* h);
+int vj_pgm_load(const char* path, uint8_t* hOut, size_t cap);
+int vj_ppm_save_gray_boxes(const char* path, const uint8_t* hGray, int w, int h,
+                           const vj_detection* hBoxes, int n);
+
+/* ==== FaceVisionEngine, include/vj/face_vision_engine.cuh:19-196 ======== */
+typedef struct vj_engine vj_engine;
+
+typedef struct vj_integral_view {  /* IntegralImageSetT, common_types.cuh:48-60 */
+    int32_t nSamples, iiWidth, iiHeight, iiArea;
+    const uint32_t* sumT;          /* [iiArea][nSamples] (row-major image when nSamples == 1) */
+    const uint64_t* sqsumT;        /* [iiArea][nSamples], full 64-bit squares                 */
+    const float* invNorm;          /* [nSamples]                                               */
+} vj_integral_view;
+
+/* FaceVisionEngine ctor, src/face_vision_engine.cu:644-725 (throws there). */
+int vj_engine_create(int winW, int winH, int maxSamples, int maxImageW, int maxImageH,
+                     int enableTilted, int maxDetections, vj_engine** out);
+void vj_engine_destroy(vj_engine* e);
+/* uploadFeaturePool, src/face_vision_engine.cu:1022-1055 */
+int vj_engine_upload_feature_pool(vj_engine* e, const vj_feature* hFeatures, int n, vj_stream s);
+/* deviceFeaturePool / featurePoolSize, include/vj/face_vision_engine.cuh:62-63 */
+int vj_engine_feature_pool(vj_engine* e, const vj_feature** dFeatures, int* n);
+/* computeIntegralBatchTransposed, src/face_vision_engine.cu:1057-1148 */
+int vj_engine_integral_batch(vj_engine* e, const uint8_t* dGray, int w, int h, int stride,
+                             int nSamples, int computeInvNorm, vj_stream s);
+/* integralSetView, src/face_vision_engine.cu:1150-1162 */
+int vj_engine_integral_view(vj_engine* e, vj_integral_view* out);
+/* setCascadeModel, src/face_vision_engine.cu:1176-1322 (copies its inputs). */
+int vj_engine_set_cascade(vj_engine* e, const vj_feature* dFeatures, int nFeatures,
+                          const vj_stump* dStumps, int nStumps,
+                          const vj_stage* hStages, int nStages, vj_stream s);
+/* clearCascadeModel, src/face_vision_engine.cu:1324-1354 */
+int vj_engine_clear_cascade(vj_engine* e, vj_stream s);
+/* detectMultiScale, src/face_vision_engine.cu:1541-1747.  Synchronises s.
+ * Raw detections are in (level, y, x) scan order and truncated to the first
+ * maxDetections of that order (the reference CPU path's order and cap).
+ * *count receives the number of detections; min(*count, cap) are copied. */
+int vj_engine_detect(vj_engine* e, const uint8_t* dImage, int w, int h, int stride,
+                     float scaleFactor, int minNeighbors, int minObjectSize, int maxDetections,
+                     int applyGrouping, vj_detection* hOut, int cap, int* count, vj_stream s);
+
+typedef struct vj_detect_stats {
+    int32_t levels;          /* pyramid levels scanned                    */
+    int64_t windows;         /* windows visited (after step skipping)     */
+    int64_t rawDetections;   /* before truncation and grouping            */
+    int64_t pyramidPixels;   /* sum of level W*H                          */
+    int64_t integralBytes;   /* algorithmic bytes of the integral kernels */
+    int64_t cascadeBytes;    /* algorithmic bytes of the cascade kernel   */
+} vj_detect_stats;
+/* Statistics of the engine's last detect (no reference counterpart). */
+int vj_engine_last_stats(vj_engine* e, vj_detect_stats* out);
+
+/* Batched detect (no reference counterpart: the reference detects one image
+ * per call).  nImages images, each through the same pyramid, integral,
+ * cascade and compaction kernels as vj_engine_detect but all images' work in
+ * the same launches; the launch sequence is captured once per batch shape in
+ * a HIP graph and replayed.  Per image the detections are identical to
+ * vj_engine_detect on that image.  hOut receives the images' detections
+ * concatenated in image order; hCounts[i] is image i's count; the return is
+ * VJ_INVALID_ARG when the total exceeds cap. */
+int vj_engine_detect_batch(vj_engine* e, int nImages, const uint8_t* const* dImages,
+                           const int* w, const int* h, const int* stride,
+                           float scaleFactor, int minNeighbors, int minObjectSize,
+                           int maxDetections, int applyGrouping,
+                           vj_detection* hOut, int cap, int* hCounts, vj_stream s);
+
+/* ==== AdaBoostTrainer, include/vj/boosting_core.cuh:20-161 =============== */
+typedef struct vj_trainer vj_trainer;
+
+/* AdaBoostTrainer ctor, src/boosting_core.cu:466-548 (throws there). */
+int vj_trainer_create(int winW, int winH, int maxSamples, int maxFeaturesPerTile, vj_trainer** out);
+void vj_trainer_destroy(vj_trainer* t);
+/* bindTrainingSet, src/boosting_core.cu:684-745 (non-owning views; n <= 65535). */
+int vj_trainer_bind(vj_trainer* t, const vj_integral_view* ii, const int8_t* dLabel,
+                    const uint8_t* dActive, const float* dWeight, int nSamples);
+/* evaluateFeatureResponses, src/boosting_core.cu:747-787: dOut [count][n] f32. */
+int vj_trainer_evaluate_responses(vj_trainer* t, const vj_feature* dFeatures, int begin, int count,
+                                  float* dOut, vj_stream s);
+/* sortSamplesPerFeature, src/boosting_core.cu:829-892: per-feature ascending
+ * (value bits, index) order of dResp rows, written as plain u16 indices. */
+int vj_trainer_sort_per_feature(vj_trainer* t, const float* dResp, int count, uint16_t* dOutSorted,
+                                vj_stream s);
+/* evaluateAndFindThreshold, src/boosting_core.cu:789-827 (FeatureBestSplit
+ * records, 16 B each: err f32, theta f32, split i32, parity i8, pad). */
+int vj_trainer_find_threshold(vj_trainer* t, const float* dResp, const uint16_t* dSorted, int count,
+                              void* dOutBest, vj_stream s);
+/* selectBestInTile, src/boosting_core.cu:894-925 (TileBestCandidate, 16 B). */
+int vj_trainer_select_best(vj_trainer* t, const void* dBest, int begin, int count,
+                           const uint8_t* dUsedMask, void* dOut, vj_stream s);
+
+/* ---- fused stage / round form of the training loop (src/main.cu:1936-2348).
+ * The samples of a stage are fixed, so each feature's sample order is sorted
+ * once per stage (stage_begin) and every boosting round is one exact
+ * weighted prefix scan over the stored orders (round). ------------------- */
+typedef struct vj_round_config {
+    float minHitRate;      /* default 0.99  (src/main.cu:1386) */
+    float maxFalseAlarm;   /* default 0.40  (src/main.cu:1387) */
+} vj_round_config;
+
+/* Stage start (src/main.cu:1936-1997): integrals and invNorm of the n
+ * 24x24 patches (positives first), presort of every feature of dFeatures,
+ * weights 0.5/numPos and 0.5/activeNeg normalised, strong scores 0, used
+ * mask cleared.  hActive may be NULL (all active). */
+int vj_trainer_stage_begin(vj_trainer* t, const uint8_t* dPatches, const int8_t* hLabels,
+                           const uint8_t* hActive, int nSamples, int numPos,
+                           const vj_feature* dFeatures, int nFeatures, vj_stream s);
+/* One boosting round (src/main.cu:2005-2300): best stump over unused
+ * features, weight update, normalisation and stage-threshold calibration. */
+int vj_trainer_round(vj_trainer* t, const vj_round_config* cfg, vj_weak* out, vj_stream s);
+/* Stage end (src/main.cu:2302-2348): negatives with strong >= threshold stay
+ * active; returns their count in *remainNeg. */
+int vj_trainer_stage_end(vj_trainer* t, float stageThreshold, int* remainNeg, vj_stream s);
+/* Copies of the trainer's per-sample state (for checks and debugging). */
+int vj_trainer_read_state(vj_trainer* t, double* hWeights, float* hStrong, uint8_t* hActive,
+                          float* hInvNorm);
+/* Presorted orders of features [begin, begin+count) (u16 entries, bit 15 =
+ * valid split after this position) for checking the presort. */
+int vj_trainer_read_orders(vj_trainer* t, int begin, int count, uint16_t* hOut);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VJ_API_H */

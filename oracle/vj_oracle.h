@@ -1,0 +1,123 @@
+/*
+ * vj_oracle.h -- CPU restatement of the reference Viola-Jones hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the smoke
+ * entry point and bench.py's cpu_baseline leg may load it; the product
+ * library (cs677s26_amd/libvj_hip.so) never links or calls it.
+ *
+ * Every function restates one reference CPU routine; the citation (file:line
+ * relative to the reference repository) is on each declaration.  Types are
+ * byte-identical to the reference PODs (include/vj/common_types.cuh:26-143).
+ *
+ * Built with -O2 -ffp-contract=off and no fast-math so every float and
+ * double operation rounds exactly as written (x86-64 SSE2, no FMA).
+ */
+#ifndef VJ_ORACLE_H
+#define VJ_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { int32_t p0, p1, p2, p3; float w; } or_rect;          /* FastRect    20 B */
+typedef struct {
+    or_rect r[3];
+    uint8_t rectCount, tilted;
+    uint16_t reserved;
+} or_feature;                                                          /* HaarFeature 64 B */
+typedef struct { int32_t feature; float theta, left, right; } or_stump; /* GpuStump4   16 B */
+typedef struct { int32_t first, ntrees; float threshold; } or_stage;   /* CascadeStage 12 B */
+typedef struct { int32_t x, y, w, h, scaleIdx; float score; } or_det;  /* Detection   24 B */
+
+/* Haar pool enumeration; mode 0 Basic, 1 Core, 2 All (tilted).
+ * src/face_vision_engine.cu:618-635 (makeFastRect), :882-1020 (pool loops).
+ * Returns the feature count; writes at most cap features when out != NULL. */
+int or_haar_pool(int winW, int winH, int mode, or_feature* out, int cap);
+
+/* Summed-area tables, (h+1) x (w+1) row-major, row 0 and column 0 zero.
+ * src/main.cu:428-455 (buildHostIntegralImage).  sum is the exact u32 value
+ * (the reference's int32 for every image up to 4K); sq keeps all 64 bits,
+ * the reference keeps the low 32 (its window deltas agree, see or_detect_raw). */
+void or_integral(const uint8_t* img, int w, int h, int stride, uint32_t* sum, uint64_t* sq);
+
+/* Training-patch normaliser on the inset rect (1,1,iiW-3,iiH-3).
+ * src/main.cu:457-472 (same arithmetic as ComputeInvNormKernel,
+ * src/face_vision_engine.cu:216-259). */
+float or_train_invnorm(const uint32_t* sum, const uint64_t* sq, int iiW, int iiH);
+
+/* Pixel-centre bilinear resize with +0.5 rounding.  src/main.cu:545-585. */
+void or_resize(const uint8_t* src, int sw, int sh, int sstride, uint8_t* dst, int dw, int dh);
+
+/* Pyramid schedule.  Fills up to cap entries of (scaleIdx, scaledW, scaledH,
+ * step, scale) for the levels that are scanned; returns their count.
+ * src/main.cu:717-747, src/face_vision_engine.cu:1596-1714. */
+typedef struct { int32_t scaleIdx, w, h, step; float scale; } or_level;
+int or_levels(int imageW, int imageH, int winW, int winH, float scaleFactor,
+              int minObjectSize, or_level* out, int cap);
+
+/* Raw multi-scale detection in (level, y, x) order, returning at maxDetections.
+ * src/main.cu:684-827 (detectMultiScaleCpu) with evalDetectionFeatureHost
+ * src/main.cu:504-543.  Returns the detection count (<= cap). */
+int or_detect_raw(const or_feature* features, int nFeatures,
+                  const or_stump* stumps, int nStumps,
+                  const or_stage* stages, int nStages,
+                  int winW, int winH,
+                  const uint8_t* img, int w, int h, int stride,
+                  float scaleFactor, int minObjectSize, int maxDetections,
+                  or_det* out, int cap,
+                  int64_t* windowsVisited);
+
+/* OpenCV-style grouping.  src/face_vision_engine.cu:585-616 (DisjointSet,
+ * similarRects), :1401-1539 (groupRectanglesCpu).  Returns the output count. */
+int or_group(const or_det* in, int n, int minNeighbors, float eps, or_det* out);
+
+/* ---- training ---------------------------------------------------------- */
+
+/* Transposed training integrals for n patches of w x h (layout [iiArea][n])
+ * plus per-sample invNorm.  src/face_vision_engine.cu:47-259 semantics. */
+void or_train_integrals(const uint8_t* patches, int w, int h, int n,
+                        uint32_t* sumT, uint64_t* sqT, float* invNorm);
+
+/* Normalised response of one feature on sample s of a transposed set.
+ * src/main.cu:476-496 (evalTrainingFeatureHost) / src/boosting_core.cu:221-261. */
+float or_train_response(const or_feature* f, const uint32_t* sumT, const float* invNorm,
+                        int n, int s);
+
+/* Per-feature ascending order of (response bits, sample index) with bit 15
+ * (or bit 31 for the u32 layout) set where fl(v_k + 1e-7f) < v_{k+1}.
+ * Split semantics: src/boosting_core.cu:395-405; order: the stable segmented
+ * radix sort at src/boosting_core.cu:863-876 (float keys, -0 before +0). */
+void or_presort(const or_feature* features, int fBegin, int fCount,
+                const uint32_t* sumT, const float* invNorm, int n,
+                uint16_t* out16, uint32_t* out32);
+
+/* Exact fixed-point weight of a float weight: trunc(|w| * 2^62) with sign. */
+int64_t or_fixed_weight(float w);
+
+typedef struct {
+    int64_t errQ;      /* error in units of 2^-62; INT64_MAX when no valid split */
+    int32_t split;     /* position k in the sorted order; -1 when none */
+    int32_t parity;    /* +1 / -1 */
+} or_feature_best;
+
+/* Best split of every feature in [fBegin, fBegin+fCount) from the presorted
+ * orders, with exact int64 prefix sums over sw32 (signed float weights:
+ * +w positive, -w negative, 0 inactive).  Comparator betterBest,
+ * src/boosting_core.cu:59-79; threshold scan :289-456. */
+void or_search(const uint16_t* ord16, const uint32_t* ord32, int fCount, int n,
+               const float* sw32, or_feature_best* out);
+
+/* Round arg-min over features (skip used, tie -> smaller index).
+ * src/boosting_core.cu:87-99,155-192, src/main.cu:991-1028.
+ * Returns the feature index or -1. */
+int or_select(const or_feature_best* best, int fCount, const uint8_t* used, int64_t* errQ);
+
+/* Exact sum of n doubles via 2^-90 fixed point (int128) rounded to double. */
+double or_exact_sum(const double* w, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

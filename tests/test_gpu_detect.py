@@ -1,0 +1,152 @@
+"""GPU parity of the detect path against the oracle (bit-exact).
+
+Integrals (u32 sum, u64 square sum), the pyramid, the cascade and the
+order-preserving compaction are checked through the C ABI against the C
+restatement of the reference CPU path (src/main.cu:428-827) and the host
+grouping (src/face_vision_engine.cu:1401-1539).
+"""
+import numpy as np
+import pytest
+
+from cs677s26_amd import synth
+from cs677s26_amd.device import DeviceArray
+from cs677s26_amd.engine import FaceVisionEngine
+from cs677s26_amd.vjtypes import STAGE, det_tuples
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def cascade3():
+    return synth.blob_cascade((1, 10, 25))
+
+
+def _engine(w, h, cascade, max_det=200000):
+    eng = FaceVisionEngine((24, 24), 1, w, h, False, max_det)
+    eng.set_cascade_host(*cascade)
+    return eng
+
+
+@pytest.mark.parametrize("w,h", [(24, 24), (65, 70), (641, 479), (1920, 1080), (257, 1031)])
+def test_integral_single_image(w, h):
+    say(f"integral single {w}x{h}")
+    rng = np.random.default_rng(w * 7 + h)
+    img = rng.integers(0, 256, size=(h, w), dtype=np.uint8)
+    eng = FaceVisionEngine((24, 24), 1, w, h)
+    d = DeviceArray.from_host(img)
+    eng.computeIntegralBatchTransposed(d, w, h, w, 1, True)
+    s, q, inv = eng.integral_to_host()
+    es, eq = O.integral(img)
+    assert np.array_equal(s.reshape(h + 1, w + 1), es)
+    assert np.array_equal(q.reshape(h + 1, w + 1), eq)
+
+
+def test_integral_all_255_4k():
+    w, h = 3840, 2160
+    say("integral all-255 4K")
+    img = np.full((h, w), 255, np.uint8)
+    eng = FaceVisionEngine((24, 24), 1, w, h)
+    d = DeviceArray.from_host(img)
+    eng.computeIntegralBatchTransposed(d, w, h, w, 1, False)
+    s, q, _ = eng.integral_to_host()
+    s = s.reshape(h + 1, w + 1)
+    q = q.reshape(h + 1, w + 1)
+    yy, xx = np.mgrid[0:h + 1, 0:w + 1]
+    assert np.array_equal(s.astype(np.int64), 255 * yy.astype(np.int64) * xx)
+    assert np.array_equal(q.astype(np.int64), 65025 * yy.astype(np.int64) * xx)
+
+
+def test_integral_batch_transposed():
+    say("integral batch 24x24 x 3000")
+    patches, _ = synth.training_set(1500, 1500, seed=3)
+    n = len(patches)
+    eng = FaceVisionEngine((24, 24), n, 24, 24)
+    d = DeviceArray.from_host(patches)
+    eng.computeIntegralBatchTransposed(d, 24, 24, 24, n, True)
+    s, q, inv = eng.integral_to_host()
+    es, eq, einv = O.train_integrals(patches)
+    assert np.array_equal(s, es) and np.array_equal(q, eq)
+    assert np.array_equal(inv.view(np.uint32), einv.view(np.uint32))
+
+
+def test_detect_config1_raw_and_grouped(cascade3):
+    say("detect 640x480 3-stage raw+grouped")
+    img = synth.detect_image(640, 480, 20, 24, 64)
+    eng = _engine(640, 480, cascade3)
+    d = DeviceArray.from_host(img)
+    raw = eng.detectMultiScale(d, 640, 480, 640, 1.25, 4, 24, 200000, applyGrouping=False)
+    want, _ = O.detect_raw(*cascade3, img)
+    assert len(raw) == len(want) > 0
+    assert det_tuples(raw) == det_tuples(want)
+    grouped = eng.detectMultiScale(d, 640, 480, 640, 1.25, 4, 24, 200000, applyGrouping=True)
+    assert det_tuples(grouped) == det_tuples(O.group(want, 4))
+    st = eng.last_stats()
+    assert st["windows"] == 653485 and st["levels"] == 14
+
+
+def test_detect_truncates_in_scan_order(cascade3):
+    say("detect maxDetections truncation")
+    img = synth.detect_image(640, 480, 20, 24, 64)
+    feats, stumps, stages = cascade3
+    eng = _engine(640, 480, cascade3, max_det=1000)
+    d = DeviceArray.from_host(img)
+    raw = eng.detectMultiScale(d, 640, 480, 640, 1.25, 4, 24, 1000, applyGrouping=False)
+    want, _ = O.detect_raw(feats, stumps, stages, img, max_det=1000)
+    assert len(want) == 1000
+    assert det_tuples(raw) == det_tuples(want)
+
+
+def test_detect_edge_cases(cascade3):
+    say("detect edge cases")
+    feats, stumps, stages = cascade3
+    eng = _engine(700, 500, cascade3)
+    # constant image: every window fails the variance gate
+    flat = np.full((200, 300), 128, np.uint8)
+    d = DeviceArray.from_host(flat)
+    assert len(eng.detectMultiScale(d, 300, 200, 300, 1.25, 4, 24, 200000, False)) == 0
+    # image smaller than the window: no levels
+    tiny = np.zeros((20, 30), np.uint8)
+    d = DeviceArray.from_host(tiny)
+    assert len(eng.detectMultiScale(d, 30, 20, 30, 1.25, 4, 24, 200000, False)) == 0
+    # row stride wider than the image, and a large minObjectSize (level 0 skipped)
+    img = synth.detect_image(333, 250, 8, 24, 60, seed=5)
+    padded = np.zeros((250, 341), np.uint8)
+    padded[:, :333] = img
+    d = DeviceArray.from_host(padded)
+    for min_obj, sf in ((24, 1.25), (40, 1.1), (31, 1.5)):
+        raw = eng.detectMultiScale(d, 333, 250, 341, sf, 4, min_obj, 200000, False)
+        want, _ = O.detect_raw(feats, stumps, stages, img, sf=sf, min_obj=min_obj)
+        assert det_tuples(raw) == det_tuples(want), (min_obj, sf)
+    # threshold -inf everywhere: exactly the windows passing the variance gate
+    st_open = np.array(stages, STAGE)
+    st_open["threshold"] = -np.inf
+    eng2 = _engine(333, 250, (feats, stumps, st_open))
+    raw = eng2.detectMultiScale(d, 333, 250, 341, 1.25, 4, 24, 10 ** 6, False)
+    want, vis = O.detect_raw(feats, stumps, st_open, img, max_det=10 ** 6)
+    assert det_tuples(raw) == det_tuples(want)
+    st_shut = np.array(stages, STAGE)
+    st_shut["threshold"] = np.inf
+    eng3 = _engine(333, 250, (feats, stumps, st_shut))
+    assert len(eng3.detectMultiScale(d, 333, 250, 341, 1.25, 4, 24, 200000, False)) == 0
+
+
+def test_detect_batch_equals_single(cascade3):
+    say("detect batch of 6 uneven images")
+    sizes = [(640, 480), (333, 250), (1000, 431), (24, 24), (800, 600), (97, 201)]
+    imgs = [synth.detect_image(w, h, max(1, w * h // 15000), 24, 64, seed=100 + i) for i, (w, h) in enumerate(sizes)]
+    eng = _engine(1000, 600, cascade3)
+    dev = [DeviceArray.from_host(im) for im in imgs]
+    for grouping in (False, True):
+        got = eng.detect_batch(dev, [(w, h, w) for w, h in sizes], 1.25, 4, 24, 200000, grouping)
+        for im, g, d, (w, h) in zip(imgs, got, dev, sizes):
+            single = eng.detectMultiScale(d, w, h, w, 1.25, 4, 24, 200000, grouping)
+            assert det_tuples(g) == det_tuples(single)
+            want, _ = O.detect_raw(*cascade3, im)
+            if grouping:
+                want = O.group(want, 4)
+            assert det_tuples(g) == det_tuples(want)
