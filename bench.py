@@ -1,0 +1,210 @@
+"""Benchmark: detect Mpix/s on 1080p with the 38-stage cascade (one MI355X).
+
+A step is one plain detect call (pyramid, integrals, cascade, compaction,
+device-to-host copy and host grouping) on a 1920x1080 u8 frame that is
+already resident in HBM.  Prints ONE JSON line (see README of this file's
+fields below).  Extra legs, each optional:
+
+* ``batched``  -- 64 uneven images per detect_batch call (Mpix/s);
+* ``stump``    -- one AdaBoost stage: presort of 162,336 features x 20,000
+  samples, then 50 boosting rounds (seconds per stage);
+* ``cpu``      -- the oracle (C restatement of the reference CPU path) on the
+  same 1080p frame, one thread (the reference CPU path is single-threaded).
+
+``roofline`` is for the cascade kernel (the dominant one): algorithmic bytes
+per launch (12*(W+1)*(H+1) summed over levels, plus model and output bytes)
+over its average duration from HIP events on the detect stream.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "detect Mpix/s (1080p, 38-stage); stump-search s/stage; p99 latency at QPS"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def detect_leg(args):
+    import numpy as np
+
+    from cs677s26_amd import workloads
+    from cs677s26_amd.device import DeviceArray, synchronize
+    from cs677s26_amd.engine import FaceVisionEngine
+
+    img, (feats, stumps, stages) = workloads.detect_1080()
+    h, w = img.shape
+    eng = FaceVisionEngine((24, 24), 1, w, h, False, 200000)
+    eng.set_cascade_host(feats, stumps, stages)
+    d_img = DeviceArray.from_host(img)
+    for _ in range(args.warmup):
+        eng.detectMultiScale(d_img, w, h, w, 1.25, 4, 24, 200000, True)
+    # timed region: K whole detect calls, device synchronised on both sides
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dets = eng.detectMultiScale(d_img, w, h, w, 1.25, 4, 24, 200000, True)
+    synchronize()
+    wall = time.perf_counter() - t0
+    stats = eng.last_stats()
+    # per-kernel device time from HIP events on the detect stream
+    eng.set_timing(True)
+    phases = np.zeros(4)
+    for _ in range(args.steps):
+        eng.detectMultiScale(d_img, w, h, w, 1.25, 4, 24, 200000, True)
+        phases += np.array(eng.phase_ms())
+    eng.set_timing(False)
+    phases /= args.steps
+    return {
+        "mpix": w * h / 1e6,
+        "wall_s": wall,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "detections": int(len(dets)),
+        "stats": stats,
+        "phase_ms": phases.tolist(),
+    }
+
+
+def batched_leg(args):
+    from cs677s26_amd import workloads
+    from cs677s26_amd.device import DeviceArray, synchronize
+    from cs677s26_amd.engine import FaceVisionEngine
+
+    sizes, imgs = workloads.batch64()
+    mw = max(s[0] for s in sizes)
+    mh = max(s[1] for s in sizes)
+    feats, stumps, stages = workloads.cascade_38()
+    eng = FaceVisionEngine((24, 24), 1, mw, mh, False, 200000)
+    eng.set_cascade_host(feats, stumps, stages)
+    dev = [DeviceArray.from_host(im) for im in imgs]
+    spec = [(w, h, w) for (w, h) in sizes]
+    eng.detect_batch(dev, spec, 1.25, 4, 24, 200000, True)
+    synchronize()
+    reps = max(1, min(args.steps, 3))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        eng.detect_batch(dev, spec, 1.25, 4, 24, 200000, True)
+    synchronize()
+    wall = time.perf_counter() - t0
+    mpix = sum(w * h for w, h in sizes) / 1e6
+    return {"mpix_per_s": mpix * reps / wall, "mpix_per_batch": mpix, "ms_per_batch": 1e3 * wall / reps}
+
+
+def stump_leg(args):
+    from cs677s26_amd import workloads
+    from cs677s26_amd.device import DeviceArray, synchronize
+    from cs677s26_amd.engine import FaceVisionEngine
+    from cs677s26_amd.trainer import AdaBoostTrainer
+
+    patches, labels = workloads.training()
+    pool = FaceVisionEngine.buildHaarFeaturePool((24, 24), FaceVisionEngine.Basic)
+    n = len(labels)
+    tr = AdaBoostTrainer((24, 24), n, 1024)
+    d_p = DeviceArray.from_host(patches)
+    d_f = DeviceArray.from_host(pool)
+    synchronize()
+    t0 = time.perf_counter()
+    tr.stage_begin(d_p, labels, int((labels > 0).sum()), d_f, len(pool))
+    synchronize()
+    t1 = time.perf_counter()
+    rounds = []
+    for _ in range(args.rounds):
+        rounds.append(tr.round(0.99, 0.40))
+    synchronize()
+    t2 = time.perf_counter()
+    return {"s_per_stage": t2 - t0, "presort_s": t1 - t0, "round_ms": 1e3 * (t2 - t1) / max(1, args.rounds),
+            "rounds": args.rounds, "features": len(pool), "samples": n,
+            "first_feature": rounds[0]["feature"] if rounds else -1}
+
+
+def cpu_leg(args):
+    from cs677s26_amd import workloads
+    from oracle import pyoracle as O
+
+    img, (feats, stumps, stages) = workloads.detect_1080()
+    h, w = img.shape
+    O.lib()
+    t0 = time.perf_counter()
+    raw, _ = O.detect_raw(feats, stumps, stages, img)
+    O.group(raw, 4)
+    dt = time.perf_counter() - t0
+    return {"value": w * h / 1e6 / dt, "unit": "Mpix/s", "cores": 1, "kind": "port",
+            "sample": "one 1920x1080 frame, 38-stage cascade, detect + grouping, oracle C restatement, 1 thread",
+            "seconds": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rounds", type=int, default=50)
+    ap.add_argument("--skip", default="", help="comma list of legs to skip: batched,stump,cpu")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X; "
+              "run without --gpus or with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    skip = set(x for x in args.skip.split(",") if x)
+
+    log("detect 1080p / 38-stage")
+    d = detect_leg(args)
+    out = {
+        "metric": METRIC,
+        "value": d["mpix"] * args.steps / d["wall_s"],
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": d["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "detect 1920x1080 u8, seeded 200 blobs 24-400 px, 38-stage 6061-stump cascade, "
+                               "sf 1.25, minNeighbors 4, minObjectSize 24, maxDetections 200000, plain path",
+                   "levels": d["stats"]["levels"], "windows": d["stats"]["windows"],
+                   "raw_detections": d["stats"]["rawDetections"], "detections": d["detections"]},
+    }
+    casc_ms = d["phase_ms"][2]
+    casc_bytes = d["stats"]["cascadeBytes"]
+    achieved = casc_bytes / (casc_ms * 1e-3) / 1e9 if casc_ms > 0 else 0.0
+    out["roofline"] = {"bound": "hbm", "kernel": "cascade_kernel", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                       "algorithmic_bytes": casc_bytes, "avg_ms": casc_ms}
+    int_ms = d["phase_ms"][1] + d["phase_ms"][0]
+    int_bytes = d["stats"]["integralBytes"]
+    out["phases_ms"] = {"pyramid": d["phase_ms"][0], "integral": d["phase_ms"][1], "cascade": d["phase_ms"][2],
+                        "compaction": d["phase_ms"][3]}
+    out["integral_roofline"] = {"achieved": int_bytes / (int_ms * 1e-3) / 1e9 if int_ms > 0 else 0.0,
+                                "peak": HBM_PEAK_GBS, "unit": "GB/s", "algorithmic_bytes": int_bytes,
+                                "avg_ms": int_ms}
+    if "batched" not in skip:
+        log("batched 64 images")
+        b = batched_leg(args)
+        out["batched_mpix_per_s"] = b["mpix_per_s"]
+        out["batched"] = b
+    if "stump" not in skip:
+        log("stump search stage (162,336 x 20,000, 50 rounds)")
+        s = stump_leg(args)
+        out["stump_search_s_per_stage"] = s["s_per_stage"]
+        out["stump"] = s
+    out["p99_latency_ms"] = None
+    if "cpu" not in skip:
+        log("cpu baseline (oracle, 1 thread)")
+        out["cpu_baseline"] = cpu_leg(args)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -91,6 +91,8 @@ _PROTOS = {
     "vj_engine_clear_cascade": (I, [P, P]),
     "vj_engine_detect": (I, [P, P, I, I, I, F, I, I, I, I, P, I, C.POINTER(I), P]),
     "vj_engine_last_stats": (I, [P, C.POINTER(DetectStats)]),
+    "vj_engine_set_timing": (I, [P, I]),
+    "vj_engine_phase_ms": (I, [P, C.POINTER(F)]),
     "vj_engine_detect_batch": (I, [P, I, P, P, P, P, F, I, I, I, I, P, I, P, P]),
     "vj_trainer_create": (I, [I, I, I, I, PP]),
     "vj_trainer_destroy": (None, [P]),

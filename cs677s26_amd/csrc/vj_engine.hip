@@ -319,10 +319,10 @@ __global__ __launch_bounds__(256) void invnorm_kernel(const uint32_t* __restrict
 __global__ __launch_bounds__(256) void cascade_kernel(const LevelDesc* __restrict__ L,
                                                       const int32_t* __restrict__ lvlIdx,
                                                       const int32_t* __restrict__ tileOffs, int nLv,
-                                                      int nTiles, int step,
+                                                      int nTiles,
                                                       const uint32_t* __restrict__ iiS,
                                                       const uint64_t* __restrict__ iiQ,
-                                                      const KStump* __restrict__ stumps,
+                                                      const KStump* __restrict__ stumpsAll, int nStumps,
                                                       const KStage* __restrict__ stages, int nStages,
                                                       int winW, int winH,
                                                       uint64_t* __restrict__ words, float* __restrict__ scores) {
@@ -345,6 +345,8 @@ __global__ __launch_bounds__(256) void cascade_kernel(const LevelDesc* __restric
     const int ty = local / lv.tilesX;
     const int tx = local - ty * lv.tilesX;
     const int TH = lv.tileRows;
+    const int step = lv.step;
+    const KStump* __restrict__ stumps = stumpsAll + static_cast<size_t>(step - 1) * nStumps;
     const int wx0 = tx * kTileWin, wy0 = ty * TH;
     const int ix0 = wx0 * step, iy0 = wy0 * step;
     const int FW = (kTileWin - 1) * step + winW + 1;
@@ -555,8 +557,9 @@ struct Plan {
     std::vector<int64_t> imgWordOff;
     std::vector<std::vector<ResizeJob>> resizeJobs;  // per depth
     std::vector<int32_t> resizeItems;                // per depth
-    std::vector<int32_t> stepLvl[3], stepTileOff[3];
-    int32_t stepTiles[3] = {0, 0, 0};
+    std::vector<int32_t> tileLvl, tileOff;  // scanned levels and their first cascade tile
+    int32_t nTiles = 0;
+    int32_t maxStep = 0;
     int32_t nItems = 0, nChunkItems = 0;
     int64_t pyrBytes = 0, iiElems = 0, colElems = 0, totElems = 0, rowElems = 0, nWords = 0;
     int64_t windows = 0, pyramidPixels = 0, integralBytes = 0, cascadeBytes = 0;
@@ -606,6 +609,17 @@ struct vj_engine {
     PinnedBuf<uint8_t> hStage;  // staging for plan uploads
 
     vj_detect_stats stats{};
+
+    // optional per-phase timing: events before resize, integral, cascade,
+    // compaction and after compaction
+    bool timing = false;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    float phaseMs[4] = {0, 0, 0, 0};
+
+    ~vj_engine() {
+        for (hipEvent_t& x : ev)
+            if (x) (void)hipEventDestroy(x);
+    }
 };
 
 namespace vj {
@@ -722,7 +736,6 @@ int build_plan(const vj_engine* e, const std::vector<ImageIn>& imgs, float sf, i
     // per-level work layout (scanned levels only; source-only entries get empty ranges)
     int64_t iiOff = 0, colOff = 0, totOff = 0, rowOff = 0, wordOff = 0;
     int32_t items = 0, chunkItems = 0;
-    int32_t tiles[3] = {0, 0, 0};
     int curImg = 0;
     for (size_t li = 0; li < P.levels.size(); ++li) {
         LevelDesc& d = P.levels[li];
@@ -766,11 +779,11 @@ int build_plan(const vj_engine* e, const std::vector<ImageIn>& imgs, float sf, i
         totOff += static_cast<int64_t>(d.nBands) * d.nChunks;
         rowOff += static_cast<int64_t>(d.h) * d.nChunks;
         wordOff += static_cast<int64_t>(d.scanH) * d.rowWords;
-        const int si = d.step - 1;
-        P.stepLvl[si].push_back(static_cast<int32_t>(li));
-        P.stepTileOff[si].push_back(tiles[si]);
-        d.tileOff = tiles[si];
-        tiles[si] += d.tilesX * d.tilesY;
+        P.tileLvl.push_back(static_cast<int32_t>(li));
+        P.tileOff.push_back(P.nTiles);
+        d.tileOff = P.nTiles;
+        P.nTiles += d.tilesX * d.tilesY;
+        P.maxStep = std::max(P.maxStep, d.step);
         P.windows += static_cast<int64_t>(d.scanW) * d.scanH;
         P.pyramidPixels += static_cast<int64_t>(d.w) * d.h;
         P.integralBytes += static_cast<int64_t>(d.w) * d.h + 12ll * (d.w + 1) * (d.h + 1);
@@ -788,7 +801,6 @@ int build_plan(const vj_engine* e, const std::vector<ImageIn>& imgs, float sf, i
     }
     P.nItems = items;
     P.nChunkItems = chunkItems;
-    for (int s = 0; s < 3; ++s) P.stepTiles[s] = tiles[s];
     P.pyrBytes = pyrOff;
     P.iiElems = iiOff;
     P.colElems = colOff;
@@ -860,13 +872,10 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     meta32.insert(meta32.end(), P.chunkOffs.begin(), P.chunkOffs.end());
     const size_t oSrc = meta32.size();
     meta32.insert(meta32.end(), P.src.begin(), P.src.end());
-    size_t oLvl[3], oTile[3];
-    for (int k = 0; k < 3; ++k) {
-        oLvl[k] = meta32.size();
-        meta32.insert(meta32.end(), P.stepLvl[k].begin(), P.stepLvl[k].end());
-        oTile[k] = meta32.size();
-        meta32.insert(meta32.end(), P.stepTileOff[k].begin(), P.stepTileOff[k].end());
-    }
+    const size_t oLvl = meta32.size();
+    meta32.insert(meta32.end(), P.tileLvl.begin(), P.tileLvl.end());
+    const size_t oTile = meta32.size();
+    meta32.insert(meta32.end(), P.tileOff.begin(), P.tileOff.end());
     std::vector<int64_t> meta64;
     const size_t oLvWord = meta64.size();
     meta64.insert(meta64.end(), P.lvWordOff.begin(), P.lvWordOff.end());
@@ -894,6 +903,7 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     const int64_t* m64 = e->dMeta64.get();
     const int nLevels = static_cast<int>(P.levels.size());
 
+    if (e->timing) VJ_HIP(hipEventRecord(e->ev[0], s));
     // 1. pyramid: one launch per resize depth (each depth needs the previous)
     for (size_t dpt = 0; dpt < P.resizeJobs.size(); ++dpt) {
         const int n = static_cast<int>(P.resizeJobs[dpt].size());
@@ -902,6 +912,7 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
                            e->dJobs.get() + jobStart[dpt], n, e->dRs.get(), m32 + oSrc);
         VJ_HIP(hipGetLastError());
     }
+    if (e->timing) VJ_HIP(hipEventRecord(e->ev[1], s));
     // 2. integrals of every level
     if (P.nItems > 0) {
         hipLaunchKernelGGL(ii_colsum_kernel, dim3(P.nItems), dim3(64), 0, s, dL, m32 + oItem, nLevels,
@@ -915,20 +926,29 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
                            e->rowQ.get(), e->iiS.get(), e->iiQ.get());
         VJ_HIP(hipGetLastError());
     }
-    // 3. cascade, one launch per window step (tile footprint differs)
-    for (int k = 0; k < 3; ++k) {
-        if (!P.stepTiles[k]) continue;
-        const int step = k + 1;
-        const int TH = tile_rows_for_step(step);
-        const int FW = (kTileWin - 1) * step + e->winW + 1;
-        const int FH = (TH - 1) * step + e->winH + 1;
-        const size_t ldsBytes = static_cast<size_t>(FW) * FH * sizeof(float);
-        hipLaunchKernelGGL(cascade_kernel, dim3(P.stepTiles[k]), dim3(256), ldsBytes, s, dL, m32 + oLvl[k],
-                           m32 + oTile[k], static_cast<int>(P.stepLvl[k].size()), P.stepTiles[k], step,
-                           e->iiS.get(), e->iiQ.get(), e->kst.get() + static_cast<size_t>(k) * e->nStumps,
-                           e->kstage.get(), e->nStages, e->winW, e->winH, e->words.get(), e->scores.get());
+    // 3. cascade: every tile of every level in one launch; LDS sized for the
+    //    largest window step present (its tile footprint is the widest)
+    if (e->timing) VJ_HIP(hipEventRecord(e->ev[2], s));
+    if (P.nTiles > 0) {
+        const int TH = tile_rows_for_step(P.maxStep);
+        const int FW = (kTileWin - 1) * P.maxStep + e->winW + 1;
+        const int FH = (TH - 1) * P.maxStep + e->winH + 1;
+        size_t ldsBytes = 0;
+        for (int st = 1; st <= P.maxStep; ++st) {
+            const int th = tile_rows_for_step(st);
+            ldsBytes = std::max(ldsBytes, static_cast<size_t>((kTileWin - 1) * st + e->winW + 1) *
+                                              static_cast<size_t>((th - 1) * st + e->winH + 1) * sizeof(float));
+        }
+        (void)TH;
+        (void)FW;
+        (void)FH;
+        hipLaunchKernelGGL(cascade_kernel, dim3(P.nTiles), dim3(256), ldsBytes, s, dL, m32 + oLvl, m32 + oTile,
+                           static_cast<int>(P.tileLvl.size()), P.nTiles, e->iiS.get(), e->iiQ.get(), e->kst.get(),
+                           e->nStumps, e->kstage.get(), e->nStages, e->winW, e->winH, e->words.get(),
+                           e->scores.get());
         VJ_HIP(hipGetLastError());
     }
+    if (e->timing) VJ_HIP(hipEventRecord(e->ev[3], s));
     // 4. counts and order-preserving emission
     hipLaunchKernelGGL(count_kernel, dim3(nImages), dim3(1024), 0, s, m64 + oImgWord, e->words.get(),
                        e->dCounts.get());
@@ -937,6 +957,7 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
                        m64 + oImgWord, e->words.get(), e->scores.get(), e->dCounts.get(), cap, e->winW,
                        e->winH, e->dOut.get());
     VJ_HIP(hipGetLastError());
+    if (e->timing) VJ_HIP(hipEventRecord(e->ev[4], s));
     return VJ_OK;
 }
 
@@ -1166,6 +1187,8 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
     VJ_TRY(e->hCounts.ensure(static_cast<size_t>(nImages)));
     VJ_HIP(hipMemcpyAsync(e->hCounts.get(), e->dCounts.get(), sizeof(int32_t) * nImages, hipMemcpyDeviceToHost, s));
     VJ_HIP(hipStreamSynchronize(s));
+    if (e->timing)
+        for (int k = 0; k < 4; ++k) VJ_HIP(hipEventElapsedTime(&e->phaseMs[k], e->ev[k], e->ev[k + 1]));
     int64_t total = 0;
     int64_t raw = 0;
     for (int i = 0; i < nImages; ++i) {
@@ -1244,6 +1267,20 @@ extern "C" int vj_engine_detect_batch(vj_engine* e, int nImages, const uint8_t* 
         set_error(ex.what());
         return VJ_RUNTIME_ERROR;
     }
+}
+
+extern "C" int vj_engine_set_timing(vj_engine* e, int on) {
+    VJ_CHECK(e, "vj_engine_set_timing: null");
+    if (on && !e->ev[0])
+        for (hipEvent_t& x : e->ev) VJ_HIP(hipEventCreate(&x));
+    e->timing = on != 0;
+    return VJ_OK;
+}
+
+extern "C" int vj_engine_phase_ms(vj_engine* e, float* ms) {
+    VJ_CHECK(e && ms, "vj_engine_phase_ms: null");
+    for (int k = 0; k < 4; ++k) ms[k] = e->phaseMs[k];
+    return VJ_OK;
 }
 
 extern "C" int vj_engine_last_stats(vj_engine* e, vj_detect_stats* out) {

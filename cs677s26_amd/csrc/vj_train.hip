@@ -975,6 +975,19 @@ extern "C" int vj_trainer_find_threshold(vj_trainer* t, const float* dResp, cons
     const int n = t->bN;
     VJ_CHECK(n > 1 && n <= 32767, "evaluateAndFindThreshold: sample count must be 2..32767");
     hipStream_t s = static_cast<hipStream_t>(sv);
+    // exact sums count 2^-62 units in int64: the weights must be normalised
+    // (every boosting round leaves them summing to 1)
+    {
+        std::vector<float> hw(static_cast<size_t>(n));
+        VJ_HIP(hipMemcpyAsync(hw.data(), t->bWeight, sizeof(float) * hw.size(), hipMemcpyDeviceToHost, s));
+        VJ_HIP(hipStreamSynchronize(s));
+        double tot = 0.0;
+        for (float x : hw) {
+            VJ_CHECK(std::isfinite(x), "evaluateAndFindThreshold: non-finite weight");
+            tot += std::fabs(static_cast<double>(x));
+        }
+        VJ_CHECK(tot <= 1.5, "evaluateAndFindThreshold: weights must be normalised (sum of |w| <= 1.5)");
+    }
     const int64_t pitch = (n + 255) / 256 * 256;
     VJ_TRY(t->opSw.ensure(static_cast<size_t>(n)));
     VJ_TRY(t->opOrd.ensure(static_cast<size_t>(pitch) * count));

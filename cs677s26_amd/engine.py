@@ -152,6 +152,15 @@ class FaceVisionEngine:
             pos += counts[i]
         return res
 
+    def set_timing(self, on=True):
+        check(lib().vj_engine_set_timing(self.handle, int(bool(on))), "set_timing")
+
+    def phase_ms(self):
+        """Device ms of the last detect: pyramid, integrals, cascade, compaction."""
+        a = (C.c_float * 4)()
+        check(lib().vj_engine_phase_ms(self.handle, a), "phase_ms")
+        return list(a)
+
     def last_stats(self):
         s = DetectStats()
         check(lib().vj_engine_last_stats(self.handle, C.byref(s)), "last_stats")

@@ -1,0 +1,60 @@
+"""The benchmark workloads, built from seeds (see synth.py).
+
+* ``detect_640``   -- 640x480, 20 blobs of 24-64 px, 3-stage 1/10/25 cascade.
+* ``detect_1080``  -- 1920x1080, 200 blobs of 24-400 px, 38 stages / 6061 stumps
+  (1, 10, 25, 25, 50, the rest spread evenly).
+* ``training``     -- 10000 + 10000 24x24 patches, Basic pool (162,336 features).
+* ``batch64``      -- 64 images, log-uniform area 320x240..3840x2160, aspect 4:3..21:9.
+
+Generated cascades are cached as .bin model files under ``.vjcache/`` at the
+repository root (or ``$VJ_CACHE_DIR``); the cache is an accelerator only, a
+missing file is regenerated from the seed.
+"""
+import os
+import zlib
+
+import numpy as np
+
+from . import synth
+from .engine import loadCascadeModel, saveCascadeModel
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CACHE = os.environ.get("VJ_CACHE_DIR", os.path.join(ROOT, ".vjcache"))
+
+
+def cascade(stage_sizes, seed=synth.DEFAULT_SEED):
+    crc = zlib.crc32(np.asarray(stage_sizes, np.int64).tobytes())
+    key = f"cascade_{len(stage_sizes)}x{sum(stage_sizes)}_s{seed}_{crc:08x}.bin"
+    path = os.path.join(CACHE, key)
+    if os.path.exists(path):
+        _, f, s, g = loadCascadeModel(path)
+        return f, s, g
+    f, s, g = synth.blob_cascade(tuple(stage_sizes), seed)
+    os.makedirs(CACHE, exist_ok=True)
+    tmp = path + f".tmp{os.getpid()}"
+    saveCascadeModel(tmp, (24, 24), f, s, g)
+    os.replace(tmp, path)
+    return f, s, g
+
+
+def detect_640():
+    img = synth.detect_image(640, 480, 20, 24, 64)
+    return img, cascade((1, 10, 25))
+
+
+def cascade_38():
+    return cascade(synth.spread_stage_sizes(38, 6061))
+
+
+def detect_1080():
+    img = synth.detect_image(1920, 1080, 200, 24, 400)
+    return img, cascade_38()
+
+
+def training(n_pos=10000, n_neg=10000, seed=synth.DEFAULT_SEED):
+    return synth.training_set(n_pos, n_neg, seed)
+
+
+def batch64(seed=synth.DEFAULT_SEED):
+    sizes = synth.batch_sizes(64, seed=seed)
+    return sizes, synth.batch_images(sizes, seed=seed)

@@ -160,6 +160,12 @@ typedef struct vj_detect_stats {
 } vj_detect_stats;
 /* Statistics of the engine's last detect (no reference counterpart). */
 int vj_engine_last_stats(vj_engine* e, vj_detect_stats* out);
+/* Per-phase device time of the last detect when timing is on (HIP events on
+ * the detect's stream): ms[0] pyramid, ms[1] integrals, ms[2] cascade,
+ * ms[3] compaction.  No reference counterpart (its detect-bench times the
+ * whole call, src/main.cu:2471-2537). */
+int vj_engine_set_timing(vj_engine* e, int on);
+int vj_engine_phase_ms(vj_engine* e, float* ms);
 
 /* Batched detect (no reference counterpart: the reference detects one image
  * per call).  nImages images, each through the same pyramid, integral,

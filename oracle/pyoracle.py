@@ -43,6 +43,7 @@ def lib():
         L.or_train_integrals.argtypes = [P, i, i, i, P, P, P]
         L.or_train_response.argtypes = [P, P, P, i, i]
         L.or_train_response.restype = f
+        L.or_train_responses.argtypes = [P, P, P, i, P]
         L.or_presort.argtypes = [P, i, i, P, P, i, P, P]
         L.or_fixed_weight.argtypes = [f]
         L.or_fixed_weight.restype = C.c_int64
@@ -176,3 +177,102 @@ def fixed_weight(w):
 def exact_sum(w):
     w = np.ascontiguousarray(w, np.float64)
     return lib().or_exact_sum(_p(w), len(w))
+
+
+def feature_responses(feature, sumT, inv):
+    """Responses of one feature on every sample (src/main.cu:476-496)."""
+    f = np.ascontiguousarray(np.asarray(feature, FEATURE).reshape(1))
+    n = sumT.shape[1]
+    out = np.zeros(n, np.float32)
+    lib().or_train_responses(_p(f), _p(sumT), _p(inv), n, _p(out))
+    return out
+
+
+def _signed_search_weights(w, labels, active):
+    f = w.astype(np.float32)
+    return np.where(active != 0, np.where(labels > 0, f, -f), np.float32(0)).astype(np.float32)
+
+
+def _normalise(w, active):
+    """ApplyActiveMask + exact sum + divide (src/main.cu:885-906, 1725-1740)."""
+    w = np.where(active != 0, w, 0.0)
+    s = exact_sum(w)
+    if s > 1e-30:
+        w = w / s
+    return w
+
+
+def train_stage(patches, labels, num_pos, features, rounds, min_hit=0.99, max_fa=0.40, active=None,
+                stop=False):
+    """One cascade stage restated from src/main.cu:1936-2348.
+
+    Stage start: integrals, invNorm, weights 0.5/numPos and 0.5/activeNeg
+    normalised, strong = 0.  Each round: exact fixed-point search over the
+    presorted orders (or_search / or_select), theta = 0.5f*(v_k + v_k+1),
+    C = log((1-e')/e') in f64 with e' = clamp(e, 1e-6, 1-1e-6), leaves
+    (float)(-+C) by parity, strong += (resp < theta) ? L : R in f32, active
+    misclassified w *= exp(C), exact-sum normalisation, calibration by the
+    cutIdx-th smallest positive score.  Returns the per-round records and the
+    final (w, strong, active).
+    """
+    import math
+
+    labels = np.asarray(labels, np.int8)
+    n = len(labels)
+    act = np.ones(n, np.uint8) if active is None else np.array(active, np.uint8)
+    act[:num_pos] = 1
+    sumT, _sq, inv = train_integrals(patches)
+    ords = presort(features, sumT, inv)
+    active_neg = int(act[num_pos:].sum())
+    w = np.where(np.arange(n) < num_pos, 0.5 / num_pos, np.where(act != 0, 0.5 / active_neg, 0.0))
+    w = _normalise(w, act)
+    strong = np.zeros(n, np.float32)
+    used = np.zeros(len(features), np.uint8)
+    recs = []
+    min_pass = max(1, int(math.ceil(np.float32(min_hit) * np.float32(num_pos))))
+    cut_idx = num_pos - min_pass
+    for _ in range(rounds):
+        sw = _signed_search_weights(w, labels, act)
+        best = search(ords, sw)
+        fi, eq = select(best, used)
+        if fi < 0:
+            break
+        k, parity = int(best[fi]["split"]), int(best[fi]["parity"])
+        resp = feature_responses(features[fi], sumT, inv)
+        s0, s1 = int(ords[fi, k]) & 0x7FFF, int(ords[fi, k + 1]) & 0x7FFF
+        theta = np.float32(0.5) * (resp[s0] + resp[s1])
+        err = math.ldexp(float(eq), -62)
+        ec = min(max(err, 1e-6), 1.0 - 1e-6)
+        c = math.log((1.0 - ec) / ec)
+        factor = math.exp(c)
+        fc = np.float32(c)
+        left, right = (-fc, fc) if parity > 0 else (fc, -fc)
+        used[fi] = 1
+        lt = resp < theta
+        strong = (strong + np.where(lt, left, right)).astype(np.float32)
+        pred = np.where(lt, -1, 1) if parity > 0 else np.where(lt, 1, -1)
+        mis = (act != 0) & (pred != labels)
+        w = np.where(mis, w * factor, w)
+        w = _normalise(w, act)
+        pos = np.sort(strong[:num_pos])
+        cut = pos[cut_idx]
+        pos_ge = int((strong[:num_pos] >= cut).sum())
+        pos_gt = int((strong[:num_pos] > cut).sum())
+        neg = strong[num_pos:][act[num_pos:] != 0]
+        neg_ge = int((neg >= cut).sum())
+        neg_gt = int((neg > cut).sum())
+        strict = pos_gt >= min_pass
+        thr = np.nextafter(cut, np.float32(np.inf)) if strict else cut
+        pos_pass = pos_gt if strict else pos_ge
+        neg_surv = neg_gt if strict else neg_ge
+        hit = np.float32(pos_pass) / np.float32(num_pos)
+        fa = np.float32(neg_surv) / np.float32(active_neg)
+        sat = (hit >= np.float32(min_hit) and fa <= np.float32(max_fa)) or neg_surv == 0
+        recs.append({"feature": int(fi), "split": k, "parity": parity, "errQ": int(eq), "err": err,
+                     "theta": float(theta), "theta_bits": int(np.float32(theta).view(np.uint32)),
+                     "alpha": c, "left": float(left), "right": float(right),
+                     "threshold_bits": int(np.float32(thr).view(np.uint32)), "posPass": pos_pass,
+                     "negSurvive": neg_surv, "satisfied": bool(sat)})
+        if stop and sat:
+            break
+    return recs, w, strong, act

@@ -516,6 +516,10 @@ float or_train_response(const or_feature* f, const uint32_t* sumT, const float* 
     return resp * invNorm[s];
 }
 
+void or_train_responses(const or_feature* f, const uint32_t* sumT, const float* invNorm, int n, float* out) {
+    for (int s = 0; s < n; ++s) out[s] = or_train_response(f, sumT, invNorm, n, s);
+}
+
 static inline uint32_t float_key(float v) {
     uint32_t b;
     memcpy(&b, &v, 4);

@@ -84,6 +84,7 @@ void or_train_integrals(const uint8_t* patches, int w, int h, int n,
  * src/main.cu:476-496 (evalTrainingFeatureHost) / src/boosting_core.cu:221-261. */
 float or_train_response(const or_feature* f, const uint32_t* sumT, const float* invNorm,
                         int n, int s);
+void or_train_responses(const or_feature* f, const uint32_t* sumT, const float* invNorm, int n, float* out);
 
 /* Per-feature ascending order of (response bits, sample index) with bit 15
  * (or bit 31 for the u32 layout) set where fl(v_k + 1e-7f) < v_{k+1}.

@@ -150,3 +150,20 @@ def test_detect_batch_equals_single(cascade3):
             if grouping:
                 want = O.group(want, 4)
             assert det_tuples(g) == det_tuples(want)
+
+
+def test_detect_config3_golden():
+    """1920x1080, 38-stage / 6061-stump cascade: raw and grouped boxes equal
+    the oracle's (golden fixture written by tests/golden/make_golden.py)."""
+    import os
+    say("detect 1080p 38-stage vs golden")
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "detect_c3.npz"))
+    from cs677s26_amd import workloads
+    img, casc = workloads.detect_1080()
+    eng = _engine(1920, 1080, casc)
+    d = DeviceArray.from_host(img)
+    raw = eng.detectMultiScale(d, 1920, 1080, 1920, 1.25, 4, 24, 200000, applyGrouping=False)
+    assert det_tuples(raw) == det_tuples(g["raw"])
+    assert eng.last_stats()["windows"] == int(g["windows"]) == 4729998
+    grouped = eng.detectMultiScale(d, 1920, 1080, 1920, 1.25, 4, 24, 200000, applyGrouping=True)
+    assert det_tuples(grouped) == det_tuples(g["grouped"])

@@ -1,0 +1,151 @@
+"""GPU parity of the stump search and boosting rounds against the oracle.
+
+Integer and index results (sample orders, valid-split bits, chosen feature,
+split position, parity, calibrated counts) are bit-exact; so are theta, the
+stage threshold, the strong scores and the f64 weights, because every sum is
+exact (fixed point) and every other step is one IEEE operation.  Errors are
+also checked against plain f64 sums of the f64 weights within 1e-6 relative.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from cs677s26_amd import synth, workloads
+from cs677s26_amd.device import DeviceArray
+from cs677s26_amd.engine import FaceVisionEngine
+from cs677s26_amd.trainer import FEATURE_BEST_SPLIT, TILE_BEST, AdaBoostTrainer
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def _subset_pool():
+    basic = O.haar_pool(24, 0)
+    core = O.haar_pool(24, 1)
+    centre = core[core["r"][:, 1]["w"] == 9.0]
+    return np.concatenate([basic[::53], centre[::97]])
+
+
+def test_presort_orders_and_valid_bits():
+    say("presort parity 3000 samples")
+    patches, labels = synth.training_set(1500, 1500, seed=5)
+    pool = _subset_pool()
+    tr = AdaBoostTrainer((24, 24), len(labels), 1024)
+    d_p, d_f = DeviceArray.from_host(patches), DeviceArray.from_host(pool)
+    tr.stage_begin(d_p, labels, 1500, d_f, len(pool))
+    got = tr.read_orders(0, len(pool))
+    sumT, _, inv = O.train_integrals(patches)
+    want = O.presort(pool, sumT, inv)
+    assert np.array_equal(got, want)
+    w, strong, act, ginv = tr.read_state()
+    assert np.array_equal(ginv.view(np.uint32), inv.view(np.uint32))
+
+
+def test_rounds_bit_exact_and_f64_errors():
+    say("30 rounds parity 2000 samples")
+    patches, labels = synth.training_set(1000, 1000, seed=9)
+    pool = _subset_pool()
+    tr = AdaBoostTrainer((24, 24), len(labels), 1024)
+    d_p, d_f = DeviceArray.from_host(patches), DeviceArray.from_host(pool)
+    tr.stage_begin(d_p, labels, 1000, d_f, len(pool))
+    recs, w_or, strong_or, act = O.train_stage(patches, labels, 1000, pool, 30)
+    sumT, _, inv = O.train_integrals(patches)
+    w_prev, _, _, _ = tr.read_state()
+    for i, rec in enumerate(recs):
+        r = tr.round(0.99, 0.40)
+        assert r["feature"] == rec["feature"], i
+        assert (r["split"], r["parity"]) == (rec["split"], rec["parity"]), i
+        assert r["err"] == rec["err"], i
+        assert np.float32(r["theta"]).view(np.uint32) == rec["theta_bits"], i
+        assert np.float32(r["stageThreshold"]).view(np.uint32) == rec["threshold_bits"], i
+        assert (r["posPass"], r["negSurvive"]) == (rec["posPass"], rec["negSurvive"]), i
+        # error of the chosen stump as a plain f64 sum of the f64 weights
+        resp = O.feature_responses(pool[r["feature"]], sumT, inv)
+        lt = resp < np.float32(r["theta"])
+        pred = np.where(lt, -1, 1) if r["parity"] > 0 else np.where(lt, 1, -1)
+        e64 = math.fsum(w_prev[pred != labels])
+        assert abs(r["err"] - e64) <= 1e-6 * max(e64, 1e-30), (i, r["err"], e64)
+        w_prev, _, _, _ = tr.read_state()
+    w, strong, a, _ = tr.read_state()
+    assert np.array_equal(w.view(np.uint64), w_or.view(np.uint64))
+    assert np.array_equal(strong.view(np.uint32), strong_or.view(np.uint32))
+    thr = np.uint32(recs[-1]["threshold_bits"]).view(np.float32)
+    remain = tr.stage_end(float(thr))
+    assert remain == int((strong[1000:] >= thr).sum())
+
+
+def test_per_op_interface_matches_oracle():
+    say("per-op trainer interface")
+    patches, labels = synth.training_set(700, 900, seed=21)
+    n = len(labels)
+    pool = _subset_pool()[:900]
+    eng = FaceVisionEngine((24, 24), n, 24, 24)
+    d_p = DeviceArray.from_host(patches)
+    eng.computeIntegralBatchTransposed(d_p, 24, 24, 24, n, True)
+    view = eng.integralSetView()
+    rng = np.random.default_rng(4)
+    weights = rng.random(n).astype(np.float32)
+    weights /= weights.sum()
+    active = (rng.random(n) > 0.1).astype(np.uint8)
+    d_lab = DeviceArray.from_host(labels)
+    d_act = DeviceArray.from_host(active)
+    d_w = DeviceArray.from_host(weights)
+    tr = AdaBoostTrainer((24, 24), n, 1024)
+    tr.bindTrainingSet(view, d_lab, d_act, d_w)
+    d_f = DeviceArray.from_host(pool)
+    d_resp = DeviceArray((len(pool), n), np.float32)
+    tr.evaluateFeatureResponses(d_f, 0, len(pool), d_resp)
+    resp = d_resp.to_host()
+    sumT, _, inv = O.train_integrals(patches)
+    want_resp = np.stack([O.feature_responses(f, sumT, inv) for f in pool])
+    assert np.array_equal(resp.view(np.uint32), want_resp.view(np.uint32))
+    d_sorted = DeviceArray((len(pool), n), np.uint16)
+    tr.sortSamplesPerFeature(d_resp, len(pool), d_sorted)
+    srt = d_sorted.to_host()
+    ords = O.presort(pool, sumT, inv)
+    assert np.array_equal(srt, ords & 0x7FFF)
+    d_best = DeviceArray(len(pool), FEATURE_BEST_SPLIT)
+    tr.evaluateAndFindThreshold(d_resp, d_sorted, len(pool), d_best)
+    best = d_best.to_host()
+    sw = np.where(active != 0, np.where(labels > 0, weights, -weights), 0).astype(np.float32)
+    ob = O.search(ords, sw)
+    assert np.array_equal(best["bestSplitPos"], ob["split"])
+    assert np.array_equal(best["bestParity"].astype(np.int32), np.where(ob["split"] >= 0, ob["parity"], 1))
+    valid = ob["split"] >= 0
+    assert np.array_equal(best["bestErr"][valid], np.ldexp(ob["errQ"][valid].astype(np.float64), -62).astype(np.float32))
+    used = np.zeros(len(pool), np.uint8)
+    used[::3] = 1
+    d_used = DeviceArray.from_host(used)
+    d_tile = DeviceArray(1, TILE_BEST)
+    tr.selectBestInTile(d_best, 0, len(pool), d_used, d_tile)
+    tile = d_tile.to_host()[0]
+    cand = [(best["bestErr"][i], i) for i in range(len(pool)) if not used[i]]
+    e, fi = min(cand)
+    assert tile["featureIdx"] == fi and tile["bestErr"] == e
+
+
+def test_config2_stage_matches_golden():
+    path = os.path.join(HERE, "golden", "stage_c2.json")
+    if not os.path.exists(path):
+        pytest.skip("golden stage fixture not generated")
+    gold = json.load(open(path))
+    say("config-2 stage: 20000 samples x 162336 features, 50 rounds")
+    patches, labels = workloads.training()
+    pool = FaceVisionEngine.buildHaarFeaturePool((24, 24), FaceVisionEngine.Basic)
+    tr = AdaBoostTrainer((24, 24), len(labels), 1024)
+    d_p, d_f = DeviceArray.from_host(patches), DeviceArray.from_host(pool)
+    tr.stage_begin(d_p, labels, 10000, d_f, len(pool))
+    for i, rec in enumerate(gold["rounds"]):
+        r = tr.round(0.99, 0.40)
+        assert (r["feature"], r["split"], r["parity"]) == (rec["feature"], rec["split"], rec["parity"]), i
+        assert r["err"] == rec["err"], i
+        assert np.float32(r["theta"]).view(np.uint32) == rec["theta_bits"], i
+        assert np.float32(r["stageThreshold"]).view(np.uint32) == rec["threshold_bits"], i

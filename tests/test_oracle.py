@@ -1,0 +1,304 @@
+"""Pins the oracle (CPU, no GPU) against what the mathematics fixes:
+closed forms, brute force on tiny inputs, invariants and hand-built cases,
+chosen so that a dropped term, a wrong index or sign, or a swapped operand in
+the C restatement fails at least one check."""
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from cs677s26_amd import synth, workloads
+from cs677s26_amd.vjtypes import DETECTION, STAGE, det_tuples
+from oracle import pyoracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+# ---- Haar pool ---------------------------------------------------------------
+
+def test_pool_counts_and_first_features():
+    basic, core = O.haar_pool(24, 0), O.haar_pool(24, 1)
+    assert len(basic) == 162336
+    assert len(core) == 210400  # "Haar features in model file: 210,400" (presentation_notes.md:14)
+    # first loop iteration x=0,y=0,dx=1,dy=1: x2, y2, x3, y3, x2y2 (then Core: x4, y4, centre)
+    iiw = 25
+    x2 = basic[0]["r"]
+    assert (x2[0]["p0"], x2[0]["p1"], x2[0]["p2"], x2[0]["p3"], x2[0]["w"]) == (0, 2, iiw, iiw + 2, -1.0)
+    assert (x2[1]["p0"], x2[1]["p3"], x2[1]["w"]) == (1, iiw + 2, 2.0)
+    assert basic[4]["rectCount"] == 3 and basic[3]["rectCount"] == 2
+    centre = core[7]
+    assert centre["r"][1]["w"] == 9.0 and centre["r"][0]["p3"] == 3 + 3 * iiw
+    # every corner of every feature lies in the 25x25 window integral
+    for pool in (basic, core):
+        for r in range(3):
+            for c in ("p0", "p1", "p2", "p3"):
+                v = pool["r"][:, r][c]
+                assert v.min() >= 0 and v.max() < iiw * iiw
+
+
+def test_pool_type_counts():
+    # per-type counts: sum over x, y, dx, dy of the placement condition
+    def count(mx, my):
+        return sum((24 - mx * dx + 1) * (24 - my * dy + 1) for dx in range(1, 25) for dy in range(1, 25)
+                   if mx * dx <= 24 and my * dy <= 24)
+    basic = count(2, 1) + count(1, 2) + count(3, 1) + count(1, 3) + count(2, 2)
+    core = basic + count(4, 1) + count(1, 4) + count(3, 3)
+    assert (basic, core) == (162336, 210400)
+
+
+# ---- integrals ---------------------------------------------------------------
+
+@pytest.mark.parametrize("w,h", [(1, 1), (24, 24), (37, 5), (640, 480)])
+def test_integral_matches_cumsum(w, h):
+    rng = np.random.default_rng(w + 1000 * h)
+    img = rng.integers(0, 256, size=(h, w), dtype=np.uint8)
+    s, q = O.integral(img)
+    es = np.zeros((h + 1, w + 1), np.int64)
+    es[1:, 1:] = img.astype(np.int64).cumsum(0).cumsum(1)
+    eq = np.zeros((h + 1, w + 1), np.int64)
+    eq[1:, 1:] = (img.astype(np.int64) ** 2).cumsum(0).cumsum(1)
+    assert np.array_equal(s.astype(np.int64), es) and np.array_equal(q.astype(np.int64), eq)
+    # recurrence: ii(x,y) - ii(x-1,y) - ii(x,y-1) + ii(x-1,y-1) = pix
+    d = s[1:, 1:].astype(np.int64) - s[:-1, 1:] - s[1:, :-1] + s[:-1, :-1]
+    assert np.array_equal(d, img)
+
+
+def test_integral_all_255():
+    img = np.full((300, 200), 255, np.uint8)
+    s, q = O.integral(img)
+    yy, xx = np.mgrid[0:301, 0:201]
+    assert np.array_equal(s.astype(np.int64), 255 * yy * xx)
+    assert np.array_equal(q.astype(np.int64), 65025 * yy * xx)
+
+
+def test_train_invnorm_closed_form():
+    yy, xx = np.mgrid[0:24, 0:24]
+    checker = np.where((xx + yy) % 2 == 0, 255, 0).astype(np.uint8)
+    flat = np.full((24, 24), 77, np.uint8)
+    ramp = (xx * 10 + yy).astype(np.uint8)
+    patches = np.stack([checker, flat, ramp])
+    _, _, inv = O.train_integrals(patches)
+    for p, got in zip(patches, inv):
+        inset = p[1:23, 1:23].astype(np.int64)  # rect (1,1,22,22)
+        s, q = int(inset.sum()), int((inset ** 2).sum())
+        nf2 = Fraction(484 * q - s * s)
+        want = np.float32(1.0 / math.sqrt(float(nf2))) if nf2 > Fraction(1, 10 ** 12) else np.float32(0)
+        assert np.float32(got) == want
+    assert inv[1] == 0.0  # constant patch
+
+
+# ---- resize and schedule -----------------------------------------------------
+
+def test_resize_identity_and_ramp():
+    rng = np.random.default_rng(1)
+    img = rng.integers(0, 256, size=(31, 45), dtype=np.uint8)
+    assert np.array_equal(O.resize(img, 45, 31), img)
+    ramp = np.tile(np.arange(0, 200, dtype=np.uint8), (8, 1))
+    half = O.resize(ramp, 100, 4)
+    # fx = 2x + 0.5 lies halfway between 2x and 2x+1: (2x + 2x+1)/2 + 0.5 -> 2x+1
+    assert np.array_equal(half[0], np.arange(100) * 2 + 1)
+
+
+def test_levels_and_window_count_640():
+    lv = O.levels(640, 480)
+    assert len(lv) == 14
+    scale = np.float32(1.0)
+    for i, l in enumerate(lv):
+        assert l["scaleIdx"] == i and l["w"] == int(np.round(np.float32(640) / scale))
+        scale = np.float32(scale * np.float32(1.25))
+    windows = sum(math.ceil((l["w"] - 23) / l["step"]) * math.ceil((l["h"] - 23) / l["step"]) for l in lv)
+    assert windows == 653485
+    # scales 1, 1.25, 1.5625, 1.95 use step 1; 2.44 step 2; 3.05 and up step 3
+    assert [int(l["step"]) for l in lv] == [1] * 4 + [2] + [3] * 9
+
+
+# ---- detection ---------------------------------------------------------------
+
+def _gate_count(img, min_obj=24):
+    """Windows passing the variance gate, counted with exact integers and f64."""
+    n = 0
+    prev = img
+    for l in O.levels(img.shape[1], img.shape[0], 1.25, min_obj):
+        lvl = img if l["scaleIdx"] == 0 else O.resize(prev, int(l["w"]), int(l["h"]))
+        prev = lvl
+        a = lvl.astype(np.int64)
+        s = np.zeros((a.shape[0] + 1, a.shape[1] + 1), np.int64)
+        q = np.zeros_like(s)
+        s[1:, 1:] = a.cumsum(0).cumsum(1)
+        q[1:, 1:] = (a * a).cumsum(0).cumsum(1)
+        st = int(l["step"])
+        ys = np.arange(0, a.shape[0] - 23, st)
+        xs = np.arange(0, a.shape[1] - 23, st)
+        Y, X = np.meshgrid(ys, xs, indexing="ij")
+        ss = s[Y + 1, X + 1] - s[Y + 1, X + 23] - s[Y + 23, X + 1] + s[Y + 23, X + 23]
+        qq = q[Y + 1, X + 1] - q[Y + 1, X + 23] - q[Y + 23, X + 1] + q[Y + 23, X + 23]
+        nf2 = 484.0 * qq.astype(np.float64) - ss.astype(np.float64) ** 2
+        ok = nf2 > 0
+        inv = np.zeros_like(nf2, dtype=np.float32)
+        inv[ok] = (1.0 / np.sqrt(nf2[ok])).astype(np.float32)
+        n += int((ok & (484.0 * inv.astype(np.float64) < 0.1)).sum())
+    return n
+
+
+def test_detect_threshold_extremes_and_flat_image():
+    img = synth.detect_image(200, 150, 6, 24, 50, seed=3)
+    feats, stumps, stages = workloads.cascade((1, 10, 25))
+    open_ = np.array(stages, STAGE)
+    open_["threshold"] = -np.inf
+    raw, _ = O.detect_raw(feats, stumps, open_, img, max_det=10 ** 6)
+    assert len(raw) == _gate_count(img) > 0
+    shut = np.array(stages, STAGE)
+    shut["threshold"] = np.inf
+    assert len(O.detect_raw(feats, stumps, shut, img)[0]) == 0
+    flat = np.full((100, 120), 9, np.uint8)
+    assert len(O.detect_raw(feats, stumps, open_, flat, max_det=10 ** 6)[0]) == 0
+
+
+def test_detect_order_boxes_and_cap():
+    img, casc = workloads.detect_640()
+    raw, vis = O.detect_raw(*casc, img)
+    assert vis == 653485
+    lv = {int(l["scaleIdx"]): l for l in O.levels(640, 480)}
+    # (level, y, x) scan order; boxes are the windows scaled with +0.5 rounding
+    key = [(int(d["scaleIdx"]), int(d["y"]), int(d["x"])) for d in raw]
+    assert key == sorted(key)
+    for d in raw[::97]:
+        s = np.float32(lv[int(d["scaleIdx"])]["scale"])
+        assert d["w"] == int(np.float32(24) * s + np.float32(0.5))
+    capped, _ = O.detect_raw(*casc, img, max_det=137)
+    assert det_tuples(capped) == det_tuples(raw[:137])
+
+
+def test_golden_detect_fixtures():
+    g = np.load(os.path.join(HERE, "golden", "detect_c1.npz"))
+    img, casc = workloads.detect_640()
+    raw, vis = O.detect_raw(*casc, img)
+    assert det_tuples(raw) == det_tuples(g["raw"]) and vis == int(g["windows"])
+    assert det_tuples(O.group(raw, 4)) == det_tuples(g["grouped"])
+
+
+# ---- grouping ----------------------------------------------------------------
+
+def _dets(rows):
+    d = np.zeros(len(rows), DETECTION)
+    for i, r in enumerate(rows):
+        d[i] = r
+    return d
+
+
+def test_group_hand_built_clusters():
+    five = [(100, 100, 30, 30, 1, 1.0 + i) for i in range(5)]
+    three = [(300, 40, 24, 24, 0, 2.0)] * 3
+    out = O.group(_dets(five + three), 4)
+    # 5 > max(1,4) survives as the integer mean with the first max score; 3 does not
+    assert det_tuples(out) == det_tuples(_dets([(100, 100, 30, 30, 1, 5.0)]))
+    jitter = [(100 + k, 100 - k, 30, 30, 2, 1.0) for k in range(6)]
+    out = O.group(_dets(jitter), 4)
+    assert len(out) == 1 and out[0]["x"] == (sum(100 + k for k in range(6)) // 6)
+    # a small cluster inside a stronger big one is dropped (n2 > max(3, n1))
+    big = [(50, 50, 100, 100, 3, 1.0)] * 9
+    small = [(60, 60, 30, 30, 1, 1.0)] * 5
+    out = O.group(_dets(big + small), 1)
+    assert det_tuples(out) == det_tuples(_dets([(50, 50, 100, 100, 3, 1.0)]))
+    assert len(O.group(_dets([]), 4)) == 0
+
+
+# ---- training ----------------------------------------------------------------
+
+def _keys(v):
+    b = v.view(np.uint32)
+    return np.where(b & 0x80000000, ~b, b | 0x80000000).astype(np.uint32)
+
+
+def test_presort_matches_lexsort():
+    patches, labels = synth.training_set(60, 70, seed=2)
+    pool = O.haar_pool(24, 1)[::1111]
+    sumT, _, inv = O.train_integrals(patches)
+    ords = O.presort(pool, sumT, inv)
+    wide = O.presort(pool, sumT, inv, wide=True)
+    resp = np.stack([O.feature_responses(f, sumT, inv) for f in pool])
+    for fi in range(len(pool)):
+        v = resp[fi]
+        order = np.lexsort((np.arange(len(v)), _keys(v)))
+        assert np.array_equal(ords[fi] & 0x7FFF, order)
+        assert np.array_equal(wide[fi] & 0x7FFFFFFF, order)
+        vs = v[order]
+        valid = np.zeros(len(v), bool)
+        valid[:-1] = (vs[:-1] + np.float32(1e-7)) < vs[1:]
+        assert np.array_equal((ords[fi] >> 15).astype(bool), valid)
+
+
+def test_fixed_weight_and_exact_sum():
+    rng = np.random.default_rng(0)
+    for w in list(rng.random(200).astype(np.float32)) + [np.float32(1.0), np.float32(2.0 ** -39),
+                                                         np.float32(2.0 ** -45), np.float32(1e-45)]:
+        assert O.fixed_weight(float(w)) == int(Fraction(float(w)) * 2 ** 62)
+        assert O.fixed_weight(-float(w)) == -int(Fraction(float(w)) * 2 ** 62)
+    x = rng.random(1000) * 10.0 ** rng.integers(-12, 3, size=1000)
+    exact = sum(Fraction(float(v)) for v in x)
+    assert O.exact_sum(x) == float(exact)
+
+
+def test_search_brute_force_small():
+    """Exhaustive threshold search with exact rational sums (N = 14)."""
+    rng = np.random.default_rng(7)
+    patches, labels = synth.training_set(7, 7, seed=13)
+    pool = O.haar_pool(24, 0)[::4001]
+    sumT, _, inv = O.train_integrals(patches)
+    ords = O.presort(pool, sumT, inv)
+    active = (rng.random(14) > 0.2).astype(np.uint8)
+    for trial in range(3):
+        w = rng.random(14)
+        w = (w / w.sum()).astype(np.float32)  # normalised, as every round leaves them
+        if trial == 2:
+            w[:] = np.float32(1 / 14)  # uniform weights: exact ties everywhere
+        sw = np.where(active != 0, np.where(labels > 0, w, -w), 0).astype(np.float32)
+        best = O.search(ords, sw)
+        q = [int(Fraction(float(abs(x))) * 2 ** 62) for x in sw]
+        for fi, f in enumerate(pool):
+            v = O.feature_responses(f, sumT, inv)
+            cands = []
+            for k in range(13):
+                left = [int(s) for s in (ords[fi] & 0x7FFF)[:k + 1]]
+                vk, vk1 = v[ords[fi][k] & 0x7FFF], v[ords[fi][k + 1] & 0x7FFF]
+                if not (vk + np.float32(1e-7) < vk1):
+                    continue
+                wpl = sum(q[s] for s in left if sw[s] > 0)
+                wnl = sum(q[s] for s in left if sw[s] < 0)
+                tp = sum(q[s] for s in range(14) if sw[s] > 0)
+                tn = sum(q[s] for s in range(14) if sw[s] < 0)
+                cands.append((wnl + (tp - wpl), k, 1))   # parity -1 (sorted first by err, split)
+                cands.append((wpl + (tn - wnl), k, 0))   # parity +1 wins ties at equal split
+            if not cands:
+                assert best[fi]["split"] == -1
+                continue
+            e, k, p = min(cands)
+            assert (int(best[fi]["errQ"]), int(best[fi]["split"]), int(best[fi]["parity"])) == \
+                (e, k, -1 if p == 1 else 1)
+        fi, eq = O.select(best)
+        assert eq == min(int(b["errQ"]) for b in best if b["split"] >= 0)
+        assert fi == min(i for i, b in enumerate(best) if b["split"] >= 0 and b["errQ"] == eq)
+
+
+def test_train_stage_weights_and_errors_f64():
+    """Errors of each round equal plain f64 sums of the f64 weights within 1e-9."""
+    patches, labels = synth.training_set(150, 150, seed=17)
+    pool = O.haar_pool(24, 0)[::211]
+    recs, w, strong, act = O.train_stage(patches, labels, 150, pool, 8)
+    assert len(recs) == 8 and abs(w.sum() - 1.0) < 1e-12
+    sumT, _, inv = O.train_integrals(patches)
+    # replay the weights in plain f64 and check each round's error
+    n = len(labels)
+    w64 = np.where(np.arange(n) < 150, 0.5 / 150, 0.5 / 150)
+    w64 /= w64.sum()
+    for r in recs:
+        resp = O.feature_responses(pool[r["feature"]], sumT, inv)
+        lt = resp < np.float32(r["theta"])
+        pred = np.where(lt, -1, 1) if r["parity"] > 0 else np.where(lt, 1, -1)
+        e = math.fsum(w64[pred != labels])
+        assert abs(e - r["err"]) <= 1e-6 * e
+        w64 = np.where(pred != labels, w64 * math.exp(r["alpha"]), w64)
+        w64 /= math.fsum(w64)
+    assert np.allclose(w64, w, rtol=1e-12, atol=0)
