@@ -123,8 +123,8 @@ __global__ __launch_bounds__(64) void ii_colsum_kernel(const LevelDesc* __restri
             rs += p[k];
             rq += p[k] * p[k];
         }
-        rs = wave_sum(rs);
-        rq = wave_sum(rq);
+        rs = wave_sum_dpp(rs);
+        rq = wave_sum_dpp(rq);
         if (lane == 0) {
             rowS[lv.rowOff + static_cast<int64_t>(y) * lv.nChunks + c] = rs;
             rowQ[lv.rowOff + static_cast<int64_t>(y) * lv.nChunks + c] = rq;
@@ -269,10 +269,10 @@ __global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restric
         load4(lv.pix + static_cast<size_t>(y) * lv.stride, x, lv.w, p);
         const uint32_t ls = p[0] + p[1] + p[2] + p[3];
         const uint32_t lq = p[0] * p[0] + p[1] * p[1] + p[2] * p[2] + p[3] * p[3];
-        const uint32_t rowCarryS = __shfl(rcS, y - y0, kWave);
-        const uint32_t rowCarryQ = __shfl(rcQ, y - y0, kWave);
-        uint32_t rs = wave_incl_scan(ls) - ls + rowCarryS;
-        uint32_t rq = wave_incl_scan(lq) - lq + rowCarryQ;
+        const uint32_t rowCarryS = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rcS), y - y0));
+        const uint32_t rowCarryQ = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rcQ), y - y0));
+        uint32_t rs = wave_incl_scan_dpp(ls) - ls + rowCarryS;
+        uint32_t rq = wave_incl_scan_dpp(lq) - lq + rowCarryQ;
         uint32_t* rowOutS = outS + static_cast<int64_t>(y + 1) * iw;
         uint64_t* rowOutQ = outQ + static_cast<int64_t>(y + 1) * iw;
 #pragma unroll
@@ -307,6 +307,16 @@ __global__ __launch_bounds__(256) void invnorm_kernel(const uint32_t* __restrict
     if (s < n) inv[s] = train_invnorm(sumT, sqT, iw, ih, n, s);
 }
 
+// Weighted rectangle sum of one stump's feature at a window origin in LDS,
+// in the CPU path's float order: ((p0 - p1) - p2) + p3 per rectangle, times
+// its weight, summed r0 + r1 (+ r2) (src/main.cu:522-542).
+__device__ __forceinline__ float haar_raw(const float* o, const KStump& sp) {
+    float raw = (((o[sp.off[0]] - o[sp.off[1]]) - o[sp.off[2]]) + o[sp.off[3]]) * sp.w[0] +
+                (((o[sp.off[4]] - o[sp.off[5]]) - o[sp.off[6]]) + o[sp.off[7]]) * sp.w[1];
+    if (sp.rectCount == 3) raw += (((o[sp.off[8]] - o[sp.off[9]]) - o[sp.off[10]]) + o[sp.off[11]]) * sp.w[2];
+    return raw;
+}
+
 // Cascade evaluation, one workgroup per tile of 64 x tileRows windows of one
 // level.  The tile's integral footprint is staged in LDS as f32 (each
 // absolute u32 value converted once: the CPU path converts the same absolute
@@ -327,8 +337,10 @@ __global__ __launch_bounds__(256) void cascade_kernel(const LevelDesc* __restric
                                                       int winW, int winH,
                                                       uint64_t* __restrict__ words, float* __restrict__ scores) {
     extern __shared__ float lds[];
-    __shared__ uint32_t listIdx[2][1024];
+    __shared__ uint16_t listIdx[2][1024];
     __shared__ float listInv[2][1024];
+    __shared__ uint32_t votes[kVoteChunk][256 / 32];
+    __shared__ float leafL[kVoteChunk], leafR[kVoteChunk];
     __shared__ int listCnt[2];
     __shared__ unsigned long long maskWords[16];
 
@@ -397,11 +409,38 @@ __global__ __launch_bounds__(256) void cascade_kernel(const LevelDesc* __restric
         wbase = __shfl(wbase, 0, kWave);
         if (live) {
             const int p = wbase + mask_rank(m);
-            listIdx[0][p] = static_cast<uint32_t>(widx);
+            listIdx[0][p] = static_cast<uint16_t>(widx);
             listInv[0][p] = invNorm;
         }
     }
     __syncthreads();
+
+    // A window that passes the last stage sets its mask bit and stores its
+    // last stage sum; one that passes an earlier stage joins the next list.
+    auto settle = [&](bool pass, bool last, uint32_t widx, float invNorm, float stageSum, int nxt) {
+        if (last) {
+            if (pass) {
+                const int wx = wx0 + static_cast<int>(widx & (kTileWin - 1));
+                const int wy = wy0 + static_cast<int>(widx >> 6);
+                const int64_t word = lv.wordOff + static_cast<int64_t>(wy) * lv.rowWords + tx;
+                scores[word * 64 + (wx & 63)] = stageSum;
+                atomicOr(&maskWords[widx >> 6], 1ull << (widx & 63));
+            }
+        } else {
+            const uint64_t m = __ballot(pass);
+            int wbase = 0;
+            if (lane == 0 && m) wbase = atomicAdd(&listCnt[nxt], __popcll(m));
+            wbase = __shfl(wbase, 0, kWave);
+            if (pass) {
+                const int p = wbase + mask_rank(m);
+                listIdx[nxt][p] = static_cast<uint16_t>(widx);
+                listInv[nxt][p] = invNorm;
+            }
+        }
+    };
+    auto origin = [&](uint32_t widx) {
+        return lds + static_cast<int>(widx >> 6) * step * FW + static_cast<int>(widx & (kTileWin - 1)) * step;
+    };
 
     int cur = 0;
     for (int si = 0; si < nStages; ++si) {
@@ -409,47 +448,98 @@ __global__ __launch_bounds__(256) void cascade_kernel(const LevelDesc* __restric
         if (n == 0) break;
         const KStage st = stages[si];
         const bool last = (si == nStages - 1);
-        for (int base = 0; base < n; base += blockDim.x) {
-            const int i = base + threadIdx.x;
-            bool pass = false;
+        if (n >= static_cast<int>(blockDim.x)) {
+            // window mode: one live window per lane, stumps in order (the
+            // stump is uniform across the wave, so its fields are scalar
+            // loads); two stumps per step so their loads overlap, the two
+            // leaf values still added in stump order.
+            for (int base = 0; base < n; base += blockDim.x) {
+                const int i = base + threadIdx.x;
+                bool pass = false;
+                float stageSum = 0.0f;
+                uint32_t widx = 0;
+                float invNorm = 0.0f;
+                if (i < n) {
+                    widx = listIdx[cur][i];
+                    invNorm = listInv[cur][i];
+                    const float* o = origin(widx);
+                    int k = 0;
+                    for (; k + 1 < st.ntrees; k += 2) {
+                        const KStump s0 = stumps[st.first + k];
+                        const KStump s1 = stumps[st.first + k + 1];
+                        const float r0 = haar_raw(o, s0), r1 = haar_raw(o, s1);
+                        stageSum += (r0 * invNorm < s0.theta) ? s0.left : s0.right;
+                        stageSum += (r1 * invNorm < s1.theta) ? s1.left : s1.right;
+                    }
+                    if (k < st.ntrees) {
+                        const KStump s0 = stumps[st.first + k];
+                        stageSum += (haar_raw(o, s0) * invNorm < s0.theta) ? s0.left : s0.right;
+                    }
+                    pass = !(stageSum < st.threshold);
+                }
+                settle(pass, last, widx, invNorm, stageSum, cur ^ 1);
+            }
+        } else {
+            // pair mode (fewer live windows than lanes): the workgroup walks
+            // (stump, window) pairs of a chunk of up to kVoteChunk stumps,
+            // windows fastest, so a wave shares one stump (scalar loads)
+            // whenever 64 or more windows are live; the leaf each stump picks
+            // is collected by ballot into per-stump window bits; then each
+            // window's lane adds the leaf values in stump order, so the float
+            // sum rounds exactly as the sequential stage loop does.
+            const int tid = threadIdx.x;
+            const bool mine = tid < n;
+            const uint32_t myIdx = mine ? listIdx[cur][tid] : 0u;
+            const float myInv = mine ? listInv[cur][tid] : 0.0f;
+            int lgw = 0;
+            while ((1 << lgw) < n) ++lgw;
+            const int nw = 1 << lgw;
             float stageSum = 0.0f;
-            uint32_t widx = 0;
-            float invNorm = 0.0f;
-            if (i < n) {
-                widx = listIdx[cur][i];
-                invNorm = listInv[cur][i];
-                const int lx = static_cast<int>(widx & (kTileWin - 1)) * step;
-                const int ly = static_cast<int>(widx >> 6) * step;
-                const float* o = lds + ly * FW + lx;
-                for (int k = 0; k < st.ntrees; ++k) {
-                    const KStump sp = stumps[st.first + k];
-                    float raw = (((o[sp.off[0]] - o[sp.off[1]]) - o[sp.off[2]]) + o[sp.off[3]]) * sp.w[0] +
-                                (((o[sp.off[4]] - o[sp.off[5]]) - o[sp.off[6]]) + o[sp.off[7]]) * sp.w[1];
-                    if (sp.rectCount == 3)
-                        raw += (((o[sp.off[8]] - o[sp.off[9]]) - o[sp.off[10]]) + o[sp.off[11]]) * sp.w[2];
-                    stageSum += (raw * invNorm < sp.theta) ? sp.left : sp.right;
+            for (int cb = 0; cb < st.ntrees; cb += kVoteChunk) {
+                const int tc = min(kVoteChunk, st.ntrees - cb);
+                if (tid < tc) {
+                    leafL[tid] = stumps[st.first + cb + tid].left;
+                    leafR[tid] = stumps[st.first + cb + tid].right;
                 }
-                pass = !(stageSum < st.threshold);
+                const int total = tc << lgw;
+                for (int base = 0; base < total; base += blockDim.x) {
+                    const int p = base + tid;
+                    const int w = p & (nw - 1);
+                    const int j = p >> lgw;
+                    // with 64+ windows a wave covers one stump: take it from
+                    // the first lane so its fields are scalar loads
+                    const int ju = __builtin_amdgcn_readfirstlane(j);
+                    bool bit = false;
+                    if (p < total && w < n) {
+                        const float* o = origin(listIdx[cur][w]);
+                        if (nw >= kWave) {
+                            const KStump sp = stumps[st.first + cb + ju];
+                            bit = haar_raw(o, sp) * listInv[cur][w] < sp.theta;
+                        } else {
+                            const KStump sp = stumps[st.first + cb + j];
+                            bit = haar_raw(o, sp) * listInv[cur][w] < sp.theta;
+                        }
+                    }
+                    const uint64_t m = __ballot(bit);
+                    if (p < total && (w & 31) == 0) {
+                        uint32_t bits = static_cast<uint32_t>(m >> lane);
+                        if (nw < 32) bits &= (1u << nw) - 1u;
+                        votes[j][w >> 5] = bits;
+                    }
+                }
+                __syncthreads();
+                if (mine) {
+                    const int word = tid >> 5, sh = tid & 31;
+#pragma unroll 8
+                    for (int k = 0; k < tc; ++k) {
+                        const bool left = (votes[k][word] >> sh) & 1u;
+                        stageSum += left ? leafL[k] : leafR[k];
+                    }
+                }
+                __syncthreads();
             }
-            if (last) {
-                if (pass) {
-                    const int wx = wx0 + static_cast<int>(widx & (kTileWin - 1));
-                    const int wy = wy0 + static_cast<int>(widx >> 6);
-                    const int64_t word = lv.wordOff + static_cast<int64_t>(wy) * lv.rowWords + tx;
-                    scores[word * 64 + (wx & 63)] = stageSum;
-                    atomicOr(&maskWords[widx >> 6], 1ull << (widx & 63));
-                }
-            } else {
-                const uint64_t m = __ballot(pass);
-                int wbase = 0;
-                if (lane == 0 && m) wbase = atomicAdd(&listCnt[cur ^ 1], __popcll(m));
-                wbase = __shfl(wbase, 0, kWave);
-                if (pass) {
-                    const int p = wbase + mask_rank(m);
-                    listIdx[cur ^ 1][p] = widx;
-                    listInv[cur ^ 1][p] = invNorm;
-                }
-            }
+            const bool pass = mine && !(stageSum < st.threshold);
+            if (tid < kWave * ((n + kWave - 1) / kWave)) settle(pass, last, myIdx, myInv, stageSum, cur ^ 1);
         }
         __syncthreads();
         if (threadIdx.x == 0) listCnt[cur] = 0;
@@ -464,57 +554,95 @@ __global__ __launch_bounds__(256) void cascade_kernel(const LevelDesc* __restric
     }
 }
 
-// Per-image count of surviving windows (popcount over the image's words).
-__global__ __launch_bounds__(1024) void count_kernel(const int64_t* __restrict__ imgWordOff,
-                                                     const uint64_t* __restrict__ words,
-                                                     int32_t* __restrict__ counts) {
-    __shared__ int32_t part[16];
-    const int img = blockIdx.x;
-    const int64_t a = imgWordOff[img], b = imgWordOff[img + 1];
+// Output compaction in three steps over blocks of kWordBlock mask words
+// (blocks never straddle images): per-block survivor counts, one scan that
+// turns them into per-image output offsets, and a block-parallel emission.
+constexpr int kWordBlock = 1024;
+
+__global__ __launch_bounds__(256) void block_count_kernel(const int64_t* __restrict__ blockWord,
+                                                          const uint64_t* __restrict__ words,
+                                                          int32_t* __restrict__ blockCount) {
+    __shared__ int32_t part[4];
+    const int b = blockIdx.x;
+    const int64_t a = blockWord[b], e = blockWord[b + 1];
     int32_t c = 0;
-    for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) c += __popcll(words[i]);
+    for (int64_t i = a + threadIdx.x; i < e; i += blockDim.x) c += __popcll(words[i]);
     c = wave_sum(c);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int32_t s = 0;
-        for (int k = 0; k < static_cast<int>(blockDim.x >> 6); ++k) s += part[k];
-        counts[img] = s;
+    if (threadIdx.x == 0) blockCount[b] = part[0] + part[1] + part[2] + part[3];
+}
+
+// One workgroup: image totals (raw counts), each image's output base
+// sum_{j<i} min(count_j, cap), and each block's offset within its image.
+__global__ __launch_bounds__(1024) void block_scan_kernel(const int32_t* __restrict__ imgBlock, int nImages,
+                                                          const int32_t* __restrict__ blockCount,
+                                                          int32_t* __restrict__ blockOff,
+                                                          int32_t* __restrict__ counts,
+                                                          int32_t* __restrict__ imgBase, int cap) {
+    __shared__ int32_t waveTot[16];
+    __shared__ int32_t carry;
+    int32_t base = 0;
+    for (int img = 0; img < nImages; ++img) {
+        const int a = imgBlock[img], e = imgBlock[img + 1];
+        if (threadIdx.x == 0) carry = 0;
+        __syncthreads();
+        for (int c0 = a; c0 < e; c0 += blockDim.x) {
+            const int i = c0 + threadIdx.x;
+            const int32_t v = i < e ? blockCount[i] : 0;
+            const int32_t inc = wave_incl_scan(v);
+            if ((threadIdx.x & 63) == 63) waveTot[threadIdx.x >> 6] = inc;
+            __syncthreads();
+            int32_t off = carry + inc - v;
+            int32_t tot = 0;
+            for (int k = 0; k < static_cast<int>(blockDim.x >> 6); ++k) {
+                if (k < static_cast<int>(threadIdx.x >> 6)) off += waveTot[k];
+                tot += waveTot[k];
+            }
+            if (i < e) blockOff[i] = off;
+            __syncthreads();
+            if (threadIdx.x == 0) carry += tot;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            counts[img] = carry;
+            imgBase[img] = base;
+        }
+        base += min(carry, cap);
+        __syncthreads();
     }
 }
 
-// Order-preserving emission: image img's detections go to
-// out[sum_{j<img} min(count_j, cap) + rank] for rank < cap, in word order,
-// i.e. (level, y, x) scan order.  Box rounding: src/main.cu:806-813.
-__global__ __launch_bounds__(1024) void emit_kernel(const LevelDesc* __restrict__ L,
-                                                    const int64_t* __restrict__ lvWordOff, int nLevels,
-                                                    const int64_t* __restrict__ imgWordOff,
-                                                    const uint64_t* __restrict__ words,
-                                                    const float* __restrict__ scores,
-                                                    const int32_t* __restrict__ counts, int cap,
-                                                    int winW, int winH, vj_detection* __restrict__ out) {
-    __shared__ int32_t wavePart[16];
-    __shared__ int32_t outBase;
-    const int img = blockIdx.x;
-    if (threadIdx.x == 0) {
-        int32_t s = 0;
-        for (int k = 0; k < img; ++k) s += min(counts[k], cap);
-        outBase = s;
-    }
-    const int64_t a = imgWordOff[img], b = imgWordOff[img + 1];
-    const int64_t nWords = b - a;
-    const int64_t per = (nWords + blockDim.x - 1) / blockDim.x;
-    const int64_t my0 = a + min(nWords, per * threadIdx.x);
-    const int64_t my1 = a + min(nWords, per * (threadIdx.x + 1));
+// Emission: detections of block b go to out[imgBase + blockOff[b] + rank]
+// while the rank within the image stays below cap; word order is (level, y,
+// x) scan order.  Box rounding: src/main.cu:806-813.
+__global__ __launch_bounds__(256) void emit_kernel(const LevelDesc* __restrict__ L,
+                                                   const int64_t* __restrict__ lvWordOff, int nLevels,
+                                                   const int64_t* __restrict__ blockWord,
+                                                   const int32_t* __restrict__ blockImg,
+                                                   const uint64_t* __restrict__ words,
+                                                   const float* __restrict__ scores,
+                                                   const int32_t* __restrict__ blockOff,
+                                                   const int32_t* __restrict__ imgBase, int cap, int winW,
+                                                   int winH, vj_detection* __restrict__ out) {
+    __shared__ int32_t wavePart[4];
+    const int b = blockIdx.x;
+    const int64_t a = blockWord[b], e = blockWord[b + 1];
+    const int32_t off0 = blockOff[b];
+    if (off0 >= cap) return;
+    const int64_t nw = e - a;
+    const int64_t per = (nw + blockDim.x - 1) / blockDim.x;
+    const int64_t my0 = a + min(nw, per * threadIdx.x);
+    const int64_t my1 = a + min(nw, per * (threadIdx.x + 1));
     int32_t c = 0;
     for (int64_t i = my0; i < my1; ++i) c += __popcll(words[i]);
     const int32_t inc = wave_incl_scan(c);
     if ((threadIdx.x & 63) == 63) wavePart[threadIdx.x >> 6] = inc;
     __syncthreads();
-    int32_t rank = inc - c;
+    int32_t rank = off0 + inc - c;
     for (int k = 0; k < static_cast<int>(threadIdx.x >> 6); ++k) rank += wavePart[k];
-    const int32_t obase = outBase;
-    if (rank >= cap) return;
+    if (c == 0 || rank >= cap) return;
+    vj_detection* dst = out + imgBase[blockImg[b]];
     int li = -1;
     LevelDesc lv{};
     for (int64_t i = my0; i < my1 && rank < cap; ++i) {
@@ -538,7 +666,7 @@ __global__ __launch_bounds__(1024) void emit_kernel(const LevelDesc* __restrict_
             d.h = static_cast<int>(static_cast<float>(winH) * lv.scale + 0.5f);
             d.scaleIdx = lv.scaleIdx;
             d.score = scores[i * 64 + bit];
-            out[obase + rank] = d;
+            dst[rank] = d;
             ++rank;
         }
     }
@@ -602,7 +730,7 @@ struct vj_engine {
     DevBuf<int32_t> dMeta32;
     DevBuf<int64_t> dMeta64;
     DevBuf<ResizeJob> dJobs;
-    DevBuf<int32_t> dCounts;
+    DevBuf<int32_t> dCounts, dBlockCount, dBlockOff, dImgBase;
     DevBuf<vj_detection> dOut;
     PinnedBuf<vj_detection> hOut;
     PinnedBuf<int32_t> hCounts;
@@ -879,8 +1007,28 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     std::vector<int64_t> meta64;
     const size_t oLvWord = meta64.size();
     meta64.insert(meta64.end(), P.lvWordOff.begin(), P.lvWordOff.end());
-    const size_t oImgWord = meta64.size();
-    meta64.insert(meta64.end(), P.imgWordOff.begin(), P.imgWordOff.end());
+    // compaction blocks: each image's words cut into runs of kWordBlock
+    std::vector<int64_t> blockWord;
+    std::vector<int32_t> blockImg, imgBlock;
+    for (int im = 0; im < nImages; ++im) {
+        imgBlock.push_back(static_cast<int32_t>(blockImg.size()));
+        for (int64_t a = P.imgWordOff[static_cast<size_t>(im)]; a < P.imgWordOff[static_cast<size_t>(im) + 1];
+             a += kWordBlock) {
+            blockWord.push_back(a);
+            blockImg.push_back(im);
+        }
+    }
+    imgBlock.push_back(static_cast<int32_t>(blockImg.size()));
+    blockWord.push_back(P.imgWordOff.back());
+    const size_t oBlockWord = meta64.size();
+    meta64.insert(meta64.end(), blockWord.begin(), blockWord.end());
+    const size_t oBlockImg = meta32.size();
+    meta32.insert(meta32.end(), blockImg.begin(), blockImg.end());
+    const size_t oImgBlock = meta32.size();
+    meta32.insert(meta32.end(), imgBlock.begin(), imgBlock.end());
+    VJ_TRY(e->dBlockCount.ensure(blockImg.size()));
+    VJ_TRY(e->dBlockOff.ensure(blockImg.size()));
+    VJ_TRY(e->dImgBase.ensure(static_cast<size_t>(nImages)));
     std::vector<ResizeJob> jobs;
     std::vector<size_t> jobStart;
     for (const auto& jv : P.resizeJobs) {
@@ -950,13 +1098,21 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     }
     if (e->timing) VJ_HIP(hipEventRecord(e->ev[3], s));
     // 4. counts and order-preserving emission
-    hipLaunchKernelGGL(count_kernel, dim3(nImages), dim3(1024), 0, s, m64 + oImgWord, e->words.get(),
-                       e->dCounts.get());
+    const int nBlocks = static_cast<int>(blockImg.size());
+    if (nBlocks > 0) {
+        hipLaunchKernelGGL(block_count_kernel, dim3(nBlocks), dim3(256), 0, s, m64 + oBlockWord, e->words.get(),
+                           e->dBlockCount.get());
+        VJ_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(block_scan_kernel, dim3(1), dim3(1024), 0, s, m32 + oImgBlock, nImages, e->dBlockCount.get(),
+                       e->dBlockOff.get(), e->dCounts.get(), e->dImgBase.get(), cap);
     VJ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(emit_kernel, dim3(nImages), dim3(1024), 0, s, dL, m64 + oLvWord, nLevels,
-                       m64 + oImgWord, e->words.get(), e->scores.get(), e->dCounts.get(), cap, e->winW,
-                       e->winH, e->dOut.get());
-    VJ_HIP(hipGetLastError());
+    if (nBlocks > 0) {
+        hipLaunchKernelGGL(emit_kernel, dim3(nBlocks), dim3(256), 0, s, dL, m64 + oLvWord, nLevels, m64 + oBlockWord,
+                           m32 + oBlockImg, e->words.get(), e->scores.get(), e->dBlockOff.get(), e->dImgBase.get(),
+                           cap, e->winW, e->winH, e->dOut.get());
+        VJ_HIP(hipGetLastError());
+    }
     if (e->timing) VJ_HIP(hipEventRecord(e->ev[4], s));
     return VJ_OK;
 }

@@ -18,6 +18,9 @@ constexpr int kChunkCols = 256;
 // Cascade tiling: 64 windows per tile row (one 64-bit mask word), and
 // kTileRows[step] window rows per tile.
 constexpr int kTileWin = 64;
+// Stumps evaluated per vote chunk when a tile has fewer live windows than lanes.
+constexpr int kVoteChunk = 128;
+
 __host__ __device__ constexpr int tile_rows_for_step(int step) {
     return step == 1 ? 16 : (step == 2 ? 8 : 6);
 }
@@ -80,6 +83,24 @@ __device__ __forceinline__ T wave_incl_scan(T v) {
         if (lane >= d) v += u;
     }
     return v;
+}
+
+// Inclusive wave64 prefix sum of a u32 with DPP moves: shifts of 1, 2, 4, 8
+// within each 16-lane row, then row_bcast15 / row_bcast31 carry the row
+// totals into the rows above (GFX9-family DPP, valid on gfx950).
+__device__ __forceinline__ uint32_t wave_incl_scan_dpp(uint32_t v) {
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x111, 0xf, 0xf, true));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x112, 0xf, 0xf, true));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x114, 0xf, 0xf, true));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x118, 0xf, 0xf, true));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x142, 0xa, 0xf, false));
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x143, 0xc, 0xf, false));
+    return v;
+}
+
+// Sum over the wave of a u32, returned in every lane.
+__device__ __forceinline__ uint32_t wave_sum_dpp(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(wave_incl_scan_dpp(v)), 63));
 }
 
 template <typename T>
