@@ -317,32 +317,36 @@ __device__ __forceinline__ float haar_raw(const float* o, const KStump& sp) {
     return raw;
 }
 
-// Cascade evaluation, one workgroup per tile of 64 x tileRows windows of one
-// level.  The tile's integral footprint is staged in LDS as f32 (each
-// absolute u32 value converted once: the CPU path converts the same absolute
-// values, so every Haar sum rounds identically).  Windows failing the
-// variance gate never enter the live list; each stage then runs over the
-// live list only and the survivors are compacted by wavefront ballot, so
-// lanes stay busy however unevenly windows reject.  Survivors of the last
-// stage set their bit in the level's window mask and store their last stage
-// sum.  Stage rule and window normaliser: src/main.cu:752-804.
-__global__ __launch_bounds__(256) void cascade_kernel(const LevelDesc* __restrict__ L,
-                                                      const int32_t* __restrict__ lvlIdx,
-                                                      const int32_t* __restrict__ tileOffs, int nLv,
-                                                      int nTiles,
-                                                      const uint32_t* __restrict__ iiS,
-                                                      const uint64_t* __restrict__ iiQ,
-                                                      const KStump* __restrict__ stumpsAll, int nStumps,
-                                                      const KStage* __restrict__ stages, int nStages,
-                                                      int winW, int winH,
-                                                      uint64_t* __restrict__ words, float* __restrict__ scores) {
+// Cascade evaluation, one workgroup of kCascadeBlock threads per tile of
+// 2^lgc x TH windows of one level.  The tile's integral footprint is staged
+// in LDS as f32 (each absolute u32 value converted once: the CPU path
+// converts the same absolute values, so every Haar sum rounds identically).
+// Windows failing the variance gate never enter the live list; each stage
+// then runs over the live list only and the survivors are compacted by
+// wavefront ballot, so lanes stay busy however unevenly windows reject.
+// Survivors of the last stage set their bit in the level's window mask and
+// store their last stage sum.  Stage rule and window normaliser:
+// src/main.cu:752-804.
+constexpr int kCascadeBlock = 256;
+
+__global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(const LevelDesc* __restrict__ L,
+                                                                const int32_t* __restrict__ lvlIdx,
+                                                                const int32_t* __restrict__ tileOffs, int nLv,
+                                                                int nTiles,
+                                                                const uint32_t* __restrict__ iiS,
+                                                                const uint64_t* __restrict__ iiQ,
+                                                                const KStump* __restrict__ stumpsAll, int nStumps,
+                                                                const KStage* __restrict__ stages, int nStages,
+                                                                int winW, int winH,
+                                                                uint64_t* __restrict__ words,
+                                                                float* __restrict__ scores) {
     extern __shared__ float lds[];
-    __shared__ uint16_t listIdx[2][1024];
-    __shared__ float listInv[2][1024];
-    __shared__ uint32_t votes[kVoteChunk][256 / 32];
+    __shared__ uint16_t listIdx[2][kTileMaxWin];
+    __shared__ float invTab[kTileMaxWin];
+    __shared__ uint32_t votes[kVoteChunk][kCascadeBlock / 32];
     __shared__ float leafL[kVoteChunk], leafR[kVoteChunk];
     __shared__ int listCnt[2];
-    __shared__ unsigned long long maskWords[16];
+    __shared__ unsigned long long maskWords[8];
 
     // XCD-aware remap: consecutive tiles land on the same XCD (shared L2).
     int t;
@@ -357,89 +361,135 @@ __global__ __launch_bounds__(256) void cascade_kernel(const LevelDesc* __restric
     const int ty = local / lv.tilesX;
     const int tx = local - ty * lv.tilesX;
     const int TH = lv.tileRows;
+    const int lgc = lv.tileLgCols;
+    const int TW = 1 << lgc;
     const int step = lv.step;
+    const int wx0 = tx * TW, wy0 = ty * TH;
+
+    // Tile row r is the TW-bit piece of mask word (wy0 + r, wx0 / 64) that
+    // starts at bit wx0 % 64 (little-endian: a 32- or 16-bit store).  Every
+    // piece of every word is written by exactly one tile, so the mask needs
+    // no clearing.
+    auto write_mask = [&](int r, uint64_t bits) {
+        const int64_t word = lv.wordOff + static_cast<int64_t>(wy0 + r) * lv.rowWords + (wx0 >> 6);
+        if (lgc == 6) {
+            words[word] = bits;
+        } else if (lgc == 5) {
+            reinterpret_cast<uint32_t*>(words)[word * 2 + ((wx0 & 63) >> 5)] = static_cast<uint32_t>(bits);
+        } else {
+            reinterpret_cast<uint16_t*>(words)[word * 4 + ((wx0 & 63) >> 4)] = static_cast<uint16_t>(bits);
+        }
+    };
+    if (wx0 >= lv.scanW) {  // padding tile past the scan width: zero pieces
+        for (int r = threadIdx.x; r < TH; r += kCascadeBlock)
+            if (wy0 + r < lv.scanH) write_mask(r, 0ull);
+        return;
+    }
+
     const KStump* __restrict__ stumps = stumpsAll + static_cast<size_t>(step - 1) * nStumps;
-    const int wx0 = tx * kTileWin, wy0 = ty * TH;
     const int ix0 = wx0 * step, iy0 = wy0 * step;
-    const int FW = (kTileWin - 1) * step + winW + 1;
+    const int FW = (TW - 1) * step + winW + 1;
     const int FH = (TH - 1) * step + winH + 1;
     const int iw = lv.w + 1;
     const uint32_t* sumL = iiS + lv.iiOff;
     const uint64_t* sqL = iiQ + lv.iiOff;
 
-    for (int e = threadIdx.x; e < FW * FH; e += blockDim.x) {
-        const int r = e / FW, cc = e - r * FW;
-        const int gy = iy0 + r, gx = ix0 + cc;
-        float v = 0.0f;
-        if (gy <= lv.h && gx <= lv.w) v = static_cast<float>(static_cast<int32_t>(sumL[static_cast<int64_t>(gy) * iw + gx]));
-        lds[e] = v;
+    // footprint staging, four loads in flight per thread before their stores
+    {
+        const int total = FW * FH;
+        for (int e0 = threadIdx.x; e0 < total; e0 += 4 * kCascadeBlock) {
+            float v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = e0 + u * kCascadeBlock;
+                v[u] = 0.0f;
+                if (e < total) {
+                    const int r = e / FW, cc = e - r * FW;
+                    const int gy = iy0 + r, gx = ix0 + cc;
+                    if (gy <= lv.h && gx <= lv.w)
+                        v[u] = static_cast<float>(static_cast<int32_t>(sumL[static_cast<int64_t>(gy) * iw + gx]));
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = e0 + u * kCascadeBlock;
+                if (e < total) lds[e] = v[u];
+            }
+        }
     }
-    if (threadIdx.x < 16) maskWords[threadIdx.x] = 0ull;
+    if (threadIdx.x < 8) maskWords[threadIdx.x] = 0ull;
     if (threadIdx.x == 0) {
         listCnt[0] = 0;
         listCnt[1] = 0;
     }
     __syncthreads();
 
-    // variance gate (exact int sum, u32 square-sum delta, f64 normaliser)
+    // variance gate (exact int sum, u32 square-sum delta, f64 normaliser);
+    // a tile has at most two windows per thread, both gated before either
+    // is appended so their global loads overlap
     const int nw = winW - 2, nh = winH - 2, narea = nw * nh;
     const int64_t np0 = 1 + iw, np1 = (1 + nw) + iw;
     const int64_t np2 = 1 + static_cast<int64_t>(iw) * (1 + nh), np3 = (1 + nw) + static_cast<int64_t>(iw) * (1 + nh);
     const int lane = lane_id();
-    for (int base = 0; base < kTileWin * TH; base += blockDim.x) {
-        const int widx = base + threadIdx.x;
-        const int wx = wx0 + (widx & (kTileWin - 1));
-        const int wy = wy0 + (widx >> 6);
-        bool live = false;
-        float invNorm = 0.0f;
-        if (widx < kTileWin * TH && wx < lv.scanW && wy < lv.scanH) {
+    const int nWin = TW * TH;
+    static_assert(kTileMaxWin <= 2 * kCascadeBlock, "two gate slots per thread");
+    bool live[2];
+    float inv[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int widx = static_cast<int>(threadIdx.x) + k * kCascadeBlock;
+        const int wx = wx0 + (widx & (TW - 1));
+        const int wy = wy0 + (widx >> lgc);
+        live[k] = false;
+        inv[k] = 0.0f;
+        if (widx < nWin && wx < lv.scanW && wy < lv.scanH) {
             const int64_t o = static_cast<int64_t>(wy * step) * iw + wx * step;
             const int32_t ns = static_cast<int32_t>(sumL[o + np0] - sumL[o + np1] - sumL[o + np2] + sumL[o + np3]);
             const uint32_t nq = static_cast<uint32_t>(sqL[o + np0] - sqL[o + np1] - sqL[o + np2] + sqL[o + np3]);
             const double normSum = static_cast<double>(ns);
             const double nf2 = static_cast<double>(narea) * static_cast<double>(nq) - normSum * normSum;
             if (nf2 > 0.0) {
-                invNorm = static_cast<float>(1.0 / sqrt(nf2));
-                live = static_cast<double>(narea) * static_cast<double>(invNorm) < 1e-1;
+                inv[k] = static_cast<float>(1.0 / sqrt(nf2));
+                live[k] = static_cast<double>(narea) * static_cast<double>(inv[k]) < 1e-1;
             }
         }
-        const uint64_t m = __ballot(live);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        if (k * kCascadeBlock >= nWin) break;
+        const int widx = static_cast<int>(threadIdx.x) + k * kCascadeBlock;
+        const uint64_t m = __ballot(live[k]);
         int wbase = 0;
         if (lane == 0 && m) wbase = atomicAdd(&listCnt[0], __popcll(m));
         wbase = __shfl(wbase, 0, kWave);
-        if (live) {
-            const int p = wbase + mask_rank(m);
-            listIdx[0][p] = static_cast<uint16_t>(widx);
-            listInv[0][p] = invNorm;
+        if (live[k]) {
+            listIdx[0][wbase + mask_rank(m)] = static_cast<uint16_t>(widx);
+            invTab[widx] = inv[k];
         }
     }
     __syncthreads();
 
     // A window that passes the last stage sets its mask bit and stores its
     // last stage sum; one that passes an earlier stage joins the next list.
-    auto settle = [&](bool pass, bool last, uint32_t widx, float invNorm, float stageSum, int nxt) {
+    auto settle = [&](bool pass, bool last, uint32_t widx, float stageSum, int nxt) {
         if (last) {
             if (pass) {
-                const int wx = wx0 + static_cast<int>(widx & (kTileWin - 1));
-                const int wy = wy0 + static_cast<int>(widx >> 6);
-                const int64_t word = lv.wordOff + static_cast<int64_t>(wy) * lv.rowWords + tx;
+                const int wx = wx0 + static_cast<int>(widx & (TW - 1));
+                const int wy = wy0 + static_cast<int>(widx >> lgc);
+                const int64_t word = lv.wordOff + static_cast<int64_t>(wy) * lv.rowWords + (wx >> 6);
                 scores[word * 64 + (wx & 63)] = stageSum;
-                atomicOr(&maskWords[widx >> 6], 1ull << (widx & 63));
+                atomicOr(&maskWords[widx >> lgc], 1ull << (widx & (TW - 1)));
             }
         } else {
             const uint64_t m = __ballot(pass);
             int wbase = 0;
             if (lane == 0 && m) wbase = atomicAdd(&listCnt[nxt], __popcll(m));
             wbase = __shfl(wbase, 0, kWave);
-            if (pass) {
-                const int p = wbase + mask_rank(m);
-                listIdx[nxt][p] = static_cast<uint16_t>(widx);
-                listInv[nxt][p] = invNorm;
-            }
+            if (pass) listIdx[nxt][wbase + mask_rank(m)] = static_cast<uint16_t>(widx);
         }
     };
     auto origin = [&](uint32_t widx) {
-        return lds + static_cast<int>(widx >> 6) * step * FW + static_cast<int>(widx & (kTileWin - 1)) * step;
+        return lds + static_cast<int>(widx >> lgc) * step * FW + static_cast<int>(widx & (TW - 1)) * step;
     };
 
     int cur = 0;
@@ -448,20 +498,19 @@ __global__ __launch_bounds__(256) void cascade_kernel(const LevelDesc* __restric
         if (n == 0) break;
         const KStage st = stages[si];
         const bool last = (si == nStages - 1);
-        if (n >= static_cast<int>(blockDim.x)) {
+        if (n >= kCascadeBlock) {
             // window mode: one live window per lane, stumps in order (the
             // stump is uniform across the wave, so its fields are scalar
             // loads); two stumps per step so their loads overlap, the two
             // leaf values still added in stump order.
-            for (int base = 0; base < n; base += blockDim.x) {
-                const int i = base + threadIdx.x;
+            for (int base = 0; base < n; base += kCascadeBlock) {
+                const int i = base + static_cast<int>(threadIdx.x);
                 bool pass = false;
                 float stageSum = 0.0f;
                 uint32_t widx = 0;
-                float invNorm = 0.0f;
                 if (i < n) {
                     widx = listIdx[cur][i];
-                    invNorm = listInv[cur][i];
+                    const float invNorm = invTab[widx];
                     const float* o = origin(widx);
                     int k = 0;
                     for (; k + 1 < st.ntrees; k += 2) {
@@ -477,7 +526,7 @@ __global__ __launch_bounds__(256) void cascade_kernel(const LevelDesc* __restric
                     }
                     pass = !(stageSum < st.threshold);
                 }
-                settle(pass, last, widx, invNorm, stageSum, cur ^ 1);
+                settle(pass, last, widx, stageSum, cur ^ 1);
             }
         } else {
             // pair mode (fewer live windows than lanes): the workgroup walks
@@ -490,10 +539,9 @@ __global__ __launch_bounds__(256) void cascade_kernel(const LevelDesc* __restric
             const int tid = threadIdx.x;
             const bool mine = tid < n;
             const uint32_t myIdx = mine ? listIdx[cur][tid] : 0u;
-            const float myInv = mine ? listInv[cur][tid] : 0.0f;
             int lgw = 0;
             while ((1 << lgw) < n) ++lgw;
-            const int nw = 1 << lgw;
+            const int nwp = 1 << lgw;
             float stageSum = 0.0f;
             for (int cb = 0; cb < st.ntrees; cb += kVoteChunk) {
                 const int tc = min(kVoteChunk, st.ntrees - cb);
@@ -502,29 +550,30 @@ __global__ __launch_bounds__(256) void cascade_kernel(const LevelDesc* __restric
                     leafR[tid] = stumps[st.first + cb + tid].right;
                 }
                 const int total = tc << lgw;
-                for (int base = 0; base < total; base += blockDim.x) {
+                for (int base = 0; base < total; base += kCascadeBlock) {
                     const int p = base + tid;
-                    const int w = p & (nw - 1);
-                    const int j = p >> lgw;
+                    const int w = p & (nwp - 1);
+                    const int js = p >> lgw;
                     // with 64+ windows a wave covers one stump: take it from
                     // the first lane so its fields are scalar loads
-                    const int ju = __builtin_amdgcn_readfirstlane(j);
+                    const int ju = __builtin_amdgcn_readfirstlane(js);
                     bool bit = false;
                     if (p < total && w < n) {
-                        const float* o = origin(listIdx[cur][w]);
-                        if (nw >= kWave) {
+                        const uint32_t wi = listIdx[cur][w];
+                        const float* o = origin(wi);
+                        if (nwp >= kWave) {
                             const KStump sp = stumps[st.first + cb + ju];
-                            bit = haar_raw(o, sp) * listInv[cur][w] < sp.theta;
+                            bit = haar_raw(o, sp) * invTab[wi] < sp.theta;
                         } else {
-                            const KStump sp = stumps[st.first + cb + j];
-                            bit = haar_raw(o, sp) * listInv[cur][w] < sp.theta;
+                            const KStump sp = stumps[st.first + cb + js];
+                            bit = haar_raw(o, sp) * invTab[wi] < sp.theta;
                         }
                     }
                     const uint64_t m = __ballot(bit);
                     if (p < total && (w & 31) == 0) {
                         uint32_t bits = static_cast<uint32_t>(m >> lane);
-                        if (nw < 32) bits &= (1u << nw) - 1u;
-                        votes[j][w >> 5] = bits;
+                        if (nwp < 32) bits &= (1u << nwp) - 1u;
+                        votes[js][w >> 5] = bits;
                     }
                 }
                 __syncthreads();
@@ -539,7 +588,7 @@ __global__ __launch_bounds__(256) void cascade_kernel(const LevelDesc* __restric
                 __syncthreads();
             }
             const bool pass = mine && !(stageSum < st.threshold);
-            if (tid < kWave * ((n + kWave - 1) / kWave)) settle(pass, last, myIdx, myInv, stageSum, cur ^ 1);
+            if (tid < kWave * ((n + kWave - 1) / kWave)) settle(pass, last, myIdx, stageSum, cur ^ 1);
         }
         __syncthreads();
         if (threadIdx.x == 0) listCnt[cur] = 0;
@@ -547,11 +596,8 @@ __global__ __launch_bounds__(256) void cascade_kernel(const LevelDesc* __restric
         __syncthreads();
     }
 
-    // every tile writes all of its mask words, so the mask needs no clearing
-    for (int r = threadIdx.x; r < TH; r += blockDim.x) {
-        const int wy = wy0 + r;
-        if (wy < lv.scanH) words[lv.wordOff + static_cast<int64_t>(wy) * lv.rowWords + tx] = maskWords[r];
-    }
+    for (int r = threadIdx.x; r < TH; r += kCascadeBlock)
+        if (wy0 + r < lv.scanH) write_mask(r, maskWords[r]);
 }
 
 // Output compaction in three steps over blocks of kWordBlock mask words
@@ -756,6 +802,14 @@ namespace {
 
 inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
+// LDS row pitch of a cascade tile's integral footprint at a window step, and
+// the footprint's size in elements.
+inline int cascade_pitch(int step, int winW) { return ((1 << tile_lgcols_for_step(step)) - 1) * step + winW + 1; }
+inline size_t cascade_footprint(int step, int winW, int winH) {
+    return static_cast<size_t>(cascade_pitch(step, winW)) *
+           static_cast<size_t>((tile_rows_for_step(step) - 1) * step + winH + 1);
+}
+
 // Pyramid schedule of one image, src/face_vision_engine.cu:1596-1714 (and the
 // CPU loop src/main.cu:717-747): float scale, round(W/scale), skip levels
 // whose object size is under minObjectSize, stop below the window.
@@ -884,6 +938,7 @@ int build_plan(const vj_engine* e, const std::vector<ImageIn>& imgs, float sf, i
             d.scanW = d.scanH = d.rowWords = 0;
             d.tilesX = d.tilesY = 0;
             d.tileRows = 1;
+            d.tileLgCols = 6;
             P.itemOffs.push_back(items);
             P.chunkOffs.push_back(chunkItems);
             P.lvWordOff.push_back(wordOff);
@@ -895,7 +950,8 @@ int build_plan(const vj_engine* e, const std::vector<ImageIn>& imgs, float sf, i
         d.scanH = (d.h - e->winH + 1 + d.step - 1) / d.step;
         d.rowWords = (d.scanW + 63) / 64;
         d.tileRows = tile_rows_for_step(d.step);
-        d.tilesX = d.rowWords;
+        d.tileLgCols = tile_lgcols_for_step(d.step);
+        d.tilesX = d.rowWords << (6 - d.tileLgCols);  // whole mask words
         d.tilesY = (d.scanH + d.tileRows - 1) / d.tileRows;
         P.itemOffs.push_back(items);
         P.chunkOffs.push_back(chunkItems);
@@ -1078,19 +1134,10 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     //    largest window step present (its tile footprint is the widest)
     if (e->timing) VJ_HIP(hipEventRecord(e->ev[2], s));
     if (P.nTiles > 0) {
-        const int TH = tile_rows_for_step(P.maxStep);
-        const int FW = (kTileWin - 1) * P.maxStep + e->winW + 1;
-        const int FH = (TH - 1) * P.maxStep + e->winH + 1;
         size_t ldsBytes = 0;
-        for (int st = 1; st <= P.maxStep; ++st) {
-            const int th = tile_rows_for_step(st);
-            ldsBytes = std::max(ldsBytes, static_cast<size_t>((kTileWin - 1) * st + e->winW + 1) *
-                                              static_cast<size_t>((th - 1) * st + e->winH + 1) * sizeof(float));
-        }
-        (void)TH;
-        (void)FW;
-        (void)FH;
-        hipLaunchKernelGGL(cascade_kernel, dim3(P.nTiles), dim3(256), ldsBytes, s, dL, m32 + oLvl, m32 + oTile,
+        for (int st = 1; st <= P.maxStep; ++st)
+            ldsBytes = std::max(ldsBytes, cascade_footprint(st, e->winW, e->winH) * sizeof(float));
+        hipLaunchKernelGGL(cascade_kernel, dim3(P.nTiles), dim3(kCascadeBlock), ldsBytes, s, dL, m32 + oLvl, m32 + oTile,
                            static_cast<int>(P.tileLvl.size()), P.nTiles, e->iiS.get(), e->iiQ.get(), e->kst.get(),
                            e->nStumps, e->kstage.get(), e->nStages, e->winW, e->winH, e->words.get(),
                            e->scores.get());
@@ -1280,7 +1327,7 @@ extern "C" int vj_engine_set_cascade(vj_engine* e, const vj_feature* dFeatures, 
         const vj_feature& f = hf[static_cast<size_t>(st.feature)];
         VJ_CHECK(f.rectCount == 2 || f.rectCount == 3, "setCascadeModel: feature rectCount must be 2 or 3");
         for (int k = 0; k < 3; ++k) {
-            const int pitch = (kTileWin - 1) * (k + 1) + e->winW + 1;
+            const int pitch = cascade_pitch(k + 1, e->winW);
             KStump& o = ks[static_cast<size_t>(k) * nS + i];
             std::memset(&o, 0, sizeof o);
             for (int r = 0; r < 3; ++r) {

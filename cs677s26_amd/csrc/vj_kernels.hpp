@@ -15,14 +15,21 @@ constexpr int kWave = 64;
 constexpr int kBandRows = 32;
 constexpr int kChunkCols = 256;
 
-// Cascade tiling: 64 windows per tile row (one 64-bit mask word), and
-// kTileRows[step] window rows per tile.
-constexpr int kTileWin = 64;
+// Cascade tiling: a tile is 2^lgcols x rows windows of one level, shaped per
+// window step so that every step's integral footprint stays near 11 KB of LDS
+// (64x8 windows at step 1, 32x4 at step 2, 16x4 at step 3: 88x32, 87x31 and
+// 70x34 integral entries for a 24x24 window) and eight 256-thread workgroups
+// fit on a CU.  A tile row is a 64-, 32- or 16-bit piece of one 64-bit mask
+// word.
+constexpr int kTileMaxWin = 512;
 // Stumps evaluated per vote chunk when a tile has fewer live windows than lanes.
-constexpr int kVoteChunk = 128;
+constexpr int kVoteChunk = 120;
 
+__host__ __device__ constexpr int tile_lgcols_for_step(int step) {
+    return step == 1 ? 6 : (step == 2 ? 5 : 4);
+}
 __host__ __device__ constexpr int tile_rows_for_step(int step) {
-    return step == 1 ? 16 : (step == 2 ? 8 : 6);
+    return step == 1 ? 8 : 4;
 }
 
 // One pyramid level of one image, as every level-parallel kernel sees it.
@@ -40,7 +47,7 @@ struct LevelDesc {
     int32_t image;        // image index within the call
     int32_t nBands, nChunks;
     int32_t scanW, scanH, rowWords;
-    int32_t tileRows, tilesX, tilesY;
+    int32_t tileRows, tileLgCols, tilesX, tilesY;
     int32_t itemOff;      // first (band, chunk) item
     int32_t chunkItemOff; // first column-chunk item
     int32_t tileOff;      // first cascade tile
