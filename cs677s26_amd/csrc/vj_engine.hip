@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "vj_common.hpp"
@@ -341,10 +342,10 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(const LevelDesc*
                                                                 uint64_t* __restrict__ words,
                                                                 float* __restrict__ scores) {
     extern __shared__ float lds[];
-    __shared__ uint16_t listIdx[2][kTileMaxWin];
+    __shared__ uint16_t listIdx[kTileMaxWin];
     __shared__ float invTab[kTileMaxWin];
-    __shared__ uint32_t votes[kVoteChunk][kCascadeBlock / 32];
-    __shared__ float leafL[kVoteChunk], leafR[kVoteChunk];
+    __shared__ uint32_t votes[2][kVoteChunk][kCascadeBlock / 32];
+    __shared__ float leafL[2][kVoteChunk], leafR[2][kVoteChunk];
     __shared__ int listCnt[2];
     __shared__ unsigned long long maskWords[8];
 
@@ -463,7 +464,7 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(const LevelDesc*
         if (lane == 0 && m) wbase = atomicAdd(&listCnt[0], __popcll(m));
         wbase = __shfl(wbase, 0, kWave);
         if (live[k]) {
-            listIdx[0][wbase + mask_rank(m)] = static_cast<uint16_t>(widx);
+            listIdx[wbase + mask_rank(m)] = static_cast<uint16_t>(widx);
             invTab[widx] = inv[k];
         }
     }
@@ -485,115 +486,141 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(const LevelDesc*
             int wbase = 0;
             if (lane == 0 && m) wbase = atomicAdd(&listCnt[nxt], __popcll(m));
             wbase = __shfl(wbase, 0, kWave);
-            if (pass) listIdx[nxt][wbase + mask_rank(m)] = static_cast<uint16_t>(widx);
+            if (pass) listIdx[wbase + mask_rank(m)] = static_cast<uint16_t>(widx);
         }
     };
     auto origin = [&](uint32_t widx) {
         return lds + static_cast<int>(widx >> lgc) * step * FW + static_cast<int>(widx & (TW - 1)) * step;
     };
 
+    // The live list is rewritten in place: every stage first reads the
+    // entries it needs into registers, then (after a barrier) appends its
+    // survivors; the two counters alternate between "read" and "append".
     int cur = 0;
     for (int si = 0; si < nStages; ++si) {
         const int n = listCnt[cur];
         if (n == 0) break;
+        const int nxt = cur ^ 1;
         const KStage st = stages[si];
         const bool last = (si == nStages - 1);
+        const int tid = threadIdx.x;
+        const KStump* __restrict__ sg = stumps + st.first;
+        const int nt = st.ntrees;
         if (n >= kCascadeBlock) {
-            // window mode: one live window per lane, stumps in order (the
-            // stump is uniform across the wave, so its fields are scalar
-            // loads); two stumps per step so their loads overlap, the two
-            // leaf values still added in stump order.
-            for (int base = 0; base < n; base += kCascadeBlock) {
-                const int i = base + static_cast<int>(threadIdx.x);
+            // window mode: one live window per lane (two rounds above 256
+            // windows), stumps in order (the stump is uniform across the
+            // wave, so its fields are scalar loads); two stumps per step so
+            // their loads overlap, the two leaf values still added in order.
+            const uint32_t e0 = listIdx[tid];
+            const uint32_t e1 = (tid + kCascadeBlock < n) ? listIdx[tid + kCascadeBlock] : 0u;
+            if (tid == 0) listCnt[nxt] = 0;
+            __syncthreads();
+            for (int r = 0; r < 2; ++r) {
+                if (r * kCascadeBlock >= n) break;
+                const int i = r * kCascadeBlock + tid;
+                const uint32_t widx = r ? e1 : e0;
                 bool pass = false;
                 float stageSum = 0.0f;
-                uint32_t widx = 0;
                 if (i < n) {
-                    widx = listIdx[cur][i];
                     const float invNorm = invTab[widx];
                     const float* o = origin(widx);
                     int k = 0;
-                    for (; k + 1 < st.ntrees; k += 2) {
-                        const KStump s0 = stumps[st.first + k];
-                        const KStump s1 = stumps[st.first + k + 1];
+                    for (; k + 1 < nt; k += 2) {
+                        const KStump s0 = sg[k];
+                        const KStump s1 = sg[k + 1];
                         const float r0 = haar_raw(o, s0), r1 = haar_raw(o, s1);
                         stageSum += (r0 * invNorm < s0.theta) ? s0.left : s0.right;
                         stageSum += (r1 * invNorm < s1.theta) ? s1.left : s1.right;
                     }
-                    if (k < st.ntrees) {
-                        const KStump s0 = stumps[st.first + k];
+                    if (k < nt) {
+                        const KStump s0 = sg[k];
                         stageSum += (haar_raw(o, s0) * invNorm < s0.theta) ? s0.left : s0.right;
                     }
                     pass = !(stageSum < st.threshold);
                 }
-                settle(pass, last, widx, stageSum, cur ^ 1);
+                settle(pass, last, widx, stageSum, nxt);
             }
         } else {
-            // pair mode (fewer live windows than lanes): the workgroup walks
-            // (stump, window) pairs of a chunk of up to kVoteChunk stumps,
-            // windows fastest, so a wave shares one stump (scalar loads)
-            // whenever 64 or more windows are live; the leaf each stump picks
-            // is collected by ballot into per-stump window bits; then each
-            // window's lane adds the leaf values in stump order, so the float
-            // sum rounds exactly as the sequential stage loop does.
-            const int tid = threadIdx.x;
-            const bool mine = tid < n;
-            const uint32_t myIdx = mine ? listIdx[cur][tid] : 0u;
-            int lgw = 0;
+            // pair mode (fewer live windows than threads): the windows are
+            // padded to nwp = 2^lgw slots and each pass evaluates spp =
+            // 256 / nwp stumps on every slot, windows fastest, so a lane
+            // keeps one window for the whole stage and a wave shares one
+            // stump (scalar loads) whenever nwp >= 64.  The next pass's
+            // stump record is fetched while the current one is evaluated.
+            // Each pass's window bits go by ballot into per-stump vote words
+            // of a chunk of kVoteChunk stumps; the window's own lane then
+            // adds the chunk's leaf values in stump order (one chunk behind
+            // the passes, so that work overlaps the next chunk's passes), and
+            // the float sum rounds exactly as the sequential stage loop does.
+            int lgw = 2;
             while ((1 << lgw) < n) ++lgw;
             const int nwp = 1 << lgw;
+            const int spp = kCascadeBlock >> lgw;
+            const int w = tid & (nwp - 1);
+            const int js0 = tid >> lgw;
+            const bool evalLive = w < n;
+            const uint32_t ew = evalLive ? listIdx[w] : 0u;
+            const bool mine = tid < n;
+            const uint32_t myIdx = mine ? listIdx[tid] : 0u;
+            if (tid == 0) listCnt[nxt] = 0;
+            __syncthreads();
+            const float* o = origin(ew);
+            const float inv = invTab[ew];
             float stageSum = 0.0f;
-            for (int cb = 0; cb < st.ntrees; cb += kVoteChunk) {
-                const int tc = min(kVoteChunk, st.ntrees - cb);
-                if (tid < tc) {
-                    leafL[tid] = stumps[st.first + cb + tid].left;
-                    leafR[tid] = stumps[st.first + cb + tid].right;
-                }
-                const int total = tc << lgw;
-                for (int base = 0; base < total; base += kCascadeBlock) {
-                    const int p = base + tid;
-                    const int w = p & (nwp - 1);
-                    const int js = p >> lgw;
-                    // with 64+ windows a wave covers one stump: take it from
-                    // the first lane so its fields are scalar loads
-                    const int ju = __builtin_amdgcn_readfirstlane(js);
-                    bool bit = false;
-                    if (p < total && w < n) {
-                        const uint32_t wi = listIdx[cur][w];
-                        const float* o = origin(wi);
-                        if (nwp >= kWave) {
-                            const KStump sp = stumps[st.first + cb + ju];
-                            bit = haar_raw(o, sp) * invTab[wi] < sp.theta;
-                        } else {
-                            const KStump sp = stumps[st.first + cb + js];
-                            bit = haar_raw(o, sp) * invTab[wi] < sp.theta;
+            const int nChunks = (nt + kVoteChunk - 1) / kVoteChunk;
+            auto run = [&](auto uniformTag) {
+                constexpr bool kUniform = decltype(uniformTag)::value;
+                KStump nx{};
+                auto prefetch = [&](int js) {
+                    if constexpr (kUniform) {
+                        const int ju = __builtin_amdgcn_readfirstlane(js);
+                        if (ju < nt) nx = sg[ju];
+                    } else {
+                        if (js < nt) nx = sg[js];
+                    }
+                };
+                prefetch(js0);
+                for (int c = 0; c <= nChunks; ++c) {
+                    const int b = c & 1;
+                    if (c < nChunks) {
+                        const int cb = c * kVoteChunk;
+                        const int tc = min(kVoteChunk, nt - cb);
+                        if (tid < tc) {
+                            leafL[b][tid] = sg[cb + tid].left;
+                            leafR[b][tid] = sg[cb + tid].right;
+                        }
+                        for (int jb = 0; jb < tc; jb += spp) {
+                            const int js = jb + js0;
+                            const KStump sp = nx;
+                            prefetch(cb + js + spp);
+                            bool bit = false;
+                            if (evalLive && js < tc) bit = haar_raw(o, sp) * inv < sp.theta;
+                            const uint64_t m = __ballot(bit);
+                            if (js < tc && (w & 31) == 0) {
+                                uint32_t bits = static_cast<uint32_t>(m >> lane);
+                                if (nwp < 32) bits &= (1u << nwp) - 1u;
+                                votes[b][js][w >> 5] = bits;
+                            }
                         }
                     }
-                    const uint64_t m = __ballot(bit);
-                    if (p < total && (w & 31) == 0) {
-                        uint32_t bits = static_cast<uint32_t>(m >> lane);
-                        if (nwp < 32) bits &= (1u << nwp) - 1u;
-                        votes[js][w >> 5] = bits;
-                    }
-                }
-                __syncthreads();
-                if (mine) {
-                    const int word = tid >> 5, sh = tid & 31;
+                    if (c > 0 && mine) {
+                        const int pb = b ^ 1;
+                        const int ptc = min(kVoteChunk, nt - (c - 1) * kVoteChunk);
+                        const int word = tid >> 5, sh = tid & 31;
 #pragma unroll 8
-                    for (int k = 0; k < tc; ++k) {
-                        const bool left = (votes[k][word] >> sh) & 1u;
-                        stageSum += left ? leafL[k] : leafR[k];
+                        for (int k = 0; k < ptc; ++k)
+                            stageSum += ((votes[pb][k][word] >> sh) & 1u) ? leafL[pb][k] : leafR[pb][k];
                     }
+                    __syncthreads();
                 }
-                __syncthreads();
-            }
+            };
+            if (nwp >= kWave) run(std::true_type{});
+            else run(std::false_type{});
             const bool pass = mine && !(stageSum < st.threshold);
-            if (tid < kWave * ((n + kWave - 1) / kWave)) settle(pass, last, myIdx, stageSum, cur ^ 1);
+            if (tid < kWave * ((n + kWave - 1) / kWave)) settle(pass, last, myIdx, stageSum, nxt);
         }
         __syncthreads();
-        if (threadIdx.x == 0) listCnt[cur] = 0;
-        cur ^= 1;
-        __syncthreads();
+        cur = nxt;
     }
 
     for (int r = threadIdx.x; r < TH; r += kCascadeBlock)

@@ -23,7 +23,7 @@ constexpr int kChunkCols = 256;
 // word.
 constexpr int kTileMaxWin = 512;
 // Stumps evaluated per vote chunk when a tile has fewer live windows than lanes.
-constexpr int kVoteChunk = 120;
+constexpr int kVoteChunk = 64;
 
 __host__ __device__ constexpr int tile_lgcols_for_step(int step) {
     return step == 1 ? 6 : (step == 2 ? 5 : 4);
