@@ -140,24 +140,24 @@ def cpu_leg(args):
             "seconds": dt}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--rounds", type=int, default=50)
-    ap.add_argument("--skip", default="", help="comma list of legs to skip: batched,stump,cpu")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has a single MI355X; "
-              "run without --gpus or with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-    skip = set(x for x in args.skip.split(",") if x)
+def measured_traffic(name):
+    """HBM bytes per dispatch of a kernel on a workload: the committed rocprofv3 --pmc
+    summary's FETCH_SIZE + WRITE_SIZE, each scaled by the same counter's
+    reading for a plain copy kernel of known size (None if not measured)."""
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_dispatch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
 
-    log("detect 1080p / 38-stage")
+
+def detect_fields(args):
     d = detect_leg(args)
     out = {
-        "metric": METRIC,
         "value": d["mpix"] * args.steps / d["wall_s"],
         "unit": "Mpix/s",
         "n_gpus": 1,
@@ -178,7 +178,8 @@ def main():
     casc_bytes = d["stats"]["cascadeBytes"]
     achieved = casc_bytes / (casc_ms * 1e-3) / 1e9 if casc_ms > 0 else 0.0
     out["roofline"] = {"bound": "hbm", "kernel": "cascade_kernel", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                       "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                       "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                       "traffic": measured_traffic("cascade_kernel_detect1080"),
                        "algorithmic_bytes": casc_bytes, "avg_ms": casc_ms}
     int_ms = d["phase_ms"][1] + d["phase_ms"][0]
     int_bytes = d["stats"]["integralBytes"]
@@ -187,6 +188,27 @@ def main():
     out["integral_roofline"] = {"achieved": int_bytes / (int_ms * 1e-3) / 1e9 if int_ms > 0 else 0.0,
                                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "algorithmic_bytes": int_bytes,
                                 "avg_ms": int_ms}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rounds", type=int, default=50)
+    ap.add_argument("--skip", default="", help="comma list of legs to skip: detect,batched,stump,cpu")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X; "
+              "run without --gpus or with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    skip = set(x for x in args.skip.split(",") if x)
+
+    out = {"metric": METRIC, "value": None}
+    if "detect" not in skip:
+        log("detect 1080p / 38-stage")
+        out.update(detect_fields(args))
     if "batched" not in skip:
         log("batched 64 images")
         b = batched_leg(args)
