@@ -302,3 +302,27 @@ def test_train_stage_weights_and_errors_f64():
         w64 = np.where(pred != labels, w64 * math.exp(r["alpha"]), w64)
         w64 /= math.fsum(w64)
     assert np.allclose(w64, w, rtol=1e-12, atol=0)
+
+
+def test_train_stage_calibration_counts():
+    """The stage threshold of the last round, applied with the detector's rule
+    (a window passes iff !(strong < threshold)), passes exactly posPass
+    positives and negSurvive active negatives; posPass meets the hit-rate
+    floor ceil(minHitRate * numPos), and the threshold is the cut score (the
+    (numPos - minPass)-th smallest positive) or the next float above it, the
+    latter exactly when it still passes minPass positives."""
+    patches, labels = synth.training_set(120, 160, seed=23)
+    pool = O.haar_pool(24, 0)[::307]
+    for rounds in (1, 3, 6):
+        recs, w, strong, act = O.train_stage(patches, labels, 120, pool, rounds)
+        r = recs[-1]
+        thr = np.uint32(r["threshold_bits"]).view(np.float32)
+        pos, neg = strong[:120], strong[120:][act[120:] != 0]
+        assert int((~(pos < thr)).sum()) == r["posPass"]
+        assert int((~(neg < thr)).sum()) == r["negSurvive"]
+        min_pass = math.ceil(np.float32(0.99) * np.float32(120))
+        assert r["posPass"] >= min_pass
+        cut = np.sort(pos)[120 - min_pass]
+        up = np.nextafter(cut, np.float32(np.inf))
+        assert thr in (cut, up)
+        assert (thr == up) == (int((pos > cut).sum()) >= min_pass)
