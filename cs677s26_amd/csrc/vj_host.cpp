@@ -261,21 +261,17 @@ struct UnionFind {
         }
         return r;
     }
-    void join(int a, int b) {
-        a = root(a);
-        b = root(b);
-        if (a == b) return;
-        if (rank[a] < rank[b]) std::swap(a, b);
-        parent[b] = a;
-        if (rank[a] == rank[b]) ++rank[a];
+    // Joins the set whose root is ra with b's set (union by rank) and returns
+    // the root of the union.
+    int join_root(int ra, int b) {
+        int rb = root(b);
+        if (ra == rb) return ra;
+        if (rank[ra] < rank[rb]) std::swap(ra, rb);
+        parent[rb] = ra;
+        if (rank[ra] == rank[rb]) ++rank[ra];
+        return ra;
     }
 };
-
-inline bool similar_boxes(const vj_detection& a, const vj_detection& b, float eps) {
-    const int delta = static_cast<int>(eps * static_cast<float>(std::min(a.w, b.w) + std::min(a.h, b.h)) * 0.5f);
-    return std::abs(a.x - b.x) <= delta && std::abs(a.y - b.y) <= delta &&
-           std::abs((a.x + a.w) - (b.x + b.w)) <= delta && std::abs((a.y + a.h) - (b.y + b.h)) <= delta;
-}
 
 std::vector<int> order_by_x(const vj_detection* d, int n) {
     std::vector<int> o(static_cast<size_t>(n));
@@ -291,14 +287,46 @@ int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps,
     *count = 0;
     if (n <= 0) return VJ_OK;
     UnionFind uf(n);
-    const std::vector<int> order = order_by_x(in, n);
-    for (int oi = 0; oi < n; ++oi) {
-        const vj_detection& a = in[order[oi]];
-        const int reach = static_cast<int>(eps * static_cast<float>(a.w + a.h) * 0.5f);
-        for (int oj = oi + 1; oj < n; ++oj) {
-            const vj_detection& b = in[order[oj]];
-            if (b.x - a.x > reach) break;
-            if (similar_boxes(a, b, eps)) uf.join(order[oi], order[oj]);
+    // sweep order (x, index) as one sorted u64 key per box, and the boxes'
+    // edges in that order as flat arrays so the sweep reads memory in order
+    std::vector<uint64_t> key(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i)
+        key[static_cast<size_t>(i)] =
+            (static_cast<uint64_t>(static_cast<uint32_t>(in[i].x) ^ 0x80000000u) << 32) | static_cast<uint32_t>(i);
+    std::sort(key.begin(), key.end());
+    std::vector<int> idx(static_cast<size_t>(n)), X(static_cast<size_t>(n)), Y(static_cast<size_t>(n)),
+        W(static_cast<size_t>(n)), H(static_cast<size_t>(n));
+    for (int k = 0; k < n; ++k) {
+        const int i = static_cast<int>(key[static_cast<size_t>(k)] & 0xffffffffu);
+        idx[static_cast<size_t>(k)] = i;
+        X[static_cast<size_t>(k)] = in[i].x;
+        Y[static_cast<size_t>(k)] = in[i].y;
+        W[static_cast<size_t>(k)] = in[i].w;
+        H[static_cast<size_t>(k)] = in[i].h;
+    }
+    // similarRects' tolerance (int)(eps * (min w + min h) * 0.5f) as a table
+    // over m = min w + min h, so the sweep does no float work per pair
+    int maxWH = 0;
+    for (int k = 0; k < n; ++k) maxWH = std::max(maxWH, std::max(W[static_cast<size_t>(k)], H[static_cast<size_t>(k)]));
+    std::vector<int> dtab(static_cast<size_t>(2 * maxWH + 1));
+    for (int m = 0; m <= 2 * maxWH; ++m)
+        dtab[static_cast<size_t>(m)] = static_cast<int>(eps * static_cast<float>(m) * 0.5f);
+    for (int a = 0; a < n; ++a) {
+        const int ax = X[static_cast<size_t>(a)], ay = Y[static_cast<size_t>(a)];
+        const int aw = W[static_cast<size_t>(a)], ah = H[static_cast<size_t>(a)];
+        const int reach = static_cast<int>(eps * static_cast<float>(aw + ah) * 0.5f);
+        int ra = -1;  // root of a's set once a has been joined
+        for (int b = a + 1; b < n; ++b) {
+            const int bx = X[static_cast<size_t>(b)];
+            if (bx - ax > reach) break;
+            const int by = Y[static_cast<size_t>(b)], bw = W[static_cast<size_t>(b)], bh = H[static_cast<size_t>(b)];
+            const int mw = std::min(aw, bw), mh = std::min(ah, bh);
+            const int d = (mw >= 0 && mh >= 0) ? dtab[static_cast<size_t>(mw + mh)]
+                                               : static_cast<int>(eps * static_cast<float>(mw + mh) * 0.5f);
+            // every edge within d of the other box's edge (one branch)
+            const bool sim = (std::abs(ax - bx) <= d) & (std::abs(ay - by) <= d) &
+                             (std::abs((ax + aw) - (bx + bw)) <= d) & (std::abs((ay + ah) - (by + bh)) <= d);
+            if (sim) ra = uf.join_root(ra < 0 ? uf.root(idx[static_cast<size_t>(a)]) : ra, idx[static_cast<size_t>(b)]);
         }
     }
     struct Acc {
