@@ -93,8 +93,84 @@ def batched_leg(args):
         eng.detect_batch(dev, spec, 1.25, 4, 24, 200000, True)
     synchronize()
     wall = time.perf_counter() - t0
+    # the plain per-image path on the same 64 images: one detect call each
+    t1 = time.perf_counter()
+    for d, (w, h) in zip(dev, sizes):
+        eng.detectMultiScale(d, w, h, w, 1.25, 4, 24, 200000, True)
+    synchronize()
+    wall_plain = time.perf_counter() - t1
     mpix = sum(w * h for w, h in sizes) / 1e6
-    return {"mpix_per_s": mpix * reps / wall, "mpix_per_batch": mpix, "ms_per_batch": 1e3 * wall / reps}
+    return {"mpix_per_s": mpix * reps / wall, "mpix_per_batch": mpix, "ms_per_batch": 1e3 * wall / reps,
+            "plain_per_image_mpix_per_s": mpix / wall_plain,
+            "speedup_vs_plain": (mpix * reps / wall) / (mpix / wall_plain)}
+
+
+def served_leg(args):
+    """Poisson stream at a fixed QPS through the served form.  Latency is
+    scheduled arrival -> result ready: the submitter's lag behind the
+    schedule plus the server's submit -> done time (so the order in which
+    this harness's threads collect results does not count)."""
+    import queue
+    import threading
+
+    import numpy as np
+
+    from cs677s26_amd import workloads
+    from cs677s26_amd.server import DetectServer
+
+    arrivals, imgs = workloads.served(args.requests, args.qps)
+    feats, stumps, stages = workloads.cascade_38()
+    srv = DetectServer(feats, stumps, stages, max_image=(3840, 2160), slots=args.slots, max_batch=8,
+                       hbm_budget=16 << 30)
+    # warm-up: every slot sees a batch of the largest requests
+    big = sorted(range(len(imgs)), key=lambda i: -imgs[i].size)[:8 * args.slots]
+    for t in [srv.submit(imgs[i]) for i in big]:
+        srv.wait(t)
+    n = len(imgs)
+    done = np.zeros(n)
+    sub = np.zeros(n)
+    srv_lat = np.zeros(n)
+    q = queue.Queue()
+
+    def waiter():
+        while True:
+            item = q.get()
+            if item is None:
+                return
+            i, tk = item
+            _, lat = srv.wait(tk)
+            done[i] = time.perf_counter()
+            srv_lat[i] = lat
+
+    ws = [threading.Thread(target=waiter) for _ in range(8)]
+    for w in ws:
+        w.start()
+    t0 = time.perf_counter()
+    for i in range(n):
+        delay = t0 + arrivals[i] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        sub[i] = time.perf_counter()
+        q.put((i, srv.submit(imgs[i])))
+    for _ in ws:
+        q.put(None)
+    for w in ws:
+        w.join()
+    lat = 1e3 * (sub - (t0 + arrivals)) + srv_lat
+    mpix = sum(im.size for im in imgs) / 1e6
+    span = max(done.max(), (sub + srv_lat / 1e3).max()) - t0
+    # saturation: the same requests submitted at once
+    t1 = time.perf_counter()
+    for t in [srv.submit(im) for im in imgs]:
+        srv.wait(t)
+    sat = time.perf_counter() - t1
+    st = srv.stats()
+    srv.close()
+    return {"qps": args.qps, "requests": n, "mpix": mpix, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "server_p50_ms": float(np.percentile(srv_lat, 50)),
+            "server_p99_ms": float(np.percentile(srv_lat, 99)), "achieved_qps": n / span,
+            "mpix_per_s": mpix / span, "saturated_mpix_per_s": mpix / sat, "slots": args.slots,
+            "max_slots_busy": st["maxSlotsBusy"], "max_batch": st["maxBatchSeen"], "batches": st["batches"]}
 
 
 def stump_leg(args):
@@ -197,7 +273,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rounds", type=int, default=50)
-    ap.add_argument("--skip", default="", help="comma list of legs to skip: detect,batched,stump,cpu")
+    ap.add_argument("--requests", type=int, default=240, help="served leg: requests in the Poisson stream")
+    ap.add_argument("--qps", type=float, default=100.0, help="served leg: arrival rate")
+    ap.add_argument("--slots", type=int, default=3, help="served leg: device slots")
+    ap.add_argument("--skip", default="", help="comma list of legs to skip: detect,batched,served,stump,cpu")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X; "
@@ -220,6 +299,11 @@ def main():
         out["stump_search_s_per_stage"] = s["s_per_stage"]
         out["stump"] = s
     out["p99_latency_ms"] = None
+    if "served" not in skip:
+        log(f"served: Poisson {args.requests} requests at {args.qps} QPS, 240p..4K")
+        v = served_leg(args)
+        out["served"] = v
+        out["p99_latency_ms"] = v["p99_ms"]
     if "cpu" not in skip:
         log("cpu baseline (oracle, 1 thread)")
         out["cpu_baseline"] = cpu_leg(args)

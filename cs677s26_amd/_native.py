@@ -47,6 +47,19 @@ class Weak(C.Structure):
                 ("posPass", C.c_int32), ("negSurvive", C.c_int32), ("stageSatisfied", C.c_int32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("winW", C.c_int32), ("winH", C.c_int32), ("maxImageW", C.c_int32), ("maxImageH", C.c_int32),
+                ("slots", C.c_int32), ("maxBatch", C.c_int32), ("hbmBudget", C.c_int64),
+                ("scaleFactor", C.c_float), ("minNeighbors", C.c_int32), ("minObjectSize", C.c_int32),
+                ("maxDetections", C.c_int32), ("applyGrouping", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
+                ("batches", C.c_int64), ("peakBytesInFlight", C.c_int64), ("maxSlotsBusy", C.c_int32),
+                ("maxBatchSeen", C.c_int32)]
+
+
 class RoundConfig(C.Structure):
     _fields_ = [("minHitRate", C.c_float), ("maxFalseAlarm", C.c_float)]
 
@@ -94,6 +107,12 @@ _PROTOS = {
     "vj_engine_set_timing": (I, [P, I]),
     "vj_engine_phase_ms": (I, [P, C.POINTER(F)]),
     "vj_engine_detect_batch": (I, [P, I, P, P, P, P, F, I, I, I, I, P, I, P, P]),
+    "vj_engine_detect_footprint": (I, [P, I, I, F, I, I, C.POINTER(I64)]),
+    "vj_server_create": (I, [C.POINTER(ServerConfig), P, I, P, I, P, I, PP]),
+    "vj_server_submit": (I, [P, P, I, I, I, C.POINTER(I64)]),
+    "vj_server_wait": (I, [P, I64, P, I, C.POINTER(I), C.POINTER(D)]),
+    "vj_server_stats_get": (I, [P, C.POINTER(ServerStats)]),
+    "vj_server_destroy": (None, [P]),
     "vj_trainer_create": (I, [I, I, I, I, PP]),
     "vj_trainer_destroy": (None, [P]),
     "vj_trainer_bind": (I, [P, C.POINTER(IntegralView), P, P, P, I]),

@@ -1499,6 +1499,26 @@ extern "C" int vj_engine_detect_batch(vj_engine* e, int nImages, const uint8_t* 
     }
 }
 
+extern "C" int vj_engine_detect_footprint(vj_engine* e, int w, int h, float sf, int minObj, int maxDet,
+                                          int64_t* bytes) {
+    VJ_CHECK(e && bytes && w > 0 && h > 0 && sf > 1.0f && minObj >= 1 && maxDet >= 1,
+             "detect_footprint: bad arguments");
+    *bytes = 0;
+    try {
+        Plan P;
+        std::vector<ImageIn> imgs{{nullptr, w, h, w}};
+        VJ_TRY(build_plan(e, imgs, sf, minObj, P));
+        const int64_t cap = std::max(1, std::min(maxDet, e->maxDet));
+        *bytes = static_cast<int64_t>(w) * h + P.pyrBytes + 12 * P.iiElems + 8 * P.colElems + 12 * P.totElems +
+                 8 * P.rowElems + 8 * P.nWords + 4 * 64 * P.nWords + 24 * cap +
+                 static_cast<int64_t>(P.levels.size()) * static_cast<int64_t>(sizeof(LevelDesc) + 64);
+        return VJ_OK;
+    } catch (const std::exception& ex) {
+        set_error(ex.what());
+        return VJ_RUNTIME_ERROR;
+    }
+}
+
 extern "C" int vj_engine_set_timing(vj_engine* e, int on) {
     VJ_CHECK(e, "vj_engine_set_timing: null");
     if (on && !e->ev[0])

@@ -58,3 +58,19 @@ def training(n_pos=10000, n_neg=10000, seed=synth.DEFAULT_SEED):
 def batch64(seed=synth.DEFAULT_SEED):
     sizes = synth.batch_sizes(64, seed=seed)
     return sizes, synth.batch_images(sizes, seed=seed)
+
+
+def served(n=240, qps=100.0, seed=synth.DEFAULT_SEED):
+    """Served detect: Poisson arrivals at `qps`, sizes log-uniform 240p..4K
+    (16:9), each request a crop of one seeded 3840x2160 scene at a seeded
+    offset.  Returns (arrival seconds, list of u8 image views)."""
+    t, sizes = synth.request_stream(n, qps, seed=seed)
+    base = synth.detect_image(3840, 2160, 400, 24, 400, seed=seed + 1)
+    rng = np.random.default_rng(seed + 2)
+    imgs = []
+    for w, h in sizes:
+        w, h = min(w, 3840), min(h, 2160)
+        x0 = int(rng.integers(0, 3840 - w + 1))
+        y0 = int(rng.integers(0, 2160 - h + 1))
+        imgs.append(base[y0:y0 + h, x0:x0 + w])
+    return t, imgs

@@ -181,6 +181,56 @@ int vj_engine_detect_batch(vj_engine* e, int nImages, const uint8_t* const* dIma
                            int maxDetections, int applyGrouping,
                            vj_detection* hOut, int cap, int* hCounts, vj_stream s);
 
+/* Device bytes one detect of a w x h image occupies in an engine's work
+ * buffers (input, pyramid, integrals and their scratch, window mask and
+ * scores, output at min(maxDetections, the engine's cap)).  The served form
+ * charges this against its HBM budget.  No reference counterpart. */
+int vj_engine_detect_footprint(vj_engine* e, int w, int h, float scaleFactor, int minObjectSize,
+                               int maxDetections, int64_t* bytes);
+
+/* ==== Served detect (no reference counterpart) ===========================
+ * A request loop with continuous batching: requests (host images of mixed
+ * sizes) queue on submit; each of `slots` device slots (one engine and one
+ * stream each, the cascade resident in every slot) takes as many queued
+ * requests as fit its share of the HBM budget (hbmBudget / slots, charged by
+ * vj_engine_detect_footprint) whenever it is free, and runs them as one
+ * batched detect, so requests that arrive while others are on the GPU are
+ * admitted as slots free up and the slots' work overlaps on the device.
+ * Every request's detections are those of vj_engine_detect on its image
+ * with the server's parameters.  A request larger than one slot's share is
+ * refused at submit with VJ_OUT_OF_MEMORY.  Thread-safe: any thread may
+ * submit and wait. */
+typedef struct vj_server vj_server;
+typedef struct vj_server_config {
+    int32_t winW, winH;             /* cascade window                            */
+    int32_t maxImageW, maxImageH;   /* largest request                           */
+    int32_t slots;                  /* batches in flight on the device, >= 1     */
+    int32_t maxBatch;               /* requests per batch, >= 1                  */
+    int64_t hbmBudget;              /* device bytes of admitted request state    */
+    float scaleFactor;              /* detectMultiScale parameters, per server   */
+    int32_t minNeighbors, minObjectSize, maxDetections, applyGrouping;
+} vj_server_config;
+typedef struct vj_server_stats {
+    int64_t submitted, completed, rejected, batches;
+    int64_t peakBytesInFlight;      /* largest sum of admitted footprints        */
+    int32_t maxSlotsBusy, maxBatchSeen;
+} vj_server_stats;
+
+/* The cascade is given as host arrays and uploaded once per slot. */
+int vj_server_create(const vj_server_config* cfg, const vj_feature* hFeatures, int nFeatures,
+                     const vj_stump* hStumps, int nStumps, const vj_stage* hStages, int nStages,
+                     vj_server** out);
+/* Copies the image (row stride `stride` bytes) and queues it; *ticket names
+ * the request for vj_server_wait. */
+int vj_server_submit(vj_server* srv, const uint8_t* hImage, int w, int h, int stride, int64_t* ticket);
+/* Blocks until the request is done; copies min(*count, cap) detections and
+ * gives the arrival-to-result latency in ms.  A ticket is waited once. */
+int vj_server_wait(vj_server* srv, int64_t ticket, vj_detection* hOut, int cap, int* count,
+                   double* latencyMs);
+int vj_server_stats_get(vj_server* srv, vj_server_stats* out);
+/* Finishes every queued request, then frees the slots. */
+void vj_server_destroy(vj_server* srv);
+
 /* ==== AdaBoostTrainer, include/vj/boosting_core.cuh:20-161 =============== */
 typedef struct vj_trainer vj_trainer;
 
