@@ -108,6 +108,9 @@ _PROTOS = {
     "vj_engine_phase_ms": (I, [P, C.POINTER(F)]),
     "vj_engine_detect_batch": (I, [P, I, P, P, P, P, F, I, I, I, I, P, I, P, P]),
     "vj_engine_detect_footprint": (I, [P, I, I, F, I, I, C.POINTER(I64)]),
+    "vj_engine_set_graphs": (I, [P, I]),
+    "vj_engine_set_autotune": (I, [P, I]),
+    "vj_engine_tile_rows": (I, [P, C.POINTER(I)]),
     "vj_server_create": (I, [C.POINTER(ServerConfig), P, I, P, I, P, I, PP]),
     "vj_server_submit": (I, [P, P, I, I, I, C.POINTER(I64)]),
     "vj_server_wait": (I, [P, I64, P, I, C.POINTER(I), C.POINTER(D)]),

@@ -9,9 +9,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <numeric>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -347,7 +352,7 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(const LevelDesc*
     __shared__ uint32_t votes[2][kVoteChunk][kCascadeBlock / 32];
     __shared__ float leafL[2][kVoteChunk], leafR[2][kVoteChunk];
     __shared__ int listCnt[2];
-    __shared__ unsigned long long maskWords[8];
+    __shared__ unsigned long long maskWords[32];
 
     // XCD-aware remap: consecutive tiles land on the same XCD (shared L2).
     int t;
@@ -418,7 +423,7 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(const LevelDesc*
             }
         }
     }
-    if (threadIdx.x < 8) maskWords[threadIdx.x] = 0ull;
+    if (threadIdx.x < 32) maskWords[threadIdx.x] = 0ull;
     if (threadIdx.x == 0) {
         listCnt[0] = 0;
         listCnt[1] = 0;
@@ -811,6 +816,19 @@ struct vj_engine {
 
     vj_detect_stats stats{};
 
+    // cascade tile rows per window step (index 1..3), chosen per batch shape
+    // by the autotuner when it is on
+    int tileRows[4] = {0, tile_rows_for_step(1), tile_rows_for_step(2), tile_rows_for_step(3)};
+    bool autotune = false;
+    std::map<std::vector<int64_t>, std::array<int, 4>> tuned;
+
+    // the detect launch sequence captured as a HIP graph, replayed while the
+    // plan shape, the tile rows and every buffer address stay the same
+    bool graphs = true;
+    std::vector<int64_t> graphKey;
+    hipGraphExec_t graphExec = nullptr;
+    hipStream_t captureStream = nullptr;  // captures run here; replays on the caller's stream
+
     // optional per-phase timing: events before resize, integral, cascade,
     // compaction and after compaction
     bool timing = false;
@@ -820,6 +838,8 @@ struct vj_engine {
     ~vj_engine() {
         for (hipEvent_t& x : ev)
             if (x) (void)hipEventDestroy(x);
+        if (graphExec) (void)hipGraphExecDestroy(graphExec);
+        if (captureStream) (void)hipStreamDestroy(captureStream);
     }
 };
 
@@ -832,9 +852,8 @@ inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 // LDS row pitch of a cascade tile's integral footprint at a window step, and
 // the footprint's size in elements.
 inline int cascade_pitch(int step, int winW) { return ((1 << tile_lgcols_for_step(step)) - 1) * step + winW + 1; }
-inline size_t cascade_footprint(int step, int winW, int winH) {
-    return static_cast<size_t>(cascade_pitch(step, winW)) *
-           static_cast<size_t>((tile_rows_for_step(step) - 1) * step + winH + 1);
+inline size_t cascade_footprint(int step, int rows, int winW, int winH) {
+    return static_cast<size_t>(cascade_pitch(step, winW)) * static_cast<size_t>((rows - 1) * step + winH + 1);
 }
 
 // Pyramid schedule of one image, src/face_vision_engine.cu:1596-1714 (and the
@@ -976,7 +995,7 @@ int build_plan(const vj_engine* e, const std::vector<ImageIn>& imgs, float sf, i
         d.scanW = (d.w - e->winW + 1 + d.step - 1) / d.step;
         d.scanH = (d.h - e->winH + 1 + d.step - 1) / d.step;
         d.rowWords = (d.scanW + 63) / 64;
-        d.tileRows = tile_rows_for_step(d.step);
+        d.tileRows = e->tileRows[d.step];
         d.tileLgCols = tile_lgcols_for_step(d.step);
         d.tilesX = d.rowWords << (6 - d.tileLgCols);  // whole mask words
         d.tilesY = (d.scanH + d.tileRows - 1) / d.tileRows;
@@ -1030,6 +1049,9 @@ int build_plan(const vj_engine* e, const std::vector<ImageIn>& imgs, float sf, i
     }
     return VJ_OK;
 }
+
+// Host threads that group a batch's images (the box's CPU share per GPU).
+constexpr int kGroupThreads = 16;
 
 int check_image_limits(const vj_engine* e, int w, int h) {
     VJ_CHECK(w <= e->maxW && h <= e->maxH, "image larger than the engine's maxImageW x maxImageH");
@@ -1134,60 +1156,122 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     const int64_t* m64 = e->dMeta64.get();
     const int nLevels = static_cast<int>(P.levels.size());
 
-    if (e->timing) VJ_HIP(hipEventRecord(e->ev[0], s));
-    // 1. pyramid: one launch per resize depth (each depth needs the previous)
-    for (size_t dpt = 0; dpt < P.resizeJobs.size(); ++dpt) {
-        const int n = static_cast<int>(P.resizeJobs[dpt].size());
-        if (!n || !P.resizeItems[dpt]) continue;
-        hipLaunchKernelGGL(resize_kernel, dim3(P.resizeItems[dpt]), dim3(256), 0, s, dL,
-                           e->dJobs.get() + jobStart[dpt], n, e->dRs.get(), m32 + oSrc);
-        VJ_HIP(hipGetLastError());
-    }
-    if (e->timing) VJ_HIP(hipEventRecord(e->ev[1], s));
-    // 2. integrals of every level
-    if (P.nItems > 0) {
-        hipLaunchKernelGGL(ii_colsum_kernel, dim3(P.nItems), dim3(64), 0, s, dL, m32 + oItem, nLevels,
-                           e->colS.get(), e->colQ.get(), e->rowS.get(), e->rowQ.get());
-        VJ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(ii_vscan_kernel, dim3(P.nChunkItems), dim3(256), 0, s, dL, m32 + oChunk, nLevels,
-                           e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get());
-        VJ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(ii_write_kernel, dim3(P.nItems), dim3(64), 0, s, dL, m32 + oItem, nLevels,
-                           e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get(), e->rowS.get(),
-                           e->rowQ.get(), e->iiS.get(), e->iiQ.get());
-        VJ_HIP(hipGetLastError());
-    }
-    // 3. cascade: every tile of every level in one launch; LDS sized for the
-    //    largest window step present (its tile footprint is the widest)
-    if (e->timing) VJ_HIP(hipEventRecord(e->ev[2], s));
-    if (P.nTiles > 0) {
-        size_t ldsBytes = 0;
-        for (int st = 1; st <= P.maxStep; ++st)
-            ldsBytes = std::max(ldsBytes, cascade_footprint(st, e->winW, e->winH) * sizeof(float));
-        hipLaunchKernelGGL(cascade_kernel, dim3(P.nTiles), dim3(kCascadeBlock), ldsBytes, s, dL, m32 + oLvl, m32 + oTile,
-                           static_cast<int>(P.tileLvl.size()), P.nTiles, e->iiS.get(), e->iiQ.get(), e->kst.get(),
-                           e->nStumps, e->kstage.get(), e->nStages, e->winW, e->winH, e->words.get(),
-                           e->scores.get());
-        VJ_HIP(hipGetLastError());
-    }
-    if (e->timing) VJ_HIP(hipEventRecord(e->ev[3], s));
-    // 4. counts and order-preserving emission
+    size_t ldsBytes = 0;
+    for (int st = 1; st <= P.maxStep; ++st)
+        ldsBytes = std::max(ldsBytes, cascade_footprint(st, e->tileRows[st], e->winW, e->winH) * sizeof(float));
     const int nBlocks = static_cast<int>(blockImg.size());
-    if (nBlocks > 0) {
-        hipLaunchKernelGGL(block_count_kernel, dim3(nBlocks), dim3(256), 0, s, m64 + oBlockWord, e->words.get(),
-                           e->dBlockCount.get());
+
+    auto launch_all = [&](hipStream_t ls) -> int {
+        if (e->timing) VJ_HIP(hipEventRecord(e->ev[0], ls));
+        // 1. pyramid: one launch per resize depth (each depth needs the previous)
+        for (size_t dpt = 0; dpt < P.resizeJobs.size(); ++dpt) {
+            const int n = static_cast<int>(P.resizeJobs[dpt].size());
+            if (!n || !P.resizeItems[dpt]) continue;
+            hipLaunchKernelGGL(resize_kernel, dim3(P.resizeItems[dpt]), dim3(256), 0, ls, dL,
+                               e->dJobs.get() + jobStart[dpt], n, e->dRs.get(), m32 + oSrc);
+            VJ_HIP(hipGetLastError());
+        }
+        if (e->timing) VJ_HIP(hipEventRecord(e->ev[1], ls));
+        // 2. integrals of every level
+        if (P.nItems > 0) {
+            hipLaunchKernelGGL(ii_colsum_kernel, dim3(P.nItems), dim3(64), 0, ls, dL, m32 + oItem, nLevels,
+                               e->colS.get(), e->colQ.get(), e->rowS.get(), e->rowQ.get());
+            VJ_HIP(hipGetLastError());
+            hipLaunchKernelGGL(ii_vscan_kernel, dim3(P.nChunkItems), dim3(256), 0, ls, dL, m32 + oChunk, nLevels,
+                               e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get());
+            VJ_HIP(hipGetLastError());
+            hipLaunchKernelGGL(ii_write_kernel, dim3(P.nItems), dim3(64), 0, ls, dL, m32 + oItem, nLevels,
+                               e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get(), e->rowS.get(),
+                               e->rowQ.get(), e->iiS.get(), e->iiQ.get());
+            VJ_HIP(hipGetLastError());
+        }
+        // 3. cascade: every tile of every level in one launch; LDS sized for the
+        //    largest window step present (its tile footprint is the widest)
+        if (e->timing) VJ_HIP(hipEventRecord(e->ev[2], ls));
+        if (P.nTiles > 0) {
+            hipLaunchKernelGGL(cascade_kernel, dim3(P.nTiles), dim3(kCascadeBlock), ldsBytes, ls, dL, m32 + oLvl, m32 + oTile,
+                               static_cast<int>(P.tileLvl.size()), P.nTiles, e->iiS.get(), e->iiQ.get(), e->kst.get(),
+                               e->nStumps, e->kstage.get(), e->nStages, e->winW, e->winH, e->words.get(),
+                               e->scores.get());
+            VJ_HIP(hipGetLastError());
+        }
+        if (e->timing) VJ_HIP(hipEventRecord(e->ev[3], ls));
+        // 4. counts and order-preserving emission
+        if (nBlocks > 0) {
+            hipLaunchKernelGGL(block_count_kernel, dim3(nBlocks), dim3(256), 0, ls, m64 + oBlockWord, e->words.get(),
+                               e->dBlockCount.get());
+            VJ_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(block_scan_kernel, dim3(1), dim3(1024), 0, ls, m32 + oImgBlock, nImages, e->dBlockCount.get(),
+                           e->dBlockOff.get(), e->dCounts.get(), e->dImgBase.get(), cap);
         VJ_HIP(hipGetLastError());
+        if (nBlocks > 0) {
+            hipLaunchKernelGGL(emit_kernel, dim3(nBlocks), dim3(256), 0, ls, dL, m64 + oLvWord, nLevels, m64 + oBlockWord,
+                               m32 + oBlockImg, e->words.get(), e->scores.get(), e->dBlockOff.get(), e->dImgBase.get(),
+                               cap, e->winW, e->winH, e->dOut.get());
+            VJ_HIP(hipGetLastError());
+        }
+        if (e->timing) VJ_HIP(hipEventRecord(e->ev[4], ls));
+        return VJ_OK;
+    };
+    if (e->timing || !e->graphs) return launch_all(s);
+
+    // Graph replay: the captured kernels read their per-call tables (levels,
+    // jobs, offsets) from the device buffers uploaded above, so the graph is
+    // valid while every grid size, scalar argument and buffer address matches.
+    std::vector<int64_t> key;
+    auto k = [&key](int64_t v) { key.push_back(v); };
+    for (size_t dpt = 0; dpt < P.resizeJobs.size(); ++dpt) {
+        k(static_cast<int64_t>(P.resizeJobs[dpt].size()));
+        k(P.resizeItems[dpt]);
+        k(static_cast<int64_t>(jobStart[dpt]));
     }
-    hipLaunchKernelGGL(block_scan_kernel, dim3(1), dim3(1024), 0, s, m32 + oImgBlock, nImages, e->dBlockCount.get(),
-                       e->dBlockOff.get(), e->dCounts.get(), e->dImgBase.get(), cap);
-    VJ_HIP(hipGetLastError());
-    if (nBlocks > 0) {
-        hipLaunchKernelGGL(emit_kernel, dim3(nBlocks), dim3(256), 0, s, dL, m64 + oLvWord, nLevels, m64 + oBlockWord,
-                           m32 + oBlockImg, e->words.get(), e->scores.get(), e->dBlockOff.get(), e->dImgBase.get(),
-                           cap, e->winW, e->winH, e->dOut.get());
-        VJ_HIP(hipGetLastError());
+    for (int64_t v : {int64_t(nLevels), int64_t(P.nItems), int64_t(P.nChunkItems), int64_t(P.nTiles),
+                      int64_t(P.tileLvl.size()), int64_t(ldsBytes), int64_t(nBlocks), int64_t(nImages), int64_t(cap),
+                      int64_t(e->nStumps), int64_t(e->nStages), int64_t(e->winW), int64_t(e->winH),
+                      int64_t(oSrc), int64_t(oItem), int64_t(oChunk), int64_t(oLvl), int64_t(oTile), int64_t(oLvWord),
+                      int64_t(oBlockWord), int64_t(oBlockImg), int64_t(oImgBlock)})
+        k(v);
+    for (const void* ptr : {static_cast<const void*>(dL), static_cast<const void*>(m32), static_cast<const void*>(m64),
+                            static_cast<const void*>(e->dJobs.get()), static_cast<const void*>(e->dRs.get()),
+                            static_cast<const void*>(e->colS.get()), static_cast<const void*>(e->colQ.get()),
+                            static_cast<const void*>(e->rowS.get()), static_cast<const void*>(e->rowQ.get()),
+                            static_cast<const void*>(e->totS.get()), static_cast<const void*>(e->totQ.get()),
+                            static_cast<const void*>(e->iiS.get()), static_cast<const void*>(e->iiQ.get()),
+                            static_cast<const void*>(e->kst.get()), static_cast<const void*>(e->kstage.get()),
+                            static_cast<const void*>(e->words.get()), static_cast<const void*>(e->scores.get()),
+                            static_cast<const void*>(e->dBlockCount.get()), static_cast<const void*>(e->dBlockOff.get()),
+                            static_cast<const void*>(e->dCounts.get()), static_cast<const void*>(e->dImgBase.get()),
+                            static_cast<const void*>(e->dOut.get())})
+        k(static_cast<int64_t>(reinterpret_cast<uintptr_t>(ptr)));
+    if (e->graphExec && key == e->graphKey) {
+        VJ_HIP(hipGraphLaunch(e->graphExec, s));
+        return VJ_OK;
     }
-    if (e->timing) VJ_HIP(hipEventRecord(e->ev[4], s));
+    if (e->graphExec) {
+        (void)hipGraphExecDestroy(e->graphExec);
+        e->graphExec = nullptr;
+        e->graphKey.clear();
+    }
+    // capture on the engine's own stream (the null stream cannot capture);
+    // capturing runs nothing, so only the replay is ordered on s
+    if (!e->captureStream) VJ_HIP(hipStreamCreateWithFlags(&e->captureStream, hipStreamNonBlocking));
+    hipGraph_t g = nullptr;
+    VJ_HIP(hipStreamBeginCapture(e->captureStream, hipStreamCaptureModeThreadLocal));
+    const int st = launch_all(e->captureStream);
+    const hipError_t ce = hipStreamEndCapture(e->captureStream, &g);
+    if (st != VJ_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture", __FILE__, __LINE__);
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) return hip_fail(ie, "hipGraphInstantiate", __FILE__, __LINE__);
+    e->graphExec = ex;
+    e->graphKey = std::move(key);
+    VJ_HIP(hipGraphLaunch(ex, s));
     return VJ_OK;
 }
 
@@ -1400,6 +1484,63 @@ extern "C" int vj_engine_clear_cascade(vj_engine* e, vj_stream) {
 
 namespace {
 
+// Cascade tile rows per window step (1, 2, 3) the autotuner tries; each keeps
+// a tile within kTileMaxWin windows and 32 rows.
+constexpr std::array<int, 4> kTileCandidates[] = {
+    {0, 8, 4, 4}, {0, 4, 4, 4}, {0, 8, 8, 8}, {0, 8, 8, 4}, {0, 4, 8, 8}};
+
+// Autotuner: the first call of a batch shape (image sizes, scale factor,
+// minimum object size, cascade size) times the cascade kernel for each
+// candidate tile shape (best of two runs, HIP events on the call's stream)
+// and keeps the fastest for that shape; later calls reuse it.  Every shape
+// gives the same detections; only the speed differs.
+int autotune(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int minObj, int capPer, hipStream_t s) {
+    std::vector<int64_t> key;
+    uint32_t sfBits;
+    std::memcpy(&sfBits, &sf, 4);
+    for (int64_t v : {int64_t(sfBits), int64_t(minObj), int64_t(e->nStages), int64_t(e->nStumps)}) key.push_back(v);
+    for (const ImageIn& in : imgs) {
+        key.push_back(in.w);
+        key.push_back(in.h);
+    }
+    auto it = e->tuned.find(key);
+    if (it == e->tuned.end()) {
+        if (!e->ev[0])
+            for (hipEvent_t& x : e->ev) VJ_HIP(hipEventCreate(&x));
+        const bool timing = e->timing;
+        e->timing = true;
+        float best = 0.0f;
+        std::array<int, 4> bestRows = kTileCandidates[0];
+        int status = VJ_OK;
+        for (const std::array<int, 4>& c : kTileCandidates) {
+            std::copy(c.begin(), c.end(), e->tileRows);
+            float t = -1.0f;
+            for (int rep = 0; rep < 2 && status == VJ_OK; ++rep) {
+                Plan P;
+                status = build_plan(e, imgs, sf, minObj, P);
+                if (status == VJ_OK) status = run_pipeline(e, P, capPer, s);
+                float ms = 0.0f;
+                if (status == VJ_OK && hipStreamSynchronize(s) == hipSuccess &&
+                    hipEventElapsedTime(&ms, e->ev[2], e->ev[3]) == hipSuccess) {
+                    if (t < 0.0f || ms < t) t = ms;
+                } else if (status == VJ_OK) {
+                    status = hip_fail(hipGetLastError(), "autotune timing", __FILE__, __LINE__);
+                }
+            }
+            if (status != VJ_OK) break;
+            if (best == 0.0f || t < best) {
+                best = t;
+                bestRows = c;
+            }
+        }
+        e->timing = timing;
+        if (status != VJ_OK) return status;
+        it = e->tuned.emplace(key, bestRows).first;
+    }
+    std::copy(it->second.begin(), it->second.end(), e->tileRows);
+    return VJ_OK;
+}
+
 int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int minNeighbors, int minObj,
                   int maxDet, int applyGrouping, vj_detection* hOut, int cap, int* hCounts, bool batch,
                   hipStream_t s) {
@@ -1410,6 +1551,7 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
         VJ_TRY(check_image_limits(e, in.w, in.h));
     }
     const int capPer = std::max(1, std::min(maxDet, e->maxDet));
+    if (e->autotune) VJ_TRY(autotune(e, imgs, sf, minObj, capPer, s));
     Plan P;
     VJ_TRY(build_plan(e, imgs, sf, minObj, P));
     const int nImages = static_cast<int>(imgs.size());
@@ -1438,18 +1580,53 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
     e->stats.integralBytes = P.integralBytes;
     e->stats.cascadeBytes = P.cascadeBytes + static_cast<int64_t>(e->nStumps) * 64 + 24 * total;
 
-    // per image: optional grouping, then copy out
-    int64_t inPos = 0, outPos = 0;
-    std::vector<vj_detection> grouped;
+    // per image: optional grouping (images in parallel on host threads,
+    // largest first), then copy out in image order
+    std::vector<int64_t> rawOff(static_cast<size_t>(nImages) + 1, 0);
+    for (int i = 0; i < nImages; ++i)
+        rawOff[static_cast<size_t>(i) + 1] = rawOff[static_cast<size_t>(i)] + std::min(e->hCounts.get()[i], capPer);
+    std::vector<std::vector<vj_detection>> grouped(applyGrouping ? static_cast<size_t>(nImages) : 0);
+    if (applyGrouping) {
+        std::vector<int> status(static_cast<size_t>(nImages), VJ_OK);
+        std::vector<int> order(static_cast<size_t>(nImages));
+        std::iota(order.begin(), order.end(), 0);
+        std::sort(order.begin(), order.end(), [&](int a, int b) {
+            return rawOff[a + 1] - rawOff[a] > rawOff[b + 1] - rawOff[b];
+        });
+        std::atomic<int> next{0};
+        auto worker = [&] {
+            for (int k = next++; k < nImages; k = next++) {
+                const int i = order[static_cast<size_t>(k)];
+                const int n = static_cast<int>(rawOff[static_cast<size_t>(i) + 1] - rawOff[static_cast<size_t>(i)]);
+                try {
+                    std::vector<vj_detection>& g = grouped[static_cast<size_t>(i)];
+                    g.resize(static_cast<size_t>(std::max(n, 1)));
+                    int m = 0;
+                    status[static_cast<size_t>(i)] = group_rectangles(e->hOut.get() + rawOff[static_cast<size_t>(i)], n,
+                                                                      minNeighbors, 0.2f, g.data(), &m);
+                    g.resize(static_cast<size_t>(m));
+                } catch (const std::exception&) {
+                    status[static_cast<size_t>(i)] = VJ_OUT_OF_MEMORY;
+                }
+            }
+        };
+        const int hw = static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+        const int nThreads = std::min({nImages, hw, kGroupThreads});
+        std::vector<std::thread> pool;
+        for (int k = 1; k < nThreads; ++k) pool.emplace_back(worker);
+        worker();
+        for (std::thread& th : pool) th.join();
+        for (int i = 0; i < nImages; ++i)
+            VJ_CHECK(status[static_cast<size_t>(i)] == VJ_OK, "groupRectangles failed (host memory)");
+    }
+    int64_t outPos = 0;
     for (int i = 0; i < nImages; ++i) {
-        const int n = std::min(e->hCounts.get()[i], capPer);
-        const vj_detection* src = e->hOut.get() + inPos;
-        inPos += n;
+        const int n = static_cast<int>(rawOff[static_cast<size_t>(i) + 1] - rawOff[static_cast<size_t>(i)]);
+        const vj_detection* src = e->hOut.get() + rawOff[static_cast<size_t>(i)];
         int m = n;
         if (applyGrouping) {
-            grouped.resize(static_cast<size_t>(std::max(n, 1)));
-            VJ_TRY(group_rectangles(src, n, minNeighbors, 0.2f, grouped.data(), &m));
-            src = grouped.data();
+            src = grouped[static_cast<size_t>(i)].data();
+            m = static_cast<int>(grouped[static_cast<size_t>(i)].size());
         }
         if (hCounts) hCounts[i] = m;
         if (batch) {
@@ -1517,6 +1694,26 @@ extern "C" int vj_engine_detect_footprint(vj_engine* e, int w, int h, float sf, 
         set_error(ex.what());
         return VJ_RUNTIME_ERROR;
     }
+}
+
+extern "C" int vj_engine_set_graphs(vj_engine* e, int on) {
+    VJ_CHECK(e, "vj_engine_set_graphs: null");
+    e->graphs = on != 0;
+    return VJ_OK;
+}
+
+extern "C" int vj_engine_set_autotune(vj_engine* e, int on) {
+    VJ_CHECK(e, "vj_engine_set_autotune: null");
+    e->autotune = on != 0;
+    if (!e->autotune)
+        for (int st = 1; st <= 3; ++st) e->tileRows[st] = tile_rows_for_step(st);
+    return VJ_OK;
+}
+
+extern "C" int vj_engine_tile_rows(vj_engine* e, int* rows) {
+    VJ_CHECK(e && rows, "vj_engine_tile_rows: null");
+    for (int st = 1; st <= 3; ++st) rows[st - 1] = e->tileRows[st];
+    return VJ_OK;
 }
 
 extern "C" int vj_engine_set_timing(vj_engine* e, int on) {

@@ -155,6 +155,20 @@ class FaceVisionEngine:
     def set_timing(self, on=True):
         check(lib().vj_engine_set_timing(self.handle, int(bool(on))), "set_timing")
 
+    def set_graphs(self, on=True):
+        """Capture the detect launch sequence in a HIP graph and replay it."""
+        check(lib().vj_engine_set_graphs(self.handle, int(bool(on))), "set_graphs")
+
+    def set_autotune(self, on=True):
+        """Pick the cascade tile shape per batch shape by timing candidates."""
+        check(lib().vj_engine_set_autotune(self.handle, int(bool(on))), "set_autotune")
+
+    def tile_rows(self):
+        """Cascade tile rows in use for window steps 1, 2 and 3."""
+        a = (C.c_int * 3)()
+        check(lib().vj_engine_tile_rows(self.handle, a), "tile_rows")
+        return list(a)
+
     def phase_ms(self):
         """Device ms of the last detect: pyramid, integrals, cascade, compaction."""
         a = (C.c_float * 4)()

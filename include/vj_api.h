@@ -171,7 +171,8 @@ int vj_engine_phase_ms(vj_engine* e, float* ms);
  * per call).  nImages images, each through the same pyramid, integral,
  * cascade and compaction kernels as vj_engine_detect but all images' work in
  * the same launches; the launch sequence is captured once per batch shape in
- * a HIP graph and replayed.  Per image the detections are identical to
+ * a HIP graph and replayed (vj_engine_set_graphs), with tile shapes from the
+ * autotuner when it is on (vj_engine_set_autotune).  Per image the detections are identical to
  * vj_engine_detect on that image.  hOut receives the images' detections
  * concatenated in image order; hCounts[i] is image i's count; the return is
  * VJ_INVALID_ARG when the total exceeds cap. */
@@ -180,6 +181,18 @@ int vj_engine_detect_batch(vj_engine* e, int nImages, const uint8_t* const* dIma
                            float scaleFactor, int minNeighbors, int minObjectSize,
                            int maxDetections, int applyGrouping,
                            vj_detection* hOut, int cap, int* hCounts, vj_stream s);
+
+/* Launch-path switches of the detect pipeline (no reference counterpart).
+ * Graphs (default on): the launch sequence of a detect is captured in a HIP
+ * graph and replayed while the plan shape and the engine's buffers are
+ * unchanged.  Autotune (default off): the first call of each batch shape
+ * (image sizes, scale factor, minimum object size, cascade) times the
+ * cascade kernel over a few candidate tile shapes and keeps the fastest for
+ * that shape.  Neither changes any detection.  vj_engine_tile_rows reports
+ * the cascade tile rows in use for window steps 1, 2 and 3. */
+int vj_engine_set_graphs(vj_engine* e, int on);
+int vj_engine_set_autotune(vj_engine* e, int on);
+int vj_engine_tile_rows(vj_engine* e, int* rows3);
 
 /* Device bytes one detect of a w x h image occupies in an engine's work
  * buffers (input, pyramid, integrals and their scratch, window mask and

@@ -167,3 +167,35 @@ def test_detect_config3_golden():
     assert eng.last_stats()["windows"] == int(g["windows"]) == 4729998
     grouped = eng.detectMultiScale(d, 1920, 1080, 1920, 1.25, 4, 24, 200000, applyGrouping=True)
     assert det_tuples(grouped) == det_tuples(g["grouped"])
+
+
+def test_graph_replays_and_autotune_match_eager(cascade3):
+    """The captured launch graph, replayed several times with eager detects of
+    other shapes in between, and the autotuned tile shapes both give exactly
+    the eager detections."""
+    say("graph replays and autotune vs eager")
+    sizes = [(640, 480), (333, 250), (97, 201), (512, 300)]
+    imgs = [synth.detect_image(w, h, max(1, w * h // 15000), 24, 64, seed=300 + i) for i, (w, h) in enumerate(sizes)]
+    dev = [DeviceArray.from_host(im) for im in imgs]
+    spec = [(w, h, w) for w, h in sizes]
+    eager = _engine(640, 480, cascade3)
+    eager.set_graphs(False)
+    want = [det_tuples(x) for x in eager.detect_batch(dev, spec, 1.25, 4, 24, 200000, False)]
+    other = synth.detect_image(300, 200, 5, 24, 60, seed=77)
+    d_other = DeviceArray.from_host(other)
+    want_other = det_tuples(eager.detectMultiScale(d_other, 300, 200, 300, 1.25, 4, 24, 200000, False))
+    eng = _engine(640, 480, cascade3)
+    for rep in range(4):
+        got = eng.detect_batch(dev, spec, 1.25, 4, 24, 200000, False)
+        assert [det_tuples(x) for x in got] == want, rep
+        # eager work of another shape on the same engine between replays
+        eng.set_graphs(False)
+        assert det_tuples(eng.detectMultiScale(d_other, 300, 200, 300, 1.25, 4, 24, 200000, False)) == want_other
+        eng.set_graphs(True)
+    tuned = _engine(640, 480, cascade3)
+    tuned.set_autotune(True)
+    for rep in range(2):
+        got = tuned.detect_batch(dev, spec, 1.25, 4, 24, 200000, False)
+        assert [det_tuples(x) for x in got] == want, rep
+    say(f"  tuned tile rows {tuned.tile_rows()}")
+    assert tuned.tile_rows() in ([8, 4, 4], [4, 4, 4], [8, 8, 8], [8, 8, 4], [4, 8, 8])
