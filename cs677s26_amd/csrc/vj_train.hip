@@ -31,7 +31,6 @@
 namespace vj {
 
 constexpr int kSortThreads = 1024;
-constexpr int kSortBuckets = 4096;
 constexpr int kSortMaxPerThread = 20;              // N <= 20480
 constexpr int kSortMaxN = kSortThreads * kSortMaxPerThread;
 constexpr int kSearchThreads = 1024;               // 16 waves, one feature each at a time
@@ -100,17 +99,25 @@ __device__ __forceinline__ float response(const vj_feature& f, const uint32_t* _
 }
 
 // ---------------------------------------------------------------------------
-// Presort: one 1024-thread workgroup per feature.  Keys are the
-// order-preserving bit patterns of the responses (-0 before +0, as the
-// reference's radix sort orders them); ties break by sample index.  The
-// samples are bucketed by a monotone linear map of their value into 4096
-// LDS buckets, each element's final rank is its bucket start plus the count
-// of smaller (key, index) pairs in its bucket.  Output per position k: the
-// sample index, with bit 15 set when fl(v_k + 1e-7f) < v_{k+1}
-// (src/boosting_core.cu:395-405).  mode 0 computes the responses from the
-// integrals; mode 1 reads them from resp (row f of [count][n]) and writes
-// plain indices (sortSamplesPerFeature semantics).
+// Presort: one 1024-thread workgroup per feature, a stable LSD radix sort in
+// LDS.  Keys are the order-preserving bit patterns of the responses (-0
+// before +0, as the reference's radix sort orders them), rebased to the
+// feature's smallest key so that only the bits in which the feature's keys
+// differ are sorted, 8 per pass.  The sorted positions are padded to whole
+// waves (nPad = n rounded up to 1024) with the largest key, which sorts them
+// after every real sample; stability keeps equal keys in sample order.
+// Each wave owns nPad/16 consecutive positions and ranks them 64 at a time:
+// eight ballots give every lane the lanes holding the same digit, so a
+// lane's rank is its count of same-digit lanes below it plus the wave's
+// running count of that digit; one exclusive scan over (digit, wave) then
+// places every element.  Output per position k: the sample index, with bit
+// 15 set when fl(v_k + 1e-7f) < v_{k+1} (src/boosting_core.cu:395-405).
+// mode 0 computes the responses from the integrals; mode 1 reads them from
+// resp (row f of [count][n]) and writes plain indices (sortSamplesPerFeature
+// semantics).
 // ---------------------------------------------------------------------------
+constexpr int kSortWaves = kSortThreads / kWave;
+
 __global__ __launch_bounds__(kSortThreads) void presort_kernel(const vj_feature* __restrict__ feats, int fBegin,
                                                                const uint32_t* __restrict__ sumT,
                                                                const float* __restrict__ inv,
@@ -118,135 +125,129 @@ __global__ __launch_bounds__(kSortThreads) void presort_kernel(const vj_feature*
                                                                uint16_t* __restrict__ out, int64_t outPitch,
                                                                int mode) {
     extern __shared__ uint8_t smem[];
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(smem);                  // [kSortBuckets + 1]
-    uint32_t* keys = cnt + kSortBuckets + 4;                             // [n]
-    uint16_t* idx = reinterpret_cast<uint16_t*>(keys + n);               // [n]
-    __shared__ float redMin[16], redMax[16];
+    const int nPad = (n + kSortThreads - 1) / kSortThreads * kSortThreads;
+    const int nb = nPad / kSortThreads;                                  // 64-position batches per wave
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(smem);                   // [256][kSortWaves]
+    uint32_t* keys = cnt + 256 * kSortWaves;                             // [nPad]
+    uint16_t* idx = reinterpret_cast<uint16_t*>(keys + nPad);            // [nPad]
+    __shared__ uint32_t redMin[kSortWaves], redMax[kSortWaves], waveTot[kSortWaves];
 
     const int f = blockIdx.x;
-    const int tid = threadIdx.x;
-    float v[kSortMaxPerThread];
-    float vmin = INFINITY, vmax = -INFINITY;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t kmin = 0xFFFFFFFFu, kmax = 0u;
     if (mode == 0) {
         const vj_feature ft = feats[fBegin + f];
 #pragma unroll
         for (int j = 0; j < kSortMaxPerThread; ++j) {
             const int s = tid + j * kSortThreads;
-            v[j] = (s < n) ? response(ft, sumT, inv, n, s) : 0.0f;
+            if (s < n) {
+                const uint32_t k = float_key(response(ft, sumT, inv, n, s));
+                keys[s] = k;
+                idx[s] = static_cast<uint16_t>(s);
+                kmin = min(kmin, k);
+                kmax = max(kmax, k);
+            }
         }
     } else {
         const float* row = resp + static_cast<size_t>(f) * n;
 #pragma unroll
         for (int j = 0; j < kSortMaxPerThread; ++j) {
             const int s = tid + j * kSortThreads;
-            v[j] = (s < n) ? row[s] : 0.0f;
+            if (s < n) {
+                const uint32_t k = float_key(row[s]);
+                keys[s] = k;
+                idx[s] = static_cast<uint16_t>(s);
+                kmin = min(kmin, k);
+                kmax = max(kmax, k);
+            }
         }
     }
-#pragma unroll
-    for (int j = 0; j < kSortMaxPerThread; ++j) {
-        const int s = tid + j * kSortThreads;
-        if (s < n) {
-            vmin = fminf(vmin, v[j]);
-            vmax = fmaxf(vmax, v[j]);
-        }
-    }
-    for (int b = tid; b < kSortBuckets; b += kSortThreads) cnt[b] = 0u;
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) {
-        vmin = fminf(vmin, __shfl_xor(vmin, d, kWave));
-        vmax = fmaxf(vmax, __shfl_xor(vmax, d, kWave));
+        kmin = min(kmin, static_cast<uint32_t>(__shfl_xor(static_cast<int>(kmin), d, kWave)));
+        kmax = max(kmax, static_cast<uint32_t>(__shfl_xor(static_cast<int>(kmax), d, kWave)));
     }
-    if ((tid & 63) == 0) {
-        redMin[tid >> 6] = vmin;
-        redMax[tid >> 6] = vmax;
+    if (lane == 0) {
+        redMin[wave] = kmin;
+        redMax[wave] = kmax;
     }
     __syncthreads();
-    vmin = redMin[0];
-    vmax = redMax[0];
-    for (int k = 1; k < kSortThreads / kWave; ++k) {
-        vmin = fminf(vmin, redMin[k]);
-        vmax = fmaxf(vmax, redMax[k]);
+    kmin = redMin[0];
+    kmax = redMax[0];
+    for (int w = 1; w < kSortWaves; ++w) {
+        kmin = min(kmin, redMin[w]);
+        kmax = max(kmax, redMax[w]);
     }
-    const float range = vmax - vmin;
-    const float scale = (range > 0.0f && range < INFINITY) ? static_cast<float>(kSortBuckets) / range : 0.0f;
+    for (int s = n + tid; s < nPad; s += kSortThreads) {
+        keys[s] = kmax;
+        idx[s] = 0xFFFFu;
+    }
+    __syncthreads();
 
-    // histogram; the atomic's old value is the element's slot in its bucket
-    uint32_t bs[kSortMaxPerThread];
+    const uint32_t span = kmax - kmin;
+    const int bits = span ? 32 - __clz(static_cast<int>(span)) : 0;
+    const int base = wave * nb * kWave;
+    for (int shift = 0; shift < bits; shift += 8) {
+        uint32_t kr[kSortMaxPerThread], rk[kSortMaxPerThread];
+        uint16_t ir[kSortMaxPerThread];
 #pragma unroll
-    for (int j = 0; j < kSortMaxPerThread; ++j) {
-        const int s = tid + j * kSortThreads;
-        bs[j] = 0;
-        if (s < n) {
-            int b = static_cast<int>((v[j] - vmin) * scale);
-            b = b < 0 ? 0 : (b >= kSortBuckets ? kSortBuckets - 1 : b);
-            const uint32_t slot = atomicAdd(&cnt[b], 1u);
-            bs[j] = (static_cast<uint32_t>(b) << 16) | slot;
-        }
-    }
-    __syncthreads();
-    // exclusive scan of the bucket counts (4 buckets per thread)
-    {
-        const int b0 = tid * (kSortBuckets / kSortThreads);
-        uint32_t c[kSortBuckets / kSortThreads];
-        uint32_t t = 0;
-#pragma unroll
-        for (int k = 0; k < kSortBuckets / kSortThreads; ++k) {
-            c[k] = cnt[b0 + k];
-            t += c[k];
-        }
-        uint32_t inc = wave_incl_scan(t);
-        __shared__ uint32_t waveTot[16];
-        if ((tid & 63) == 63) waveTot[tid >> 6] = inc;
-        __syncthreads();
-        uint32_t base = inc - t;
-        for (int k = 0; k < (tid >> 6); ++k) base += waveTot[k];
-#pragma unroll
-        for (int k = 0; k < kSortBuckets / kSortThreads; ++k) {
-            cnt[b0 + k] = base;
-            base += c[k];
-        }
-        if (tid == kSortThreads - 1) cnt[kSortBuckets] = base;
-    }
-    __syncthreads();
-    // scatter into bucket order
-#pragma unroll
-    for (int j = 0; j < kSortMaxPerThread; ++j) {
-        const int s = tid + j * kSortThreads;
-        if (s < n) {
-            const uint32_t pos = cnt[bs[j] >> 16] + (bs[j] & 0xFFFFu);
-            keys[pos] = float_key(v[j]);
-            idx[pos] = static_cast<uint16_t>(s);
-        }
-    }
-    __syncthreads();
-    // rank within the bucket by (key, index)
-    uint32_t rank[kSortMaxPerThread];
-#pragma unroll
-    for (int j = 0; j < kSortMaxPerThread; ++j) {
-        const int s = tid + j * kSortThreads;
-        rank[j] = 0;
-        if (s < n) {
-            const uint32_t b = bs[j] >> 16;
-            const uint32_t a = cnt[b], e = cnt[b + 1];
-            const uint32_t k = float_key(v[j]);
-            uint32_t r = a;
-            for (uint32_t p = a; p < e; ++p) {
-                const uint32_t kp = keys[p];
-                r += (kp < k || (kp == k && idx[p] < static_cast<uint32_t>(s))) ? 1u : 0u;
+        for (int i = 0; i < kSortMaxPerThread; ++i) {
+            if (i < nb) {
+                kr[i] = keys[base + i * kWave + lane];
+                ir[i] = idx[base + i * kWave + lane];
             }
-            rank[j] = r;
         }
-    }
-    __syncthreads();
+        for (int d = lane; d < 256; d += kWave) cnt[d * kSortWaves + wave] = 0u;
 #pragma unroll
-    for (int j = 0; j < kSortMaxPerThread; ++j) {
-        const int s = tid + j * kSortThreads;
-        if (s < n) {
-            keys[rank[j]] = float_key(v[j]);
-            idx[rank[j]] = static_cast<uint16_t>(s);
+        for (int i = 0; i < kSortMaxPerThread; ++i) {
+            if (i < nb) {
+                const uint32_t dg = ((kr[i] - kmin) >> shift) & 0xFFu;
+                uint64_t m = ~0ull;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const bool bit = (dg >> b) & 1u;
+                    const uint64_t bb = __ballot(bit);
+                    m &= bit ? bb : ~bb;
+                }
+                const uint32_t below = static_cast<uint32_t>(mask_rank(m));
+                const uint32_t c = cnt[dg * kSortWaves + wave];
+                rk[i] = c + below;
+                if (below == 0) cnt[dg * kSortWaves + wave] = c + static_cast<uint32_t>(__popcll(m));
+            }
         }
+        __syncthreads();
+        // exclusive scan over (digit, wave): 4 consecutive entries per thread
+        {
+            uint32_t c4[4];
+            uint32_t t = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                c4[k] = cnt[tid * 4 + k];
+                t += c4[k];
+            }
+            const uint32_t inc = wave_incl_scan_dpp(t);
+            if (lane == 63) waveTot[wave] = inc;
+            __syncthreads();
+            uint32_t off = inc - t;
+            for (int w = 0; w < wave; ++w) off += waveTot[w];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                cnt[tid * 4 + k] = off;
+                off += c4[k];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kSortMaxPerThread; ++i) {
+            if (i < nb) {
+                const uint32_t dg = ((kr[i] - kmin) >> shift) & 0xFFu;
+                const uint32_t pos = cnt[dg * kSortWaves + wave] + rk[i];
+                keys[pos] = kr[i];
+                idx[pos] = ir[i];
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
     uint16_t* o = out + static_cast<int64_t>(f) * outPitch;
     for (int k = tid; k < n; k += kSortThreads) {
         uint32_t e = idx[k];
@@ -260,11 +261,15 @@ __global__ __launch_bounds__(kSortThreads) void presort_kernel(const vj_feature*
 
 // ---------------------------------------------------------------------------
 // Threshold scan: every wave takes one feature at a time and walks its
-// sorted order, 4 entries per lane per step; D_k = sum_{j<=k} q(order j)
+// sorted order, 8 entries per lane per step; D_k = sum_{j<=k} q(order j)
 // exactly in int64 (signed: +w positives, -w negatives, 0 inactive);
 // errN (parity +1) = totNeg + D_k, errP (parity -1) = totPos - D_k at every
-// position whose valid bit is set (src/boosting_core.cu:289-456).  The
-// search weights sit in LDS as float; each WG keeps the best of its features.
+// position whose valid bit is set (src/boosting_core.cu:289-456).  So the
+// best parity -1 split is the first valid position of largest D and the best
+// parity +1 split the first valid position of smallest D; the two are
+// compared at the end with the reference's tie rules.  The search weights
+// sit in LDS packed one word per sample; each WG keeps the best of its
+// features.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool better(int64_t e, int32_t k, int32_t p, int64_t be, int32_t bk, int32_t bp) {
     if (e != be) return e < be;
@@ -272,73 +277,136 @@ __device__ __forceinline__ bool better(int64_t e, int32_t k, int32_t p, int64_t 
     return p > bp;
 }
 
+// A search weight packed in one word: bits 0-24 hold the signed significand
+// sm (|sm| < 2^24) and bits 25-31 a shift sh <= 39, with sm << sh equal to
+// fixed_weight(w) exactly (significands of weights below 2^-39 are truncated
+// first, as fixed_weight does).
+__device__ __forceinline__ uint32_t pack_weight(float w) {
+    const uint32_t b = __float_as_uint(w);
+    int e = static_cast<int>((b >> 23) & 0xFFu);
+    uint32_t m = b & 0x7FFFFFu;
+    if (e == 0) e = 1;
+    else m |= 0x800000u;
+    int sh = min(e - 88, 39);
+    if (sh < 0) {
+        m = sh > -32 ? (m >> (-sh)) : 0u;
+        sh = 0;
+    }
+    const int32_t sm = (b >> 31) ? -static_cast<int32_t>(m) : static_cast<int32_t>(m);
+    return (static_cast<uint32_t>(sh) << 25) | (static_cast<uint32_t>(sm) & 0x1FFFFFFu);
+}
+
+__device__ __forceinline__ int64_t unpack_weight(uint32_t p) {
+    const int32_t sm = static_cast<int32_t>(p << 7) >> 7;
+    return static_cast<int64_t>(static_cast<uint64_t>(static_cast<int64_t>(sm)) << (p >> 25));
+}
+
+// Inclusive wave64 prefix sum of an int64 with DPP moves of both halves
+// (the same row shifts and row broadcasts as wave_incl_scan_dpp).
+__device__ __forceinline__ int64_t wave_incl_scan_dpp64(int64_t v) {
+#define VJ_DPP64_STEP(ctrl, rmask, bc)                                                                          \
+    {                                                                                                           \
+        const uint32_t lo = static_cast<uint32_t>(                                                              \
+            __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint64_t>(v)), ctrl, rmask, 0xf, bc));  \
+        const uint32_t hi = static_cast<uint32_t>(                                                              \
+            __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint64_t>(v) >> 32), ctrl, rmask, 0xf, bc)); \
+        v += static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);                                      \
+    }
+    VJ_DPP64_STEP(0x111, 0xf, true)
+    VJ_DPP64_STEP(0x112, 0xf, true)
+    VJ_DPP64_STEP(0x114, 0xf, true)
+    VJ_DPP64_STEP(0x118, 0xf, true)
+    VJ_DPP64_STEP(0x142, 0xa, false)
+    VJ_DPP64_STEP(0x143, 0xc, false)
+#undef VJ_DPP64_STEP
+    return v;
+}
+
+constexpr int kScanPer = 8;  // sorted entries per lane per step (one 16-byte load)
+
 // Exact threshold scan of one feature's sorted order by one wavefront; returns
-// the feature's best (error, split, parity) in every lane.
-__device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, int n, const float* lsw,
+// the feature's best (error, split, parity) in every lane (split -1: none).
+__device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, int n, const uint32_t* lsw,
                                              int64_t totPos, int64_t totNeg, int lane,
                                              int64_t& fE, int32_t& fK, int32_t& fP) {
     int64_t carry = 0;
-    fE = INT64_MAX;
-    fK = -1;
-    fP = +1;
-    for (int base = 0; base < n; base += 256) {
-        const int k0 = base + lane * 4;
-        uint64_t e4 = 0;
-        if (k0 < n) e4 = *reinterpret_cast<const uint64_t*>(row + k0);  // rows padded to 256 entries
-        int64_t q[4];
+    int64_t hiD = INT64_MIN, loD = INT64_MAX;
+    int32_t hiK = INT_MAX, loK = INT_MAX;
+    for (int base = 0; base < n; base += kWave * kScanPer) {
+        const int k0 = base + lane * kScanPer;
+        uint4 e8 = make_uint4(0u, 0u, 0u, 0u);
+        if (k0 < n) e8 = *reinterpret_cast<const uint4*>(row + k0);  // rows padded to 256 entries
+        const uint32_t ew[4] = {e8.x, e8.y, e8.z, e8.w};
+        int64_t q[kScanPer];
         int64_t c = 0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const uint32_t e = static_cast<uint32_t>(e4 >> (16 * j)) & 0xFFFFu;
-            q[j] = (k0 + j < n) ? fixed_weight(lsw[e & 0x7FFFu]) : 0;
+        for (int j = 0; j < kScanPer; ++j) {
+            const uint32_t e = (ew[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+            q[j] = (k0 + j < n) ? unpack_weight(lsw[e & 0x7FFFu]) : 0;
             c += q[j];
         }
-        const int64_t inc = wave_incl_scan(c);
+        const int64_t inc = wave_incl_scan_dpp64(c);
         int64_t D = carry + inc - c;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < kScanPer; ++j) {
             D += q[j];
-            const uint32_t e = static_cast<uint32_t>(e4 >> (16 * j)) & 0xFFFFu;
+            const uint32_t e = (ew[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
             if ((e & 0x8000u) && k0 + j < n) {
-                const int32_t k = k0 + j;
-                const int64_t errP = totPos - D;
-                const int64_t errN = totNeg + D;
-                if (better(errP, k, -1, fE, fK, fP)) {
-                    fE = errP;
-                    fK = k;
-                    fP = -1;
+                if (D > hiD) {
+                    hiD = D;
+                    hiK = k0 + j;
                 }
-                if (better(errN, k, +1, fE, fK, fP)) {
-                    fE = errN;
-                    fK = k;
-                    fP = +1;
+                if (D < loD) {
+                    loD = D;
+                    loK = k0 + j;
                 }
             }
         }
-        carry += __shfl(inc, 63, kWave);
+        const uint64_t last = static_cast<uint64_t>(inc);
+        carry += static_cast<int64_t>(
+            (static_cast<uint64_t>(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(last >> 32), 63)))
+             << 32) |
+            static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(last), 63)));
     }
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) {
-        const int64_t oE = __shfl_xor(fE, d, kWave);
-        const int32_t oK = __shfl_xor(fK, d, kWave);
-        const int32_t oP = __shfl_xor(fP, d, kWave);
-        if (better(oE, oK, oP, fE, fK, fP)) {
-            fE = oE;
-            fK = oK;
-            fP = oP;
+        const int64_t oH = __shfl_xor(hiD, d, kWave);
+        const int32_t oHK = __shfl_xor(hiK, d, kWave);
+        if (oH > hiD || (oH == hiD && oHK < hiK)) {
+            hiD = oH;
+            hiK = oHK;
+        }
+        const int64_t oL = __shfl_xor(loD, d, kWave);
+        const int32_t oLK = __shfl_xor(loK, d, kWave);
+        if (oL < loD || (oL == loD && oLK < loK)) {
+            loD = oL;
+            loK = oLK;
+        }
+    }
+    fE = INT64_MAX;
+    fK = -1;
+    fP = +1;
+    if (loK != INT_MAX) {
+        fE = totNeg + loD;
+        fK = loK;
+        const int64_t eP = totPos - hiD;
+        if (better(eP, hiK, -1, fE, fK, fP)) {
+            fE = eP;
+            fK = hiK;
+            fP = -1;
         }
     }
 }
 
-// Loads the float search weights into LDS and returns the exact totals.
-__device__ __forceinline__ void load_weights(const float* __restrict__ sw32, int n, float* lsw, int64_t* totP,
+// Packs the search weights into LDS and returns the exact totals.
+__device__ __forceinline__ void load_weights(const float* __restrict__ sw32, int n, uint32_t* lsw, int64_t* totP,
                                              int64_t* totN, int64_t& totPos, int64_t& totNeg) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int64_t tp = 0, tn = 0;
     for (int s = tid; s < n; s += blockDim.x) {
-        const float w = sw32[s];
-        lsw[s] = w;
-        const int64_t q = fixed_weight(w);
+        const uint32_t pw = pack_weight(sw32[s]);
+        lsw[s] = pw;
+        const int64_t q = unpack_weight(pw);
         if (q > 0) tp += q;
         else tn -= q;
     }
@@ -362,7 +430,7 @@ __global__ __launch_bounds__(kSearchThreads) void search_kernel(const uint16_t* 
                                                                const float* __restrict__ sw32,
                                                                const uint8_t* __restrict__ used,
                                                                WaveBest* __restrict__ blockBest) {
-    extern __shared__ float lsw[];  // [n]
+    extern __shared__ uint32_t lsw[];  // [n] packed search weights
     __shared__ int64_t totP[16], totN[16];
     __shared__ WaveBest wb[16];
     const int tid = threadIdx.x;
@@ -731,7 +799,7 @@ __global__ __launch_bounds__(kSearchThreads) void search_each_kernel(const uint1
                                                                     const float* __restrict__ sw32,
                                                                     const float* __restrict__ resp,
                                                                     FeatureBestSplit* __restrict__ out) {
-    extern __shared__ float lsw[];
+    extern __shared__ uint32_t lsw[];  // [n] packed search weights
     __shared__ int64_t totP[16], totN[16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int64_t totPos, totNeg;
@@ -861,7 +929,10 @@ int search_blocks() {
     return 2 * cus;
 }
 
-size_t presort_lds(int n) { return (kSortBuckets + 4) * 4 + static_cast<size_t>(n) * 4 + static_cast<size_t>(n) * 2 + 16; }
+size_t presort_lds(int n) {
+    const size_t nPad = static_cast<size_t>((n + kSortThreads - 1) / kSortThreads * kSortThreads);
+    return 256 * kSortWaves * 4 + nPad * 4 + nPad * 2;
+}
 
 // Every corner of every feature must index the (winW+1) x (winH+1) window
 // integral; checked on the host before any kernel reads through them.
