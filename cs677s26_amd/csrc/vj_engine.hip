@@ -323,133 +323,106 @@ __device__ __forceinline__ float haar_raw(const float* o, const KStump& sp) {
     return raw;
 }
 
-// Cascade evaluation, one workgroup of kCascadeBlock threads per tile of
-// 2^lgc x TH windows of one level.  The tile's integral footprint is staged
-// in LDS as f32 (each absolute u32 value converted once: the CPU path
-// converts the same absolute values, so every Haar sum rounds identically).
-// Windows failing the variance gate never enter the live list; each stage
-// then runs over the live list only and the survivors are compacted by
-// wavefront ballot, so lanes stay busy however unevenly windows reject.
-// Survivors of the last stage set their bit in the level's window mask and
-// store their last stage sum.  Stage rule and window normaliser:
-// src/main.cu:752-804.
 constexpr int kCascadeBlock = 256;
+// Survivors of the first pass are handed to the deep pass in slices of up to
+// kDeepSlice windows (at most kTileMaxWin / kDeepSlice = 8 slices per tile).
+constexpr int kDeepSlice = 64;
+// Stumps of the leading stages the first pass runs, and the deep pass's
+// persistent grid (eight workgroups on each of 256 CUs).
+constexpr int kSplitStumps = 128;
+constexpr int kDeepGrid = 2048;
 
-__global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(const LevelDesc* __restrict__ L,
-                                                                const int32_t* __restrict__ lvlIdx,
-                                                                const int32_t* __restrict__ tileOffs, int nLv,
-                                                                int nTiles,
-                                                                const uint32_t* __restrict__ iiS,
-                                                                const uint64_t* __restrict__ iiQ,
-                                                                const KStump* __restrict__ stumpsAll, int nStumps,
-                                                                const KStage* __restrict__ stages, int nStages,
-                                                                int winW, int winH,
-                                                                uint64_t* __restrict__ words,
-                                                                float* __restrict__ scores) {
-    extern __shared__ float lds[];
-    __shared__ uint16_t listIdx[kTileMaxWin];
-    __shared__ float invTab[kTileMaxWin];
-    __shared__ uint32_t votes[2][kVoteChunk][kCascadeBlock / 32];
-    __shared__ float leafL[2][kVoteChunk], leafR[2][kVoteChunk];
-    __shared__ int listCnt[2];
-    __shared__ unsigned long long maskWords[32];
+// LDS of one cascade workgroup besides the staged footprint.
+struct CascadeSmem {
+    uint16_t listIdx[kTileMaxWin];               // live windows (tile-local index)
+    float invTab[kTileMaxWin];                   // normaliser by tile-local index
+    uint32_t votes[2][kVoteChunk][kCascadeBlock / 32];
+    float leafL[2][kVoteChunk], leafR[2][kVoteChunk];
+    int listCnt[2];
+    int item;                                    // deep pass: current work item
+    unsigned long long maskWords[32];            // final bits per tile row
+};
 
-    // XCD-aware remap: consecutive tiles land on the same XCD (shared L2).
-    int t;
-    {
-        const int i = blockIdx.x, q = nTiles >> 3, r = nTiles & 7;
-        const int xcd = i & 7, pos = i >> 3;
-        t = (xcd < r) ? xcd * (q + 1) + pos : r * (q + 1) + (xcd - r) * q + pos;
-    }
+// A tile of windows: level, position and shape.
+struct TileGeo {
+    LevelDesc lv;
+    int tx, ty, TH, lgc, TW, step, wx0, wy0;
+};
+
+__device__ __forceinline__ TileGeo tile_geo(const LevelDesc* __restrict__ L, const int32_t* __restrict__ lvlIdx,
+                                            const int32_t* __restrict__ tileOffs, int nLv, int t) {
+    TileGeo g;
     const int j = find_segment(tileOffs, nLv, t);
-    const LevelDesc lv = L[lvlIdx[j]];
+    g.lv = L[lvlIdx[j]];
     const int local = t - tileOffs[j];
-    const int ty = local / lv.tilesX;
-    const int tx = local - ty * lv.tilesX;
-    const int TH = lv.tileRows;
-    const int lgc = lv.tileLgCols;
-    const int TW = 1 << lgc;
-    const int step = lv.step;
-    const int wx0 = tx * TW, wy0 = ty * TH;
+    g.ty = local / g.lv.tilesX;
+    g.tx = local - g.ty * g.lv.tilesX;
+    g.TH = g.lv.tileRows;
+    g.lgc = g.lv.tileLgCols;
+    g.TW = 1 << g.lgc;
+    g.step = g.lv.step;
+    g.wx0 = g.tx * g.TW;
+    g.wy0 = g.ty * g.TH;
+    return g;
+}
 
-    // Tile row r is the TW-bit piece of mask word (wy0 + r, wx0 / 64) that
-    // starts at bit wx0 % 64 (little-endian: a 32- or 16-bit store).  Every
-    // piece of every word is written by exactly one tile, so the mask needs
-    // no clearing.
-    auto write_mask = [&](int r, uint64_t bits) {
-        const int64_t word = lv.wordOff + static_cast<int64_t>(wy0 + r) * lv.rowWords + (wx0 >> 6);
-        if (lgc == 6) {
-            words[word] = bits;
-        } else if (lgc == 5) {
-            reinterpret_cast<uint32_t*>(words)[word * 2 + ((wx0 & 63) >> 5)] = static_cast<uint32_t>(bits);
-        } else {
-            reinterpret_cast<uint16_t*>(words)[word * 4 + ((wx0 & 63) >> 4)] = static_cast<uint16_t>(bits);
-        }
-    };
-    if (wx0 >= lv.scanW) {  // padding tile past the scan width: zero pieces
-        for (int r = threadIdx.x; r < TH; r += kCascadeBlock)
-            if (wy0 + r < lv.scanH) write_mask(r, 0ull);
-        return;
-    }
-
-    const KStump* __restrict__ stumps = stumpsAll + static_cast<size_t>(step - 1) * nStumps;
-    const int ix0 = wx0 * step, iy0 = wy0 * step;
-    const int FW = (TW - 1) * step + winW + 1;
-    const int FH = (TH - 1) * step + winH + 1;
-    const int iw = lv.w + 1;
-    const uint32_t* sumL = iiS + lv.iiOff;
-    const uint64_t* sqL = iiQ + lv.iiOff;
-
-    // footprint staging, four loads in flight per thread before their stores
-    {
-        const int total = FW * FH;
-        for (int e0 = threadIdx.x; e0 < total; e0 += 4 * kCascadeBlock) {
-            float v[4];
+// Stages the tile's integral footprint into LDS as f32, four loads in
+// flight per thread before their stores.
+__device__ __forceinline__ void stage_footprint(const TileGeo& g, const uint32_t* __restrict__ iiS, float* lds,
+                                                int winW, int winH) {
+    const int FW = (g.TW - 1) * g.step + winW + 1;
+    const int FH = (g.TH - 1) * g.step + winH + 1;
+    const int ix0 = g.wx0 * g.step, iy0 = g.wy0 * g.step;
+    const int iw = g.lv.w + 1;
+    const uint32_t* sumL = iiS + g.lv.iiOff;
+    const int total = FW * FH;
+    for (int e0 = threadIdx.x; e0 < total; e0 += 4 * kCascadeBlock) {
+        float v[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int e = e0 + u * kCascadeBlock;
-                v[u] = 0.0f;
-                if (e < total) {
-                    const int r = e / FW, cc = e - r * FW;
-                    const int gy = iy0 + r, gx = ix0 + cc;
-                    if (gy <= lv.h && gx <= lv.w)
-                        v[u] = static_cast<float>(static_cast<int32_t>(sumL[static_cast<int64_t>(gy) * iw + gx]));
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int e = e0 + u * kCascadeBlock;
-                if (e < total) lds[e] = v[u];
+        for (int u = 0; u < 4; ++u) {
+            const int e = e0 + u * kCascadeBlock;
+            v[u] = 0.0f;
+            if (e < total) {
+                const int r = e / FW, cc = e - r * FW;
+                const int gy = iy0 + r, gx = ix0 + cc;
+                if (gy <= g.lv.h && gx <= g.lv.w)
+                    v[u] = static_cast<float>(static_cast<int32_t>(sumL[static_cast<int64_t>(gy) * iw + gx]));
             }
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = e0 + u * kCascadeBlock;
+            if (e < total) lds[e] = v[u];
+        }
     }
-    if (threadIdx.x < 32) maskWords[threadIdx.x] = 0ull;
-    if (threadIdx.x == 0) {
-        listCnt[0] = 0;
-        listCnt[1] = 0;
-    }
-    __syncthreads();
+}
 
-    // variance gate (exact int sum, u32 square-sum delta, f64 normaliser);
-    // a tile has at most two windows per thread, both gated before either
-    // is appended so their global loads overlap
+// Variance gate (exact int sum, u32 square-sum delta, f64 normaliser) into
+// the live list; a tile has at most two windows per thread, both gated
+// before either is appended so their global loads overlap.
+__device__ __forceinline__ void gate_windows(const TileGeo& g, const uint32_t* __restrict__ iiS,
+                                             const uint64_t* __restrict__ iiQ, CascadeSmem& sm, int winW,
+                                             int winH) {
+    const int iw = g.lv.w + 1;
+    const uint32_t* sumL = iiS + g.lv.iiOff;
+    const uint64_t* sqL = iiQ + g.lv.iiOff;
     const int nw = winW - 2, nh = winH - 2, narea = nw * nh;
     const int64_t np0 = 1 + iw, np1 = (1 + nw) + iw;
     const int64_t np2 = 1 + static_cast<int64_t>(iw) * (1 + nh), np3 = (1 + nw) + static_cast<int64_t>(iw) * (1 + nh);
     const int lane = lane_id();
-    const int nWin = TW * TH;
+    const int nWin = g.TW * g.TH;
     static_assert(kTileMaxWin <= 2 * kCascadeBlock, "two gate slots per thread");
     bool live[2];
     float inv[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int widx = static_cast<int>(threadIdx.x) + k * kCascadeBlock;
-        const int wx = wx0 + (widx & (TW - 1));
-        const int wy = wy0 + (widx >> lgc);
+        const int wx = g.wx0 + (widx & (g.TW - 1));
+        const int wy = g.wy0 + (widx >> g.lgc);
         live[k] = false;
         inv[k] = 0.0f;
-        if (widx < nWin && wx < lv.scanW && wy < lv.scanH) {
-            const int64_t o = static_cast<int64_t>(wy * step) * iw + wx * step;
+        if (widx < nWin && wx < g.lv.scanW && wy < g.lv.scanH) {
+            const int64_t o = static_cast<int64_t>(wy * g.step) * iw + wx * g.step;
             const int32_t ns = static_cast<int32_t>(sumL[o + np0] - sumL[o + np1] - sumL[o + np2] + sumL[o + np3]);
             const uint32_t nq = static_cast<uint32_t>(sqL[o + np0] - sqL[o + np1] - sqL[o + np2] + sqL[o + np3]);
             const double normSum = static_cast<double>(ns);
@@ -466,44 +439,52 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(const LevelDesc*
         const int widx = static_cast<int>(threadIdx.x) + k * kCascadeBlock;
         const uint64_t m = __ballot(live[k]);
         int wbase = 0;
-        if (lane == 0 && m) wbase = atomicAdd(&listCnt[0], __popcll(m));
+        if (lane == 0 && m) wbase = atomicAdd(&sm.listCnt[0], __popcll(m));
         wbase = __shfl(wbase, 0, kWave);
         if (live[k]) {
-            listIdx[wbase + mask_rank(m)] = static_cast<uint16_t>(widx);
-            invTab[widx] = inv[k];
+            sm.listIdx[wbase + mask_rank(m)] = static_cast<uint16_t>(widx);
+            sm.invTab[widx] = inv[k];
         }
     }
-    __syncthreads();
+}
 
-    // A window that passes the last stage sets its mask bit and stores its
-    // last stage sum; one that passes an earlier stage joins the next list.
+// Runs stages [si0, si1) over the live list sm.listIdx[0 .. sm.listCnt[0]).
+// A window that passes the cascade's last stage sets its bit in
+// sm.maskWords and stores its last stage sum; one that passes an earlier
+// stage joins the next list.  The list is rewritten in place: every stage
+// first reads the entries it needs into registers, then (after a barrier)
+// appends its survivors; the two counters alternate between "read" and
+// "append".  Returns the counter slot holding the survivors of stage si1-1.
+// Stage rule and window normaliser: src/main.cu:752-804.
+__device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, const KStump* __restrict__ stumps,
+                          const KStage* __restrict__ stages, int si0, int si1, int nStages, int winW,
+                          float* __restrict__ scores) {
+    const int lane = lane_id();
+    const int FW = (g.TW - 1) * g.step + winW + 1;
     auto settle = [&](bool pass, bool last, uint32_t widx, float stageSum, int nxt) {
         if (last) {
             if (pass) {
-                const int wx = wx0 + static_cast<int>(widx & (TW - 1));
-                const int wy = wy0 + static_cast<int>(widx >> lgc);
-                const int64_t word = lv.wordOff + static_cast<int64_t>(wy) * lv.rowWords + (wx >> 6);
+                const int wx = g.wx0 + static_cast<int>(widx & (g.TW - 1));
+                const int wy = g.wy0 + static_cast<int>(widx >> g.lgc);
+                const int64_t word = g.lv.wordOff + static_cast<int64_t>(wy) * g.lv.rowWords + (wx >> 6);
                 scores[word * 64 + (wx & 63)] = stageSum;
-                atomicOr(&maskWords[widx >> lgc], 1ull << (widx & (TW - 1)));
+                atomicOr(&sm.maskWords[widx >> g.lgc], 1ull << (widx & (g.TW - 1)));
             }
         } else {
             const uint64_t m = __ballot(pass);
             int wbase = 0;
-            if (lane == 0 && m) wbase = atomicAdd(&listCnt[nxt], __popcll(m));
+            if (lane == 0 && m) wbase = atomicAdd(&sm.listCnt[nxt], __popcll(m));
             wbase = __shfl(wbase, 0, kWave);
-            if (pass) listIdx[wbase + mask_rank(m)] = static_cast<uint16_t>(widx);
+            if (pass) sm.listIdx[wbase + mask_rank(m)] = static_cast<uint16_t>(widx);
         }
     };
     auto origin = [&](uint32_t widx) {
-        return lds + static_cast<int>(widx >> lgc) * step * FW + static_cast<int>(widx & (TW - 1)) * step;
+        return lds + static_cast<int>(widx >> g.lgc) * g.step * FW + static_cast<int>(widx & (g.TW - 1)) * g.step;
     };
 
-    // The live list is rewritten in place: every stage first reads the
-    // entries it needs into registers, then (after a barrier) appends its
-    // survivors; the two counters alternate between "read" and "append".
     int cur = 0;
-    for (int si = 0; si < nStages; ++si) {
-        const int n = listCnt[cur];
+    for (int si = si0; si < si1; ++si) {
+        const int n = sm.listCnt[cur];
         if (n == 0) break;
         const int nxt = cur ^ 1;
         const KStage st = stages[si];
@@ -516,9 +497,9 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(const LevelDesc*
             // windows), stumps in order (the stump is uniform across the
             // wave, so its fields are scalar loads); two stumps per step so
             // their loads overlap, the two leaf values still added in order.
-            const uint32_t e0 = listIdx[tid];
-            const uint32_t e1 = (tid + kCascadeBlock < n) ? listIdx[tid + kCascadeBlock] : 0u;
-            if (tid == 0) listCnt[nxt] = 0;
+            const uint32_t e0 = sm.listIdx[tid];
+            const uint32_t e1 = (tid + kCascadeBlock < n) ? sm.listIdx[tid + kCascadeBlock] : 0u;
+            if (tid == 0) sm.listCnt[nxt] = 0;
             __syncthreads();
             for (int r = 0; r < 2; ++r) {
                 if (r * kCascadeBlock >= n) break;
@@ -527,7 +508,7 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(const LevelDesc*
                 bool pass = false;
                 float stageSum = 0.0f;
                 if (i < n) {
-                    const float invNorm = invTab[widx];
+                    const float invNorm = sm.invTab[widx];
                     const float* o = origin(widx);
                     int k = 0;
                     for (; k + 1 < nt; k += 2) {
@@ -564,13 +545,13 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(const LevelDesc*
             const int w = tid & (nwp - 1);
             const int js0 = tid >> lgw;
             const bool evalLive = w < n;
-            const uint32_t ew = evalLive ? listIdx[w] : 0u;
+            const uint32_t ew = evalLive ? sm.listIdx[w] : 0u;
             const bool mine = tid < n;
-            const uint32_t myIdx = mine ? listIdx[tid] : 0u;
-            if (tid == 0) listCnt[nxt] = 0;
+            const uint32_t myIdx = mine ? sm.listIdx[tid] : 0u;
+            if (tid == 0) sm.listCnt[nxt] = 0;
             __syncthreads();
             const float* o = origin(ew);
-            const float inv = invTab[ew];
+            const float inv = sm.invTab[ew];
             float stageSum = 0.0f;
             const int nChunks = (nt + kVoteChunk - 1) / kVoteChunk;
             auto run = [&](auto uniformTag) {
@@ -591,8 +572,8 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(const LevelDesc*
                         const int cb = c * kVoteChunk;
                         const int tc = min(kVoteChunk, nt - cb);
                         if (tid < tc) {
-                            leafL[b][tid] = sg[cb + tid].left;
-                            leafR[b][tid] = sg[cb + tid].right;
+                            sm.leafL[b][tid] = sg[cb + tid].left;
+                            sm.leafR[b][tid] = sg[cb + tid].right;
                         }
                         for (int jb = 0; jb < tc; jb += spp) {
                             const int js = jb + js0;
@@ -604,7 +585,7 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(const LevelDesc*
                             if (js < tc && (w & 31) == 0) {
                                 uint32_t bits = static_cast<uint32_t>(m >> lane);
                                 if (nwp < 32) bits &= (1u << nwp) - 1u;
-                                votes[b][js][w >> 5] = bits;
+                                sm.votes[b][js][w >> 5] = bits;
                             }
                         }
                     }
@@ -614,7 +595,7 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(const LevelDesc*
                         const int word = tid >> 5, sh = tid & 31;
 #pragma unroll 8
                         for (int k = 0; k < ptc; ++k)
-                            stageSum += ((votes[pb][k][word] >> sh) & 1u) ? leafL[pb][k] : leafR[pb][k];
+                            stageSum += ((sm.votes[pb][k][word] >> sh) & 1u) ? sm.leafL[pb][k] : sm.leafR[pb][k];
                     }
                     __syncthreads();
                 }
@@ -627,9 +608,182 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(const LevelDesc*
         __syncthreads();
         cur = nxt;
     }
+    return cur;
+}
 
-    for (int r = threadIdx.x; r < TH; r += kCascadeBlock)
-        if (wy0 + r < lv.scanH) write_mask(r, maskWords[r]);
+// Cascade evaluation, pass 1: one workgroup of kCascadeBlock threads per tile
+// of 2^lgc x TH windows of one level.  The tile's integral footprint is
+// staged in LDS as f32 (each absolute u32 value converted once: the CPU path
+// converts the same absolute values, so every Haar sum rounds identically).
+// Windows failing the variance gate never enter the live list; each stage
+// then runs over the live list only and the survivors are compacted by
+// wavefront ballot, so lanes stay busy however unevenly windows reject.
+// Stages [0, stageSplit) run here.  When that is the whole cascade, the
+// survivors' mask bits are written here; otherwise the tile's mask pieces
+// are written as zero and its survivors (tile-local index and normaliser)
+// are appended to a global list for the deep pass.
+__global__ __launch_bounds__(kCascadeBlock) __attribute__((amdgpu_waves_per_eu(8))) void cascade_kernel(
+    const LevelDesc* __restrict__ L, const int32_t* __restrict__ lvlIdx, const int32_t* __restrict__ tileOffs,
+    int nLv, int nTiles, const uint32_t* __restrict__ iiS, const uint64_t* __restrict__ iiQ,
+    const KStump* __restrict__ stumpsAll, int nStumps, const KStage* __restrict__ stages, int nStages,
+    int stageSplit, int winW, int winH, uint64_t* __restrict__ words, float* __restrict__ scores,
+    int32_t* __restrict__ survTotal, uint16_t* __restrict__ survIdx, float* __restrict__ survInv,
+    int32_t* __restrict__ tileSurvOff, int32_t* __restrict__ tileCount) {
+    extern __shared__ float lds[];
+    __shared__ CascadeSmem sm;
+    __shared__ int survBase;
+
+    // XCD-aware remap: consecutive tiles land on the same XCD (shared L2).
+    int t;
+    {
+        const int i = blockIdx.x, q = nTiles >> 3, r = nTiles & 7;
+        const int xcd = i & 7, pos = i >> 3;
+        t = (xcd < r) ? xcd * (q + 1) + pos : r * (q + 1) + (xcd - r) * q + pos;
+    }
+    const TileGeo g = tile_geo(L, lvlIdx, tileOffs, nLv, t);
+    const bool deep = stageSplit < nStages;
+
+    // Tile row r is the TW-bit piece of mask word (wy0 + r, wx0 / 64) that
+    // starts at bit wx0 % 64 (little-endian: a 32- or 16-bit store).  Every
+    // piece of every word is written by exactly one tile, so the mask needs
+    // no clearing.
+    auto write_mask = [&](int r, uint64_t bits) {
+        const int64_t word = g.lv.wordOff + static_cast<int64_t>(g.wy0 + r) * g.lv.rowWords + (g.wx0 >> 6);
+        if (g.lgc == 6) {
+            words[word] = bits;
+        } else if (g.lgc == 5) {
+            reinterpret_cast<uint32_t*>(words)[word * 2 + ((g.wx0 & 63) >> 5)] = static_cast<uint32_t>(bits);
+        } else {
+            reinterpret_cast<uint16_t*>(words)[word * 4 + ((g.wx0 & 63) >> 4)] = static_cast<uint16_t>(bits);
+        }
+    };
+    if (g.wx0 >= g.lv.scanW) {  // padding tile past the scan width: zero pieces
+        for (int r = threadIdx.x; r < g.TH; r += kCascadeBlock)
+            if (g.wy0 + r < g.lv.scanH) write_mask(r, 0ull);
+        if (deep && threadIdx.x == 0) tileCount[t] = 0;
+        return;
+    }
+
+    stage_footprint(g, iiS, lds, winW, winH);
+    if (threadIdx.x < 32) sm.maskWords[threadIdx.x] = 0ull;
+    if (threadIdx.x == 0) {
+        sm.listCnt[0] = 0;
+        sm.listCnt[1] = 0;
+    }
+    __syncthreads();
+    gate_windows(g, iiS, iiQ, sm, winW, winH);
+    __syncthreads();
+    const KStump* __restrict__ stumps = stumpsAll + static_cast<size_t>(g.step - 1) * nStumps;
+    const int slot = run_stages(g, sm, lds, stumps, stages, 0, min(stageSplit, nStages), nStages, winW, scores);
+
+    for (int r = threadIdx.x; r < g.TH; r += kCascadeBlock)
+        if (g.wy0 + r < g.lv.scanH) write_mask(r, deep ? 0ull : sm.maskWords[r]);
+    if (deep) {
+        const int n = sm.listCnt[slot];
+        if (threadIdx.x == 0) {
+            survBase = n ? atomicAdd(survTotal, n) : 0;
+            tileSurvOff[t] = survBase;
+            tileCount[t] = n;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += kCascadeBlock) {
+            const uint32_t widx = sm.listIdx[i];
+            survIdx[survBase + i] = static_cast<uint16_t>(widx);
+            survInv[survBase + i] = sm.invTab[widx];
+        }
+    }
+}
+
+__global__ void reset_counter_kernel(int32_t* __restrict__ c) {
+    if (threadIdx.x == 0) c[0] = 0;
+}
+
+// Deep-pass work list: every tile with survivors after the first pass
+// contributes ceil(count / kDeepSlice) items (tile * 8 + slice), ordered by
+// survivor count, largest first, so the longest items start first.  One
+// workgroup; also resets the deep pass's work counter.
+__global__ __launch_bounds__(1024) void deep_list_kernel(const int32_t* __restrict__ tileCount, int nTiles,
+                                                         int32_t* __restrict__ items, int32_t* __restrict__ nItems,
+                                                         int32_t* __restrict__ workCounter) {
+    __shared__ uint32_t hist[kTileMaxWin + 1];
+    __shared__ uint32_t base[kTileMaxWin + 1];
+    for (int c = threadIdx.x; c <= kTileMaxWin; c += blockDim.x) hist[c] = 0u;
+    __syncthreads();
+    for (int t = threadIdx.x; t < nTiles; t += blockDim.x) {
+        const int c = tileCount[t];
+        if (c > 0) atomicAdd(&hist[c], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int c = kTileMaxWin; c >= 1; --c) {
+            base[c] = run;
+            run += hist[c] * static_cast<uint32_t>((c + kDeepSlice - 1) / kDeepSlice);
+            hist[c] = 0u;
+        }
+        *nItems = static_cast<int32_t>(run);
+        *workCounter = 0;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nTiles; t += blockDim.x) {
+        const int c = tileCount[t];
+        if (c > 0) {
+            const int ns = (c + kDeepSlice - 1) / kDeepSlice;
+            const uint32_t pos = base[c] + atomicAdd(&hist[c], 1u) * static_cast<uint32_t>(ns);
+            for (int s = 0; s < ns; ++s) items[pos + s] = t * 8 + s;
+        }
+    }
+}
+
+// Cascade evaluation, deep pass: persistent workgroups take work items
+// (a slice of up to kDeepSlice first-pass survivors of one tile) from a
+// counter until none are left, restage the tile's footprint, run stages
+// [stageSplit, nStages) over the slice and OR the final bits into the mask
+// words (several slices of a tile share words).
+__global__ __launch_bounds__(kCascadeBlock) __attribute__((amdgpu_waves_per_eu(8))) void cascade_deep_kernel(
+    const LevelDesc* __restrict__ L, const int32_t* __restrict__ lvlIdx, const int32_t* __restrict__ tileOffs,
+    int nLv, const uint32_t* __restrict__ iiS, const KStump* __restrict__ stumpsAll, int nStumps,
+    const KStage* __restrict__ stages, int nStages, int stageSplit, int winW, int winH,
+    unsigned long long* __restrict__ words, float* __restrict__ scores, const uint16_t* __restrict__ survIdx,
+    const float* __restrict__ survInv, const int32_t* __restrict__ tileSurvOff,
+    const int32_t* __restrict__ tileCount, const int32_t* __restrict__ items, const int32_t* __restrict__ nItems,
+    int32_t* __restrict__ workCounter) {
+    extern __shared__ float lds[];
+    __shared__ CascadeSmem sm;
+    const int total = *nItems;
+    for (;;) {
+        if (threadIdx.x == 0) sm.item = atomicAdd(workCounter, 1);
+        __syncthreads();
+        const int item = sm.item;
+        if (item >= total) break;
+        const int t = items[item] >> 3, slice = items[item] & 7;
+        const TileGeo g = tile_geo(L, lvlIdx, tileOffs, nLv, t);
+        stage_footprint(g, iiS, lds, winW, winH);
+        const int first = slice * kDeepSlice;
+        const int m = min(kDeepSlice, tileCount[t] - first);
+        const int off = tileSurvOff[t] + first;
+        if (threadIdx.x < 32) sm.maskWords[threadIdx.x] = 0ull;
+        if (static_cast<int>(threadIdx.x) < m) {
+            const uint32_t widx = survIdx[off + threadIdx.x];
+            sm.listIdx[threadIdx.x] = static_cast<uint16_t>(widx);
+            sm.invTab[widx] = survInv[off + threadIdx.x];
+        }
+        if (threadIdx.x == 0) {
+            sm.listCnt[0] = m;
+            sm.listCnt[1] = 0;
+        }
+        __syncthreads();
+        const KStump* __restrict__ stumps = stumpsAll + static_cast<size_t>(g.step - 1) * nStumps;
+        run_stages(g, sm, lds, stumps, stages, stageSplit, nStages, nStages, winW, scores);
+        for (int r = threadIdx.x; r < g.TH; r += kCascadeBlock) {
+            const unsigned long long bits = sm.maskWords[r];
+            if (bits) {
+                const int64_t word = g.lv.wordOff + static_cast<int64_t>(g.wy0 + r) * g.lv.rowWords + (g.wx0 >> 6);
+                atomicOr(&words[word], bits << (g.wx0 & 63));
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // Output compaction in three steps over blocks of kWordBlock mask words
@@ -809,6 +963,11 @@ struct vj_engine {
     DevBuf<int64_t> dMeta64;
     DevBuf<ResizeJob> dJobs;
     DevBuf<int32_t> dCounts, dBlockCount, dBlockOff, dImgBase;
+    // two-pass cascade: first-pass survivors and the deep pass's work list
+    int stageSplit = 0;                        // stages run by the first pass
+    DevBuf<uint16_t> survIdx;
+    DevBuf<float> survInv;
+    DevBuf<int32_t> tileSurvOff, tileCount, deepItems, deepCtl;  // deepCtl: total, items, counter
     DevBuf<vj_detection> dOut;
     PinnedBuf<vj_detection> hOut;
     PinnedBuf<int32_t> hCounts;
@@ -1093,6 +1252,15 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     VJ_TRY(e->rowQ.ensure(static_cast<size_t>(P.rowElems)));
     VJ_TRY(e->words.ensure(static_cast<size_t>(P.nWords)));
     VJ_TRY(e->scores.ensure(static_cast<size_t>(P.nWords) * 64));
+    const bool deep = e->stageSplit < e->nStages && P.nTiles > 0;
+    if (deep) {
+        VJ_TRY(e->survIdx.ensure(static_cast<size_t>(std::max<int64_t>(P.windows, 1))));
+        VJ_TRY(e->survInv.ensure(static_cast<size_t>(std::max<int64_t>(P.windows, 1))));
+        VJ_TRY(e->tileSurvOff.ensure(static_cast<size_t>(P.nTiles)));
+        VJ_TRY(e->tileCount.ensure(static_cast<size_t>(P.nTiles)));
+        VJ_TRY(e->deepItems.ensure(static_cast<size_t>(P.nTiles) * 8));
+        VJ_TRY(e->deepCtl.ensure(4));
+    }
     const int nImages = static_cast<int>(P.imgWordOff.size()) - 1;
     VJ_TRY(e->dCounts.ensure(static_cast<size_t>(nImages)));
     VJ_TRY(e->dOut.ensure(static_cast<size_t>(cap) * nImages));
@@ -1160,6 +1328,7 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     for (int st = 1; st <= P.maxStep; ++st)
         ldsBytes = std::max(ldsBytes, cascade_footprint(st, e->tileRows[st], e->winW, e->winH) * sizeof(float));
     const int nBlocks = static_cast<int>(blockImg.size());
+    const int deepGrid = deep ? std::min(P.nTiles * 8, kDeepGrid) : 0;
 
     auto launch_all = [&](hipStream_t ls) -> int {
         if (e->timing) VJ_HIP(hipEventRecord(e->ev[0], ls));
@@ -1189,11 +1358,28 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
         //    largest window step present (its tile footprint is the widest)
         if (e->timing) VJ_HIP(hipEventRecord(e->ev[2], ls));
         if (P.nTiles > 0) {
-            hipLaunchKernelGGL(cascade_kernel, dim3(P.nTiles), dim3(kCascadeBlock), ldsBytes, ls, dL, m32 + oLvl, m32 + oTile,
-                               static_cast<int>(P.tileLvl.size()), P.nTiles, e->iiS.get(), e->iiQ.get(), e->kst.get(),
-                               e->nStumps, e->kstage.get(), e->nStages, e->winW, e->winH, e->words.get(),
-                               e->scores.get());
+            if (deep) {
+                hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, ls, e->deepCtl.get());
+                VJ_HIP(hipGetLastError());
+            }
+            hipLaunchKernelGGL(cascade_kernel, dim3(P.nTiles), dim3(kCascadeBlock), ldsBytes, ls, dL, m32 + oLvl,
+                               m32 + oTile, static_cast<int>(P.tileLvl.size()), P.nTiles, e->iiS.get(), e->iiQ.get(),
+                               e->kst.get(), e->nStumps, e->kstage.get(), e->nStages, e->stageSplit, e->winW, e->winH,
+                               e->words.get(), e->scores.get(), e->deepCtl.get(), e->survIdx.get(), e->survInv.get(),
+                               e->tileSurvOff.get(), e->tileCount.get());
             VJ_HIP(hipGetLastError());
+            if (deep) {
+                hipLaunchKernelGGL(deep_list_kernel, dim3(1), dim3(1024), 0, ls, e->tileCount.get(), P.nTiles,
+                                   e->deepItems.get(), e->deepCtl.get() + 1, e->deepCtl.get() + 2);
+                VJ_HIP(hipGetLastError());
+                hipLaunchKernelGGL(cascade_deep_kernel, dim3(deepGrid), dim3(kCascadeBlock), ldsBytes, ls, dL,
+                                   m32 + oLvl, m32 + oTile, static_cast<int>(P.tileLvl.size()), e->iiS.get(),
+                                   e->kst.get(), e->nStumps, e->kstage.get(), e->nStages, e->stageSplit, e->winW,
+                                   e->winH, reinterpret_cast<unsigned long long*>(e->words.get()), e->scores.get(),
+                                   e->survIdx.get(), e->survInv.get(), e->tileSurvOff.get(), e->tileCount.get(),
+                                   e->deepItems.get(), e->deepCtl.get() + 1, e->deepCtl.get() + 2);
+                VJ_HIP(hipGetLastError());
+            }
         }
         if (e->timing) VJ_HIP(hipEventRecord(e->ev[3], ls));
         // 4. counts and order-preserving emission
@@ -1226,7 +1412,7 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
         k(P.resizeItems[dpt]);
         k(static_cast<int64_t>(jobStart[dpt]));
     }
-    for (int64_t v : {int64_t(nLevels), int64_t(P.nItems), int64_t(P.nChunkItems), int64_t(P.nTiles),
+    for (int64_t v : {int64_t(e->stageSplit), int64_t(deepGrid), int64_t(nLevels), int64_t(P.nItems), int64_t(P.nChunkItems), int64_t(P.nTiles),
                       int64_t(P.tileLvl.size()), int64_t(ldsBytes), int64_t(nBlocks), int64_t(nImages), int64_t(cap),
                       int64_t(e->nStumps), int64_t(e->nStages), int64_t(e->winW), int64_t(e->winH),
                       int64_t(oSrc), int64_t(oItem), int64_t(oChunk), int64_t(oLvl), int64_t(oTile), int64_t(oLvWord),
@@ -1242,7 +1428,10 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
                             static_cast<const void*>(e->words.get()), static_cast<const void*>(e->scores.get()),
                             static_cast<const void*>(e->dBlockCount.get()), static_cast<const void*>(e->dBlockOff.get()),
                             static_cast<const void*>(e->dCounts.get()), static_cast<const void*>(e->dImgBase.get()),
-                            static_cast<const void*>(e->dOut.get())})
+                            static_cast<const void*>(e->dOut.get()), static_cast<const void*>(e->survIdx.get()),
+                            static_cast<const void*>(e->survInv.get()), static_cast<const void*>(e->tileSurvOff.get()),
+                            static_cast<const void*>(e->tileCount.get()), static_cast<const void*>(e->deepItems.get()),
+                            static_cast<const void*>(e->deepCtl.get())})
         k(static_cast<int64_t>(reinterpret_cast<uintptr_t>(ptr)));
     if (e->graphExec && key == e->graphKey) {
         VJ_HIP(hipGraphLaunch(e->graphExec, s));
@@ -1470,6 +1659,14 @@ extern "C" int vj_engine_set_cascade(vj_engine* e, const vj_feature* dFeatures, 
     e->nModelFeat = nF;
     e->nStumps = nS;
     e->nStages = nSt;
+    // The first pass runs the leading stages whose stumps add up to at most
+    // kSplitStumps (at least one stage); the rest go to the deep pass.
+    {
+        int64_t cum = 0;
+        int k = 0;
+        while (k < nSt && (k == 0 || cum + hStages[k].ntrees <= kSplitStumps)) cum += hStages[k++].ntrees;
+        e->stageSplit = k;
+    }
     return VJ_OK;
 }
 
@@ -1687,7 +1884,8 @@ extern "C" int vj_engine_detect_footprint(vj_engine* e, int w, int h, float sf, 
         VJ_TRY(build_plan(e, imgs, sf, minObj, P));
         const int64_t cap = std::max(1, std::min(maxDet, e->maxDet));
         *bytes = static_cast<int64_t>(w) * h + P.pyrBytes + 12 * P.iiElems + 8 * P.colElems + 12 * P.totElems +
-                 8 * P.rowElems + 8 * P.nWords + 4 * 64 * P.nWords + 24 * cap +
+                 8 * P.rowElems + 8 * P.nWords + 4 * 64 * P.nWords + 24 * cap + 6 * P.windows +
+                 44 * static_cast<int64_t>(P.nTiles) +
                  static_cast<int64_t>(P.levels.size()) * static_cast<int64_t>(sizeof(LevelDesc) + 64);
         return VJ_OK;
     } catch (const std::exception& ex) {

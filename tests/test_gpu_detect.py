@@ -199,3 +199,31 @@ def test_graph_replays_and_autotune_match_eager(cascade3):
         assert [det_tuples(x) for x in got] == want, rep
     say(f"  tuned tile rows {tuned.tile_rows()}")
     assert tuned.tile_rows() in ([8, 4, 4], [4, 4, 4], [8, 8, 8], [8, 8, 4], [4, 8, 8])
+
+
+@pytest.fixture(scope="module")
+def cascade7():
+    return synth.blob_cascade((1, 10, 25, 25, 50, 40, 40), seed=11)
+
+
+def test_two_pass_cascade_matches_oracle(cascade7):
+    """A 7-stage / 191-stump cascade: the first pass runs the leading stages
+    (111 stumps), the deep pass the last two over slices of the survivors;
+    raw and grouped boxes equal the oracle's, per image and in a batch."""
+    say("two-pass cascade 7 stages vs oracle")
+    sizes = [(640, 480), (333, 250), (1000, 431)]
+    imgs = [synth.detect_image(w, h, max(2, w * h // 12000), 24, 96, seed=500 + i) for i, (w, h) in enumerate(sizes)]
+    eng = _engine(1000, 480, cascade7)
+    dev = [DeviceArray.from_host(im) for im in imgs]
+    wants = []
+    for im, d, (w, h) in zip(imgs, dev, sizes):
+        want, _ = O.detect_raw(*cascade7, im)
+        assert len(want) > 0
+        wants.append(want)
+        raw = eng.detectMultiScale(d, w, h, w, 1.25, 4, 24, 200000, applyGrouping=False)
+        assert det_tuples(raw) == det_tuples(want)
+        grouped = eng.detectMultiScale(d, w, h, w, 1.25, 4, 24, 200000, applyGrouping=True)
+        assert det_tuples(grouped) == det_tuples(O.group(want, 4))
+    got = eng.detect_batch(dev, [(w, h, w) for w, h in sizes], 1.25, 4, 24, 200000, False)
+    for g, want in zip(got, wants):
+        assert det_tuples(g) == det_tuples(want)
