@@ -531,8 +531,9 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
             // padded to nwp = 2^lgw slots and each pass evaluates spp =
             // 256 / nwp stumps on every slot, windows fastest, so a lane
             // keeps one window for the whole stage and a wave shares one
-            // stump (scalar loads) whenever nwp >= 64.  The next pass's
-            // stump record is fetched while the current one is evaluated.
+            // stump (scalar loads) whenever nwp >= 64; below that, the next
+            // pass's per-lane stump record is fetched while the current one
+            // is evaluated.
             // Each pass's window bits go by ballot into per-stump vote words
             // of a chunk of kVoteChunk stumps; the window's own lane then
             // adds the chunk's leaf values in stump order (one chunk behind
@@ -558,14 +559,9 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
                 constexpr bool kUniform = decltype(uniformTag)::value;
                 KStump nx{};
                 auto prefetch = [&](int js) {
-                    if constexpr (kUniform) {
-                        const int ju = __builtin_amdgcn_readfirstlane(js);
-                        if (ju < nt) nx = sg[ju];
-                    } else {
-                        if (js < nt) nx = sg[js];
-                    }
+                    if (js < nt) nx = sg[js];
                 };
-                prefetch(js0);
+                if constexpr (!kUniform) prefetch(js0);
                 for (int c = 0; c <= nChunks; ++c) {
                     const int b = c & 1;
                     if (c < nChunks) {
@@ -577,10 +573,21 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
                         }
                         for (int jb = 0; jb < tc; jb += spp) {
                             const int js = jb + js0;
-                            const KStump sp = nx;
-                            prefetch(cb + js + spp);
                             bool bit = false;
-                            if (evalLive && js < tc) bit = haar_raw(o, sp) * inv < sp.theta;
+                            if constexpr (kUniform) {
+                                // the record is a scalar load used at once: a
+                                // scalar load still in flight would make every
+                                // wait on the LDS gathers below wait for it too
+                                const int ju = __builtin_amdgcn_readfirstlane(cb + js);
+                                if (evalLive && js < tc) {
+                                    const KStump sp = sg[ju];
+                                    bit = haar_raw(o, sp) * inv < sp.theta;
+                                }
+                            } else {
+                                const KStump sp = nx;
+                                prefetch(cb + js + spp);
+                                if (evalLive && js < tc) bit = haar_raw(o, sp) * inv < sp.theta;
+                            }
                             const uint64_t m = __ballot(bit);
                             if (js < tc && (w & 31) == 0) {
                                 uint32_t bits = static_cast<uint32_t>(m >> lane);
