@@ -313,13 +313,15 @@ __global__ __launch_bounds__(256) void invnorm_kernel(const uint32_t* __restrict
     if (s < n) inv[s] = train_invnorm(sumT, sqT, iw, ih, n, s);
 }
 
-// Weighted rectangle sum of one stump's feature at a window origin in LDS,
+// Weighted rectangle sum of one stump's feature at a window origin in LDS
+// (a byte pointer; the corner offsets are in bytes),
 // in the CPU path's float order: ((p0 - p1) - p2) + p3 per rectangle, times
 // its weight, summed r0 + r1 (+ r2) (src/main.cu:522-542).
-__device__ __forceinline__ float haar_raw(const float* o, const KStump& sp) {
-    float raw = (((o[sp.off[0]] - o[sp.off[1]]) - o[sp.off[2]]) + o[sp.off[3]]) * sp.w[0] +
-                (((o[sp.off[4]] - o[sp.off[5]]) - o[sp.off[6]]) + o[sp.off[7]]) * sp.w[1];
-    if (sp.rectCount == 3) raw += (((o[sp.off[8]] - o[sp.off[9]]) - o[sp.off[10]]) + o[sp.off[11]]) * sp.w[2];
+__device__ __forceinline__ float haar_raw(const char* ob, const KStump& sp) {
+    auto at = [ob](uint32_t off) { return *reinterpret_cast<const float*>(ob + off); };
+    float raw = (((at(sp.off[0]) - at(sp.off[1])) - at(sp.off[2])) + at(sp.off[3])) * sp.w[0] +
+                (((at(sp.off[4]) - at(sp.off[5])) - at(sp.off[6])) + at(sp.off[7])) * sp.w[1];
+    if (sp.rectCount == 3) raw += (((at(sp.off[8]) - at(sp.off[9])) - at(sp.off[10])) + at(sp.off[11])) * sp.w[2];
     return raw;
 }
 
@@ -479,7 +481,8 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
         }
     };
     auto origin = [&](uint32_t widx) {
-        return lds + static_cast<int>(widx >> g.lgc) * g.step * FW + static_cast<int>(widx & (g.TW - 1)) * g.step;
+        return reinterpret_cast<const char*>(lds + static_cast<int>(widx >> g.lgc) * g.step * FW +
+                                             static_cast<int>(widx & (g.TW - 1)) * g.step);
     };
 
     int cur = 0;
@@ -509,7 +512,7 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
                 float stageSum = 0.0f;
                 if (i < n) {
                     const float invNorm = sm.invTab[widx];
-                    const float* o = origin(widx);
+                    const char* o = origin(widx);
                     int k = 0;
                     for (; k + 1 < nt; k += 2) {
                         const KStump s0 = sg[k];
@@ -551,7 +554,7 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
             const uint32_t myIdx = mine ? sm.listIdx[tid] : 0u;
             if (tid == 0) sm.listCnt[nxt] = 0;
             __syncthreads();
-            const float* o = origin(ew);
+            const char* o = origin(ew);
             const float inv = sm.invTab[ew];
             float stageSum = 0.0f;
             const int nChunks = (nt + kVoteChunk - 1) / kVoteChunk;
@@ -1644,7 +1647,7 @@ extern "C" int vj_engine_set_cascade(vj_engine* e, const vj_feature* dFeatures, 
                     if (r >= f.rectCount) v = 0;
                     VJ_CHECK(v >= 0 && v < area, "setCascadeModel: feature corner outside the window");
                     const int dy = v / mW, dx = v - dy * mW;
-                    o.off[r * 4 + c] = static_cast<uint16_t>(dy * pitch + dx);
+                    o.off[r * 4 + c] = static_cast<uint16_t>((dy * pitch + dx) * 4);  // LDS byte offset
                 }
                 o.w[r] = (r < f.rectCount) ? f.r[r].w : 0.0f;
             }

@@ -57,7 +57,7 @@ struct LevelDesc {
 // Cascade stump with its corner offsets decoded for one LDS tile pitch.
 // 64 bytes so a wavefront reads it with one scalar load.
 struct alignas(16) KStump {
-    uint16_t off[12];     // r0.p0..p3, r1.p0..p3, r2.p0..p3 as dy*pitch + dx
+    uint16_t off[12];     // r0.p0..p3, r1.p0..p3, r2.p0..p3 as LDS byte offsets 4 * (dy*pitch + dx)
     float w[3];
     float theta, left, right;
     int32_t rectCount;
