@@ -632,7 +632,7 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
 // survivors' mask bits are written here; otherwise the tile's mask pieces
 // are written as zero and its survivors (tile-local index and normaliser)
 // are appended to a global list for the deep pass.
-__global__ __launch_bounds__(kCascadeBlock) __attribute__((amdgpu_waves_per_eu(8))) void cascade_kernel(
+__global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(
     const LevelDesc* __restrict__ L, const int32_t* __restrict__ lvlIdx, const int32_t* __restrict__ tileOffs,
     int nLv, int nTiles, const uint32_t* __restrict__ iiS, const uint64_t* __restrict__ iiQ,
     const KStump* __restrict__ stumpsAll, int nStumps, const KStage* __restrict__ stages, int nStages,
@@ -750,7 +750,7 @@ __global__ __launch_bounds__(1024) void deep_list_kernel(const int32_t* __restri
 // counter until none are left, restage the tile's footprint, run stages
 // [stageSplit, nStages) over the slice and OR the final bits into the mask
 // words (several slices of a tile share words).
-__global__ __launch_bounds__(kCascadeBlock) __attribute__((amdgpu_waves_per_eu(8))) void cascade_deep_kernel(
+__global__ __launch_bounds__(kCascadeBlock) void cascade_deep_kernel(
     const LevelDesc* __restrict__ L, const int32_t* __restrict__ lvlIdx, const int32_t* __restrict__ tileOffs,
     int nLv, const uint32_t* __restrict__ iiS, const KStump* __restrict__ stumpsAll, int nStumps,
     const KStage* __restrict__ stages, int nStages, int stageSplit, int winW, int winH,
