@@ -2,7 +2,12 @@
 // the Haar feature pool, the .bin model format, rectangle grouping and
 // PGM/PPM I/O.  None of this runs on the GPU in the reference either.
 #include <algorithm>
+#include <atomic>
 #include <cctype>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -273,6 +278,83 @@ struct UnionFind {
     }
 };
 
+// A few persistent host threads for data-parallel host loops.  run() hands
+// tasks 0..n-1 out dynamically to the workers and the caller and returns
+// when all are done; a caller that finds the pool busy (another thread is
+// using it) runs its tasks itself.
+class HostPool {
+public:
+    static HostPool& get() {
+        static HostPool pool;
+        return pool;
+    }
+    void run(int nTasks, const std::function<void(int)>& fn) {
+        std::unique_lock<std::mutex> use(useMu_, std::try_to_lock);
+        if (!use.owns_lock() || workers_.empty() || nTasks <= 1) {
+            for (int t = 0; t < nTasks; ++t) fn(t);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &fn;
+            total_ = nTasks;
+            next_.store(0);
+            active_ = static_cast<int>(workers_.size());
+            ++gen_;
+        }
+        cv_.notify_all();
+        drain(fn);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return active_ == 0; });
+        job_ = nullptr;
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+            ++gen_;
+        }
+        cv_.notify_all();
+        for (std::thread& t : workers_) t.join();
+    }
+
+private:
+    HostPool() {
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int n = static_cast<int>(std::min(hw > 1 ? hw - 1 : 0u, 15u));
+        for (int k = 0; k < n; ++k) workers_.emplace_back([this] { loop(); });
+    }
+    void drain(const std::function<void(int)>& fn) {
+        for (int t = next_.fetch_add(1); t < total_; t = next_.fetch_add(1)) fn(t);
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            const std::function<void(int)>* job;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+                if (stop_) return;
+                job = job_;
+            }
+            if (job) drain(*job);
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (--active_ == 0) done_.notify_all();
+            }
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex useMu_, mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int)>* job_ = nullptr;
+    std::atomic<int> next_{0};
+    int total_ = 0, active_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
 std::vector<int> order_by_x(const vj_detection* d, int n) {
     std::vector<int> o(static_cast<size_t>(n));
     std::iota(o.begin(), o.end(), 0);
@@ -311,11 +393,14 @@ int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps,
     std::vector<int> dtab(static_cast<size_t>(2 * maxWH + 1));
     for (int m = 0; m <= 2 * maxWH; ++m)
         dtab[static_cast<size_t>(m)] = static_cast<int>(eps * static_cast<float>(m) * 0.5f);
-    for (int a = 0; a < n; ++a) {
+    // The similar pairs (a, b), a before b in sweep order, are found in
+    // blocks of sweep positions in parallel, then joined in exactly the
+    // sweep's order (a ascending, then b ascending), which is the sequence of
+    // unions the sequential sweep performs.
+    auto similar_after = [&](int a, std::vector<int>& hits) {
         const int ax = X[static_cast<size_t>(a)], ay = Y[static_cast<size_t>(a)];
         const int aw = W[static_cast<size_t>(a)], ah = H[static_cast<size_t>(a)];
         const int reach = static_cast<int>(eps * static_cast<float>(aw + ah) * 0.5f);
-        int ra = -1;  // root of a's set once a has been joined
         for (int b = a + 1; b < n; ++b) {
             const int bx = X[static_cast<size_t>(b)];
             if (bx - ax > reach) break;
@@ -326,7 +411,36 @@ int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps,
             // every edge within d of the other box's edge (one branch)
             const bool sim = (std::abs(ax - bx) <= d) & (std::abs(ay - by) <= d) &
                              (std::abs((ax + aw) - (bx + bw)) <= d) & (std::abs((ay + ah) - (by + bh)) <= d);
-            if (sim) ra = uf.join_root(ra < 0 ? uf.root(idx[static_cast<size_t>(a)]) : ra, idx[static_cast<size_t>(b)]);
+            if (sim) hits.push_back(b);
+        }
+    };
+    constexpr int kBlock = 32;
+    const int nBlocks = (n + kBlock - 1) / kBlock;
+    // per block: for each a of the block, its similar b's after a marker -1 - a
+    std::vector<std::vector<int>> edges(static_cast<size_t>(nBlocks));
+    auto find_block = [&](int blk) {
+        std::vector<int>& e = edges[static_cast<size_t>(blk)];
+        for (int a = blk * kBlock; a < std::min(n, (blk + 1) * kBlock); ++a) {
+            const size_t mark = e.size();
+            e.push_back(-1 - a);
+            similar_after(a, e);
+            if (e.size() == mark + 1) e.pop_back();
+        }
+    };
+    if (n >= 1024) {
+        HostPool::get().run(nBlocks, find_block);
+    } else {
+        for (int blk = 0; blk < nBlocks; ++blk) find_block(blk);
+    }
+    for (const std::vector<int>& e : edges) {
+        int ra = -1, a = -1;  // ra: root of a's set once a has been joined
+        for (int v : e) {
+            if (v < 0) {
+                a = -1 - v;
+                ra = -1;
+            } else {
+                ra = uf.join_root(ra < 0 ? uf.root(idx[static_cast<size_t>(a)]) : ra, idx[static_cast<size_t>(v)]);
+            }
         }
     }
     struct Acc {

@@ -121,3 +121,16 @@ def test_cli_usage_and_errors(tmp_path):
     assert r.returncode == 1 and "[ERR] loadCascadeModel failed" in r.stderr
     r = subprocess.run([exe, "train", str(tmp_path / "m.bin")], capture_output=True, text=True)
     assert r.returncode == 1 and "stageNegCacheDir is required" in r.stderr
+
+
+def test_grouping_large_input_equals_golden():
+    """4,488 raw boxes of the 1080p workload (the parallel pair search runs
+    above 1,024 boxes): grouped boxes equal the oracle's golden fixture."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", "detect_c3.npz"))
+    assert det_tuples(group_rectangles(g["raw"], 4)) == det_tuples(g["grouped"])
+    assert det_tuples(group_rectangles(g["raw"], 4)) == det_tuples(O.group(g["raw"], 4))
+    # a shuffled copy: the same set of boxes in another input order groups
+    # exactly as the oracle groups that order
+    rng = np.random.default_rng(4)
+    sh = g["raw"][rng.permutation(len(g["raw"]))]
+    assert det_tuples(group_rectangles(sh, 3)) == det_tuples(O.group(sh, 3))
