@@ -237,6 +237,39 @@ def measured_traffic(name):
     return None
 
 
+def cpu_stump_leg(args):
+    """The oracle's stump search (presort + exact sorted sweep, the
+    reference's runCpuWeakSearch semantics) on one thread over the first
+    2,048 features of the 162,336-feature pool and all 20,000 samples, timed
+    for one presort and one round and scaled to the full pool and 50 rounds
+    (extrapolated, as the reference's own compare mode samples features)."""
+    import numpy as np
+
+    from cs677s26_amd import workloads
+    from oracle import pyoracle as O
+
+    patches, labels = workloads.training()
+    O.set_threads(1)
+    sumT, _, inv = O.train_integrals(patches)
+    pool = O.haar_pool(24, 0)
+    sub = pool[:2048]
+    n = len(labels)
+    w = np.full(n, 1.0 / n)
+    sw = np.where(labels > 0, w, -w).astype(np.float32)
+    t0 = time.perf_counter()
+    ords = O.presort(sub, sumT, inv)
+    t1 = time.perf_counter()
+    best = O.search(ords, sw)
+    O.select(best)
+    t2 = time.perf_counter()
+    scale = len(pool) / len(sub)
+    per_stage = (t1 - t0) * scale + args.rounds * (t2 - t1) * scale
+    return {"value": per_stage, "unit": "s/stage (extrapolated)", "cores": 1, "kind": "port",
+            "sample": f"{len(sub)} of {len(pool)} features x {n} samples: one presort ({t1 - t0:.3f} s) and one "
+                      f"round ({t2 - t1:.3f} s), scaled to the full pool and {args.rounds} rounds",
+            "higher_is_better": False}
+
+
 def detect_fields(args):
     d = detect_leg(args)
     out = {
@@ -313,6 +346,8 @@ def main():
     if "cpu" not in skip:
         log("cpu baseline (oracle, 1 thread)")
         out["cpu_baseline"] = cpu_leg(args)
+        log("cpu baseline: stump search (oracle, 1 thread, 2,048-feature sample)")
+        out["cpu_baseline_stump"] = cpu_stump_leg(args)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -59,6 +59,10 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
+def set_threads(n):
+    lib().or_set_threads(int(n))
+
+
 def haar_pool(win=24, mode=0):
     n = lib().or_haar_pool(win, win, mode, None, 0)
     out = np.zeros(n, FEATURE)

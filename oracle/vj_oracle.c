@@ -14,6 +14,7 @@
 #include "vj_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -637,3 +638,5 @@ double or_exact_sum(const double* w, int n) {
     for (int i = 0; i < n; ++i) acc += (__int128)ldexp(w[i], 90);
     return ldexp((double)acc, -90);
 }
+
+void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }

@@ -118,6 +118,10 @@ int or_select(const or_feature_best* best, int fCount, const uint8_t* used, int6
 /* Exact sum of n doubles via 2^-90 fixed point (int128) rounded to double. */
 double or_exact_sum(const double* w, int n);
 
+/* Threads the OpenMP loops of or_presort / or_search use (the CPU baseline
+ * times them on one thread, as the reference's CPU path runs). */
+void or_set_threads(int n);
+
 #ifdef __cplusplus
 }
 #endif
