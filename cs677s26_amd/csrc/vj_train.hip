@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "vj_common.hpp"
@@ -332,17 +333,22 @@ __device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, i
     int64_t carry = 0;
     int64_t hiD = INT64_MIN, loD = INT64_MAX;
     int32_t hiK = INT_MAX, loK = INT_MAX;
-    for (int base = 0; base < n; base += kWave * kScanPer) {
+    // one step of 64 * kScanPer sorted entries; kTail adds the bounds checks
+    // that only the last, partial step needs (without them the eight LDS
+    // reads of a lane issue together)
+    auto step = [&](int base, const uint4 e8, auto tailTag) {
+        constexpr bool kTail = decltype(tailTag)::value;
         const int k0 = base + lane * kScanPer;
-        uint4 e8 = make_uint4(0u, 0u, 0u, 0u);
-        if (k0 < n) e8 = *reinterpret_cast<const uint4*>(row + k0);  // rows padded to 256 entries
         const uint32_t ew[4] = {e8.x, e8.y, e8.z, e8.w};
+        // all eight weight words are read before any is used
+        uint32_t pw[kScanPer];
+#pragma unroll
+        for (int j = 0; j < kScanPer; ++j) pw[j] = lsw[(ew[j >> 1] >> (16 * (j & 1))) & 0x7FFFu];
         int64_t q[kScanPer];
         int64_t c = 0;
 #pragma unroll
         for (int j = 0; j < kScanPer; ++j) {
-            const uint32_t e = (ew[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
-            q[j] = (k0 + j < n) ? unpack_weight(lsw[e & 0x7FFFu]) : 0;
+            q[j] = (!kTail || k0 + j < n) ? unpack_weight(pw[j]) : 0;
             c += q[j];
         }
         const int64_t inc = wave_incl_scan_dpp64(c);
@@ -351,7 +357,7 @@ __device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, i
         for (int j = 0; j < kScanPer; ++j) {
             D += q[j];
             const uint32_t e = (ew[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
-            if ((e & 0x8000u) && k0 + j < n) {
+            if ((e & 0x8000u) && (!kTail || k0 + j < n)) {
                 if (D > hiD) {
                     hiD = D;
                     hiK = k0 + j;
@@ -367,6 +373,24 @@ __device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, i
             (static_cast<uint64_t>(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(last >> 32), 63)))
              << 32) |
             static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(last), 63)));
+    };
+    // whole steps with the next step's entries loaded one step ahead, then
+    // the partial last step (rows are padded to 256 entries, so a lane's
+    // 16-byte load starting below n stays inside its row)
+    constexpr int kStep = kWave * kScanPer;
+    auto load8 = [&](int base) { return *reinterpret_cast<const uint4*>(row + base + lane * kScanPer); };
+    int base = 0;
+    if (n >= kStep) {
+        uint4 cur = load8(0);
+        for (; base + kStep <= n; base += kStep) {
+            const uint4 nxt = (base + 2 * kStep <= n) ? load8(base + kStep) : make_uint4(0u, 0u, 0u, 0u);
+            step(base, cur, std::false_type{});
+            cur = nxt;
+        }
+    }
+    if (base < n) {
+        const uint4 e8 = (base + lane * kScanPer < n) ? load8(base) : make_uint4(0u, 0u, 0u, 0u);
+        step(base, e8, std::true_type{});
     }
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) {
@@ -425,7 +449,7 @@ __device__ __forceinline__ void load_weights(const float* __restrict__ sw32, int
     }
 }
 
-__global__ __launch_bounds__(kSearchThreads) void search_kernel(const uint16_t* __restrict__ ord, int64_t pitch,
+__global__ __launch_bounds__(kSearchThreads) __attribute__((amdgpu_waves_per_eu(1, 4))) void search_kernel(const uint16_t* __restrict__ ord, int64_t pitch,
                                                                int n, int nFeatures,
                                                                const float* __restrict__ sw32,
                                                                const uint8_t* __restrict__ used,
