@@ -35,6 +35,7 @@ constexpr int kSortThreads = 1024;
 constexpr int kSortMaxPerThread = 20;              // N <= 20480
 constexpr int kSortMaxN = kSortThreads * kSortMaxPerThread;
 constexpr int kSearchThreads = 1024;               // 16 waves, one feature each at a time
+constexpr size_t kSearchWideLds = 160 * 1024 - 2048; // dynamic LDS limit of the search kernels
 
 struct RoundBest {
     int64_t errQ;
@@ -327,7 +328,15 @@ constexpr int kScanPer = 8;  // sorted entries per lane per step (one 16-byte lo
 
 // Exact threshold scan of one feature's sorted order by one wavefront; returns
 // the feature's best (error, split, parity) in every lane (split -1: none).
-__device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, int n, const uint32_t* lsw,
+// A search weight as the scan reads it from LDS: a packed word, or (when the
+// samples' int64 weights fit the workgroup's LDS) the int64 value itself.
+__device__ __forceinline__ int64_t weight_value(uint32_t w) { return unpack_weight(w); }
+__device__ __forceinline__ int64_t weight_value(int64_t w) { return w; }
+__device__ __forceinline__ void store_weight(uint32_t* lsw, int s, uint32_t packed) { lsw[s] = packed; }
+__device__ __forceinline__ void store_weight(int64_t* lsw, int s, uint32_t packed) { lsw[s] = unpack_weight(packed); }
+
+template <typename W>
+__device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, int n, const W* lsw,
                                              int64_t totPos, int64_t totNeg, int lane,
                                              int64_t& fE, int32_t& fK, int32_t& fP) {
     int64_t carry = 0;
@@ -341,14 +350,14 @@ __device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, i
         const int k0 = base + lane * kScanPer;
         const uint32_t ew[4] = {e8.x, e8.y, e8.z, e8.w};
         // all eight weight words are read before any is used
-        uint32_t pw[kScanPer];
+        W pw[kScanPer];
 #pragma unroll
         for (int j = 0; j < kScanPer; ++j) pw[j] = lsw[(ew[j >> 1] >> (16 * (j & 1))) & 0x7FFFu];
         int64_t q[kScanPer];
         int64_t c = 0;
 #pragma unroll
         for (int j = 0; j < kScanPer; ++j) {
-            q[j] = (!kTail || k0 + j < n) ? unpack_weight(pw[j]) : 0;
+            q[j] = (!kTail || k0 + j < n) ? weight_value(pw[j]) : 0;
             c += q[j];
         }
         const int64_t inc = wave_incl_scan_dpp64(c);
@@ -423,13 +432,14 @@ __device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, i
 }
 
 // Packs the search weights into LDS and returns the exact totals.
-__device__ __forceinline__ void load_weights(const float* __restrict__ sw32, int n, uint32_t* lsw, int64_t* totP,
+template <typename W>
+__device__ __forceinline__ void load_weights(const float* __restrict__ sw32, int n, W* lsw, int64_t* totP,
                                              int64_t* totN, int64_t& totPos, int64_t& totNeg) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int64_t tp = 0, tn = 0;
     for (int s = tid; s < n; s += blockDim.x) {
         const uint32_t pw = pack_weight(sw32[s]);
-        lsw[s] = pw;
+        store_weight(lsw, s, pw);
         const int64_t q = unpack_weight(pw);
         if (q > 0) tp += q;
         else tn -= q;
@@ -449,12 +459,14 @@ __device__ __forceinline__ void load_weights(const float* __restrict__ sw32, int
     }
 }
 
+template <typename W>
 __global__ __launch_bounds__(kSearchThreads) __attribute__((amdgpu_waves_per_eu(1, 4))) void search_kernel(const uint16_t* __restrict__ ord, int64_t pitch,
                                                                int n, int nFeatures,
                                                                const float* __restrict__ sw32,
                                                                const uint8_t* __restrict__ used,
                                                                WaveBest* __restrict__ blockBest) {
-    extern __shared__ uint32_t lsw[];  // [n] packed search weights
+    extern __shared__ int64_t lsmem[];
+    W* lsw = reinterpret_cast<W*>(lsmem);  // [n] search weights
     __shared__ int64_t totP[16], totN[16];
     __shared__ WaveBest wb[16];
     const int tid = threadIdx.x;
@@ -980,7 +992,9 @@ int set_lds_limits() {
     if (done) return VJ_OK;
     VJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(presort_kernel),
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024));
-    VJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(search_kernel),
+    VJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(search_kernel<uint32_t>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 2048));
+    VJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(search_kernel<int64_t>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 2048));
     VJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(search_each_kernel),
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 2048));
@@ -1179,8 +1193,16 @@ extern "C" int vj_trainer_round(vj_trainer* t, const vj_round_config* cfg, vj_we
     std::memset(out, 0, sizeof *out);
     out->feature = -1;
     const int blocks = std::min((t->nFeat + 15) / 16, t->searchBlocks);
-    hipLaunchKernelGGL(search_kernel, dim3(blocks), dim3(kSearchThreads), static_cast<size_t>(t->n) * 4, s,
-                       t->ord.get(), t->pitch, t->n, t->nFeat, t->sw32.get(), t->used.get(), t->blockBest.get());
+    // int64 weights in LDS when they fit beside the kernel's static LDS
+    if (static_cast<size_t>(t->n) * 8 <= kSearchWideLds) {
+        hipLaunchKernelGGL(search_kernel<int64_t>, dim3(blocks), dim3(kSearchThreads), static_cast<size_t>(t->n) * 8,
+                           s, t->ord.get(), t->pitch, t->n, t->nFeat, t->sw32.get(), t->used.get(),
+                           t->blockBest.get());
+    } else {
+        hipLaunchKernelGGL(search_kernel<uint32_t>, dim3(blocks), dim3(kSearchThreads), static_cast<size_t>(t->n) * 4,
+                           s, t->ord.get(), t->pitch, t->n, t->nFeat, t->sw32.get(), t->used.get(),
+                           t->blockBest.get());
+    }
     VJ_HIP(hipGetLastError());
     hipLaunchKernelGGL(reduce_best_kernel, dim3(1), dim3(256), 0, s, t->blockBest.get(), blocks, t->ord.get(), t->pitch,
                        t->feats, t->sumT.get(), t->inv.get(), t->n, t->roundBest.get());
