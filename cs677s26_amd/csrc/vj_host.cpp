@@ -253,9 +253,10 @@ namespace vj {
 
 namespace {
 
+// Union-find over caller-owned arrays (union by rank, path compression).
 struct UnionFind {
-    std::vector<int> parent, rank;
-    explicit UnionFind(int n) : parent(n), rank(n, 0) { std::iota(parent.begin(), parent.end(), 0); }
+    int* parent;
+    int* rank;
     int root(int x) {
         int r = x;
         while (parent[r] != r) r = parent[r];
@@ -355,11 +356,46 @@ private:
     bool stop_ = false;
 };
 
-std::vector<int> order_by_x(const vj_detection* d, int n) {
-    std::vector<int> o(static_cast<size_t>(n));
-    std::iota(o.begin(), o.end(), 0);
-    std::sort(o.begin(), o.end(), [d](int a, int b) { return d[a].x != d[b].x ? d[a].x < d[b].x : a < b; });
-    return o;
+struct ClusterAcc {
+    int64_t x, y, w, h;
+    int count;
+    float best;
+    int scaleIdx;
+};
+
+// Working memory of one thread's group_rectangles calls, kept between calls
+// so that a call allocates only when its input outgrows every earlier one.
+struct GroupScratch {
+    std::vector<int> parent, rank, idx, X, Y, W, H, cnt, members, corder;
+    std::vector<uint64_t> key;
+    std::vector<std::vector<int>> edges;
+    std::vector<ClusterAcc> acc;
+    std::vector<vj_detection> clusters;
+};
+
+// Positions 0..n-1 of boxes in (x, index) order into idx: a stable counting
+// sort over x when the x range is moderate, else a sort of (x, index) keys.
+void sweep_order(const vj_detection* d, int n, GroupScratch& g) {
+    g.idx.resize(static_cast<size_t>(n));
+    int lo = d[0].x, hi = d[0].x;
+    for (int i = 1; i < n; ++i) {
+        lo = std::min(lo, d[i].x);
+        hi = std::max(hi, d[i].x);
+    }
+    const int64_t range = static_cast<int64_t>(hi) - lo + 1;
+    if (range <= 4ll * n + 65536) {
+        g.cnt.assign(static_cast<size_t>(range) + 1, 0);
+        for (int i = 0; i < n; ++i) ++g.cnt[static_cast<size_t>(d[i].x - lo) + 1];
+        for (int64_t v = 1; v <= range; ++v) g.cnt[static_cast<size_t>(v)] += g.cnt[static_cast<size_t>(v - 1)];
+        for (int i = 0; i < n; ++i) g.idx[static_cast<size_t>(g.cnt[static_cast<size_t>(d[i].x - lo)]++)] = i;
+        return;
+    }
+    g.key.resize(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i)
+        g.key[static_cast<size_t>(i)] =
+            (static_cast<uint64_t>(static_cast<uint32_t>(d[i].x) ^ 0x80000000u) << 32) | static_cast<uint32_t>(i);
+    std::sort(g.key.begin(), g.key.end());
+    for (int k = 0; k < n; ++k) g.idx[static_cast<size_t>(k)] = static_cast<int>(g.key[static_cast<size_t>(k)] & 0xffffffffu);
 }
 
 }  // namespace
@@ -368,58 +404,72 @@ int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps,
                      int* count) {
     *count = 0;
     if (n <= 0) return VJ_OK;
-    UnionFind uf(n);
-    // sweep order (x, index) as one sorted u64 key per box, and the boxes'
-    // edges in that order as flat arrays so the sweep reads memory in order
-    std::vector<uint64_t> key(static_cast<size_t>(n));
-    for (int i = 0; i < n; ++i)
-        key[static_cast<size_t>(i)] =
-            (static_cast<uint64_t>(static_cast<uint32_t>(in[i].x) ^ 0x80000000u) << 32) | static_cast<uint32_t>(i);
-    std::sort(key.begin(), key.end());
-    std::vector<int> idx(static_cast<size_t>(n)), X(static_cast<size_t>(n)), Y(static_cast<size_t>(n)),
-        W(static_cast<size_t>(n)), H(static_cast<size_t>(n));
+    // (a named reference, so that the pool's workers reach this thread's
+    // scratch: a thread_local named inside a lambda is the running thread's)
+    thread_local GroupScratch scratch;
+    GroupScratch& g = scratch;
+    const size_t un = static_cast<size_t>(n);
+    g.parent.resize(un);
+    g.rank.assign(un, 0);
+    std::iota(g.parent.begin(), g.parent.end(), 0);
+    UnionFind uf{g.parent.data(), g.rank.data()};
+    // sweep order (x, index) and the boxes' edges in that order as flat
+    // arrays, so the sweep reads memory in order
+    sweep_order(in, n, g);
+    g.X.resize(un);
+    g.Y.resize(un);
+    g.W.resize(un);
+    g.H.resize(un);
+    const int* idx = g.idx.data();
+    int* X = g.X.data();
+    int* Y = g.Y.data();
+    int* W = g.W.data();
+    int* H = g.H.data();
     for (int k = 0; k < n; ++k) {
-        const int i = static_cast<int>(key[static_cast<size_t>(k)] & 0xffffffffu);
-        idx[static_cast<size_t>(k)] = i;
-        X[static_cast<size_t>(k)] = in[i].x;
-        Y[static_cast<size_t>(k)] = in[i].y;
-        W[static_cast<size_t>(k)] = in[i].w;
-        H[static_cast<size_t>(k)] = in[i].h;
+        const vj_detection& b = in[idx[k]];
+        X[k] = b.x;
+        Y[k] = b.y;
+        W[k] = b.w;
+        H[k] = b.h;
     }
-    // similarRects' tolerance (int)(eps * (min w + min h) * 0.5f) as a table
-    // over m = min w + min h, so the sweep does no float work per pair
-    int maxWH = 0;
-    for (int k = 0; k < n; ++k) maxWH = std::max(maxWH, std::max(W[static_cast<size_t>(k)], H[static_cast<size_t>(k)]));
-    std::vector<int> dtab(static_cast<size_t>(2 * maxWH + 1));
-    for (int m = 0; m <= 2 * maxWH; ++m)
-        dtab[static_cast<size_t>(m)] = static_cast<int>(eps * static_cast<float>(m) * 0.5f);
     // The similar pairs (a, b), a before b in sweep order, are found in
     // blocks of sweep positions in parallel, then joined in exactly the
     // sweep's order (a ascending, then b ascending), which is the sequence of
-    // unions the sequential sweep performs.
+    // unions the sequential sweep performs.  For each a, the candidates are
+    // the b's whose x is within a's reach; their similarRects test (the
+    // tolerance (int)(eps * (min w + min h) * 0.5f), every edge within it) is
+    // evaluated branch-free over runs of 64 candidates, then the hits are
+    // collected in order.
     auto similar_after = [&](int a, std::vector<int>& hits) {
-        const int ax = X[static_cast<size_t>(a)], ay = Y[static_cast<size_t>(a)];
-        const int aw = W[static_cast<size_t>(a)], ah = H[static_cast<size_t>(a)];
+        const int ax = X[a], ay = Y[a], aw = W[a], ah = H[a];
+        const int ar = ax + aw, ab = ay + ah;
         const int reach = static_cast<int>(eps * static_cast<float>(aw + ah) * 0.5f);
-        for (int b = a + 1; b < n; ++b) {
-            const int bx = X[static_cast<size_t>(b)];
-            if (bx - ax > reach) break;
-            const int by = Y[static_cast<size_t>(b)], bw = W[static_cast<size_t>(b)], bh = H[static_cast<size_t>(b)];
-            const int mw = std::min(aw, bw), mh = std::min(ah, bh);
-            const int d = (mw >= 0 && mh >= 0) ? dtab[static_cast<size_t>(mw + mh)]
-                                               : static_cast<int>(eps * static_cast<float>(mw + mh) * 0.5f);
-            // every edge within d of the other box's edge (one branch)
-            const bool sim = (std::abs(ax - bx) <= d) & (std::abs(ay - by) <= d) &
-                             (std::abs((ax + aw) - (bx + bw)) <= d) & (std::abs((ay + ah) - (by + bh)) <= d);
-            if (sim) hits.push_back(b);
+        int end = a + 1;
+        while (end < n && X[end] - ax <= reach) ++end;
+        uint8_t flag[64];
+        for (int b0 = a + 1; b0 < end; b0 += 64) {
+            const int m = std::min(64, end - b0);
+            const int* xb = X + b0;
+            const int* yb = Y + b0;
+            const int* wb = W + b0;
+            const int* hb = H + b0;
+            for (int k = 0; k < m; ++k) {
+                const int d = static_cast<int>(eps * static_cast<float>(std::min(aw, wb[k]) + std::min(ah, hb[k])) * 0.5f);
+                flag[k] = static_cast<uint8_t>((std::abs(ax - xb[k]) <= d) & (std::abs(ay - yb[k]) <= d) &
+                                               (std::abs(ar - (xb[k] + wb[k])) <= d) &
+                                               (std::abs(ab - (yb[k] + hb[k])) <= d));
+            }
+            for (int k = 0; k < m; ++k)
+                if (flag[k]) hits.push_back(b0 + k);
         }
     };
     constexpr int kBlock = 32;
     const int nBlocks = (n + kBlock - 1) / kBlock;
     // per block: for each a of the block, its similar b's after a marker -1 - a
-    std::vector<std::vector<int>> edges(static_cast<size_t>(nBlocks));
+    if (g.edges.size() < static_cast<size_t>(nBlocks)) g.edges.resize(static_cast<size_t>(nBlocks));
     auto find_block = [&](int blk) {
-        std::vector<int>& e = edges[static_cast<size_t>(blk)];
+        std::vector<int>& e = g.edges[static_cast<size_t>(blk)];
+        e.clear();
         for (int a = blk * kBlock; a < std::min(n, (blk + 1) * kBlock); ++a) {
             const size_t mark = e.size();
             e.push_back(-1 - a);
@@ -432,26 +482,22 @@ int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps,
     } else {
         for (int blk = 0; blk < nBlocks; ++blk) find_block(blk);
     }
-    for (const std::vector<int>& e : edges) {
+    for (int blk = 0; blk < nBlocks; ++blk) {
         int ra = -1, a = -1;  // ra: root of a's set once a has been joined
-        for (int v : e) {
+        for (int v : g.edges[static_cast<size_t>(blk)]) {
             if (v < 0) {
                 a = -1 - v;
                 ra = -1;
             } else {
-                ra = uf.join_root(ra < 0 ? uf.root(idx[static_cast<size_t>(a)]) : ra, idx[static_cast<size_t>(v)]);
+                ra = uf.join_root(ra < 0 ? uf.root(idx[a]) : ra, idx[v]);
             }
         }
     }
-    struct Acc {
-        int64_t x = 0, y = 0, w = 0, h = 0;
-        int count = 0;
-        float best = -std::numeric_limits<float>::infinity();
-        int scaleIdx = 0;
-    };
-    std::vector<Acc> acc(static_cast<size_t>(n));
+    g.acc.resize(un);
+    ClusterAcc* acc = g.acc.data();
+    for (int i = 0; i < n; ++i) acc[i] = ClusterAcc{0, 0, 0, 0, 0, -std::numeric_limits<float>::infinity(), 0};
     for (int i = 0; i < n; ++i) {
-        Acc& c = acc[static_cast<size_t>(uf.root(i))];
+        ClusterAcc& c = acc[uf.root(i)];
         c.x += in[i].x;
         c.y += in[i].y;
         c.w += in[i].w;
@@ -463,11 +509,11 @@ int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps,
         }
     }
     const int keep = std::max(1, minNeighbors);
-    std::vector<vj_detection> clusters;
-    std::vector<int> members;
+    g.clusters.clear();
+    g.members.clear();
     for (int i = 0; i < n; ++i) {
-        const Acc& c = acc[static_cast<size_t>(i)];
-        if (uf.root(i) != i || c.count <= keep) continue;
+        const ClusterAcc& c = acc[i];
+        if (g.parent[static_cast<size_t>(i)] != i || c.count <= keep) continue;
         vj_detection d;
         d.x = static_cast<int>(c.x / c.count);
         d.y = static_cast<int>(c.y / c.count);
@@ -475,26 +521,30 @@ int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps,
         d.h = static_cast<int>(c.h / c.count);
         d.scaleIdx = c.scaleIdx;
         d.score = c.best;
-        clusters.push_back(d);
-        members.push_back(c.count);
+        g.clusters.push_back(d);
+        g.members.push_back(c.count);
     }
-    const int nc = static_cast<int>(clusters.size());
+    const int nc = static_cast<int>(g.clusters.size());
+    const vj_detection* clusters = g.clusters.data();
     int maxW = 0;
-    for (const vj_detection& d : clusters) maxW = std::max(maxW, d.w);
-    const std::vector<int> corder = order_by_x(clusters.data(), nc);
+    for (int i = 0; i < nc; ++i) maxW = std::max(maxW, clusters[i].w);
+    g.corder.resize(static_cast<size_t>(nc));
+    std::iota(g.corder.begin(), g.corder.end(), 0);
+    std::sort(g.corder.begin(), g.corder.end(),
+              [clusters](int a, int b) { return clusters[a].x != clusters[b].x ? clusters[a].x < clusters[b].x : a < b; });
     int no = 0;
     for (int i = 0; i < nc; ++i) {
-        const vj_detection& r1 = clusters[static_cast<size_t>(i)];
-        const int n1 = members[static_cast<size_t>(i)];
+        const vj_detection& r1 = clusters[i];
+        const int n1 = g.members[static_cast<size_t>(i)];
         const float lo = static_cast<float>(r1.x + r1.w) - (1.0f + eps) * static_cast<float>(maxW);
         const float hi = static_cast<float>(r1.x) + eps * static_cast<float>(maxW);
         bool inside = false;
-        for (int o : corder) {
-            const vj_detection& r2 = clusters[static_cast<size_t>(o)];
+        for (int o : g.corder) {
+            const vj_detection& r2 = clusters[o];
             if (r2.x < static_cast<int>(lo)) continue;
             if (r2.x > static_cast<int>(hi)) break;
             if (o == i) continue;
-            const int n2 = members[static_cast<size_t>(o)];
+            const int n2 = g.members[static_cast<size_t>(o)];
             const int dx = static_cast<int>(static_cast<float>(r2.w) * eps);
             const int dy = static_cast<int>(static_cast<float>(r2.h) * eps);
             if (r1.x >= r2.x - dx && r1.y >= r2.y - dy && r1.x + r1.w <= r2.x + r2.w + dx &&

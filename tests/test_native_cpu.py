@@ -85,6 +85,22 @@ def test_grouping_equals_oracle_on_random_boxes():
         assert det_tuples(group_rectangles(d, mn)) == det_tuples(O.group(d, mn)), trial
 
 
+def test_grouping_sweep_order_negative_and_wide_x():
+    """The sweep order is a counting sort over x for moderate x ranges and a
+    key sort otherwise: negative x, and x spread over 10^7 (the key sort),
+    group as the oracle groups them."""
+    rng = np.random.default_rng(5)
+    for trial, (lo, span) in enumerate([(-500, 1200), (-3000000, 10000000), (0, 1), (2**30, 5000)]):
+        n = 600
+        d = np.zeros(n, DETECTION)
+        centres = np.stack([rng.integers(lo, lo + span, 60), rng.integers(0, 800, 60)], 1)
+        for i in range(n):
+            cx, cy = centres[rng.integers(0, len(centres))]
+            s = int(rng.integers(24, 90))
+            d[i] = (cx + rng.integers(-5, 6), cy + rng.integers(-5, 6), s, s, 0, np.float32(rng.normal()))
+        assert det_tuples(group_rectangles(d, 2)) == det_tuples(O.group(d, 2)), trial
+
+
 def test_pgm_reader(tmp_path):
     from cs677s26_amd._native import lib
     import ctypes as C
