@@ -119,6 +119,10 @@ __device__ __forceinline__ float response(const vj_feature& f, const uint32_t* _
 // semantics).
 // ---------------------------------------------------------------------------
 constexpr int kSortWaves = kSortThreads / kWave;
+// Digit counters: one row of 256 per wave, rows kCntPitch words apart, so a
+// wave's lanes reading the counters of distinct digits hit distinct banks.
+constexpr int kCntPitch = 257;
+static_assert(kSortWaves == 16 && 256 * kSortWaves == 4 * kSortThreads, "the counter scan takes 4 entries per thread");
 
 __global__ __launch_bounds__(kSortThreads) void presort_kernel(const vj_feature* __restrict__ feats, int fBegin,
                                                                const uint32_t* __restrict__ sumT,
@@ -129,8 +133,8 @@ __global__ __launch_bounds__(kSortThreads) void presort_kernel(const vj_feature*
     extern __shared__ uint8_t smem[];
     const int nPad = (n + kSortThreads - 1) / kSortThreads * kSortThreads;
     const int nb = nPad / kSortThreads;                                  // 64-position batches per wave
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(smem);                   // [256][kSortWaves]
-    uint32_t* keys = cnt + 256 * kSortWaves;                             // [nPad]
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(smem);                   // [kSortWaves][kCntPitch]
+    uint32_t* keys = cnt + kCntPitch * kSortWaves;                       // [nPad]
     uint16_t* idx = reinterpret_cast<uint16_t*>(keys + nPad);            // [nPad]
     __shared__ uint32_t redMin[kSortWaves], redMax[kSortWaves], waveTot[kSortWaves];
 
@@ -199,7 +203,8 @@ __global__ __launch_bounds__(kSortThreads) void presort_kernel(const vj_feature*
                 ir[i] = idx[base + i * kWave + lane];
             }
         }
-        for (int d = lane; d < 256; d += kWave) cnt[d * kSortWaves + wave] = 0u;
+        uint32_t* wcnt = cnt + wave * kCntPitch;
+        for (int d = lane; d < 256; d += kWave) wcnt[d] = 0u;
 #pragma unroll
         for (int i = 0; i < kSortMaxPerThread; ++i) {
             if (i < nb) {
@@ -212,19 +217,21 @@ __global__ __launch_bounds__(kSortThreads) void presort_kernel(const vj_feature*
                     m &= bit ? bb : ~bb;
                 }
                 const uint32_t below = static_cast<uint32_t>(mask_rank(m));
-                const uint32_t c = cnt[dg * kSortWaves + wave];
+                const uint32_t c = wcnt[dg];
                 rk[i] = c + below;
-                if (below == 0) cnt[dg * kSortWaves + wave] = c + static_cast<uint32_t>(__popcll(m));
+                if (below == 0) wcnt[dg] = c + static_cast<uint32_t>(__popcll(m));
             }
         }
         __syncthreads();
-        // exclusive scan over (digit, wave): 4 consecutive entries per thread
+        // exclusive scan over (digit, wave) in that order: 4 consecutive
+        // entries per thread
         {
             uint32_t c4[4];
             uint32_t t = 0;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                c4[k] = cnt[tid * 4 + k];
+                const int e = tid * 4 + k;
+                c4[k] = cnt[(e & (kSortWaves - 1)) * kCntPitch + (e >> 4)];
                 t += c4[k];
             }
             const uint32_t inc = wave_incl_scan_dpp(t);
@@ -234,7 +241,8 @@ __global__ __launch_bounds__(kSortThreads) void presort_kernel(const vj_feature*
             for (int w = 0; w < wave; ++w) off += waveTot[w];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                cnt[tid * 4 + k] = off;
+                const int e = tid * 4 + k;
+                cnt[(e & (kSortWaves - 1)) * kCntPitch + (e >> 4)] = off;
                 off += c4[k];
             }
         }
@@ -243,7 +251,7 @@ __global__ __launch_bounds__(kSortThreads) void presort_kernel(const vj_feature*
         for (int i = 0; i < kSortMaxPerThread; ++i) {
             if (i < nb) {
                 const uint32_t dg = ((kr[i] - kmin) >> shift) & 0xFFu;
-                const uint32_t pos = cnt[dg * kSortWaves + wave] + rk[i];
+                const uint32_t pos = wcnt[dg] + rk[i];
                 keys[pos] = kr[i];
                 idx[pos] = ir[i];
             }
@@ -967,7 +975,7 @@ int search_blocks() {
 
 size_t presort_lds(int n) {
     const size_t nPad = static_cast<size_t>((n + kSortThreads - 1) / kSortThreads * kSortThreads);
-    return 256 * kSortWaves * 4 + nPad * 4 + nPad * 2;
+    return static_cast<size_t>(kCntPitch) * kSortWaves * 4 + nPad * 4 + nPad * 2;
 }
 
 // Every corner of every feature must index the (winW+1) x (winH+1) window
