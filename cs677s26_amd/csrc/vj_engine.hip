@@ -13,6 +13,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <numeric>
 #include <string>
@@ -338,8 +339,7 @@ constexpr int kDeepGrid = 2048;
 struct CascadeSmem {
     uint16_t listIdx[kTileMaxWin];               // live windows (tile-local index)
     float invTab[kTileMaxWin];                   // normaliser by tile-local index
-    uint32_t votes[2][kVoteChunk][kCascadeBlock / 32];
-    float leafL[2][kVoteChunk], leafR[2][kVoteChunk];
+    float red[kCascadeBlock];                    // pair mode: partial stage sums
     int listCnt[2];
     int item;                                    // deep pass: current work item
     unsigned long long maskWords[32];            // final bits per tile row
@@ -534,15 +534,19 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
             // padded to nwp = 2^lgw slots and each pass evaluates spp =
             // 256 / nwp stumps on every slot, windows fastest, so a lane
             // keeps one window for the whole stage and a wave shares one
-            // stump (scalar loads) whenever nwp >= 64; below that, the next
-            // pass's per-lane stump record is fetched while the current one
-            // is evaluated.
-            // Each pass's window bits go by ballot into per-stump vote words
-            // of a chunk of kVoteChunk stumps; the window's own lane then
-            // adds the chunk's leaf values in stump order (one chunk behind
-            // the passes, so that work overlaps the next chunk's passes), and
-            // the float sum rounds exactly as the sequential stage loop does.
-            int lgw = 2;
+            // stump (scalar loads) whenever nwp >= 64; below that, each
+            // lane's next stump record is fetched while the current one is
+            // evaluated.  Every lane adds its stumps' leaf values into a
+            // partial sum; the partial sums of a window are then added up
+            // (within the wave, then across the waves).  That sum S differs
+            // from the sequential stage sum (the reference's float order) by
+            // at most the stage's bound B (both are sums of the same leaf
+            // values, each within gamma_(n+16) * sum|leaf| of the exact
+            // sum), so S < lo = thr - B fails and S > hi = thr + B passes
+            // exactly as the sequential sum would.  Windows in between, and
+            // every window the cascade's last stage may pass (its score is
+            // the sequential sum), are summed again sequentially.
+            int lgw = 0;
             while ((1 << lgw) < n) ++lgw;
             const int nwp = 1 << lgw;
             const int spp = kCascadeBlock >> lgw;
@@ -553,66 +557,56 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
             const bool mine = tid < n;
             const uint32_t myIdx = mine ? sm.listIdx[tid] : 0u;
             if (tid == 0) sm.listCnt[nxt] = 0;
-            __syncthreads();
             const char* o = origin(ew);
             const float inv = sm.invTab[ew];
-            float stageSum = 0.0f;
-            const int nChunks = (nt + kVoteChunk - 1) / kVoteChunk;
-            auto run = [&](auto uniformTag) {
-                constexpr bool kUniform = decltype(uniformTag)::value;
-                KStump nx{};
-                auto prefetch = [&](int js) {
-                    if (js < nt) nx = sg[js];
-                };
-                if constexpr (!kUniform) prefetch(js0);
-                for (int c = 0; c <= nChunks; ++c) {
-                    const int b = c & 1;
-                    if (c < nChunks) {
-                        const int cb = c * kVoteChunk;
-                        const int tc = min(kVoteChunk, nt - cb);
-                        if (tid < tc) {
-                            sm.leafL[b][tid] = sg[cb + tid].left;
-                            sm.leafR[b][tid] = sg[cb + tid].right;
-                        }
-                        for (int jb = 0; jb < tc; jb += spp) {
-                            const int js = jb + js0;
-                            bool bit = false;
-                            if constexpr (kUniform) {
-                                // the record is a scalar load used at once: a
-                                // scalar load still in flight would make every
-                                // wait on the LDS gathers below wait for it too
-                                const int ju = __builtin_amdgcn_readfirstlane(cb + js);
-                                if (evalLive && js < tc) {
-                                    const KStump sp = sg[ju];
-                                    bit = haar_raw(o, sp) * inv < sp.theta;
-                                }
-                            } else {
-                                const KStump sp = nx;
-                                prefetch(cb + js + spp);
-                                if (evalLive && js < tc) bit = haar_raw(o, sp) * inv < sp.theta;
-                            }
-                            const uint64_t m = __ballot(bit);
-                            if (js < tc && (w & 31) == 0) {
-                                uint32_t bits = static_cast<uint32_t>(m >> lane);
-                                if (nwp < 32) bits &= (1u << nwp) - 1u;
-                                sm.votes[b][js][w >> 5] = bits;
-                            }
-                        }
+            float part = 0.0f;
+            if (nwp >= kWave) {
+                for (int jb = 0; jb < nt; jb += spp) {
+                    // js is uniform across the wave; the record is a scalar
+                    // load used at once (a scalar load still in flight would
+                    // make every wait on the LDS gathers wait for it too)
+                    const int js = __builtin_amdgcn_readfirstlane(jb + js0);
+                    if (js < nt && evalLive) {
+                        const KStump sp = sg[js];
+                        part += (haar_raw(o, sp) * inv < sp.theta) ? sp.left : sp.right;
                     }
-                    if (c > 0 && mine) {
-                        const int pb = b ^ 1;
-                        const int ptc = min(kVoteChunk, nt - (c - 1) * kVoteChunk);
-                        const int word = tid >> 5, sh = tid & 31;
-#pragma unroll 8
-                        for (int k = 0; k < ptc; ++k)
-                            stageSum += ((sm.votes[pb][k][word] >> sh) & 1u) ? sm.leafL[pb][k] : sm.leafR[pb][k];
-                    }
-                    __syncthreads();
                 }
-            };
-            if (nwp >= kWave) run(std::true_type{});
-            else run(std::false_type{});
-            const bool pass = mine && !(stageSum < st.threshold);
+            } else {
+                KStump curRec{};
+                if (js0 < nt) curRec = sg[js0];
+                for (int jb = 0; jb < nt; jb += spp) {
+                    const int js = jb + js0;
+                    KStump nextRec{};
+                    if (js + spp < nt) nextRec = sg[js + spp];
+                    if (evalLive && js < nt) part += (haar_raw(o, curRec) * inv < curRec.theta) ? curRec.left : curRec.right;
+                    curRec = nextRec;
+                }
+            }
+            // lanes l and l ^ d (d >= nwp) of a wave hold the same window
+            for (int d = nwp; d < kWave; d <<= 1) part += __shfl_xor(part, d, kWave);
+            sm.red[tid] = part;
+            __syncthreads();
+            float stageSum = 0.0f;
+            bool pass = false;
+            if (mine) {
+                // window tid's partial sums: lanes tid + k * max(nwp, 64)
+                const int per = max(nwp, kWave);
+                float S = sm.red[tid];
+                for (int k = per; k < kCascadeBlock; k += per) S += sm.red[tid + k];
+                if (S < st.lo) {
+                    pass = false;
+                } else if (S > st.hi && !last) {
+                    pass = true;
+                } else {
+                    const char* om = origin(myIdx);
+                    const float invm = sm.invTab[myIdx];
+                    for (int k = 0; k < nt; ++k) {
+                        const KStump sp = sg[k];
+                        stageSum += (haar_raw(om, sp) * invm < sp.theta) ? sp.left : sp.right;
+                    }
+                    pass = !(stageSum < st.threshold);
+                }
+            }
             if (tid < kWave * ((n + kWave - 1) / kWave)) settle(pass, last, myIdx, stageSum, nxt);
         }
         __syncthreads();
@@ -1610,6 +1604,28 @@ extern "C" int vj_engine_integral_view(vj_engine* e, vj_integral_view* out) {
     return VJ_OK;
 }
 
+// The pair mode's decision margins of one stage.  Any float summation of n
+// terms x_i, in any order, is within gamma_m * sum|x_i| of the exact sum
+// (gamma_m = m u / (1 - m u), u = 2^-24, m >= the number of roundings any
+// term goes through); the pair mode's sum and the sequential one are two such
+// summations of the same chosen leaf values, |x_i| <= max(|left_i|,
+// |right_i|).  So with B = 2 gamma_m sum_i max(|left_i|, |right_i|) and m =
+// n + 16 (room for the additions of zero partial sums), lo is a float at or
+// below threshold - B and hi a float at or above threshold + B.
+void stage_margins(const vj_stump* st, int n, float threshold, float& lo, float& hi) {
+    double M = 0.0;
+    for (int k = 0; k < n; ++k) M += std::max(std::fabs(static_cast<double>(st[k].left)), std::fabs(static_cast<double>(st[k].right)));
+    const double u = std::ldexp(1.0, -24);
+    const double m = static_cast<double>(n) + 16.0;
+    const double gamma = m * u / (1.0 - m * u);
+    const double B = n > 1 ? 2.0 * gamma * M * (1.0 + 1e-6) + 1e-30 : 0.0;
+    const double dlo = static_cast<double>(threshold) - B, dhi = static_cast<double>(threshold) + B;
+    lo = static_cast<float>(dlo);
+    if (static_cast<double>(lo) > dlo) lo = std::nextafter(lo, -std::numeric_limits<float>::infinity());
+    hi = static_cast<float>(dhi);
+    if (static_cast<double>(hi) < dhi) hi = std::nextafter(hi, std::numeric_limits<float>::infinity());
+}
+
 extern "C" int vj_engine_set_cascade(vj_engine* e, const vj_feature* dFeatures, int nF, const vj_stump* dStumps,
                                      int nS, const vj_stage* hStages, int nSt, vj_stream sv) {
     // src/face_vision_engine.cu:1182-1185
@@ -1658,7 +1674,11 @@ extern "C" int vj_engine_set_cascade(vj_engine* e, const vj_feature* dFeatures, 
         }
     }
     std::vector<KStage> kg(static_cast<size_t>(nSt));
-    for (int k = 0; k < nSt; ++k) kg[static_cast<size_t>(k)] = {hStages[k].first, hStages[k].ntrees, hStages[k].threshold, 0};
+    for (int k = 0; k < nSt; ++k) {
+        KStage& g = kg[static_cast<size_t>(k)];
+        g = KStage{hStages[k].first, hStages[k].ntrees, hStages[k].threshold, 0.0f, 0.0f, {0, 0, 0}};
+        stage_margins(hs.data() + hStages[k].first, hStages[k].ntrees, hStages[k].threshold, g.lo, g.hi);
+    }
     VJ_TRY(e->modelFeat.ensure(static_cast<size_t>(nF)));
     VJ_TRY(e->kst.ensure(ks.size()));
     VJ_TRY(e->kstage.ensure(kg.size()));

@@ -22,8 +22,6 @@ constexpr int kChunkCols = 256;
 // fit on a CU.  A tile row is a 64-, 32- or 16-bit piece of one 64-bit mask
 // word.
 constexpr int kTileMaxWin = 512;
-// Stumps evaluated per vote chunk when a tile has fewer live windows than lanes.
-constexpr int kVoteChunk = 64;
 
 __host__ __device__ constexpr int tile_lgcols_for_step(int step) {
     return step == 1 ? 6 : (step == 2 ? 5 : 4);
@@ -65,10 +63,12 @@ struct alignas(16) KStump {
 };
 static_assert(sizeof(KStump) == 64, "KStump is one 64-byte scalar load");
 
+// Cascade stage.  lo <= threshold - B and hi >= threshold + B, where B bounds
+// how far any two float summations of the stage's leaf values can differ.
 struct KStage {
     int32_t first, ntrees;
-    float threshold;
-    int32_t pad;
+    float threshold, lo, hi;
+    int32_t pad[3];
 };
 
 __device__ __forceinline__ int lane_id() {

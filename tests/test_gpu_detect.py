@@ -227,3 +227,32 @@ def test_two_pass_cascade_matches_oracle(cascade7):
     got = eng.detect_batch(dev, [(w, h, w) for w, h in sizes], 1.25, 4, 24, 200000, False)
     for g, want in zip(got, wants):
         assert det_tuples(g) == det_tuples(want)
+
+
+@pytest.mark.parametrize("offset", [0.0, 3.5, -17.25])
+def test_pair_mode_margins_near_threshold(cascade7, offset):
+    """Leaves of +-(1e6 + a few units) make every stage sum's rounding depend
+    on the order of its additions, and thresholds near zero put the windows
+    with as many positive as negative leaves within the pair mode's margin of
+    the threshold, so its sequential re-summation decides them (and computes
+    every last-stage score).  Raw boxes and scores equal the oracle's."""
+    say(f"pair-mode margins, threshold offset {offset}")
+    feats, stumps, _ = cascade7
+    rng = np.random.default_rng(23)
+    st = stumps.copy()
+    n = len(st)
+    st["left"] = np.where(stumps["left"] >= 0, 1, -1) * (1e6 + rng.uniform(-30, 30, n)).astype(np.float32)
+    st["right"] = np.where(stumps["right"] >= 0, 1, -1) * (1e6 + rng.uniform(-30, 30, n)).astype(np.float32)
+    sizes = (1, 10, 24, 26, 50, 40, 40)
+    assert sum(sizes) == n
+    stages = np.zeros(len(sizes), STAGE)
+    stages["first"] = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    stages["ntrees"] = sizes
+    stages["threshold"] = np.float32(offset)
+    img = synth.detect_image(640, 480, 20, 24, 64, seed=77)
+    want, _ = O.detect_raw(feats, st, stages, img)
+    say(f"  oracle raw detections {len(want)}")
+    assert len(want) > 100
+    eng = _engine(640, 480, (feats, st, stages))
+    raw = eng.detectMultiScale(DeviceArray.from_host(img), 640, 480, 640, 1.25, 4, 24, 200000, applyGrouping=False)
+    assert det_tuples(raw) == det_tuples(want)
