@@ -328,8 +328,10 @@ __device__ __forceinline__ float haar_raw(const char* ob, const KStump& sp) {
 
 constexpr int kCascadeBlock = 256;
 // Survivors of the first pass are handed to the deep pass in slices of up to
-// kDeepSlice windows (at most kTileMaxWin / kDeepSlice = 8 slices per tile).
+// kDeepSlice windows (at most kDeepSlices = kTileMaxWin / kDeepSlice slices
+// per tile; a work item is tile * kDeepSlices + slice).
 constexpr int kDeepSlice = 64;
+constexpr int kDeepSlices = kTileMaxWin / kDeepSlice;
 // Stumps of the leading stages the first pass runs, and the deep pass's
 // persistent grid (eight workgroups on each of 256 CUs).
 constexpr int kSplitStumps = 128;
@@ -703,7 +705,7 @@ __global__ void reset_counter_kernel(int32_t* __restrict__ c) {
 }
 
 // Deep-pass work list: every tile with survivors after the first pass
-// contributes ceil(count / kDeepSlice) items (tile * 8 + slice), ordered by
+// contributes ceil(count / kDeepSlice) items (tile * kDeepSlices + slice), ordered by
 // survivor count, largest first, so the longest items start first.  One
 // workgroup; also resets the deep pass's work counter.
 __global__ __launch_bounds__(1024) void deep_list_kernel(const int32_t* __restrict__ tileCount, int nTiles,
@@ -734,7 +736,7 @@ __global__ __launch_bounds__(1024) void deep_list_kernel(const int32_t* __restri
         if (c > 0) {
             const int ns = (c + kDeepSlice - 1) / kDeepSlice;
             const uint32_t pos = base[c] + atomicAdd(&hist[c], 1u) * static_cast<uint32_t>(ns);
-            for (int s = 0; s < ns; ++s) items[pos + s] = t * 8 + s;
+            for (int s = 0; s < ns; ++s) items[pos + s] = t * kDeepSlices + s;
         }
     }
 }
@@ -760,7 +762,7 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_deep_kernel(
         __syncthreads();
         const int item = sm.item;
         if (item >= total) break;
-        const int t = items[item] >> 3, slice = items[item] & 7;
+        const int t = items[item] / kDeepSlices, slice = items[item] % kDeepSlices;
         const TileGeo g = tile_geo(L, lvlIdx, tileOffs, nLv, t);
         stage_footprint(g, iiS, lds, winW, winH);
         const int first = slice * kDeepSlice;
@@ -1262,7 +1264,7 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
         VJ_TRY(e->survInv.ensure(static_cast<size_t>(std::max<int64_t>(P.windows, 1))));
         VJ_TRY(e->tileSurvOff.ensure(static_cast<size_t>(P.nTiles)));
         VJ_TRY(e->tileCount.ensure(static_cast<size_t>(P.nTiles)));
-        VJ_TRY(e->deepItems.ensure(static_cast<size_t>(P.nTiles) * 8));
+        VJ_TRY(e->deepItems.ensure(static_cast<size_t>(P.nTiles) * kDeepSlices));
         VJ_TRY(e->deepCtl.ensure(4));
     }
     const int nImages = static_cast<int>(P.imgWordOff.size()) - 1;
@@ -1332,7 +1334,7 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     for (int st = 1; st <= P.maxStep; ++st)
         ldsBytes = std::max(ldsBytes, cascade_footprint(st, e->tileRows[st], e->winW, e->winH) * sizeof(float));
     const int nBlocks = static_cast<int>(blockImg.size());
-    const int deepGrid = deep ? std::min(P.nTiles * 8, kDeepGrid) : 0;
+    const int deepGrid = deep ? std::min(P.nTiles * kDeepSlices, kDeepGrid) : 0;
 
     auto launch_all = [&](hipStream_t ls) -> int {
         if (e->timing) VJ_HIP(hipEventRecord(e->ev[0], ls));
