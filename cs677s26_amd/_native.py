@@ -94,6 +94,7 @@ _PROTOS = {
     "vj_pgm_peek": (I, [C.c_char_p, C.POINTER(I), C.POINTER(I)]),
     "vj_pgm_load": (I, [C.c_char_p, P, SZ]),
     "vj_ppm_save_gray_boxes": (I, [C.c_char_p, P, I, I, P, I]),
+    "vj_crop_resize": (I, [P, I, I, I, P, I, I, P]),
     "vj_engine_create": (I, [I, I, I, I, I, I, I, PP]),
     "vj_engine_destroy": (None, [P]),
     "vj_engine_upload_feature_pool": (I, [P, P, I, P]),

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <iostream>
 #include <limits>
@@ -182,27 +183,36 @@ int main(int argc, char** argv) {
 
 // ---------------------------------------------------------------------------
 // train (src/main.cu:1370-2392).  Positives from faces_u8.bin ([N][24][24]
-// u8); negatives per stage from <stageNegCacheDir>/stage_negatives/
-// non_faces_stageNN.bin (the files the reference's sampling script writes);
-// the boosting rounds run on the fused trainer (presort once per stage).
+// u8), a seeded subset per stage.  Negatives per stage as the reference
+// gathers them (src/main.cu:1765-1925):
+//   * stage 0: <stageNegCacheDir>/stage_negatives/non_faces_stage00.bin when
+//     it holds enough patches, else random square crops of the PGM images in
+//     <stageNegImagesDir>, written to that file (the reference runs its
+//     python sampler for this, scripts/stage_negatives.py:149-228);
+//   * later stages: hard negatives mined with the cascade trained so far --
+//     raw detections on the shuffled PGM images, the 256 best-scoring boxes
+//     of each image cropped and resized to the window (cropResizeTo24) --
+//     topped up, when too few, by random picks from a fresh sample of
+//     max(numNeg, missing * hardNegCandidateMultiplier) crops.
+// The boosting rounds run on the fused trainer (presort once per stage).
 // ---------------------------------------------------------------------------
 namespace {
 
-bool load_bin(const std::string& path, int px, std::vector<uint8_t>& out, int& count) {
+bool load_bin(const std::string& path, int px, std::vector<uint8_t>& out, int& count, bool quiet = false) {
     std::ifstream ifs(path, std::ios::binary);
     if (!ifs) {
-        std::cerr << "[ERR] Cannot open: " << path << "\n";
+        if (!quiet) std::cerr << "[ERR] Cannot open: " << path << "\n";
         return false;
     }
     ifs.seekg(0, std::ios::end);
     const std::streamoff sz = ifs.tellg();
     ifs.seekg(0, std::ios::beg);
     if (sz <= 0) {
-        std::cerr << "[ERR] Empty file: " << path << "\n";
+        if (!quiet) std::cerr << "[ERR] Empty file: " << path << "\n";
         return false;
     }
     if (sz % px != 0) {
-        std::cerr << "[ERR] File size is not divisible by samplePixels: " << path << "\n";
+        if (!quiet) std::cerr << "[ERR] File size is not divisible by samplePixels: " << path << "\n";
         return false;
     }
     out.resize(static_cast<size_t>(sz));
@@ -211,10 +221,100 @@ bool load_bin(const std::string& path, int px, std::vector<uint8_t>& out, int& c
     return true;
 }
 
+// Every .pgm file under dir (any depth, extension in any case), sorted so a
+// seed gives the same order on every file system.
+std::vector<std::string> collect_pgm_paths(const std::string& dir) {
+    namespace fs = std::filesystem;
+    std::vector<std::string> out;
+    std::error_code ec;
+    if (dir.empty() || !fs::exists(dir, ec)) return out;
+    for (fs::recursive_directory_iterator it(dir, ec), end; !ec && it != end; it.increment(ec)) {
+        if (!it->is_regular_file(ec)) continue;
+        std::string ext = it->path().extension().string();
+        std::transform(ext.begin(), ext.end(), ext.begin(), [](unsigned char c) { return static_cast<char>(std::tolower(c)); });
+        if (ext == ".pgm") out.push_back(it->path().string());
+    }
+    std::sort(out.begin(), out.end());
+    return out;
+}
+
+struct Gray {
+    int w = 0, h = 0;
+    std::vector<uint8_t> px;
+};
+
+bool load_gray(const std::string& path, Gray& g) {
+    if (vj_pgm_peek(path.c_str(), &g.w, &g.h) != VJ_OK) return false;
+    g.px.resize(static_cast<size_t>(g.w) * g.h);
+    return vj_pgm_load(path.c_str(), g.px.data(), g.px.size()) == VJ_OK;
+}
+
+// count random square crops (side uniform in [win, min(w, h)], position
+// uniform), each resized to win x win, from images drawn uniformly from the
+// shuffled usable files, at most 6 crops per draw.
+bool sample_negatives(const std::vector<std::string>& files, int count, int win, uint32_t seed,
+                      std::vector<uint8_t>& out) {
+    std::mt19937 rng(seed);
+    std::vector<std::string> order = files;
+    std::shuffle(order.begin(), order.end(), rng);
+    std::vector<std::string> usable;
+    for (const std::string& f : order) {
+        int w = 0, h = 0;
+        if (vj_pgm_peek(f.c_str(), &w, &h) == VJ_OK && std::min(w, h) >= win) usable.push_back(f);
+    }
+    if (usable.empty()) return false;
+    const size_t px = static_cast<size_t>(win) * win;
+    out.assign(static_cast<size_t>(count) * px, 0);
+    std::uniform_int_distribution<size_t> pickImage(0, usable.size() - 1);
+    Gray g;
+    std::string loaded;
+    int n = 0;
+    while (n < count) {
+        const std::string& f = usable[pickImage(rng)];
+        if (f != loaded) {
+            if (!load_gray(f, g)) return false;
+            loaded = f;
+        }
+        const int k = std::min(count - n, 6);
+        for (int i = 0; i < k; ++i) {
+            const int side = std::uniform_int_distribution<int>(win, std::min(g.w, g.h))(rng);
+            const int x0 = std::uniform_int_distribution<int>(0, g.w - side)(rng);
+            const int y0 = std::uniform_int_distribution<int>(0, g.h - side)(rng);
+            const vj_detection box{x0, y0, side, side, 0, 0.0f};
+            if (vj_crop_resize(g.px.data(), g.w, g.h, g.w, &box, win, win, out.data() + static_cast<size_t>(n) * px) !=
+                VJ_OK)
+                return false;
+            ++n;
+        }
+    }
+    return true;
+}
+
+// count negative patches for a stage: the cached stage file when it holds at
+// least count patches, else a fresh sample (written to the cache file).
+bool stage_negatives(const std::string& cacheDir, const std::vector<std::string>& images, int stage, int count,
+                     int win, int seed, std::vector<uint8_t>& out) {
+    char name[64];
+    std::snprintf(name, sizeof name, "/stage_negatives/non_faces_stage%02d.bin", stage);
+    const std::string path = cacheDir + name;
+    const int px = win * win;
+    int have = 0;
+    if (load_bin(path, px, out, have, true) && have >= count) {
+        out.resize(static_cast<size_t>(count) * px);
+        return true;
+    }
+    if (!sample_negatives(images, count, win, static_cast<uint32_t>(seed), out)) return false;
+    std::error_code ec;
+    std::filesystem::create_directories(cacheDir + "/stage_negatives", ec);
+    std::ofstream ofs(path, std::ios::binary);
+    if (ofs) ofs.write(reinterpret_cast<const char*>(out.data()), static_cast<std::streamsize>(out.size()));
+    return true;
+}
+
 }  // namespace
 
 int run_train(int argc, char** argv) {
-    constexpr int kStages = 38, kTotal = 6061, kPos = 10000, kNeg = 10000;
+    constexpr int kStages = 38, kTotal = 6061, kPos = 10000, kNeg = 10000, kWin = 24, kPerImageKeep = 256;
     const int modelArg = argc - 1;
     if (modelArg < 1) {
         std::cerr << "Usage: cuda_hello train [faces_u8.bin] [numNeg] [maxWeakPerStage] [maxDetections] [minNeighbors]"
@@ -224,15 +324,21 @@ int run_train(int argc, char** argv) {
     }
     const std::string outPath = argv[modelArg];
     const std::string posPath = modelArg > 1 ? argv[1] : "faces_u8.bin";
-    int numNeg = kNeg, maxWeak = 250, maxStages = kStages, seed = 677;
-    float minHit = 0.99f, maxFa = 0.40f;
-    std::string cacheDir;
+    int numNeg = kNeg, maxWeak = 250, maxStages = kStages, seed = 677, maxDetections = 200000, minNeighbors = 4;
+    int minObjectSize = 24, candMul = 3;
+    float minHit = 0.99f, maxFa = 0.40f, scaleFactor = 1.25f;
+    std::string cacheDir, imagesDir = "data_cache/coco_val2017_nonperson_pgm";
     try {
         if (modelArg > 2) numNeg = std::max(1, std::stoi(argv[2]));
         if (modelArg > 3) maxWeak = std::max(1, std::stoi(argv[3]));
-        if (modelArg > 6 && std::stof(argv[6]) <= 1.0f) {
-            std::cerr << "[ERR] scaleFactor must be > 1.0\n";
-            return 1;
+        if (modelArg > 4) maxDetections = std::max(100, std::stoi(argv[4]));
+        if (modelArg > 5) minNeighbors = std::max(1, std::stoi(argv[5]));
+        if (modelArg > 6) {
+            scaleFactor = std::stof(argv[6]);
+            if (scaleFactor <= 1.0f) {
+                std::cerr << "[ERR] scaleFactor must be > 1.0\n";
+                return 1;
+            }
         }
         if (modelArg > 7) maxStages = std::max(1, std::stoi(argv[7]));
         if (modelArg > 8) {
@@ -249,8 +355,11 @@ int run_train(int argc, char** argv) {
                 return 1;
             }
         }
+        if (modelArg > 10) minObjectSize = std::max(1, std::stoi(argv[10]));
         if (modelArg > 11) cacheDir = argv[11];
+        if (modelArg > 12) imagesDir = argv[12];
         if (modelArg > 13) seed = std::stoi(argv[13]);
+        if (modelArg > 14) candMul = std::max(1, std::stoi(argv[14]));
     } catch (...) {
         std::cerr << "[ERR] invalid numeric argument\n";
         return 1;
@@ -263,43 +372,66 @@ int run_train(int argc, char** argv) {
         std::cout << "[INFO] maxStages capped to " << kStages << "\n";
         maxStages = kStages;
     }
+    std::cout << "[INFO] Config: maxWeakPerStage=" << maxWeak << " maxStages=" << maxStages
+              << " scaleFactor=" << scaleFactor << " minNeighbors=" << minNeighbors
+              << " minObjectSize=" << minObjectSize << " maxDetections=" << maxDetections
+              << " minHitRate=" << minHit << " maxFalseAlarm=" << maxFa << " numNeg=" << numNeg
+              << " hardNegCandMul=" << candMul << "\n";
     if (cacheDir.empty()) {
         std::cerr << "[ERR] stageNegCacheDir is required. Dynamic negatives are mandatory.\n";
         return 1;
     }
-    const int px = 24 * 24;
+    const int px = kWin * kWin;
     std::vector<uint8_t> posRaw;
     int posPool = 0;
     if (!load_bin(posPath, px, posRaw, posPool)) return 1;
     const int numPos = std::min(posPool, kPos);
     const int n = numPos + numNeg;
     int poolN = 0;
-    vj_haar_pool(24, 24, 1, nullptr, 0, &poolN);
+    vj_haar_pool(kWin, kWin, 1, nullptr, 0, &poolN);
     std::vector<vj_feature> pool(static_cast<size_t>(poolN));
-    vj_haar_pool(24, 24, 1, pool.data(), poolN, &poolN);
+    vj_haar_pool(kWin, kWin, 1, pool.data(), poolN, &poolN);
     std::cout << "[INFO] Feature count: " << poolN << "\n";
+    const std::vector<std::string> images = collect_pgm_paths(imagesDir);
 
     vj_trainer* tr = nullptr;
-    if (vj_trainer_create(24, 24, n, 1024, &tr) != VJ_OK) return fail("AdaBoostTrainer");
+    if (vj_trainer_create(kWin, kWin, n, 1024, &tr) != VJ_OK) return fail("AdaBoostTrainer");
     struct TGuard {
         vj_trainer* t;
         ~TGuard() { vj_trainer_destroy(t); }
     } tguard{tr};
-    DevMem dPool, dTrain;
+    // the mining detector: one image of up to 2048 x 2048 at a time
+    // (src/main.cu:1614-1626)
+    const int mineMaxW = std::max(2048, kWin), mineMaxH = std::max(2048, kWin);
+    vj_engine* miner = nullptr;
+    if (vj_engine_create(kWin, kWin, 1, mineMaxW, mineMaxH, 0, maxDetections, &miner) != VJ_OK)
+        return fail("FaceVisionEngine");
+    struct EGuard {
+        vj_engine* e;
+        ~EGuard() { vj_engine_destroy(e); }
+    } eguard{miner};
+    DevMem dPool, dTrain, dStumps, dMineImage;
     if (vj_malloc(&dPool.p, pool.size() * sizeof(vj_feature)) != VJ_OK ||
         vj_memcpy_htod(dPool.p, pool.data(), pool.size() * sizeof(vj_feature), nullptr) != VJ_OK ||
-        vj_malloc(&dTrain.p, static_cast<size_t>(n) * px) != VJ_OK)
+        vj_malloc(&dTrain.p, static_cast<size_t>(n) * px) != VJ_OK ||
+        vj_malloc(&dStumps.p, static_cast<size_t>(kTotal) * sizeof(vj_stump)) != VJ_OK ||
+        vj_malloc(&dMineImage.p, static_cast<size_t>(mineMaxW) * mineMaxH) != VJ_OK)
         return fail("device buffers");
     std::vector<int8_t> labels(static_cast<size_t>(n), -1);
     for (int i = 0; i < numPos; ++i) labels[static_cast<size_t>(i)] = 1;
     std::vector<uint8_t> trainRaw(static_cast<size_t>(n) * px);
     std::vector<vj_stump> stumps;
     std::vector<vj_stage> stages;
+    const int mineCap = std::max(20000, numNeg * 2);
+    std::vector<vj_detection> mined(static_cast<size_t>(mineCap));
     const auto t0 = std::chrono::high_resolution_clock::now();
     std::cout << "[TRAIN] Start cascade training: maxStages=" << maxStages << " maxWeakPerStage=" << maxWeak
               << " totalFeatureBudget=" << kTotal << "\n";
     for (int stage = 0; stage < maxStages; ++stage) {
-        if (static_cast<int>(stumps.size()) >= kTotal) break;
+        if (static_cast<int>(stumps.size()) >= kTotal) {
+            std::cout << "[TRAIN] Reached total feature budget " << kTotal << ", stop at stage " << stage << "\n";
+            break;
+        }
         const int stageSeed = seed + stage * 1000003;
         // positives: all, or a seeded subset of the pool (src/main.cu:1582-1598)
         if (posPool <= numPos) {
@@ -313,15 +445,71 @@ int run_train(int argc, char** argv) {
                 std::memcpy(trainRaw.data() + static_cast<size_t>(i) * px,
                             posRaw.data() + static_cast<size_t>(idx[static_cast<size_t>(i)]) * px, px);
         }
-        char name[64];
-        std::snprintf(name, sizeof name, "/stage_negatives/non_faces_stage%02d.bin", stage);
-        std::vector<uint8_t> negRaw;
-        int negCount = 0;
-        if (!load_bin(cacheDir + name, px, negRaw, negCount) || negCount < numNeg) {
-            std::cerr << "[ERR] stage negatives missing or short for stage " << stage << "\n";
-            return 1;
+        uint8_t* neg = trainRaw.data() + static_cast<size_t>(numPos) * px;
+        int hard = 0;
+        std::vector<uint8_t> sample;
+        if (stage == 0) {
+            if (!stage_negatives(cacheDir, images, stage, numNeg, kWin, stageSeed, sample)) {
+                std::cerr << "[ERR] stage negative sampling failed at stage " << stage << "\n";
+                return 1;
+            }
+            std::memcpy(neg, sample.data(), static_cast<size_t>(numNeg) * px);
+            hard = numNeg;
+        } else {
+            // hard negatives: what the cascade so far still accepts
+            if (vj_memcpy_htod(dStumps.p, stumps.data(), stumps.size() * sizeof(vj_stump), nullptr) != VJ_OK ||
+                vj_engine_set_cascade(miner, static_cast<const vj_feature*>(dPool.p), poolN,
+                                      static_cast<const vj_stump*>(dStumps.p), static_cast<int>(stumps.size()),
+                                      stages.data(), static_cast<int>(stages.size()), nullptr) != VJ_OK)
+                return fail("set mining cascade model");
+            if (images.empty()) {
+                std::cerr << "[ERR] No PGM mining images found in " << imagesDir << "\n";
+                return 1;
+            }
+            std::vector<std::string> order = images;
+            std::mt19937 mineRng(static_cast<uint32_t>(stageSeed));
+            std::shuffle(order.begin(), order.end(), mineRng);
+            Gray g;
+            int scanned = 0;
+            for (const std::string& path : order) {
+                if (hard >= numNeg) break;
+                if (!load_gray(path, g)) continue;
+                if (g.w < kWin || g.h < kWin || g.w > mineMaxW || g.h > mineMaxH) continue;
+                if (vj_memcpy_htod(dMineImage.p, g.px.data(), g.px.size(), nullptr) != VJ_OK) continue;
+                int found = 0;
+                if (vj_engine_detect(miner, static_cast<const uint8_t*>(dMineImage.p), g.w, g.h, g.w, scaleFactor, 0,
+                                     kWin, mineCap, 0, mined.data(), mineCap, &found, nullptr) != VJ_OK)
+                    continue;
+                found = std::min(found, mineCap);
+                // the best-scoring boxes of each image only, so that a few
+                // images cannot fill the stage (src/main.cu:1866-1878)
+                const int keep = std::min(found, kPerImageKeep);
+                std::partial_sort(mined.begin(), mined.begin() + keep, mined.begin() + found,
+                                  [](const vj_detection& a, const vj_detection& b) { return a.score > b.score; });
+                for (int i = 0; i < keep && hard < numNeg; ++i, ++hard)
+                    vj_crop_resize(g.px.data(), g.w, g.h, g.w, &mined[static_cast<size_t>(i)], kWin, kWin,
+                                   neg + static_cast<size_t>(hard) * px);
+                ++scanned;
+                if (scanned % 20 == 0)
+                    std::cout << "[MINE] stage=" << stage << " scanned=" << scanned << " hard=" << hard << "/"
+                              << numNeg << "\n";
+            }
+            std::cout << "[TRAIN] stage=" << stage << " hardNegSelected=" << hard << "/" << numNeg << "\n";
         }
-        std::memcpy(trainRaw.data() + static_cast<size_t>(numPos) * px, negRaw.data(), static_cast<size_t>(numNeg) * px);
+        if (hard < numNeg) {
+            // too few hard negatives: random picks from a fresh sample
+            // (src/main.cu:1900-1925)
+            const int fallback = std::max(numNeg, (numNeg - hard) * candMul);
+            if (!stage_negatives(cacheDir, images, stage, fallback, kWin, stageSeed ^ 0x9e3779b9, sample)) {
+                std::cerr << "[ERR] fallback stage negative sampling failed at stage " << stage << "\n";
+                return 1;
+            }
+            std::mt19937 rng(static_cast<uint32_t>(stageSeed) ^ 0x5bd1e995u);
+            std::uniform_int_distribution<int> pick(0, fallback - 1);
+            for (; hard < numNeg; ++hard)
+                std::memcpy(neg + static_cast<size_t>(hard) * px, sample.data() + static_cast<size_t>(pick(rng)) * px,
+                            px);
+        }
         if (vj_memcpy_htod(dTrain.p, trainRaw.data(), trainRaw.size(), nullptr) != VJ_OK) return fail("copy stage set");
         if (vj_trainer_stage_begin(tr, static_cast<const uint8_t*>(dTrain.p), labels.data(), nullptr, n, numPos,
                                    static_cast<const vj_feature*>(dPool.p), poolN, nullptr) != VJ_OK)

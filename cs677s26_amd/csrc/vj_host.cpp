@@ -639,6 +639,36 @@ extern "C" int vj_pgm_load(const char* path, uint8_t* out, size_t cap) {
     return VJ_OK;
 }
 
+// Crop and resample of a mined box, src/main.cu:245-284: source position
+// (x + (o + 0.5) * (w / outW) - 0.5) per axis in float, floor clamped to the
+// image (the fractional weight is taken from the clamped tap, as there),
+// float lerp across then down, round half up, clamp to 0..255.
+extern "C" int vj_crop_resize(const uint8_t* gray, int w, int h, int stride, const vj_detection* box, int outW,
+                              int outH, uint8_t* out) {
+    VJ_CHECK(gray && box && out && w > 0 && h > 0 && stride >= w && outW > 0 && outH > 0,
+             "vj_crop_resize: bad arguments");
+    const float sx = static_cast<float>(std::max(1, box->w));
+    const float sy = static_cast<float>(std::max(1, box->h));
+    for (int oy = 0; oy < outH; ++oy) {
+        for (int ox = 0; ox < outW; ++ox) {
+            const float fx = static_cast<float>(box->x) + (static_cast<float>(ox) + 0.5f) * (sx / outW) - 0.5f;
+            const float fy = static_cast<float>(box->y) + (static_cast<float>(oy) + 0.5f) * (sy / outH) - 0.5f;
+            const int x0 = std::clamp(static_cast<int>(std::floor(fx)), 0, w - 1);
+            const int y0 = std::clamp(static_cast<int>(std::floor(fy)), 0, h - 1);
+            const int x1 = std::min(x0 + 1, w - 1), y1 = std::min(y0 + 1, h - 1);
+            const float ax = fx - static_cast<float>(x0), ay = fy - static_cast<float>(y0);
+            const uint8_t* r0 = gray + static_cast<size_t>(y0) * stride;
+            const uint8_t* r1 = gray + static_cast<size_t>(y1) * stride;
+            const float p00 = r0[x0], p01 = r0[x1], p10 = r1[x0], p11 = r1[x1];
+            const float top = p00 + (p01 - p00) * ax;
+            const float bot = p10 + (p11 - p10) * ax;
+            const float v = top + (bot - top) * ay;
+            out[static_cast<size_t>(oy) * outW + ox] = static_cast<uint8_t>(std::clamp(static_cast<int>(v + 0.5f), 0, 255));
+        }
+    }
+    return VJ_OK;
+}
+
 extern "C" int vj_ppm_save_gray_boxes(const char* path, const uint8_t* gray, int w, int h,
                                       const vj_detection* boxes, int n) {
     VJ_CHECK(path && gray && w > 0 && h > 0 && n >= 0, "vj_ppm_save_gray_boxes: bad arguments");

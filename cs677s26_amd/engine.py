@@ -181,6 +181,19 @@ class FaceVisionEngine:
         return {f: getattr(s, f) for f, _ in DetectStats._fields_}
 
 
+def crop_resize(gray, box, outW=24, outH=24):
+    """cropResizeTo24 (src/main.cu:245-284): box (x, y, w, h) of a host gray
+    image resampled to outW x outH, host C++."""
+    g = np.ascontiguousarray(gray, np.uint8)
+    h, w = g.shape
+    b = np.zeros(1, DETECTION)
+    b["x"], b["y"], b["w"], b["h"] = box[:4]
+    out = np.zeros((outH, outW), np.uint8)
+    check(lib().vj_crop_resize(g.ctypes.data, w, h, w, b.ctypes.data, int(outW), int(outH), out.ctypes.data),
+          "cropResize")
+    return out
+
+
 def group_rectangles(dets, minNeighbors=4, eps=0.2):
     """groupRectanglesCpu (src/face_vision_engine.cu:1401-1539), host C++."""
     d = np.ascontiguousarray(dets, DETECTION)

@@ -113,6 +113,13 @@ int vj_pgm_load(const char* path, uint8_t* hOut, size_t cap);
 int vj_ppm_save_gray_boxes(const char* path, const uint8_t* hGray, int w, int h,
                            const vj_detection* hBoxes, int n);
 
+/* ---- hard-negative crops: cropResizeTo24, src/main.cu:245-284 (host) -----
+ * The box (x, y, w, h) of a gray image (row pitch stride) resampled to
+ * outW x outH by pixel-centre bilinear interpolation with clamped taps (the
+ * reference's window is 24 x 24). ---- */
+int vj_crop_resize(const uint8_t* hGray, int w, int h, int stride, const vj_detection* box,
+                   int outW, int outH, uint8_t* hOut);
+
 /* ==== FaceVisionEngine, include/vj/face_vision_engine.cuh:19-196 ======== */
 typedef struct vj_engine vj_engine;
 

@@ -37,6 +37,7 @@ def lib():
         L.or_train_invnorm.argtypes = [P, P, i, i]
         L.or_train_invnorm.restype = f
         L.or_resize.argtypes = [P, i, i, i, P, i, i]
+        L.or_crop_resize.argtypes = [P, i, i, i, i, i, i, i, i, i, P]
         L.or_levels.argtypes = [i, i, i, i, f, i, P, i]
         L.or_detect_raw.argtypes = [P, i, P, i, P, i, i, i, P, i, i, i, f, i, i, P, i, P]
         L.or_group.argtypes = [P, i, i, f, P]
@@ -84,6 +85,15 @@ def resize(img, dw, dh):
     h, w = img.shape
     out = np.zeros((dh, dw), np.uint8)
     lib().or_resize(_p(img), w, h, w, _p(out), dw, dh)
+    return out
+
+
+def crop_resize(img, box, out_w=24, out_h=24):
+    """cropResizeTo24 (src/main.cu:245-284); box = (x, y, w, h)."""
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape
+    out = np.zeros((out_h, out_w), np.uint8)
+    lib().or_crop_resize(_p(img), w, h, w, int(box[0]), int(box[1]), int(box[2]), int(box[3]), out_w, out_h, _p(out))
     return out
 
 

@@ -178,6 +178,36 @@ void or_resize(const uint8_t* src, int sw, int sh, int sstride, uint8_t* dst, in
     }
 }
 
+/* ---- mined-box crop: cropResizeTo24, src/main.cu:245-284 ----------------- */
+
+void or_crop_resize(const uint8_t* img, int w, int h, int stride, int bx, int by, int bw, int bh,
+                    int outW, int outH, uint8_t* out) {
+    const float sx = (float)MAXI(1, bw);
+    const float sy = (float)MAXI(1, bh);
+    for (int oy = 0; oy < outH; ++oy) {
+        for (int ox = 0; ox < outW; ++ox) {
+            const float srcFx = (float)bx + ((float)ox + 0.5f) * (sx / (float)outW) - 0.5f;
+            const float srcFy = (float)by + ((float)oy + 0.5f) * (sy / (float)outH) - 0.5f;
+            const int x0 = MAXI(0, MINI((int)floorf(srcFx), w - 1));
+            const int y0 = MAXI(0, MINI((int)floorf(srcFy), h - 1));
+            const int x1 = MINI(x0 + 1, w - 1);
+            const int y1 = MINI(y0 + 1, h - 1);
+            const float ax = srcFx - (float)x0;
+            const float ay = srcFy - (float)y0;
+            const float p00 = (float)img[(size_t)y0 * stride + x0];
+            const float p01 = (float)img[(size_t)y0 * stride + x1];
+            const float p10 = (float)img[(size_t)y1 * stride + x0];
+            const float p11 = (float)img[(size_t)y1 * stride + x1];
+            const float top = p00 + (p01 - p00) * ax;
+            const float bot = p10 + (p11 - p10) * ax;
+            const float val = top + (bot - top) * ay;
+            int v = (int)(val + 0.5f);
+            v = v < 0 ? 0 : (v > 255 ? 255 : v);
+            out[(size_t)oy * outW + ox] = (uint8_t)v;
+        }
+    }
+}
+
 /* ---- pyramid schedule: src/main.cu:717-747 ------------------------------ */
 
 int or_levels(int imageW, int imageH, int winW, int winH, float sf, int minObj,

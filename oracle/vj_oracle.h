@@ -50,6 +50,10 @@ float or_train_invnorm(const uint32_t* sum, const uint64_t* sq, int iiW, int iiH
 /* Pixel-centre bilinear resize with +0.5 rounding.  src/main.cu:545-585. */
 void or_resize(const uint8_t* src, int sw, int sh, int sstride, uint8_t* dst, int dw, int dh);
 
+/* cropResizeTo24 (src/main.cu:245-284) for an outW x outH window. */
+void or_crop_resize(const uint8_t* img, int w, int h, int stride, int bx, int by, int bw, int bh,
+                    int outW, int outH, uint8_t* out);
+
 /* Pyramid schedule.  Fills up to cap entries of (scaleIdx, scaledW, scaledH,
  * step, scale) for the levels that are scanned; returns their count.
  * src/main.cu:717-747, src/face_vision_engine.cu:1596-1714. */

@@ -1,0 +1,81 @@
+"""The drop-in command line on the GPU: `cuda_hello detect` against the
+oracle, and `cuda_hello train` end to end (seeded stage-0 negatives, hard
+negatives mined with the cascade trained so far, the top-up sample, and a
+model file the detector loads)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from cs677s26_amd import synth
+from cs677s26_amd.engine import loadCascadeModel, saveCascadeModel
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "cs677s26_amd", "bin", "cuda_hello")
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def write_pgm(path, img):
+    h, w = img.shape
+    with open(path, "wb") as f:
+        f.write(b"P5\n%d %d\n255\n" % (w, h))
+        f.write(np.ascontiguousarray(img, np.uint8).tobytes())
+
+
+def test_cli_detect_matches_oracle(tmp_path):
+    say("cli detect vs oracle")
+    feats, stumps, stages = synth.blob_cascade((1, 10, 25))
+    model = tmp_path / "m.bin"
+    saveCascadeModel(model, (24, 24), feats, stumps, stages)
+    img = synth.detect_image(640, 480, 20, 24, 64, seed=31)
+    write_pgm(tmp_path / "a.pgm", img)
+    r = subprocess.run([EXE, "detect", str(model), str(tmp_path / "a.pgm"), str(tmp_path / "o.ppm")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw, _ = O.detect_raw(feats, stumps, stages, img)
+    want = len(O.group(raw, 4))
+    m = re.search(r"\[DETECT\] detections=(\d+)", r.stdout)
+    assert m and int(m.group(1)) == want
+    ppm = (tmp_path / "o.ppm").read_bytes()
+    assert ppm.startswith(b"P6\n640 480\n255\n") and len(ppm) == len(b"P6\n640 480\n255\n") + 640 * 480 * 3
+
+
+def test_cli_train_mines_hard_negatives(tmp_path):
+    """Three stages on 2,000 synthetic faces: stage 0 samples its negatives
+    from the PGM images (and caches them), stages 1-2 mine hard negatives
+    with the cascade so far and top them up from a seeded sample; the saved
+    model loads and its stages index its stumps."""
+    say("cli train with hard-negative mining")
+    rng = np.random.default_rng(8)
+    faces = synth.face_patches(2000, rng)
+    faces.astype(np.uint8).tofile(tmp_path / "faces_u8.bin")
+    imgs = tmp_path / "neg"
+    imgs.mkdir()
+    for i, (w, h) in enumerate([(320, 240), (400, 300), (256, 256), (500, 200), (300, 420), (640, 480)]):
+        write_pgm(imgs / f"n{i}.pgm", synth.detect_image(w, h, 6, 24, 80, seed=900 + i))
+    cache = tmp_path / "cache"
+    out = tmp_path / "model.bin"
+    args = [EXE, "train", str(tmp_path / "faces_u8.bin"), "10000", "12", "200000", "4", "1.25", "3", "0.99", "0.4",
+            "24", str(cache), str(imgs), "677", "3", str(out)]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    say(r.stdout[-2000:])
+    assert r.returncode == 0, r.stderr
+    assert (cache / "stage_negatives" / "non_faces_stage00.bin").stat().st_size == 10000 * 576
+    assert re.search(r"\[TRAIN\] stage=1 hardNegSelected=(\d+)/10000", r.stdout)
+    win, f, s, g = loadCascadeModel(out)
+    assert win == (24, 24) and len(g) >= 2
+    assert int(g["first"][-1]) + int(g["ntrees"][-1]) == len(s)
+    assert (s["feature"] >= 0).all() and (s["feature"] < len(f)).all()
+    # the same seed gives the same model
+    out2 = tmp_path / "model2.bin"
+    r2 = subprocess.run(args[:-1] + [str(out2)], capture_output=True, text=True, timeout=300)
+    assert r2.returncode == 0, r2.stderr
+    assert out.read_bytes() == out2.read_bytes()

@@ -101,6 +101,18 @@ def test_grouping_sweep_order_negative_and_wide_x():
         assert det_tuples(group_rectangles(d, 2)) == det_tuples(O.group(d, 2)), trial
 
 
+def test_crop_resize_equals_oracle():
+    """Mined-box crops (src/main.cu:245-284) equal the oracle's, byte for
+    byte, for boxes inside, across and beyond the image border."""
+    from cs677s26_amd.engine import crop_resize
+    rng = np.random.default_rng(6)
+    img = rng.integers(0, 256, size=(120, 161), dtype=np.uint8)
+    for _ in range(200):
+        s = int(rng.integers(1, 140))
+        box = (int(rng.integers(-20, 160)), int(rng.integers(-20, 120)), s, int(rng.integers(1, 140)))
+        assert np.array_equal(crop_resize(img, box), O.crop_resize(img, box)), box
+
+
 def test_pgm_reader(tmp_path):
     from cs677s26_amd._native import lib
     import ctypes as C

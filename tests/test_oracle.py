@@ -101,6 +101,29 @@ def test_resize_identity_and_ramp():
     assert np.array_equal(half[0], np.arange(100) * 2 + 1)
 
 
+def test_crop_resize_pinned():
+    """cropResizeTo24's arithmetic: a whole-image box equals the pyramid
+    resize to the same size (the same pixel-centre formula), a same-size box
+    is the identity, a 2x box averages pixel pairs (rounded half up), and a
+    box past the border extrapolates from the clamped taps."""
+    rng = np.random.default_rng(2)
+    img = rng.integers(0, 256, size=(37, 53), dtype=np.uint8)
+    for ow, oh in [(24, 24), (53, 37), (17, 40), (90, 70)]:
+        assert np.array_equal(O.crop_resize(img, (0, 0, 53, 37), ow, oh), O.resize(img, ow, oh))
+    assert np.array_equal(O.crop_resize(img, (7, 5, 24, 24)), img[5:29, 7:31])
+    two = O.crop_resize(img, (4, 2, 48, 30), 24, 15)
+    a = img[2:32, 4:52].astype(np.float32)
+    top = a[0::2, 0::2] + (a[0::2, 1::2] - a[0::2, 0::2]) * np.float32(0.5)
+    bot = a[1::2, 0::2] + (a[1::2, 1::2] - a[1::2, 0::2]) * np.float32(0.5)
+    want = np.floor(top + (bot - top) * np.float32(0.5) + np.float32(0.5)).astype(np.uint8)
+    assert np.array_equal(two, want)
+    # a box starting at -3: the first taps clamp to column 0 with weight -2.5
+    ramp = np.tile((np.arange(53) * 4).astype(np.uint8), (37, 1))
+    edge = O.crop_resize(ramp, (-3, 0, 24, 24))
+    assert int(edge[0, 0]) == 0            # 0 + (4 - 0) * -2.5 = -10 -> clamped to 0
+    assert int(edge[0, 3]) == 0 and int(edge[0, 4]) == 4
+
+
 def test_levels_and_window_count_640():
     lv = O.levels(640, 480)
     assert len(lv) == 14
