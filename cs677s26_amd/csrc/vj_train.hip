@@ -209,13 +209,20 @@ __global__ __launch_bounds__(kSortThreads) void presort_kernel(const vj_feature*
         for (int i = 0; i < kSortMaxPerThread; ++i) {
             if (i < nb) {
                 const uint32_t dg = ((kr[i] - kmin) >> shift) & 0xFFu;
-                uint64_t m = ~0ull;
+                // lanes whose digit differs from this lane's in some bit: OR
+                // over the bits of (lanes with the bit set) XOR (all ones if
+                // this lane has it); the other lanes hold the same digit
+                uint32_t xlo = 0u, xhi = 0u;
 #pragma unroll
                 for (int b = 0; b < 8; ++b) {
-                    const bool bit = (dg >> b) & 1u;
-                    const uint64_t bb = __ballot(bit);
-                    m &= bit ? bb : ~bb;
+                    const uint32_t ext = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(dg), b, 1));
+                    // the ballot of ext != 0, compared on ext itself
+                    uint64_t bb;
+                    asm("v_cmp_ne_u32_e64 %0, 0, %1" : "=s"(bb) : "v"(ext));
+                    xlo = __builtin_amdgcn_bitop3_b32(ext, xlo, static_cast<uint32_t>(bb), 0xDE);   // xlo | (bb ^ ext)
+                    xhi = __builtin_amdgcn_bitop3_b32(ext, xhi, static_cast<uint32_t>(bb >> 32), 0xDE);
                 }
+                const uint64_t m = ~((static_cast<uint64_t>(xhi) << 32) | xlo);
                 const uint32_t below = static_cast<uint32_t>(mask_rank(m));
                 const uint32_t c = wcnt[dg];
                 rk[i] = c + below;
