@@ -381,16 +381,14 @@ __device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, i
         for (int j = 0; j < kScanPer; ++j) {
             D += q[j];
             const uint32_t e = (ew[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
-            if ((e & 0x8000u) && (!kTail || k0 + j < n)) {
-                if (D > hiD) {
-                    hiD = D;
-                    hiK = k0 + j;
-                }
-                if (D < loD) {
-                    loD = D;
-                    loK = k0 + j;
-                }
-            }
+            // branch-free: the two selects chains are independent
+            const bool valid = (e & 0x8000u) && (!kTail || k0 + j < n);
+            const bool up = valid && D > hiD;
+            const bool down = valid && D < loD;
+            hiD = up ? D : hiD;
+            hiK = up ? k0 + j : hiK;
+            loD = down ? D : loD;
+            loK = down ? k0 + j : loK;
         }
         const uint64_t last = static_cast<uint64_t>(inc);
         carry += static_cast<int64_t>(
