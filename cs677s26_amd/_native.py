@@ -91,6 +91,7 @@ _PROTOS = {
                           C.POINTER(I64)]),
     "vj_model_load": (I, [C.c_char_p, P, I64, P, I64, P, I64]),
     "vj_group_rectangles": (I, [P, I, I, F, P, C.POINTER(I)]),
+    "vj_group_rectangles_many": (I, [P, P, I, I, F, P, P]),
     "vj_pgm_peek": (I, [C.c_char_p, C.POINTER(I), C.POINTER(I)]),
     "vj_pgm_load": (I, [C.c_char_p, P, SZ]),
     "vj_ppm_save_gray_boxes": (I, [C.c_char_p, P, I, I, P, I]),

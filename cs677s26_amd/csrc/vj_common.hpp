@@ -119,5 +119,9 @@ private:
 // Host-side helpers implemented in vj_host.cpp and used by the engine.
 int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps,
                      vj_detection* out, int* count);
+// The same for many images at once (in[i], n[i] boxes -> out[i], count[i]),
+// their steps spread over the host thread pool.
+int group_rectangles_many(const vj_detection* const* in, const int* n, int nImages, int minNeighbors, float eps,
+                          vj_detection* const* out, int* count);
 
 }  // namespace vj

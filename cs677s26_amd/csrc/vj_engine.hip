@@ -1215,9 +1215,6 @@ int build_plan(const vj_engine* e, const std::vector<ImageIn>& imgs, float sf, i
     return VJ_OK;
 }
 
-// Host threads that group a batch's images (the box's CPU share per GPU).
-constexpr int kGroupThreads = 16;
-
 int check_image_limits(const vj_engine* e, int w, int h) {
     VJ_CHECK(w <= e->maxW && h <= e->maxH, "image larger than the engine's maxImageW x maxImageH");
     VJ_CHECK(static_cast<int64_t>(w) * h * 255 < (int64_t{1} << 31),
@@ -1809,44 +1806,27 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
     e->stats.integralBytes = P.integralBytes;
     e->stats.cascadeBytes = P.cascadeBytes + static_cast<int64_t>(e->nStumps) * 64 + 24 * total;
 
-    // per image: optional grouping (images in parallel on host threads,
-    // largest first), then copy out in image order
+    // per image: optional grouping (every image's steps spread over the host
+    // thread pool, largest images first), then copy out in image order
     std::vector<int64_t> rawOff(static_cast<size_t>(nImages) + 1, 0);
     for (int i = 0; i < nImages; ++i)
         rawOff[static_cast<size_t>(i) + 1] = rawOff[static_cast<size_t>(i)] + std::min(e->hCounts.get()[i], capPer);
     std::vector<std::vector<vj_detection>> grouped(applyGrouping ? static_cast<size_t>(nImages) : 0);
     if (applyGrouping) {
-        std::vector<int> status(static_cast<size_t>(nImages), VJ_OK);
-        std::vector<int> order(static_cast<size_t>(nImages));
-        std::iota(order.begin(), order.end(), 0);
-        std::sort(order.begin(), order.end(), [&](int a, int b) {
-            return rawOff[a + 1] - rawOff[a] > rawOff[b + 1] - rawOff[b];
-        });
-        std::atomic<int> next{0};
-        auto worker = [&] {
-            for (int k = next++; k < nImages; k = next++) {
-                const int i = order[static_cast<size_t>(k)];
-                const int n = static_cast<int>(rawOff[static_cast<size_t>(i) + 1] - rawOff[static_cast<size_t>(i)]);
-                try {
-                    std::vector<vj_detection>& g = grouped[static_cast<size_t>(i)];
-                    g.resize(static_cast<size_t>(std::max(n, 1)));
-                    int m = 0;
-                    status[static_cast<size_t>(i)] = group_rectangles(e->hOut.get() + rawOff[static_cast<size_t>(i)], n,
-                                                                      minNeighbors, 0.2f, g.data(), &m);
-                    g.resize(static_cast<size_t>(m));
-                } catch (const std::exception&) {
-                    status[static_cast<size_t>(i)] = VJ_OUT_OF_MEMORY;
-                }
-            }
-        };
-        const int hw = static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
-        const int nThreads = std::min({nImages, hw, kGroupThreads});
-        std::vector<std::thread> pool;
-        for (int k = 1; k < nThreads; ++k) pool.emplace_back(worker);
-        worker();
-        for (std::thread& th : pool) th.join();
-        for (int i = 0; i < nImages; ++i)
-            VJ_CHECK(status[static_cast<size_t>(i)] == VJ_OK, "groupRectangles failed (host memory)");
+        std::vector<const vj_detection*> ins(static_cast<size_t>(nImages));
+        std::vector<vj_detection*> outs(static_cast<size_t>(nImages));
+        std::vector<int> ns(static_cast<size_t>(nImages)), cnt(static_cast<size_t>(nImages));
+        for (int i = 0; i < nImages; ++i) {
+            const size_t k = static_cast<size_t>(i);
+            ns[k] = static_cast<int>(rawOff[k + 1] - rawOff[k]);
+            grouped[k].resize(static_cast<size_t>(std::max(ns[k], 1)));
+            ins[k] = e->hOut.get() + rawOff[k];
+            outs[k] = grouped[k].data();
+        }
+        VJ_CHECK(group_rectangles_many(ins.data(), ns.data(), nImages, minNeighbors, 0.2f, outs.data(), cnt.data()) ==
+                     VJ_OK,
+                 "groupRectangles failed (host memory)");
+        for (int i = 0; i < nImages; ++i) grouped[static_cast<size_t>(i)].resize(static_cast<size_t>(cnt[static_cast<size_t>(i)]));
     }
     int64_t outPos = 0;
     for (int i = 0; i < nImages; ++i) {

@@ -363,15 +363,20 @@ struct ClusterAcc {
     int scaleIdx;
 };
 
-// Working memory of one thread's group_rectangles calls, kept between calls
-// so that a call allocates only when its input outgrows every earlier one.
+// Working memory of one grouping, kept between calls so that a call
+// allocates only when its input outgrows every earlier one.
 struct GroupScratch {
+    const vj_detection* in = nullptr;
+    int n = 0, nBlocks = 0;
+    float eps = 0.0f;
     std::vector<int> parent, rank, idx, X, Y, W, H, cnt, members, corder;
     std::vector<uint64_t> key;
     std::vector<std::vector<int>> edges;
     std::vector<ClusterAcc> acc;
     std::vector<vj_detection> clusters;
 };
+
+constexpr int kGroupBlock = 32;  // sweep positions per pair-search task
 
 // Positions 0..n-1 of boxes in (x, index) order into idx: a stable counting
 // sort over x when the x range is moderate, else a sort of (x, index) keys.
@@ -398,55 +403,58 @@ void sweep_order(const vj_detection* d, int n, GroupScratch& g) {
     for (int k = 0; k < n; ++k) g.idx[static_cast<size_t>(k)] = static_cast<int>(g.key[static_cast<size_t>(k)] & 0xffffffffu);
 }
 
-}  // namespace
-
-int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps, vj_detection* out,
-                     int* count) {
-    *count = 0;
-    if (n <= 0) return VJ_OK;
-    // (a named reference, so that the pool's workers reach this thread's
-    // scratch: a thread_local named inside a lambda is the running thread's)
-    thread_local GroupScratch scratch;
-    GroupScratch& g = scratch;
+// Grouping in three steps: prepare (sweep order, the boxes' edges in that
+// order as flat arrays, union-find reset), the pair search in blocks of
+// sweep positions (independent, any thread), and finish (the unions, the
+// clusters, the containment pass).  n >= 1.
+void group_prepare(GroupScratch& g, const vj_detection* in, int n, float eps) {
+    g.in = in;
+    g.n = n;
+    g.eps = eps;
     const size_t un = static_cast<size_t>(n);
     g.parent.resize(un);
     g.rank.assign(un, 0);
     std::iota(g.parent.begin(), g.parent.end(), 0);
-    UnionFind uf{g.parent.data(), g.rank.data()};
-    // sweep order (x, index) and the boxes' edges in that order as flat
-    // arrays, so the sweep reads memory in order
     sweep_order(in, n, g);
     g.X.resize(un);
     g.Y.resize(un);
     g.W.resize(un);
     g.H.resize(un);
-    const int* idx = g.idx.data();
-    int* X = g.X.data();
-    int* Y = g.Y.data();
-    int* W = g.W.data();
-    int* H = g.H.data();
     for (int k = 0; k < n; ++k) {
-        const vj_detection& b = in[idx[k]];
-        X[k] = b.x;
-        Y[k] = b.y;
-        W[k] = b.w;
-        H[k] = b.h;
+        const vj_detection& b = in[g.idx[static_cast<size_t>(k)]];
+        g.X[static_cast<size_t>(k)] = b.x;
+        g.Y[static_cast<size_t>(k)] = b.y;
+        g.W[static_cast<size_t>(k)] = b.w;
+        g.H[static_cast<size_t>(k)] = b.h;
     }
-    // The similar pairs (a, b), a before b in sweep order, are found in
-    // blocks of sweep positions in parallel, then joined in exactly the
-    // sweep's order (a ascending, then b ascending), which is the sequence of
-    // unions the sequential sweep performs.  For each a, the candidates are
-    // the b's whose x is within a's reach; their similarRects test (the
-    // tolerance (int)(eps * (min w + min h) * 0.5f), every edge within it) is
-    // evaluated branch-free over runs of 64 candidates, then the hits are
-    // collected in order.
-    auto similar_after = [&](int a, std::vector<int>& hits) {
+    g.nBlocks = (n + kGroupBlock - 1) / kGroupBlock;
+    if (g.edges.size() < static_cast<size_t>(g.nBlocks)) g.edges.resize(static_cast<size_t>(g.nBlocks));
+}
+
+// The similar pairs (a, b), a before b in sweep order, for the a's of one
+// block: for each a the candidates are the b's whose x is within a's reach;
+// their similarRects test (the tolerance (int)(eps * (min w + min h) *
+// 0.5f), every edge within it) is evaluated branch-free over runs of 64
+// candidates, then the hits are collected in order.  The block's list holds,
+// for each a with hits, a marker -1 - a followed by its b's.
+void group_find_block(GroupScratch& g, int blk) {
+    const int n = g.n;
+    const float eps = g.eps;
+    const int* X = g.X.data();
+    const int* Y = g.Y.data();
+    const int* W = g.W.data();
+    const int* H = g.H.data();
+    std::vector<int>& e = g.edges[static_cast<size_t>(blk)];
+    e.clear();
+    uint8_t flag[64];
+    for (int a = blk * kGroupBlock; a < std::min(n, (blk + 1) * kGroupBlock); ++a) {
+        const size_t mark = e.size();
+        e.push_back(-1 - a);
         const int ax = X[a], ay = Y[a], aw = W[a], ah = H[a];
         const int ar = ax + aw, ab = ay + ah;
         const int reach = static_cast<int>(eps * static_cast<float>(aw + ah) * 0.5f);
         int end = a + 1;
         while (end < n && X[end] - ax <= reach) ++end;
-        uint8_t flag[64];
         for (int b0 = a + 1; b0 < end; b0 += 64) {
             const int m = std::min(64, end - b0);
             const int* xb = X + b0;
@@ -460,29 +468,21 @@ int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps,
                                                (std::abs(ab - (yb[k] + hb[k])) <= d));
             }
             for (int k = 0; k < m; ++k)
-                if (flag[k]) hits.push_back(b0 + k);
+                if (flag[k]) e.push_back(b0 + k);
         }
-    };
-    constexpr int kBlock = 32;
-    const int nBlocks = (n + kBlock - 1) / kBlock;
-    // per block: for each a of the block, its similar b's after a marker -1 - a
-    if (g.edges.size() < static_cast<size_t>(nBlocks)) g.edges.resize(static_cast<size_t>(nBlocks));
-    auto find_block = [&](int blk) {
-        std::vector<int>& e = g.edges[static_cast<size_t>(blk)];
-        e.clear();
-        for (int a = blk * kBlock; a < std::min(n, (blk + 1) * kBlock); ++a) {
-            const size_t mark = e.size();
-            e.push_back(-1 - a);
-            similar_after(a, e);
-            if (e.size() == mark + 1) e.pop_back();
-        }
-    };
-    if (n >= 1024) {
-        HostPool::get().run(nBlocks, find_block);
-    } else {
-        for (int blk = 0; blk < nBlocks; ++blk) find_block(blk);
+        if (e.size() == mark + 1) e.pop_back();
     }
-    for (int blk = 0; blk < nBlocks; ++blk) {
+}
+
+int group_finish(GroupScratch& g, int minNeighbors, vj_detection* out, int* count) {
+    const int n = g.n;
+    const float eps = g.eps;
+    const vj_detection* in = g.in;
+    const int* idx = g.idx.data();
+    UnionFind uf{g.parent.data(), g.rank.data()};
+    // the unions in exactly the sequential sweep's order (a ascending, then
+    // b ascending)
+    for (int blk = 0; blk < g.nBlocks; ++blk) {
         int ra = -1, a = -1;  // ra: root of a's set once a has been joined
         for (int v : g.edges[static_cast<size_t>(blk)]) {
             if (v < 0) {
@@ -493,7 +493,7 @@ int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps,
             }
         }
     }
-    g.acc.resize(un);
+    g.acc.resize(static_cast<size_t>(n));
     ClusterAcc* acc = g.acc.data();
     for (int i = 0; i < n; ++i) acc[i] = ClusterAcc{0, 0, 0, 0, 0, -std::numeric_limits<float>::infinity(), 0};
     for (int i = 0; i < n; ++i) {
@@ -539,9 +539,13 @@ int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps,
         const float lo = static_cast<float>(r1.x + r1.w) - (1.0f + eps) * static_cast<float>(maxW);
         const float hi = static_cast<float>(r1.x) + eps * static_cast<float>(maxW);
         bool inside = false;
-        for (int o : g.corder) {
+        // the clusters left of lo are skipped by the scan; start past them
+        const int loI = static_cast<int>(lo);
+        auto it = std::lower_bound(g.corder.begin(), g.corder.end(), loI,
+                                   [clusters](int o, int v) { return clusters[o].x < v; });
+        for (; it != g.corder.end(); ++it) {
+            const int o = *it;
             const vj_detection& r2 = clusters[o];
-            if (r2.x < static_cast<int>(lo)) continue;
             if (r2.x > static_cast<int>(hi)) break;
             if (o == i) continue;
             const int n2 = g.members[static_cast<size_t>(o)];
@@ -559,7 +563,104 @@ int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps,
     return VJ_OK;
 }
 
+}  // namespace
+
+int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps, vj_detection* out,
+                     int* count) {
+    *count = 0;
+    if (n <= 0) return VJ_OK;
+    thread_local GroupScratch scratch;
+    GroupScratch& g = scratch;
+    group_prepare(g, in, n, eps);
+    if (n >= 1024) {
+        // (an allocation failure on a pool thread is reported, not thrown there)
+        std::vector<char> failed(static_cast<size_t>(g.nBlocks), 0);
+        HostPool::get().run(g.nBlocks, [&](int blk) {
+            try {
+                group_find_block(g, blk);
+            } catch (const std::exception&) {
+                failed[static_cast<size_t>(blk)] = 1;
+            }
+        });
+        for (char f : failed)
+            if (f) return VJ_OUT_OF_MEMORY;
+    } else {
+        for (int blk = 0; blk < g.nBlocks; ++blk) group_find_block(g, blk);
+    }
+    return group_finish(g, minNeighbors, out, count);
+}
+
+int group_rectangles_many(const vj_detection* const* in, const int* n, int nImages, int minNeighbors, float eps,
+                          vj_detection* const* out, int* count) {
+    // every image's steps on the host pool: prepares, then all pair-search
+    // blocks of all images, then finishes, largest images first
+    // (a named reference, so that the pool's workers reach this thread's
+    // scratch: a thread_local named inside a lambda is the running thread's)
+    thread_local std::vector<GroupScratch> scratchTls;
+    std::vector<GroupScratch>& scratch = scratchTls;
+    if (scratch.size() < static_cast<size_t>(nImages)) scratch.resize(static_cast<size_t>(nImages));
+    std::vector<int> order;
+    for (int i = 0; i < nImages; ++i) {
+        count[i] = 0;
+        if (n[i] > 0) order.push_back(i);
+    }
+    std::sort(order.begin(), order.end(), [n](int a, int b) { return n[a] != n[b] ? n[a] > n[b] : a < b; });
+    const int m = static_cast<int>(order.size());
+    HostPool& pool = HostPool::get();
+    // (an allocation failure on a pool thread is recorded, not thrown there)
+    std::vector<char> failed(static_cast<size_t>(m), 0);
+    pool.run(m, [&](int k) {
+        const int i = order[static_cast<size_t>(k)];
+        try {
+            group_prepare(scratch[static_cast<size_t>(i)], in[i], n[i], eps);
+        } catch (const std::exception&) {
+            failed[static_cast<size_t>(k)] = 1;
+        }
+    });
+    for (char f : failed)
+        if (f) return VJ_OUT_OF_MEMORY;
+    std::vector<std::pair<int, int>> tasks;
+    for (int i : order)
+        for (int blk = 0; blk < scratch[static_cast<size_t>(i)].nBlocks; ++blk) tasks.emplace_back(i, blk);
+    std::vector<char> taskFailed(tasks.size(), 0);
+    pool.run(static_cast<int>(tasks.size()), [&](int t) {
+        try {
+            group_find_block(scratch[static_cast<size_t>(tasks[static_cast<size_t>(t)].first)],
+                             tasks[static_cast<size_t>(t)].second);
+        } catch (const std::exception&) {
+            taskFailed[static_cast<size_t>(t)] = 1;
+        }
+    });
+    for (char f : taskFailed)
+        if (f) return VJ_OUT_OF_MEMORY;
+    std::vector<int> status(static_cast<size_t>(nImages), VJ_OK);
+    pool.run(m, [&](int k) {
+        const int i = order[static_cast<size_t>(k)];
+        try {
+            status[static_cast<size_t>(i)] = group_finish(scratch[static_cast<size_t>(i)], minNeighbors, out[i], &count[i]);
+        } catch (const std::exception&) {
+            status[static_cast<size_t>(i)] = VJ_OUT_OF_MEMORY;
+        }
+    });
+    for (int i = 0; i < nImages; ++i)
+        if (status[static_cast<size_t>(i)] != VJ_OK) return status[static_cast<size_t>(i)];
+    return VJ_OK;
+}
+
 }  // namespace vj
+
+extern "C" int vj_group_rectangles_many(const vj_detection* const* in, const int* n, int nImages, int minNeighbors,
+                                        float eps, vj_detection* const* out, int* count) {
+    VJ_CHECK(nImages >= 0 && (nImages == 0 || (in && n && out && count)), "vj_group_rectangles_many: bad arguments");
+    for (int i = 0; i < nImages; ++i)
+        VJ_CHECK(n[i] >= 0 && (n[i] == 0 || (in[i] && out[i])), "vj_group_rectangles_many: bad list");
+    try {
+        return vj::group_rectangles_many(in, n, nImages, minNeighbors, eps, out, count);
+    } catch (const std::exception& ex) {
+        vj::set_error(std::string("vj_group_rectangles_many: ") + ex.what());
+        return VJ_OUT_OF_MEMORY;
+    }
+}
 
 extern "C" int vj_group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps,
                                    vj_detection* out, int* count) {

@@ -106,6 +106,12 @@ int vj_model_load(const char* path, vj_feature* hFeatures, int64_t capFeatures,
  * src/face_vision_engine.cu:1401-1539 (host).  out needs room for n. ---- */
 int vj_group_rectangles(const vj_detection* hIn, int n, int minNeighbors, float eps,
                         vj_detection* hOut, int* count);
+/* The same for nImages boxes lists at once (hIn[i] with n[i] boxes into
+ * hOut[i], which needs room for n[i]; count[i] set), each result equal to
+ * vj_group_rectangles on that list; the work of all lists is spread over the
+ * library's host threads.  No reference counterpart (batched detect uses it). */
+int vj_group_rectangles_many(const vj_detection* const* hIn, const int* n, int nImages, int minNeighbors,
+                             float eps, vj_detection* const* hOut, int* count);
 
 /* ---- PGM / PPM: loadPGM / savePPM, src/main.cu:286-371 -------------------- */
 int vj_pgm_peek(const char* path, int* w, int* h);

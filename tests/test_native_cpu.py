@@ -101,6 +101,34 @@ def test_grouping_sweep_order_negative_and_wide_x():
         assert det_tuples(group_rectangles(d, 2)) == det_tuples(O.group(d, 2)), trial
 
 
+def test_grouping_many_lists_equals_one_by_one():
+    """vj_group_rectangles_many (every list's steps on the host pool) returns
+    for each list what vj_group_rectangles returns for it alone: lists of 0,
+    1, a few hundred and several thousand boxes."""
+    import ctypes as C
+    from cs677s26_amd._native import lib
+    rng = np.random.default_rng(12)
+    g = np.load(os.path.join(ROOT, "tests", "golden", "detect_c3.npz"))
+    lists = [np.zeros(0, DETECTION), g["raw"][:1].copy(), g["raw"], g["raw"][::-1].copy()]
+    for n in (300, 2500, 7000):
+        d = np.zeros(n, DETECTION)
+        centres = rng.integers(0, 3000, size=(n // 20 + 1, 2))
+        for i in range(n):
+            cx, cy = centres[rng.integers(0, len(centres))]
+            s = int(rng.integers(24, 120))
+            d[i] = (cx + rng.integers(-8, 9), cy + rng.integers(-8, 9), s, s, 0, np.float32(rng.normal()))
+        lists.append(d)
+    k = len(lists)
+    outs = [np.zeros(max(1, len(x)), DETECTION) for x in lists]
+    ins = (C.c_void_p * k)(*[x.ctypes.data if len(x) else None for x in lists])
+    ons = (C.c_void_p * k)(*[o.ctypes.data for o in outs])
+    ns = (C.c_int * k)(*[len(x) for x in lists])
+    cnt = (C.c_int * k)()
+    assert lib().vj_group_rectangles_many(ins, ns, k, 3, C.c_float(0.2), ons, cnt) == 0
+    for x, o, c in zip(lists, outs, cnt):
+        assert det_tuples(o[:c]) == det_tuples(group_rectangles(x, 3))
+
+
 def test_crop_resize_equals_oracle():
     """Mined-box crops (src/main.cu:245-284) equal the oracle's, byte for
     byte, for boxes inside, across and beyond the image border."""
