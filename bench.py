@@ -128,10 +128,10 @@ def served_leg(args):
 
     arrivals, imgs = workloads.served(args.requests, args.qps)
     feats, stumps, stages = workloads.cascade_38()
-    srv = DetectServer(feats, stumps, stages, max_image=(3840, 2160), slots=args.slots, max_batch=8,
+    srv = DetectServer(feats, stumps, stages, max_image=(3840, 2160), slots=args.slots, max_batch=args.max_batch,
                        hbm_budget=16 << 30)
     # warm-up: every slot sees a batch of the largest requests
-    big = sorted(range(len(imgs)), key=lambda i: -imgs[i].size)[:8 * args.slots]
+    big = sorted(range(len(imgs)), key=lambda i: -imgs[i].size)[:args.max_batch * args.slots]
     for t in [srv.submit(imgs[i]) for i in big]:
         srv.wait(t)
     n = len(imgs)
@@ -174,10 +174,24 @@ def served_leg(args):
     sat = time.perf_counter() - t1
     st = srv.stats()
     srv.close()
+    # the plain per-image path on the same requests (one engine, image by image)
+    from cs677s26_amd.device import DeviceArray, synchronize
+    from cs677s26_amd.engine import FaceVisionEngine
+    eng = FaceVisionEngine((24, 24), 1, 3840, 2160, False, 200000)
+    eng.set_cascade_host(feats, stumps, stages)
+    dev = [DeviceArray.from_host(im) for im in imgs]
+    eng.detectMultiScale(dev[0], imgs[0].shape[1], imgs[0].shape[0], imgs[0].shape[1], 1.25, 4, 24, 200000, True)
+    synchronize()
+    t2 = time.perf_counter()
+    for d, im in zip(dev, imgs):
+        eng.detectMultiScale(d, im.shape[1], im.shape[0], im.shape[1], 1.25, 4, 24, 200000, True)
+    synchronize()
+    plain = time.perf_counter() - t2
     return {"qps": args.qps, "requests": n, "mpix": mpix, "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)), "server_p50_ms": float(np.percentile(srv_lat, 50)),
             "server_p99_ms": float(np.percentile(srv_lat, 99)), "achieved_qps": n / span,
-            "mpix_per_s": mpix / span, "saturated_mpix_per_s": mpix / sat, "slots": args.slots,
+            "mpix_per_s": mpix / span, "saturated_mpix_per_s": mpix / sat,
+            "plain_per_image_mpix_per_s": mpix / plain, "saturated_speedup_vs_plain": plain / sat, "slots": args.slots,
             "max_slots_busy": st["maxSlotsBusy"], "max_batch": st["maxBatchSeen"], "batches": st["batches"]}
 
 
@@ -318,6 +332,7 @@ def main():
     ap.add_argument("--requests", type=int, default=240, help="served leg: requests in the Poisson stream")
     ap.add_argument("--qps", type=float, default=100.0, help="served leg: arrival rate")
     ap.add_argument("--slots", type=int, default=3, help="served leg: device slots")
+    ap.add_argument("--max-batch", type=int, default=8, help="served leg: requests per slot batch")
     ap.add_argument("--skip", default="", help="comma list of legs to skip: detect,batched,served,stump,cpu")
     args = ap.parse_args()
     if args.gpus != 1:
