@@ -217,9 +217,18 @@ def stump_leg(args):
         rounds.append(tr.round(0.99, 0.40))
     synchronize()
     t2 = time.perf_counter()
-    return {"s_per_stage": t2 - t0, "presort_s": t1 - t0, "round_ms": 1e3 * (t2 - t1) / max(1, args.rounds),
+    round_ms = 1e3 * (t2 - t1) / max(1, args.rounds)
+    # the threshold scan streams each feature's u16 order with its split bit:
+    # F*N*2 + F*N/8 bytes per round, over the whole round's wall time
+    scan_bytes = len(pool) * n * 2 + len(pool) * n // 8
+    scan_gbs = scan_bytes / (round_ms * 1e-3) / 1e9 if round_ms > 0 else 0.0
+    return {"s_per_stage": t2 - t0, "presort_s": t1 - t0, "round_ms": round_ms,
             "rounds": args.rounds, "features": len(pool), "samples": n,
-            "first_feature": rounds[0]["feature"] if rounds else -1}
+            "first_feature": rounds[0]["feature"] if rounds else -1,
+            "search_roofline": {"bound": "hbm", "kernel": "search_kernel (whole round)", "achieved": scan_gbs,
+                                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": scan_gbs / HBM_PEAK_GBS,
+                                "algorithmic_bytes": scan_bytes, "avg_ms": round_ms,
+                                "traffic": measured_traffic_search()}}
 
 
 def cpu_leg(args):
@@ -247,6 +256,20 @@ def measured_traffic(name):
         with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_dispatch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
+def measured_traffic_search():
+    """HBM bytes of one search_kernel dispatch from the committed rocprofv3
+    --pmc summary of the stump search (None if not measured)."""
+    path = os.path.join(ROOT, "profiles", "stump_search_162336x20000_kernels.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_dispatch (search_kernel):"):
                     return float(line.split(":", 1)[1])
     except OSError:
         return None

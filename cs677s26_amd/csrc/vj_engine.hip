@@ -938,6 +938,198 @@ struct Plan {
 
 using namespace vj;
 
+// ---------------------------------------------------------------------------
+// Grouping help on the device: the sweep order and the similar pairs of each
+// image's raw boxes (src/face_vision_engine.cu:1417-1445), so that the host
+// only replays the unions and builds the clusters.  Box a's sweep rank is the
+// number of boxes before it in (x, index) order; its pairs are the boxes b
+// after it in that order with similarRects(a, b) (the tolerance (int)(eps *
+// (min w + min h) * 0.5f), every edge within it), which all lie within a's
+// x reach, eps * (w + h) / 2.  Images with more than kGroupDevMax raw boxes
+// are left to the host.  Every kernel derives the images' box counts from
+// the device counts.
+// ---------------------------------------------------------------------------
+constexpr int kGroupDevMax = 16384;
+constexpr int kPairTile = 256;
+
+__device__ __forceinline__ int grouped_boxes(const int32_t* __restrict__ counts, int i, int cap) {
+    const int n = min(counts[i], cap);
+    return n <= kGroupDevMax ? n : 0;
+}
+
+__device__ __forceinline__ int64_t all_boxes(const int32_t* __restrict__ counts, int nImages, int cap) {
+    int64_t total = 0;
+    for (int i = 0; i < nImages; ++i) total += min(counts[i], cap);
+    return total;
+}
+
+// The image of box g (boxes of the images concatenated, base imgBase[i]).
+__device__ __forceinline__ int image_of(const int32_t* __restrict__ imgBase, int nImages, int64_t g) {
+    int lo = 0, hi = nImages - 1;  // the last image whose base is at most g
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (imgBase[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Zeroes the sweep ranks of all raw boxes of the call.
+__global__ __launch_bounds__(256) void pair_zero_kernel(const int32_t* __restrict__ counts, int nImages, int cap,
+                                                       int32_t* __restrict__ rank) {
+    const int64_t total = all_boxes(counts, nImages, cap);
+    for (int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; g < total;
+         g += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        rank[g] = 0;
+}
+
+// Sweep ranks: work items (image, tile of 256 a's, tile of 256 b's); each
+// adds the count of the tile's boxes before a to rank[a].
+__global__ __launch_bounds__(kPairTile) void pair_rank_kernel(const vj_detection* __restrict__ dets,
+                                                             const int32_t* __restrict__ counts,
+                                                             const int32_t* __restrict__ imgBase, int nImages,
+                                                             int cap, int32_t* __restrict__ rank) {
+    __shared__ int sx[kPairTile];
+    int64_t totalItems = 0;
+    for (int i = 0; i < nImages; ++i) {
+        const int64_t nt = (grouped_boxes(counts, i, cap) + kPairTile - 1) / kPairTile;
+        totalItems += nt * nt;
+    }
+    for (int64_t item = blockIdx.x; item < totalItems; item += gridDim.x) {
+        int i = 0, n = 0;
+        int64_t base = 0, nt = 0;
+        for (;; ++i) {
+            n = grouped_boxes(counts, i, cap);
+            nt = (n + kPairTile - 1) / kPairTile;
+            if (item < base + nt * nt) break;
+            base += nt * nt;
+        }
+        const int64_t local = item - base;
+        const int at = static_cast<int>(local / nt), jt = static_cast<int>(local - at * nt);
+        const vj_detection* D = dets + imgBase[i];
+        const int j0 = jt * kPairTile;
+        const int jn = min(kPairTile, n - j0);
+        __syncthreads();  // the previous item's tile is no longer read
+        if (static_cast<int>(threadIdx.x) < jn) sx[threadIdx.x] = D[j0 + threadIdx.x].x;
+        __syncthreads();
+        const int a = at * kPairTile + static_cast<int>(threadIdx.x);
+        if (a >= n) continue;
+        const int ax = D[a].x;
+        int before = 0;
+        for (int k = 0; k < jn; ++k) {
+            const int bx = sx[k];
+            before += (bx < ax) | ((bx == ax) & (j0 + k < a));
+        }
+        if (before) atomicAdd(&rank[imgBase[i] + a], before);
+    }
+}
+
+// The boxes in sweep order: sorted[base + rank] = the box's edges, order[base
+// + rank] = its index in its image.
+__global__ __launch_bounds__(256) void pair_place_kernel(const vj_detection* __restrict__ dets,
+                                                        const int32_t* __restrict__ counts,
+                                                        const int32_t* __restrict__ imgBase, int nImages, int cap,
+                                                        const int32_t* __restrict__ rank, int4* __restrict__ sorted,
+                                                        int32_t* __restrict__ order) {
+    const int64_t total = all_boxes(counts, nImages, cap);
+    for (int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; g < total;
+         g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int i = image_of(imgBase, nImages, g);
+        if (grouped_boxes(counts, i, cap) == 0) continue;
+        const int64_t p = imgBase[i] + rank[g];
+        const vj_detection d = dets[g];
+        sorted[p] = make_int4(d.x, d.y, d.w, d.h);
+        order[p] = static_cast<int32_t>(g - imgBase[i]);
+    }
+}
+
+// Per sweep position p, one wavefront: the similar boxes after it, 64
+// candidates at a time in sweep order until the x reach is passed (kWrite
+// false: their count into pairCnt[p]; true: their sweep ranks, ascending,
+// from pairOff[p], placed by ballot rank so that the order is kept).
+template <bool kWrite>
+__global__ __launch_bounds__(256) void pair_find_kernel(const int32_t* __restrict__ counts,
+                                                       const int32_t* __restrict__ imgBase, int nImages, int cap,
+                                                       float eps, const int4* __restrict__ sorted,
+                                                       int32_t* __restrict__ pairCnt,
+                                                       const int32_t* __restrict__ pairOff,
+                                                       int32_t* __restrict__ pairs, int64_t pairCap) {
+    const int64_t total = all_boxes(counts, nImages, cap);
+    const int lane = lane_id();
+    const int64_t wavesTotal = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
+    for (int64_t p = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; p < total;
+         p += wavesTotal) {
+        const int i = image_of(imgBase, nImages, p);
+        const int n = grouped_boxes(counts, i, cap);
+        if (n == 0) {  // an image left to the host: no pairs here
+            if constexpr (!kWrite) {
+                if (lane == 0) pairCnt[p] = 0;
+            }
+            continue;
+        }
+        const int64_t base = imgBase[i], end = base + n;
+        const int4 A = sorted[p];
+        const int ar = A.x + A.z, ab = A.y + A.w;
+        const int reach = static_cast<int>(eps * static_cast<float>(A.z + A.w) * 0.5f);
+        int64_t o = 0;
+        if constexpr (kWrite) {
+            o = pairOff[p];
+            if (o + pairCnt[p] > pairCap) continue;
+        }
+        int found = 0;
+        for (int64_t q0 = p + 1; q0 < end; q0 += kWave) {
+            const int64_t q = q0 + lane;
+            const bool valid = q < end;
+            const int4 B = valid ? sorted[q] : A;
+            const bool inReach = valid && B.x - A.x <= reach;
+            bool sim = false;
+            if (inReach) {
+                const int d = static_cast<int>(eps * static_cast<float>(min(A.z, B.z) + min(A.w, B.w)) * 0.5f);
+                sim = abs(A.x - B.x) <= d && abs(A.y - B.y) <= d && abs(ar - (B.x + B.z)) <= d &&
+                      abs(ab - (B.y + B.w)) <= d;
+            }
+            const uint64_t m = __ballot(sim);
+            if constexpr (kWrite) {
+                if (sim) pairs[o + found + mask_rank(m)] = static_cast<int32_t>(q - base);
+            }
+            found += __popcll(m);
+            if (__ballot(valid && !inReach)) break;  // sweep order: the rest is out of reach too
+        }
+        if constexpr (!kWrite) {
+            if (lane == 0) pairCnt[p] = found;
+        }
+    }
+}
+
+// Exclusive scan of the pair counts of all sweep positions (one workgroup):
+// list offsets, and the total into ctl[0].
+__global__ __launch_bounds__(1024) void pair_scan_kernel(const int32_t* __restrict__ counts, int nImages, int cap,
+                                                         const int32_t* __restrict__ pairCnt,
+                                                         int32_t* __restrict__ pairOff, int64_t* __restrict__ ctl) {
+    __shared__ int64_t waveTot[16];
+    __shared__ int64_t carry;
+    const int64_t total = all_boxes(counts, nImages, cap);
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int64_t c0 = 0; c0 < total; c0 += blockDim.x) {
+        const int64_t g = c0 + threadIdx.x;
+        const int64_t v = g < total ? pairCnt[g] : 0;
+        const int64_t inc = static_cast<int64_t>(wave_incl_scan(static_cast<uint64_t>(v)));
+        if ((threadIdx.x & 63) == 63) waveTot[threadIdx.x >> 6] = inc;
+        __syncthreads();
+        int64_t off = carry + inc - v, tot = 0;
+        for (int k = 0; k < static_cast<int>(blockDim.x >> 6); ++k) {
+            if (k < static_cast<int>(threadIdx.x >> 6)) off += waveTot[k];
+            tot += waveTot[k];
+        }
+        if (g < total) pairOff[g] = static_cast<int32_t>(min<int64_t>(off, INT32_MAX));
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ctl[0] = carry;
+}
+
 struct vj_engine {
     int winW = 24, winH = 24, maxSamples = 1, maxW = 1, maxH = 1, maxDet = 1;
     bool tilted = false;
@@ -977,6 +1169,14 @@ struct vj_engine {
     DevBuf<vj_detection> dOut;
     PinnedBuf<vj_detection> hOut;
     PinnedBuf<int32_t> hCounts;
+    // grouping help: per raw box its sweep rank, pair count, pair-list offset
+    // and a cursor; per image the boxes in sweep order; the pair lists
+    DevBuf<int32_t> gRank, gPairCnt, gPairOff, gOrder, gPairs;
+    DevBuf<int4> gSorted;
+    DevBuf<int64_t> gCtl;
+    PinnedBuf<int32_t> hOrder, hPairOff, hPairCnt, hPairs;
+    PinnedBuf<int64_t> hCtl;
+    int64_t pairCap = 0;  // capacity of gPairs (grows after an overflow)
     PinnedBuf<uint8_t> hStage;  // staging for plan uploads
 
     vj_detect_stats stats{};
@@ -1782,6 +1982,41 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
     VJ_TRY(build_plan(e, imgs, sf, minObj, P));
     const int nImages = static_cast<int>(imgs.size());
     VJ_TRY(run_pipeline(e, P, capPer, s));
+    // grouping help on the device: sweep ranks and similar pairs per image
+    const bool devGroup = applyGrouping != 0;
+    if (devGroup) {
+        const size_t capTotal = static_cast<size_t>(capPer) * static_cast<size_t>(nImages);
+        if (e->pairCap == 0) e->pairCap = static_cast<int64_t>(std::min<size_t>(capTotal, kGroupDevMax) * 16 + 65536);
+        VJ_TRY(e->gRank.ensure(capTotal));
+        VJ_TRY(e->gPairCnt.ensure(capTotal));
+        VJ_TRY(e->gPairOff.ensure(capTotal));
+        VJ_TRY(e->gSorted.ensure(capTotal));
+        VJ_TRY(e->gOrder.ensure(capTotal));
+        VJ_TRY(e->gPairs.ensure(static_cast<size_t>(e->pairCap)));
+        VJ_TRY(e->gCtl.ensure(1));
+        constexpr int kPairGrid = 1024;
+        hipLaunchKernelGGL(pair_zero_kernel, dim3(64), dim3(256), 0, s, e->dCounts.get(), nImages, capPer,
+                           e->gRank.get());
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(pair_rank_kernel, dim3(kPairGrid), dim3(kPairTile), 0, s, e->dOut.get(), e->dCounts.get(),
+                           e->dImgBase.get(), nImages, capPer, e->gRank.get());
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(pair_place_kernel, dim3(1024), dim3(256), 0, s, e->dOut.get(), e->dCounts.get(),
+                           e->dImgBase.get(), nImages, capPer, e->gRank.get(), e->gSorted.get(), e->gOrder.get());
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(pair_find_kernel<false>, dim3(2048), dim3(256), 0, s, e->dCounts.get(), e->dImgBase.get(),
+                           nImages, capPer, 0.2f, e->gSorted.get(), e->gPairCnt.get(), nullptr, nullptr, int64_t{0});
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(pair_scan_kernel, dim3(1), dim3(1024), 0, s, e->dCounts.get(), nImages, capPer,
+                           e->gPairCnt.get(), e->gPairOff.get(), e->gCtl.get());
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(pair_find_kernel<true>, dim3(2048), dim3(256), 0, s, e->dCounts.get(), e->dImgBase.get(),
+                           nImages, capPer, 0.2f, e->gSorted.get(), e->gPairCnt.get(), e->gPairOff.get(),
+                           e->gPairs.get(), e->pairCap);
+        VJ_HIP(hipGetLastError());
+        VJ_TRY(e->hCtl.ensure(1));
+        VJ_HIP(hipMemcpyAsync(e->hCtl.get(), e->gCtl.get(), sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    }
     VJ_TRY(e->hCounts.ensure(static_cast<size_t>(nImages)));
     VJ_HIP(hipMemcpyAsync(e->hCounts.get(), e->dCounts.get(), sizeof(int32_t) * nImages, hipMemcpyDeviceToHost, s));
     VJ_HIP(hipStreamSynchronize(s));
@@ -1794,8 +2029,34 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
         total += std::min(e->hCounts.get()[i], capPer);
     }
     VJ_TRY(e->hOut.ensure(static_cast<size_t>(std::max<int64_t>(total, 1))));
+    // the device pairs are used when they all fit; otherwise the host finds
+    // them, and the next call has room
+    bool usePairs = false;
+    int64_t nPairs = 0;
+    if (devGroup && total > 0) {
+        nPairs = e->hCtl.get()[0];
+        if (nPairs <= e->pairCap) {
+            usePairs = true;
+        } else {
+            e->pairCap = nPairs + nPairs / 4 + 65536;
+        }
+    }
     if (total > 0) {
         VJ_HIP(hipMemcpyAsync(e->hOut.get(), e->dOut.get(), sizeof(vj_detection) * total, hipMemcpyDeviceToHost, s));
+        if (usePairs) {
+            VJ_TRY(e->hOrder.ensure(static_cast<size_t>(total)));
+            VJ_TRY(e->hPairOff.ensure(static_cast<size_t>(total)));
+            VJ_TRY(e->hPairCnt.ensure(static_cast<size_t>(total)));
+            VJ_TRY(e->hPairs.ensure(static_cast<size_t>(std::max<int64_t>(nPairs, 1))));
+            VJ_HIP(hipMemcpyAsync(e->hOrder.get(), e->gOrder.get(), sizeof(int32_t) * total, hipMemcpyDeviceToHost, s));
+            VJ_HIP(hipMemcpyAsync(e->hPairOff.get(), e->gPairOff.get(), sizeof(int32_t) * total, hipMemcpyDeviceToHost,
+                                  s));
+            VJ_HIP(hipMemcpyAsync(e->hPairCnt.get(), e->gPairCnt.get(), sizeof(int32_t) * total, hipMemcpyDeviceToHost,
+                                  s));
+            if (nPairs > 0)
+                VJ_HIP(hipMemcpyAsync(e->hPairs.get(), e->gPairs.get(), sizeof(int32_t) * nPairs,
+                                      hipMemcpyDeviceToHost, s));
+        }
         VJ_HIP(hipStreamSynchronize(s));
     }
     e->stats.levels = 0;
@@ -1823,8 +2084,19 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
             ins[k] = e->hOut.get() + rawOff[k];
             outs[k] = grouped[k].data();
         }
-        VJ_CHECK(group_rectangles_many(ins.data(), ns.data(), nImages, minNeighbors, 0.2f, outs.data(), cnt.data()) ==
-                     VJ_OK,
+        std::vector<GroupRanked> ranked(static_cast<size_t>(nImages));
+        if (usePairs) {
+            for (int i = 0; i < nImages; ++i) {
+                const size_t k = static_cast<size_t>(i);
+                if (ns[k] == 0 || ns[k] > kGroupDevMax) continue;
+                ranked[k].order = e->hOrder.get() + rawOff[k];
+                ranked[k].pairOff = e->hPairOff.get() + rawOff[k];
+                ranked[k].pairCnt = e->hPairCnt.get() + rawOff[k];
+                ranked[k].pairs = e->hPairs.get();
+            }
+        }
+        VJ_CHECK(group_rectangles_many(ins.data(), ns.data(), nImages, minNeighbors, 0.2f, outs.data(), cnt.data(),
+                                       ranked.data()) == VJ_OK,
                  "groupRectangles failed (host memory)");
         for (int i = 0; i < nImages; ++i) grouped[static_cast<size_t>(i)].resize(static_cast<size_t>(cnt[static_cast<size_t>(i)]));
     }

@@ -403,6 +403,17 @@ void sweep_order(const vj_detection* d, int n, GroupScratch& g) {
     for (int k = 0; k < n; ++k) g.idx[static_cast<size_t>(k)] = static_cast<int>(g.key[static_cast<size_t>(k)] & 0xffffffffu);
 }
 
+// Union-find reset for n boxes (the sweep order comes from elsewhere).
+void group_reset(GroupScratch& g, const vj_detection* in, int n, float eps) {
+    g.in = in;
+    g.n = n;
+    g.eps = eps;
+    g.parent.resize(static_cast<size_t>(n));
+    g.rank.assign(static_cast<size_t>(n), 0);
+    std::iota(g.parent.begin(), g.parent.end(), 0);
+    g.nBlocks = 0;
+}
+
 // Grouping in three steps: prepare (sweep order, the boxes' edges in that
 // order as flat arrays, union-find reset), the pair search in blocks of
 // sweep positions (independent, any thread), and finish (the unions, the
@@ -474,25 +485,14 @@ void group_find_block(GroupScratch& g, int blk) {
     }
 }
 
-int group_finish(GroupScratch& g, int minNeighbors, vj_detection* out, int* count) {
+// The clusters of the unions made so far (integer-mean boxes with the first
+// best score, more than max(1, minNeighbors) members), then the containment
+// pass, src/face_vision_engine.cu:1447-1537.
+int group_clusters(GroupScratch& g, int minNeighbors, vj_detection* out, int* count) {
     const int n = g.n;
     const float eps = g.eps;
     const vj_detection* in = g.in;
-    const int* idx = g.idx.data();
     UnionFind uf{g.parent.data(), g.rank.data()};
-    // the unions in exactly the sequential sweep's order (a ascending, then
-    // b ascending)
-    for (int blk = 0; blk < g.nBlocks; ++blk) {
-        int ra = -1, a = -1;  // ra: root of a's set once a has been joined
-        for (int v : g.edges[static_cast<size_t>(blk)]) {
-            if (v < 0) {
-                a = -1 - v;
-                ra = -1;
-            } else {
-                ra = uf.join_root(ra < 0 ? uf.root(idx[a]) : ra, idx[v]);
-            }
-        }
-    }
     g.acc.resize(static_cast<size_t>(n));
     ClusterAcc* acc = g.acc.data();
     for (int i = 0; i < n; ++i) acc[i] = ClusterAcc{0, 0, 0, 0, 0, -std::numeric_limits<float>::infinity(), 0};
@@ -563,6 +563,40 @@ int group_finish(GroupScratch& g, int minNeighbors, vj_detection* out, int* coun
     return VJ_OK;
 }
 
+// The unions in exactly the sequential sweep's order (a ascending, then b
+// ascending) from the pair-search blocks, then the clusters.
+int group_finish(GroupScratch& g, int minNeighbors, vj_detection* out, int* count) {
+    const int* idx = g.idx.data();
+    UnionFind uf{g.parent.data(), g.rank.data()};
+    for (int blk = 0; blk < g.nBlocks; ++blk) {
+        int ra = -1, a = -1;  // ra: root of a's set once a has been joined
+        for (int v : g.edges[static_cast<size_t>(blk)]) {
+            if (v < 0) {
+                a = -1 - v;
+                ra = -1;
+            } else {
+                ra = uf.join_root(ra < 0 ? uf.root(idx[a]) : ra, idx[v]);
+            }
+        }
+    }
+    return group_clusters(g, minNeighbors, out, count);
+}
+
+// The same unions from a sweep order and pair lists computed elsewhere:
+// order[r] is the box of sweep rank r; its pairs are the sweep ranks
+// pairs[pairOff[r] .. pairOff[r] + pairCnt[r]), ascending.
+int group_finish_ranked(GroupScratch& g, const GroupRanked& rk, int minNeighbors, vj_detection* out, int* count) {
+    UnionFind uf{g.parent.data(), g.rank.data()};
+    for (int r = 0; r < g.n; ++r) {
+        const int a = rk.order[r];
+        const int32_t* L = rk.pairs + rk.pairOff[r];
+        const int m = rk.pairCnt[r];
+        int ra = -1;
+        for (int k = 0; k < m; ++k) ra = uf.join_root(ra < 0 ? uf.root(a) : ra, rk.order[L[k]]);
+    }
+    return group_clusters(g, minNeighbors, out, count);
+}
+
 }  // namespace
 
 int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps, vj_detection* out,
@@ -591,7 +625,7 @@ int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps,
 }
 
 int group_rectangles_many(const vj_detection* const* in, const int* n, int nImages, int minNeighbors, float eps,
-                          vj_detection* const* out, int* count) {
+                          vj_detection* const* out, int* count, const GroupRanked* ranked) {
     // every image's steps on the host pool: prepares, then all pair-search
     // blocks of all images, then finishes, largest images first
     // (a named reference, so that the pool's workers reach this thread's
@@ -612,7 +646,10 @@ int group_rectangles_many(const vj_detection* const* in, const int* n, int nImag
     pool.run(m, [&](int k) {
         const int i = order[static_cast<size_t>(k)];
         try {
-            group_prepare(scratch[static_cast<size_t>(i)], in[i], n[i], eps);
+            if (ranked && ranked[i].order)
+                group_reset(scratch[static_cast<size_t>(i)], in[i], n[i], eps);
+            else
+                group_prepare(scratch[static_cast<size_t>(i)], in[i], n[i], eps);
         } catch (const std::exception&) {
             failed[static_cast<size_t>(k)] = 1;
         }
@@ -637,7 +674,10 @@ int group_rectangles_many(const vj_detection* const* in, const int* n, int nImag
     pool.run(m, [&](int k) {
         const int i = order[static_cast<size_t>(k)];
         try {
-            status[static_cast<size_t>(i)] = group_finish(scratch[static_cast<size_t>(i)], minNeighbors, out[i], &count[i]);
+            GroupScratch& g = scratch[static_cast<size_t>(i)];
+            status[static_cast<size_t>(i)] = (ranked && ranked[i].order)
+                                                 ? group_finish_ranked(g, ranked[i], minNeighbors, out[i], &count[i])
+                                                 : group_finish(g, minNeighbors, out[i], &count[i]);
         } catch (const std::exception&) {
             status[static_cast<size_t>(i)] = VJ_OUT_OF_MEMORY;
         }
@@ -655,7 +695,7 @@ extern "C" int vj_group_rectangles_many(const vj_detection* const* in, const int
     for (int i = 0; i < nImages; ++i)
         VJ_CHECK(n[i] >= 0 && (n[i] == 0 || (in[i] && out[i])), "vj_group_rectangles_many: bad list");
     try {
-        return vj::group_rectangles_many(in, n, nImages, minNeighbors, eps, out, count);
+        return vj::group_rectangles_many(in, n, nImages, minNeighbors, eps, out, count, nullptr);
     } catch (const std::exception& ex) {
         vj::set_error(std::string("vj_group_rectangles_many: ") + ex.what());
         return VJ_OUT_OF_MEMORY;

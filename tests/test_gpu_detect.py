@@ -256,3 +256,25 @@ def test_pair_mode_margins_near_threshold(cascade7, offset):
     eng = _engine(640, 480, (feats, st, stages))
     raw = eng.detectMultiScale(DeviceArray.from_host(img), 640, 480, 640, 1.25, 4, 24, 200000, applyGrouping=False)
     assert det_tuples(raw) == det_tuples(want)
+
+
+def test_grouping_device_pairs_capacity_and_host_images():
+    """Grouping with the device's sweep order and pairs: a dense 16,000-box
+    image (about 40 pairs per box) overflows the first call's pair capacity,
+    so that call groups on the host and the next one on the device pairs;
+    a 20,000-box image (above the device limit) always groups on the host.
+    Every result equals the oracle's grouping of the oracle's raw boxes."""
+    feats, stumps, stages = synth.blob_cascade((1,), seed=5, n_sample=200)
+    stages = stages.copy()
+    stages["threshold"] = np.float32(-1e30)
+    img = synth.detect_image(640, 480, 20, 24, 64, seed=41)
+    d = DeviceArray.from_host(img)
+    eng = _engine(640, 480, (feats, stumps, stages))
+    for max_det in (16000, 20000):
+        want_raw, _ = O.detect_raw(feats, stumps, stages, img, max_det=max_det)
+        assert len(want_raw) == max_det
+        want = det_tuples(O.group(want_raw, 4))
+        say(f"  maxDetections {max_det}: {len(want)} groups")
+        for rep in range(2):
+            got = eng.detectMultiScale(d, 640, 480, 640, 1.25, 4, 24, max_det, applyGrouping=True)
+            assert det_tuples(got) == want, (max_det, rep)
