@@ -8,9 +8,14 @@
 //              [maxFalseAlarm] [minObjectSize] [stageNegCacheDir]
 //              [stageNegImagesDir] [stageNegSeed] [hardNegCandidateMultiplier]
 //              <out_model.bin>
+//   cuda_hello train_compare <faces_u8.bin> <negatives_u8.bin> [numPos=256]
+//              [numNeg=256] [featureLimit=1024] [runs=3]
+//   cuda_hello detect_compare <in_model.bin> <image.pgm> <out.ppm>
+//              [scaleFactor=1.25] [minObjectSize=24] [maxDetections=200000]
+//              [maxScales=0] [runs=3]
 //
 // Same verbs, positional order, defaults, clamps and stdout tags as the
-// reference (src/main.cu:1370-1531, 2394-2579); exit status 0 or 1.
+// reference (src/main.cu:1032-1368, 1370-1531, 2394-2579); exit status 0 or 1.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -162,11 +167,15 @@ int run_detect(int argc, char** argv) {
 void usage(const char* prog) {
     std::cerr << "Usage: " << prog << " train ... <out_model.bin>\n";
     std::cerr << "   or: " << prog << " detect <in_model.bin> <image.pgm> <out.ppm>\n";
+    std::cerr << "   or: " << prog << " train_compare <faces_u8.bin> <negatives_u8.bin>\n";
+    std::cerr << "   or: " << prog << " detect_compare <in_model.bin> <image.pgm> <out.ppm>\n";
 }
 
 }  // namespace
 
 int run_train(int argc, char** argv);
+int run_train_compare(int argc, char** argv);
+int run_detect_compare(int argc, char** argv);
 
 int main(int argc, char** argv) {
     if (argc < 2) {
@@ -176,6 +185,8 @@ int main(int argc, char** argv) {
     const std::string mode = argv[1];
     if (mode == "detect") return run_detect(argc - 1, argv + 1);
     if (mode == "train") return run_train(argc - 1, argv + 1);
+    if (mode == "train_compare") return run_train_compare(argc - 1, argv + 1);
+    if (mode == "detect_compare") return run_detect_compare(argc - 1, argv + 1);
     std::cerr << "[ERR] Unknown mode: " << mode << "\n";
     usage(argv[0]);
     return 1;
@@ -564,5 +575,261 @@ int run_train(int argc, char** argv) {
     }
     std::cout << "[OK] Saved model to: " << outPath << " (features=" << poolN << ", stumps=" << stumps.size()
               << ", stages=" << stages.size() << ")\n";
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// train_compare / detect_compare (src/main.cu:1032-1368): the same work on
+// the host (vj_cpu_*) and on the GPU, averaged over `runs`, with both
+// results and the speed-up printed under the reference's tags.  Besides the
+// reference's lines each verb prints one "match=" line saying whether the
+// two legs agree (the same stump; the same raw detections in order).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Timer {
+    vj_event a = nullptr, b = nullptr;
+    Timer() {
+        vj_event_create(&a);
+        vj_event_create(&b);
+    }
+    ~Timer() {
+        if (a) vj_event_destroy(a);
+        if (b) vj_event_destroy(b);
+    }
+};
+
+double ms_since(std::chrono::high_resolution_clock::time_point t0) {
+    const auto t1 = std::chrono::high_resolution_clock::now();
+    return static_cast<double>(std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count()) / 1000.0;
+}
+
+}  // namespace
+
+int run_train_compare(int argc, char** argv) {
+    if (argc < 3) {
+        std::cerr << "Usage: cuda_hello train_compare <faces_u8.bin> <negatives_u8.bin>"
+                  << " [numPos=256] [numNeg=256] [featureLimit=1024] [runs=3]\n";
+        return 1;
+    }
+    constexpr int kWin = 24, kPx = kWin * kWin;
+    int reqPos = 256, reqNeg = 256, featureLimit = 1024, runs = 3;
+    try {
+        if (argc > 3) reqPos = std::max(1, std::stoi(argv[3]));
+        if (argc > 4) reqNeg = std::max(1, std::stoi(argv[4]));
+        if (argc > 5) featureLimit = std::max(1, std::stoi(argv[5]));
+        if (argc > 6) runs = std::max(1, std::stoi(argv[6]));
+    } catch (...) {
+        std::cerr << "[ERR] invalid numeric argument\n";
+        return 1;
+    }
+    std::vector<uint8_t> posRaw, negRaw;
+    int posCount = 0, negCount = 0;
+    if (!load_bin(argv[1], kPx, posRaw, posCount) || !load_bin(argv[2], kPx, negRaw, negCount)) return 1;
+    const int numPos = std::min(reqPos, posCount), numNeg = std::min(reqNeg, negCount);
+    const int n = numPos + numNeg;
+    if (n <= 1) {
+        std::cerr << "[ERR] Need at least 2 samples for train_compare\n";
+        return 1;
+    }
+    // positives first, then negatives; weights 0.5/numPos and 0.5/numNeg
+    std::vector<uint8_t> train(static_cast<size_t>(n) * kPx);
+    std::memcpy(train.data(), posRaw.data(), static_cast<size_t>(numPos) * kPx);
+    std::memcpy(train.data() + static_cast<size_t>(numPos) * kPx, negRaw.data(), static_cast<size_t>(numNeg) * kPx);
+    std::vector<int8_t> labels(static_cast<size_t>(n), -1);
+    std::vector<uint8_t> active(static_cast<size_t>(n), 1);
+    std::vector<float> weights(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) {
+        if (i < numPos) labels[static_cast<size_t>(i)] = +1;
+        weights[static_cast<size_t>(i)] = i < numPos ? 0.5f / static_cast<float>(numPos) : 0.5f / static_cast<float>(numNeg);
+    }
+    int poolN = 0;
+    if (vj_haar_pool(kWin, kWin, 1, nullptr, 0, &poolN) != VJ_OK) return fail("buildHaarFeaturePool");
+    std::vector<vj_feature> pool(static_cast<size_t>(poolN));
+    if (vj_haar_pool(kWin, kWin, 1, pool.data(), poolN, &poolN) != VJ_OK) return fail("buildHaarFeaturePool");
+    featureLimit = std::min(featureLimit, poolN);
+
+    vj_tile_best cpu{};
+    double cpuMs = 0.0;
+    for (int r = 0; r < runs; ++r) {
+        const auto t0 = std::chrono::high_resolution_clock::now();
+        if (vj_cpu_best_weak(train.data(), n, kWin, kWin, labels.data(), active.data(), weights.data(), pool.data(),
+                             featureLimit, &cpu) != VJ_OK)
+            return fail("CPU weak search");
+        cpuMs += ms_since(t0);
+    }
+    cpuMs /= runs;
+
+    vj_engine* eng = nullptr;
+    if (vj_engine_create(kWin, kWin, n, kWin, kWin, 0, 1, &eng) != VJ_OK) return fail("FaceVisionEngine");
+    struct EngineGuard {
+        vj_engine* e;
+        ~EngineGuard() { vj_engine_destroy(e); }
+    } eg{eng};
+    vj_trainer* tr = nullptr;
+    if (vj_trainer_create(kWin, kWin, n, featureLimit, &tr) != VJ_OK) return fail("AdaBoostTrainer");
+    struct TrainerGuard {
+        vj_trainer* t;
+        ~TrainerGuard() { vj_trainer_destroy(t); }
+    } tg{tr};
+    if (vj_engine_upload_feature_pool(eng, pool.data(), poolN, nullptr) != VJ_OK) return fail("uploadFeaturePool");
+    const size_t fn = static_cast<size_t>(featureLimit) * n;
+    DevMem dTrain, dLabel, dActive, dWeight, dResp, dSorted, dBest, dTile;
+    if (vj_malloc(&dTrain.p, train.size()) != VJ_OK || vj_malloc(&dLabel.p, labels.size()) != VJ_OK ||
+        vj_malloc(&dActive.p, active.size()) != VJ_OK || vj_malloc(&dWeight.p, weights.size() * 4) != VJ_OK ||
+        vj_malloc(&dResp.p, fn * 4) != VJ_OK || vj_malloc(&dSorted.p, fn * 2) != VJ_OK ||
+        vj_malloc(&dBest.p, static_cast<size_t>(featureLimit) * 16) != VJ_OK || vj_malloc(&dTile.p, 16) != VJ_OK)
+        return fail("device allocation");
+    if (vj_memcpy_htod(dTrain.p, train.data(), train.size(), nullptr) != VJ_OK ||
+        vj_memcpy_htod(dLabel.p, labels.data(), labels.size(), nullptr) != VJ_OK ||
+        vj_memcpy_htod(dActive.p, active.data(), active.size(), nullptr) != VJ_OK ||
+        vj_memcpy_htod(dWeight.p, weights.data(), weights.size() * 4, nullptr) != VJ_OK)
+        return fail("cudaMemcpy failed");
+    if (vj_engine_integral_batch(eng, static_cast<const uint8_t*>(dTrain.p), kWin, kWin, kWin, n, 1, nullptr) != VJ_OK)
+        return fail("computeIntegralBatchTransposed failed");
+    vj_integral_view view{};
+    if (vj_engine_integral_view(eng, &view) != VJ_OK ||
+        vj_trainer_bind(tr, &view, static_cast<const int8_t*>(dLabel.p), static_cast<const uint8_t*>(dActive.p),
+                        static_cast<const float*>(dWeight.p), n) != VJ_OK)
+        return fail("bindTrainingSet failed");
+    const vj_feature* dPool = nullptr;
+    int dPoolN = 0;
+    vj_engine_feature_pool(eng, &dPool, &dPoolN);
+    Timer tm;
+    float gpuMs = 0.0f;
+    for (int r = 0; r < runs; ++r) {
+        vj_event_record(tm.a, nullptr);
+        int st = vj_trainer_evaluate_responses(tr, dPool, 0, featureLimit, static_cast<float*>(dResp.p), nullptr);
+        if (st == VJ_OK)
+            st = vj_trainer_sort_per_feature(tr, static_cast<const float*>(dResp.p), featureLimit,
+                                             static_cast<uint16_t*>(dSorted.p), nullptr);
+        if (st == VJ_OK)
+            st = vj_trainer_find_threshold(tr, static_cast<const float*>(dResp.p),
+                                           static_cast<const uint16_t*>(dSorted.p), featureLimit, dBest.p, nullptr);
+        if (st == VJ_OK) st = vj_trainer_select_best(tr, dBest.p, 0, featureLimit, nullptr, dTile.p, nullptr);
+        vj_event_record(tm.b, nullptr);
+        vj_event_synchronize(tm.b);
+        if (st != VJ_OK) return fail("GPU benchmark failed");
+        float ms = 0.0f;
+        vj_event_elapsed_ms(tm.a, tm.b, &ms);
+        gpuMs += ms;
+    }
+    gpuMs /= static_cast<float>(runs);
+    vj_tile_best gpu{};
+    if (vj_memcpy_dtoh(&gpu, dTile.p, sizeof gpu, nullptr) != VJ_OK) return fail("cudaMemcpy failed");
+
+    std::cout << "[TRAIN_COMPARE] samples=" << n << " numPos=" << numPos << " numNeg=" << numNeg
+              << " features=" << featureLimit << " runs=" << runs << "\n";
+    std::cout << "[TRAIN_COMPARE] CPU best feature=" << cpu.feature << " theta=" << cpu.theta
+              << " parity=" << static_cast<int>(cpu.parity) << " err=" << cpu.err << " avg(ms)=" << cpuMs << "\n";
+    std::cout << "[TRAIN_COMPARE] GPU best feature=" << gpu.feature << " theta=" << gpu.theta
+              << " parity=" << static_cast<int>(gpu.parity) << " err=" << gpu.err << " avg(ms)=" << gpuMs << "\n";
+    if (gpuMs > 0.0f) std::cout << "[TRAIN_COMPARE] speedup=" << (cpuMs / static_cast<double>(gpuMs)) << "x\n";
+    const bool same = cpu.feature == gpu.feature && cpu.parity == gpu.parity &&
+                      std::memcmp(&cpu.theta, &gpu.theta, 4) == 0 && std::memcmp(&cpu.err, &gpu.err, 4) == 0;
+    std::cout << "[TRAIN_COMPARE] match=" << (same ? "yes" : "no") << "\n";
+    return 0;
+}
+
+int run_detect_compare(int argc, char** argv) {
+    if (argc < 4) {
+        std::cerr << "Usage: cuda_hello detect_compare <in_model.bin> <image.pgm> <out.ppm>"
+                  << " [scaleFactor=1.25] [minObjectSize=24] [maxDetections=200000] [maxScales=0] [runs=3]\n";
+        return 1;
+    }
+    const std::string modelPath = argv[1], imagePath = argv[2], outPath = argv[3];
+    float scaleFactor = 1.25f;
+    int minObjectSize = 24, maxDetections = 200000, maxScales = 0, runs = 3;
+    try {
+        if (argc > 4) scaleFactor = std::stof(argv[4]);
+        if (argc > 5) minObjectSize = std::max(1, std::stoi(argv[5]));
+        if (argc > 6) maxDetections = std::max(100, std::stoi(argv[6]));
+        if (argc > 7) maxScales = std::max(0, std::stoi(argv[7]));
+        if (argc > 8) runs = std::max(1, std::stoi(argv[8]));
+    } catch (...) {
+        std::cerr << "[ERR] invalid numeric argument\n";
+        return 1;
+    }
+    if (maxScales != 0) {
+        std::cerr << "[ERR] detect_compare currently requires maxScales=0 so CPU/GPU use the same scale schedule\n";
+        return 1;
+    }
+    Model m;
+    if (!load_model(modelPath, m)) {
+        std::cerr << "[ERR] loadCascadeModel failed: " << modelPath << "\n";
+        return 1;
+    }
+    Gray img;
+    if (!load_gray(imagePath, img)) return fail("loadPGM");
+    const int w = img.w, h = img.h;
+
+    std::vector<vj_detection> cpuDets(static_cast<size_t>(maxDetections));
+    int cpuCount = 0;
+    double cpuMs = 0.0;
+    for (int r = 0; r < runs; ++r) {
+        const auto t0 = std::chrono::high_resolution_clock::now();
+        if (vj_cpu_detect_raw(m.features.data(), static_cast<int>(m.features.size()), m.stumps.data(),
+                              static_cast<int>(m.stumps.size()), m.stages.data(), static_cast<int>(m.stages.size()),
+                              m.winW, m.winH, img.px.data(), w, h, w, scaleFactor, minObjectSize, maxDetections,
+                              maxScales, cpuDets.data(), static_cast<int>(cpuDets.size()), &cpuCount) != VJ_OK)
+            return fail("detectMultiScaleCpu");
+        cpuMs += ms_since(t0);
+    }
+    cpuMs /= runs;
+
+    vj_engine* eng = nullptr;
+    if (vj_engine_create(m.winW, m.winH, 1, std::max(m.winW, w), std::max(m.winH, h), 0, maxDetections, &eng) != VJ_OK)
+        return fail("FaceVisionEngine");
+    struct Guard {
+        vj_engine* e;
+        ~Guard() { vj_engine_destroy(e); }
+    } guard{eng};
+    if (vj_engine_upload_feature_pool(eng, m.features.data(), static_cast<int>(m.features.size()), nullptr) != VJ_OK)
+        return fail("uploadFeaturePool");
+    DevMem dStumps, dImage;
+    if (vj_malloc(&dStumps.p, m.stumps.size() * sizeof(vj_stump)) != VJ_OK ||
+        vj_malloc(&dImage.p, img.px.size()) != VJ_OK ||
+        vj_memcpy_htod(dStumps.p, m.stumps.data(), m.stumps.size() * sizeof(vj_stump), nullptr) != VJ_OK ||
+        vj_memcpy_htod(dImage.p, img.px.data(), img.px.size(), nullptr) != VJ_OK)
+        return fail("cudaMemcpy failed");
+    const vj_feature* dPool = nullptr;
+    int poolN = 0;
+    vj_engine_feature_pool(eng, &dPool, &poolN);
+    if (vj_engine_set_cascade(eng, dPool, poolN, static_cast<const vj_stump*>(dStumps.p),
+                              static_cast<int>(m.stumps.size()), m.stages.data(), static_cast<int>(m.stages.size()),
+                              nullptr) != VJ_OK)
+        return fail("setCascadeModel");
+    std::vector<vj_detection> gpuDets(static_cast<size_t>(maxDetections));
+    int gpuCount = 0;
+    Timer tm;
+    float gpuMs = 0.0f;
+    for (int r = 0; r < runs; ++r) {
+        vj_event_record(tm.a, nullptr);
+        const int st = vj_engine_detect(eng, static_cast<const uint8_t*>(dImage.p), w, h, w, scaleFactor, 0,
+                                        minObjectSize, maxDetections, 0, gpuDets.data(),
+                                        static_cast<int>(gpuDets.size()), &gpuCount, nullptr);
+        vj_event_record(tm.b, nullptr);
+        vj_event_synchronize(tm.b);
+        if (st != VJ_OK) return fail("detectMultiScale failed");
+        float ms = 0.0f;
+        vj_event_elapsed_ms(tm.a, tm.b, &ms);
+        gpuMs += ms;
+    }
+    gpuMs /= static_cast<float>(runs);
+    const int shown = std::min(gpuCount, static_cast<int>(gpuDets.size()));
+
+    std::cout << "[DETECT_COMPARE] cpuDetections=" << cpuCount << " gpuDetections=" << gpuCount << " runs=" << runs
+              << " maxScales=" << maxScales << "\n";
+    std::cout << "[DETECT_COMPARE] CPU avg(ms)=" << cpuMs << "\n";
+    std::cout << "[DETECT_COMPARE] GPU avg(ms)=" << gpuMs << "\n";
+    if (gpuMs > 0.0f) std::cout << "[DETECT_COMPARE] speedup=" << (cpuMs / static_cast<double>(gpuMs)) << "x\n";
+    const bool same = cpuCount == gpuCount &&
+                      std::memcmp(cpuDets.data(), gpuDets.data(), sizeof(vj_detection) * static_cast<size_t>(shown)) == 0;
+    std::cout << "[DETECT_COMPARE] match=" << (same ? "yes" : "no") << "\n";
+    if (vj_ppm_save_gray_boxes(outPath.c_str(), img.px.data(), w, h, gpuDets.data(), shown) != VJ_OK) {
+        std::cerr << "[ERR] savePPM failed\n";
+        return 1;
+    }
+    std::cout << "[OK] Saved GPU detection visualization to: " << outPath << "\n";
     return 0;
 }

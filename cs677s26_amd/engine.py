@@ -204,6 +204,25 @@ def group_rectangles(dets, minNeighbors=4, eps=0.2):
     return out[:n.value].copy()
 
 
+def detect_multiscale_cpu(features, stumps, stages, image, win=(24, 24), scaleFactor=1.25, minObjectSize=24,
+                          maxDetections=200000, maxScales=0):
+    """detectMultiScaleCpu (src/main.cu:684-827): the host leg of
+    detect_compare, single threaded.  Raw detections in (level, y, x) order,
+    stopping at maxDetections."""
+    f = np.ascontiguousarray(features, FEATURE)
+    s = np.ascontiguousarray(stumps, STUMP)
+    g = np.ascontiguousarray(stages, STAGE)
+    img = np.ascontiguousarray(image, np.uint8)
+    h, w = img.shape
+    out = np.zeros(max(1, int(maxDetections)), DETECTION)
+    n = C.c_int()
+    check(lib().vj_cpu_detect_raw(f.ctypes.data, len(f), s.ctypes.data, len(s), g.ctypes.data, len(g), int(win[0]),
+                                  int(win[1]), img.ctypes.data, w, h, w, float(scaleFactor), int(minObjectSize),
+                                  int(maxDetections), int(maxScales), out.ctypes.data, len(out), C.byref(n)),
+          "detectMultiScaleCpu")
+    return out[:n.value].copy()
+
+
 # -- ModelIO (src/model_io.cpp) ---------------------------------------------
 
 def saveCascadeModel(path, win, features, stumps, stages):

@@ -97,3 +97,20 @@ class AdaBoostTrainer:
         out = np.empty((count, self.n), np.uint16)
         check(lib().vj_trainer_read_orders(self.handle, int(begin), int(count), out.ctypes.data), "read_orders")
         return out
+
+
+def cpu_best_weak(patches, labels, weights, features, active=None, win=(24, 24)):
+    """The host leg of train_compare (runCpuWeakSearch, src/main.cu:587-682),
+    single threaded, with the search semantics of the GPU per-op path
+    (evaluateAndFindThreshold + selectBestInTile): returns a TILE_BEST
+    record for features [0, len(features))."""
+    p = np.ascontiguousarray(patches, np.uint8)
+    lab = np.ascontiguousarray(labels, np.int8)
+    w = np.ascontiguousarray(weights, np.float32)
+    f = np.ascontiguousarray(features)
+    act = None if active is None else np.ascontiguousarray(active, np.uint8)
+    out = np.zeros(1, TILE_BEST)
+    check(lib().vj_cpu_best_weak(p.ctypes.data, len(lab), int(win[0]), int(win[1]), lab.ctypes.data,
+                                 None if act is None else act.ctypes.data, w.ctypes.data, f.ctypes.data, len(f),
+                                 out.ctypes.data), "cpuBestWeak")
+    return out[0]

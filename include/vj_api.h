@@ -310,6 +310,36 @@ int vj_trainer_read_state(vj_trainer* t, double* hWeights, float* hStrong, uint8
  * valid split after this position) for checking the presort. */
 int vj_trainer_read_orders(vj_trainer* t, int begin, int count, uint16_t* hOut);
 
+/* TileBestCandidate, include/vj/boosting_core.cuh (16 B): what
+ * vj_trainer_select_best writes.  featureIdx is INT_MAX only when every
+ * feature of the range is masked as used. */
+typedef struct vj_tile_best {
+    float err, theta;
+    int32_t feature;
+    int8_t parity, pad0, pad1, pad2;
+} vj_tile_best;
+
+/* ==== CPU legs of the compare verbs (host, single thread) ================
+ * cuda_hello detect_compare / train_compare (src/main.cu:1032-1368) time the
+ * same work on the CPU and on the GPU.  These are the CPU halves; no GPU
+ * entry point uses them. */
+/* detectMultiScaleCpu, src/main.cu:684-827: raw detections in (level, y, x)
+ * order, stopping at maxDetections; maxScales > 0 limits the scale indices
+ * visited.  *count receives the number found; min(*count, cap) are copied. */
+int vj_cpu_detect_raw(const vj_feature* hFeatures, int nFeatures, const vj_stump* hStumps, int nStumps,
+                      const vj_stage* hStages, int nStages, int winW, int winH, const uint8_t* hImage,
+                      int w, int h, int stride, float scaleFactor, int minObjectSize, int maxDetections,
+                      int maxScales, vj_detection* hOut, int cap, int* count);
+/* The best stump over features [0, count) of n winW x winH patches
+ * (positives labelled +1), as runCpuWeakSearch (src/main.cu:587-682) finds
+ * it, with the search semantics of vj_trainer_find_threshold +
+ * vj_trainer_select_best (exact fixed-point sums, GPU tie rules), so both
+ * legs report the same stump.  hActive may be NULL (all active); weights
+ * must be normalised (sum of |w| <= 1.5). */
+int vj_cpu_best_weak(const uint8_t* hPatches, int n, int winW, int winH, const int8_t* hLabels,
+                     const uint8_t* hActive, const float* hWeights, const vj_feature* hFeatures, int count,
+                     vj_tile_best* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,4 +1,5 @@
-"""The drop-in command line on the GPU: `cuda_hello detect` against the
+"""The drop-in command line on the GPU: `cuda_hello detect` and the compare
+verbs (`detect_compare`, `train_compare`) against the
 oracle, and `cuda_hello train` end to end (seeded stage-0 negatives, hard
 negatives mined with the cascade trained so far, the top-up sample, and a
 model file the detector loads)."""
@@ -79,3 +80,58 @@ def test_cli_train_mines_hard_negatives(tmp_path):
     r2 = subprocess.run(args[:-1] + [str(out2)], capture_output=True, text=True, timeout=300)
     assert r2.returncode == 0, r2.stderr
     assert out.read_bytes() == out2.read_bytes()
+
+
+def test_cli_detect_compare_legs_agree(tmp_path):
+    """detect_compare (src/main.cu:1216-1368): the host leg and the GPU leg
+    report the oracle's raw detection count and agree box for box; the PPM
+    is written; maxScales other than 0 is refused as in the reference."""
+    say("cli detect_compare")
+    feats, stumps, stages = synth.blob_cascade((1, 10, 25))
+    model = tmp_path / "m.bin"
+    saveCascadeModel(model, (24, 24), feats, stumps, stages)
+    img = synth.detect_image(480, 360, 12, 24, 64, seed=33)
+    write_pgm(tmp_path / "a.pgm", img)
+    raw, _ = O.detect_raw(feats, stumps, stages, img)
+    r = subprocess.run([EXE, "detect_compare", str(model), str(tmp_path / "a.pgm"), str(tmp_path / "o.ppm"),
+                        "1.25", "24", "200000", "0", "2"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    m = re.search(r"\[DETECT_COMPARE\] cpuDetections=(\d+) gpuDetections=(\d+) runs=2 maxScales=0", r.stdout)
+    assert m and int(m.group(1)) == len(raw) and int(m.group(2)) == len(raw), r.stdout
+    assert "[DETECT_COMPARE] match=yes" in r.stdout
+    assert re.search(r"\[DETECT_COMPARE\] speedup=[0-9.e+]+x", r.stdout)
+    assert (tmp_path / "o.ppm").read_bytes().startswith(b"P6\n480 360\n255\n")
+    # the cap: both legs stop at the first 100 in scan order
+    r = subprocess.run([EXE, "detect_compare", str(model), str(tmp_path / "a.pgm"), str(tmp_path / "o.ppm"),
+                        "1.25", "24", "100", "0", "1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "[DETECT_COMPARE] match=yes" in r.stdout, r.stdout + r.stderr
+    assert f"cpuDetections={min(100, len(raw))} gpuDetections={min(100, len(raw))}" in r.stdout
+    r = subprocess.run([EXE, "detect_compare", str(model), str(tmp_path / "a.pgm"), str(tmp_path / "o.ppm"),
+                        "1.25", "24", "200000", "3"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "requires maxScales=0" in r.stderr
+
+
+def test_cli_train_compare_legs_agree(tmp_path):
+    """train_compare (src/main.cu:1032-1214): both legs report the same stump
+    over the first 1,500 Core features, and it is the oracle's (per-feature
+    exact search, then the smallest float error with the smaller index)."""
+    say("cli train_compare")
+    patches, labels = synth.training_set(300, 300, seed=21)
+    patches[:300].tofile(tmp_path / "pos.bin")
+    patches[300:].tofile(tmp_path / "neg.bin")
+    r = subprocess.run([EXE, "train_compare", str(tmp_path / "pos.bin"), str(tmp_path / "neg.bin"), "256", "200",
+                        "1500", "2"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "[TRAIN_COMPARE] samples=456 numPos=256 numNeg=200 features=1500 runs=2" in r.stdout, r.stdout
+    assert "[TRAIN_COMPARE] match=yes" in r.stdout, r.stdout
+    pool = O.haar_pool(24, 1)[:1500]
+    sel = np.concatenate([patches[:256], patches[300:500]])
+    sumT, _, inv = O.train_integrals(sel)
+    w = np.concatenate([np.full(256, np.float32(0.5) / np.float32(256), np.float32),
+                        np.full(200, np.float32(0.5) / np.float32(200), np.float32)])
+    sw = np.concatenate([w[:256], -w[256:]]).astype(np.float32)
+    best = O.search(O.presort(pool, sumT, inv), sw)
+    errs = [np.float32(np.ldexp(float(b["errQ"]), -62)) for b in best]
+    fi = min(range(len(best)), key=lambda i: (errs[i], i))
+    m = re.search(r"\[TRAIN_COMPARE\] GPU best feature=(-?\d+) theta=\S+ parity=(-?\d+)", r.stdout)
+    assert m and int(m.group(1)) == fi and int(m.group(2)) == int(best[fi]["parity"]), r.stdout

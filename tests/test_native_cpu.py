@@ -190,3 +190,98 @@ def test_grouping_large_input_equals_golden():
     rng = np.random.default_rng(4)
     sh = g["raw"][rng.permutation(len(g["raw"]))]
     assert det_tuples(group_rectangles(sh, 3)) == det_tuples(O.group(sh, 3))
+
+
+# ---- CPU legs of the compare verbs (src/main.cu:1032-1368) -----------------
+
+def test_cpu_detect_leg_equals_oracle_and_golden():
+    """detect_compare's host leg (detectMultiScaleCpu, src/main.cu:684-827)
+    gives the oracle's raw detections, in order, on the 640x480 workload and
+    under a small cap (the early return at maxDetections)."""
+    from cs677s26_amd.engine import detect_multiscale_cpu
+    g = np.load(os.path.join(ROOT, "tests", "golden", "detect_c1.npz"))
+    feats, stumps, stages = synth.blob_cascade((1, 10, 25))
+    img = synth.detect_image(640, 480, 20, 24, 64)
+    got = detect_multiscale_cpu(feats, stumps, stages, img)
+    assert det_tuples(got) == det_tuples(g["raw"])
+    want, _ = O.detect_raw(feats, stumps, stages, img)
+    assert det_tuples(got) == det_tuples(want)
+    for cap in (100, 257):
+        got = detect_multiscale_cpu(feats, stumps, stages, img, maxDetections=cap)
+        assert det_tuples(got) == det_tuples(want[:cap])
+
+
+def test_cpu_detect_leg_max_scales_and_parameters():
+    """maxScales stops the scale loop at that index (skipped levels count);
+    a larger minimum object size skips leading levels; scale 1.1 and a
+    non-multiple-of-4 image follow the oracle too."""
+    from cs677s26_amd.engine import detect_multiscale_cpu
+    feats, stumps, stages = synth.blob_cascade((1, 10, 25))
+    img = synth.detect_image(333, 251, 8, 24, 64, seed=3)
+    for sf, mo in ((1.25, 24), (1.1, 40), (1.5, 30)):
+        want, _ = O.detect_raw(feats, stumps, stages, img, sf=sf, min_obj=mo)
+        got = detect_multiscale_cpu(feats, stumps, stages, img, scaleFactor=sf, minObjectSize=mo)
+        assert det_tuples(got) == det_tuples(want), (sf, mo)
+        for ms in (1, 3):
+            got = detect_multiscale_cpu(feats, stumps, stages, img, scaleFactor=sf, minObjectSize=mo, maxScales=ms)
+            assert det_tuples(got) == det_tuples(want[want["scaleIdx"] < ms]), (sf, mo, ms)
+    # a stump that names a feature outside the pool is refused
+    bad = stumps.copy()
+    bad["feature"][0] = len(feats)
+    with pytest.raises(VJError):
+        detect_multiscale_cpu(feats, bad, stages, img)
+
+
+@pytest.mark.parametrize("weights", ["uniform", "random"])
+def test_cpu_search_leg_equals_oracle(weights):
+    """train_compare's host leg: the best stump of features [0, F) equals the
+    oracle's per-feature exact search followed by the per-op selection
+    (float errors, smaller index on ties), with theta from the oracle's
+    responses.  Uniform weights make exact ties between features common."""
+    from cs677s26_amd.trainer import cpu_best_weak
+    patches, labels = synth.training_set(150, 150, seed=5)
+    n = len(labels)
+    pool = FaceVisionEngine.buildHaarFeaturePool((24, 24), FaceVisionEngine.Core)[:1500]
+    rng = np.random.default_rng(2)
+    if weights == "uniform":
+        w = np.full(n, 0.5 / 150, np.float32)
+    else:
+        w = rng.random(n).astype(np.float32)
+        w = (w / w.sum()).astype(np.float32)
+    active = (rng.random(n) > 0.1).astype(np.uint8)
+    for act in (None, active):
+        got = cpu_best_weak(patches, labels, w, pool, active=act)
+        sumT, _, inv = O.train_integrals(patches)
+        ords = O.presort(pool, sumT, inv)
+        sw = np.where(labels > 0, w, -w).astype(np.float32)
+        if act is not None:
+            sw = np.where(act != 0, sw, np.float32(0))
+        best = O.search(ords, sw)
+        errs = [np.float32(np.ldexp(float(b["errQ"]), -62)) if b["split"] >= 0 else np.float32(np.finfo(np.float32).max)
+                for b in best]
+        fi = min(range(len(best)), key=lambda i: (errs[i], i))
+        assert got["featureIdx"] == fi
+        assert got["bestErr"] == errs[fi]
+        assert got["bestParity"] == best[fi]["parity"]
+        v = O.feature_responses(pool[fi], sumT, inv)
+        k = int(best[fi]["split"])
+        s0, s1 = int(ords[fi, k]) & 0x7FFF, int(ords[fi, k + 1]) & 0x7FFF
+        assert got["bestTheta"] == np.float32(np.float32(0.5) * (v[s0] + v[s1]))
+
+
+def test_cli_compare_verbs_usage_and_errors(tmp_path):
+    exe = os.path.join(ROOT, "cs677s26_amd", "bin", "cuda_hello")
+    r = subprocess.run([exe, "train_compare", "a.bin"], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage: cuda_hello train_compare" in r.stderr
+    r = subprocess.run([exe, "train_compare", str(tmp_path / "p.bin"), str(tmp_path / "n.bin")], capture_output=True,
+                       text=True)
+    assert r.returncode == 1 and "[ERR] Cannot open" in r.stderr
+    (tmp_path / "odd.bin").write_bytes(bytes(577))
+    r = subprocess.run([exe, "train_compare", str(tmp_path / "odd.bin"), str(tmp_path / "odd.bin")],
+                       capture_output=True, text=True)
+    assert r.returncode == 1 and "not divisible by samplePixels" in r.stderr
+    r = subprocess.run([exe, "detect_compare", "m.bin", "a.pgm"], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage: cuda_hello detect_compare" in r.stderr
+    r = subprocess.run([exe, "detect_compare", str(tmp_path / "m.bin"), "a.pgm", "o.ppm"], capture_output=True,
+                       text=True)
+    assert r.returncode == 1 and "[ERR] loadCascadeModel failed" in r.stderr
