@@ -12,7 +12,7 @@ fields below).  Extra legs, each optional:
   same 1080p frame, one thread (the reference CPU path is single-threaded).
 
 ``roofline`` is for the cascade phase (the dominant one: ``cascade_kernel``,
-the first pass over every tile, then ``deep_list_kernel`` and the persistent
+the first pass over every tile, then the deep work-list kernels and the persistent
 ``cascade_deep_kernel`` over the survivors): algorithmic bytes per frame
 (12*(W+1)*(H+1) summed over levels, plus model and output bytes) over the
 phase's average duration from HIP events on the detect stream.
@@ -331,7 +331,7 @@ def detect_fields(args):
     casc_ms = d["phase_ms"][2]
     casc_bytes = d["stats"]["cascadeBytes"]
     achieved = casc_bytes / (casc_ms * 1e-3) / 1e9 if casc_ms > 0 else 0.0
-    out["roofline"] = {"bound": "hbm", "kernel": "cascade_kernel+deep_list_kernel+cascade_deep_kernel",
+    out["roofline"] = {"bound": "hbm", "kernel": "cascade_kernel+deep_hist/base/scatter_kernel+cascade_deep_kernel",
                        "achieved": achieved, "peak": HBM_PEAK_GBS,
                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                        "traffic": measured_traffic("cascade_kernel_detect1080"),

@@ -700,44 +700,75 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(
     }
 }
 
-__global__ void reset_counter_kernel(int32_t* __restrict__ c) {
-    if (threadIdx.x == 0) c[0] = 0;
+// Deep-pass work list: every tile with survivors after the first pass
+// contributes ceil(count / kDeepSlice) items (tile * kDeepSlices + slice),
+// ordered by survivor count, largest first, so the longest items start
+// first.  Built in three launches: a histogram of the tiles' survivor
+// counts, each count's first item (counts in descending order) with the item
+// total and a zeroed work counter, then every tile's items at its count's
+// offset (tiles of one count in any order).  ctl layout: [0] survivor total,
+// [1] items, [2] work counter, then per count its tile histogram (replaced
+// by its first item) and its cursor.
+constexpr int kDeepHist = 4, kDeepCursor = kDeepHist + kTileMaxWin + 1, kDeepCtlInts = kDeepCursor + kTileMaxWin + 1;
+
+__global__ __launch_bounds__(256) void deep_reset_kernel(int32_t* __restrict__ ctl) {
+    for (int i = threadIdx.x; i < kDeepCtlInts; i += blockDim.x) ctl[i] = 0;
 }
 
-// Deep-pass work list: every tile with survivors after the first pass
-// contributes ceil(count / kDeepSlice) items (tile * kDeepSlices + slice), ordered by
-// survivor count, largest first, so the longest items start first.  One
-// workgroup; also resets the deep pass's work counter.
-__global__ __launch_bounds__(1024) void deep_list_kernel(const int32_t* __restrict__ tileCount, int nTiles,
-                                                         int32_t* __restrict__ items, int32_t* __restrict__ nItems,
-                                                         int32_t* __restrict__ workCounter) {
-    __shared__ uint32_t hist[kTileMaxWin + 1];
-    __shared__ uint32_t base[kTileMaxWin + 1];
-    for (int c = threadIdx.x; c <= kTileMaxWin; c += blockDim.x) hist[c] = 0u;
+__global__ __launch_bounds__(256) void deep_hist_kernel(const int32_t* __restrict__ tileCount, int nTiles,
+                                                        int32_t* __restrict__ ctl) {
+    __shared__ int32_t h[kTileMaxWin + 1];
+    for (int c = threadIdx.x; c <= kTileMaxWin; c += blockDim.x) h[c] = 0;
     __syncthreads();
-    for (int t = threadIdx.x; t < nTiles; t += blockDim.x) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nTiles; t += gridDim.x * blockDim.x) {
         const int c = tileCount[t];
-        if (c > 0) atomicAdd(&hist[c], 1u);
+        if (c > 0) atomicAdd(&h[c], 1);
     }
     __syncthreads();
+    for (int c = threadIdx.x; c <= kTileMaxWin; c += blockDim.x)
+        if (h[c]) atomicAdd(&ctl[kDeepHist + c], h[c]);
+}
+
+// One workgroup of kTileMaxWin threads; thread i takes count kTileMaxWin - i.
+__global__ __launch_bounds__(kTileMaxWin) void deep_base_kernel(int32_t* __restrict__ ctl) {
+    __shared__ int32_t waveTot[kTileMaxWin / kWave];
+    const int c = kTileMaxWin - static_cast<int>(threadIdx.x);
+    const int32_t v = ctl[kDeepHist + c] * ((c + kDeepSlice - 1) / kDeepSlice);
+    const int32_t inc = wave_incl_scan(v);
+    if (lane_id() == kWave - 1) waveTot[threadIdx.x / kWave] = inc;
+    __syncthreads();
+    int32_t base = inc - v, total = 0;
+    for (int w = 0; w < kTileMaxWin / kWave; ++w) {
+        if (w < static_cast<int>(threadIdx.x / kWave)) base += waveTot[w];
+        total += waveTot[w];
+    }
+    ctl[kDeepHist + c] = base;
     if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (int c = kTileMaxWin; c >= 1; --c) {
-            base[c] = run;
-            run += hist[c] * static_cast<uint32_t>((c + kDeepSlice - 1) / kDeepSlice);
-            hist[c] = 0u;
-        }
-        *nItems = static_cast<int32_t>(run);
-        *workCounter = 0;
+        ctl[1] = total;
+        ctl[2] = 0;
     }
-    __syncthreads();
-    for (int t = threadIdx.x; t < nTiles; t += blockDim.x) {
-        const int c = tileCount[t];
+}
+
+__global__ __launch_bounds__(256) void deep_scatter_kernel(const int32_t* __restrict__ tileCount, int nTiles,
+                                                           int32_t* __restrict__ ctl, int32_t* __restrict__ items) {
+    __shared__ int32_t cnt[kTileMaxWin + 1];
+    __shared__ int32_t off[kTileMaxWin + 1];
+    for (int t0 = blockIdx.x * blockDim.x; t0 < nTiles; t0 += gridDim.x * blockDim.x) {
+        for (int c = threadIdx.x; c <= kTileMaxWin; c += blockDim.x) cnt[c] = 0;
+        __syncthreads();
+        const int t = t0 + static_cast<int>(threadIdx.x);
+        const int c = t < nTiles ? tileCount[t] : 0;
+        const int local = c > 0 ? atomicAdd(&cnt[c], 1) : 0;
+        __syncthreads();
+        for (int b = threadIdx.x; b <= kTileMaxWin; b += blockDim.x)
+            if (cnt[b]) off[b] = atomicAdd(&ctl[kDeepCursor + b], cnt[b]);
+        __syncthreads();
         if (c > 0) {
             const int ns = (c + kDeepSlice - 1) / kDeepSlice;
-            const uint32_t pos = base[c] + atomicAdd(&hist[c], 1u) * static_cast<uint32_t>(ns);
-            for (int s = 0; s < ns; ++s) items[pos + s] = t * kDeepSlices + s;
+            const int pos = ctl[kDeepHist + c] + (off[c] + local) * ns;
+            for (int k = 0; k < ns; ++k) items[pos + k] = t * kDeepSlices + k;
         }
+        __syncthreads();  // cnt and off are rewritten by the next chunk
     }
 }
 
@@ -1101,33 +1132,75 @@ __global__ __launch_bounds__(256) void pair_find_kernel(const int32_t* __restric
     }
 }
 
-// Exclusive scan of the pair counts of all sweep positions (one workgroup):
-// list offsets, and the total into ctl[0].
-__global__ __launch_bounds__(1024) void pair_scan_kernel(const int32_t* __restrict__ counts, int nImages, int cap,
-                                                         const int32_t* __restrict__ pairCnt,
-                                                         int32_t* __restrict__ pairOff, int64_t* __restrict__ ctl) {
-    __shared__ int64_t waveTot[16];
-    __shared__ int64_t carry;
+// Exclusive scan of the pair counts of all sweep positions, in two launches
+// over blocks of kScanItems counts (each thread kScanPer consecutive ones):
+// pair_scan_local_kernel writes each block's own exclusive offsets and its
+// total; pair_scan_add_kernel adds the totals of the blocks before (offsets
+// saturate at INT32_MAX, past any pair capacity) and block 0 of the last
+// block writes the grand total into ctl[0].
+constexpr int kScanThreads = 256, kScanPer = 16, kScanItems = kScanThreads * kScanPer;
+
+__global__ __launch_bounds__(kScanThreads) void pair_scan_local_kernel(const int32_t* __restrict__ counts,
+                                                                      int nImages, int cap,
+                                                                      const int32_t* __restrict__ pairCnt,
+                                                                      int32_t* __restrict__ pairOff,
+                                                                      int64_t* __restrict__ blockTot) {
+    __shared__ int64_t waveTot[kScanThreads / kWave];
     const int64_t total = all_boxes(counts, nImages, cap);
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int64_t c0 = 0; c0 < total; c0 += blockDim.x) {
-        const int64_t g = c0 + threadIdx.x;
-        const int64_t v = g < total ? pairCnt[g] : 0;
-        const int64_t inc = static_cast<int64_t>(wave_incl_scan(static_cast<uint64_t>(v)));
-        if ((threadIdx.x & 63) == 63) waveTot[threadIdx.x >> 6] = inc;
-        __syncthreads();
-        int64_t off = carry + inc - v, tot = 0;
-        for (int k = 0; k < static_cast<int>(blockDim.x >> 6); ++k) {
-            if (k < static_cast<int>(threadIdx.x >> 6)) off += waveTot[k];
-            tot += waveTot[k];
+    const int64_t nb = (total + kScanItems - 1) / kScanItems;
+    const int lane = lane_id(), wave = threadIdx.x / kWave;
+    for (int64_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int64_t g0 = b * kScanItems + static_cast<int64_t>(threadIdx.x) * kScanPer;
+        int32_t v[kScanPer];
+        int64_t t = 0;
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            v[k] = g0 + k < total ? pairCnt[g0 + k] : 0;
+            t += v[k];
         }
-        if (g < total) pairOff[g] = static_cast<int32_t>(min<int64_t>(off, INT32_MAX));
+        const int64_t inc = static_cast<int64_t>(wave_incl_scan(static_cast<uint64_t>(t)));
+        if (lane == kWave - 1) waveTot[wave] = inc;
         __syncthreads();
-        if (threadIdx.x == 0) carry += tot;
-        __syncthreads();
+        int64_t off = inc - t, blk = 0;
+        for (int w = 0; w < kScanThreads / kWave; ++w) {
+            if (w < wave) off += waveTot[w];
+            blk += waveTot[w];
+        }
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            if (g0 + k < total) pairOff[g0 + k] = static_cast<int32_t>(off);
+            off += v[k];
+        }
+        if (threadIdx.x == 0) blockTot[b] = blk;
+        __syncthreads();  // waveTot is rewritten by the next block
     }
-    if (threadIdx.x == 0) ctl[0] = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void pair_scan_add_kernel(const int32_t* __restrict__ counts,
+                                                                    int nImages, int cap,
+                                                                    const int64_t* __restrict__ blockTot,
+                                                                    int32_t* __restrict__ pairOff,
+                                                                    int64_t* __restrict__ ctl) {
+    __shared__ int64_t part[kScanThreads / kWave];
+    const int64_t total = all_boxes(counts, nImages, cap);
+    const int64_t nb = (total + kScanItems - 1) / kScanItems;
+    const int lane = lane_id(), wave = threadIdx.x / kWave;
+    if (nb == 0 && blockIdx.x == 0 && threadIdx.x == 0) ctl[0] = 0;
+    for (int64_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        int64_t pre = 0;
+        for (int64_t k = threadIdx.x; k < b; k += kScanThreads) pre += blockTot[k];
+        pre = wave_sum(pre);
+        if (lane == 0) part[wave] = pre;
+        __syncthreads();
+        pre = 0;
+        for (int w = 0; w < kScanThreads / kWave; ++w) pre += part[w];
+        const int64_t g0 = b * kScanItems + static_cast<int64_t>(threadIdx.x) * kScanPer;
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k)
+            if (g0 + k < total) pairOff[g0 + k] = static_cast<int32_t>(min<int64_t>(pre + pairOff[g0 + k], INT32_MAX));
+        if (b == nb - 1 && threadIdx.x == 0) ctl[0] = pre + blockTot[b];
+        __syncthreads();  // part is rewritten by the next block
+    }
 }
 
 struct vj_engine {
@@ -1165,7 +1238,7 @@ struct vj_engine {
     int stageSplit = 0;                        // stages run by the first pass
     DevBuf<uint16_t> survIdx;
     DevBuf<float> survInv;
-    DevBuf<int32_t> tileSurvOff, tileCount, deepItems, deepCtl;  // deepCtl: total, items, counter
+    DevBuf<int32_t> tileSurvOff, tileCount, deepItems, deepCtl;  // deepCtl: total, items, counter, list scratch
     DevBuf<vj_detection> dOut;
     PinnedBuf<vj_detection> hOut;
     PinnedBuf<int32_t> hCounts;
@@ -1174,6 +1247,7 @@ struct vj_engine {
     DevBuf<int32_t> gRank, gPairCnt, gPairOff, gOrder, gPairs;
     DevBuf<int4> gSorted;
     DevBuf<int64_t> gCtl;
+    DevBuf<int64_t> gScanTot;  // pair-count scan: per-block totals
     PinnedBuf<int32_t> hOrder, hPairOff, hPairCnt, hPairs;
     PinnedBuf<int64_t> hCtl;
     int64_t pairCap = 0;  // capacity of gPairs (grows after an overflow)
@@ -1188,9 +1262,12 @@ struct vj_engine {
     std::map<std::vector<int64_t>, std::array<int, 4>> tuned;
 
     // the detect launch sequence captured as a HIP graph, replayed while the
-    // plan shape, the tile rows and every buffer address stay the same
+    // plan shape, the tile rows and every buffer address stay the same.  A
+    // shape is captured on its second consecutive call (lastKey): a caller
+    // whose image sizes change every call launches directly and pays no
+    // capture.
     bool graphs = true;
-    std::vector<int64_t> graphKey;
+    std::vector<int64_t> graphKey, lastKey;
     hipGraphExec_t graphExec = nullptr;
     hipStream_t captureStream = nullptr;  // captures run here; replays on the caller's stream
 
@@ -1462,7 +1539,7 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
         VJ_TRY(e->tileSurvOff.ensure(static_cast<size_t>(P.nTiles)));
         VJ_TRY(e->tileCount.ensure(static_cast<size_t>(P.nTiles)));
         VJ_TRY(e->deepItems.ensure(static_cast<size_t>(P.nTiles) * kDeepSlices));
-        VJ_TRY(e->deepCtl.ensure(4));
+        VJ_TRY(e->deepCtl.ensure(kDeepCtlInts));
     }
     const int nImages = static_cast<int>(P.imgWordOff.size()) - 1;
     VJ_TRY(e->dCounts.ensure(static_cast<size_t>(nImages)));
@@ -1562,7 +1639,7 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
         if (e->timing) VJ_HIP(hipEventRecord(e->ev[2], ls));
         if (P.nTiles > 0) {
             if (deep) {
-                hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, ls, e->deepCtl.get());
+                hipLaunchKernelGGL(deep_reset_kernel, dim3(1), dim3(256), 0, ls, e->deepCtl.get());
                 VJ_HIP(hipGetLastError());
             }
             hipLaunchKernelGGL(cascade_kernel, dim3(P.nTiles), dim3(kCascadeBlock), ldsBytes, ls, dL, m32 + oLvl,
@@ -1572,8 +1649,14 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
                                e->tileSurvOff.get(), e->tileCount.get());
             VJ_HIP(hipGetLastError());
             if (deep) {
-                hipLaunchKernelGGL(deep_list_kernel, dim3(1), dim3(1024), 0, ls, e->tileCount.get(), P.nTiles,
-                                   e->deepItems.get(), e->deepCtl.get() + 1, e->deepCtl.get() + 2);
+                const int listGrid = std::min((P.nTiles + 255) / 256, 1024);
+                hipLaunchKernelGGL(deep_hist_kernel, dim3(listGrid), dim3(256), 0, ls, e->tileCount.get(), P.nTiles,
+                                   e->deepCtl.get());
+                VJ_HIP(hipGetLastError());
+                hipLaunchKernelGGL(deep_base_kernel, dim3(1), dim3(kTileMaxWin), 0, ls, e->deepCtl.get());
+                VJ_HIP(hipGetLastError());
+                hipLaunchKernelGGL(deep_scatter_kernel, dim3(listGrid), dim3(256), 0, ls, e->tileCount.get(),
+                                   P.nTiles, e->deepCtl.get(), e->deepItems.get());
                 VJ_HIP(hipGetLastError());
                 hipLaunchKernelGGL(cascade_deep_kernel, dim3(deepGrid), dim3(kCascadeBlock), ldsBytes, ls, dL,
                                    m32 + oLvl, m32 + oTile, static_cast<int>(P.tileLvl.size()), e->iiS.get(),
@@ -1639,6 +1722,11 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     if (e->graphExec && key == e->graphKey) {
         VJ_HIP(hipGraphLaunch(e->graphExec, s));
         return VJ_OK;
+    }
+    if (key != e->lastKey) {
+        // first call of this shape: launch directly, capture if it repeats
+        e->lastKey = std::move(key);
+        return launch_all(s);
     }
     if (e->graphExec) {
         (void)hipGraphExecDestroy(e->graphExec);
@@ -1994,6 +2082,7 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
         VJ_TRY(e->gOrder.ensure(capTotal));
         VJ_TRY(e->gPairs.ensure(static_cast<size_t>(e->pairCap)));
         VJ_TRY(e->gCtl.ensure(1));
+        VJ_TRY(e->gScanTot.ensure((capTotal + kScanItems - 1) / kScanItems));
         constexpr int kPairGrid = 1024;
         hipLaunchKernelGGL(pair_zero_kernel, dim3(64), dim3(256), 0, s, e->dCounts.get(), nImages, capPer,
                            e->gRank.get());
@@ -2007,8 +2096,12 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
         hipLaunchKernelGGL(pair_find_kernel<false>, dim3(2048), dim3(256), 0, s, e->dCounts.get(), e->dImgBase.get(),
                            nImages, capPer, 0.2f, e->gSorted.get(), e->gPairCnt.get(), nullptr, nullptr, int64_t{0});
         VJ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(pair_scan_kernel, dim3(1), dim3(1024), 0, s, e->dCounts.get(), nImages, capPer,
-                           e->gPairCnt.get(), e->gPairOff.get(), e->gCtl.get());
+        const int scanGrid = static_cast<int>(std::min<size_t>((capTotal + kScanItems - 1) / kScanItems, 1024));
+        hipLaunchKernelGGL(pair_scan_local_kernel, dim3(scanGrid), dim3(kScanThreads), 0, s, e->dCounts.get(),
+                           nImages, capPer, e->gPairCnt.get(), e->gPairOff.get(), e->gScanTot.get());
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(pair_scan_add_kernel, dim3(scanGrid), dim3(kScanThreads), 0, s, e->dCounts.get(),
+                           nImages, capPer, e->gScanTot.get(), e->gPairOff.get(), e->gCtl.get());
         VJ_HIP(hipGetLastError());
         hipLaunchKernelGGL(pair_find_kernel<true>, dim3(2048), dim3(256), 0, s, e->dCounts.get(), e->dImgBase.get(),
                            nImages, capPer, 0.2f, e->gSorted.get(), e->gPairCnt.get(), e->gPairOff.get(),
