@@ -980,8 +980,7 @@ using namespace vj;
 // are left to the host.  Every kernel derives the images' box counts from
 // the device counts.
 // ---------------------------------------------------------------------------
-constexpr int kGroupDevMax = 16384;
-constexpr int kPairTile = 256;
+constexpr int kGroupDevMax = 1 << 24;
 
 __device__ __forceinline__ int grouped_boxes(const int32_t* __restrict__ counts, int i, int cap) {
     const int n = min(counts[i], cap);
@@ -1005,53 +1004,107 @@ __device__ __forceinline__ int image_of(const int32_t* __restrict__ imgBase, int
     return lo;
 }
 
-// Zeroes the sweep ranks of all raw boxes of the call.
-__global__ __launch_bounds__(256) void pair_zero_kernel(const int32_t* __restrict__ counts, int nImages, int cap,
-                                                       int32_t* __restrict__ rank) {
-    const int64_t total = all_boxes(counts, nImages, cap);
-    for (int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; g < total;
-         g += static_cast<int64_t>(gridDim.x) * blockDim.x)
-        rank[g] = 0;
+// Sweep ranks without a sort.  A box's rank in its image's (x, index) order
+// is the number of its image's boxes with a smaller x plus the number with
+// the same x and a smaller index.  Per image an x histogram (bins xOff[i] ..
+// xOff[i + 1], one per column 0 .. w) is scanned into each x's first rank;
+// each x's boxes are listed (in any order) in a bucket at that rank, and a
+// box's ties are the bucket entries with a smaller index.
+__global__ __launch_bounds__(256) void pair_xzero_kernel(int64_t nBins, int32_t* __restrict__ hist,
+                                                        int32_t* __restrict__ cursor) {
+    for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; k < nBins;
+         k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        hist[k] = 0;
+        cursor[k] = 0;
+    }
 }
 
-// Sweep ranks: work items (image, tile of 256 a's, tile of 256 b's); each
-// adds the count of the tile's boxes before a to rank[a].
-__global__ __launch_bounds__(kPairTile) void pair_rank_kernel(const vj_detection* __restrict__ dets,
-                                                             const int32_t* __restrict__ counts,
-                                                             const int32_t* __restrict__ imgBase, int nImages,
-                                                             int cap, int32_t* __restrict__ rank) {
-    __shared__ int sx[kPairTile];
-    int64_t totalItems = 0;
-    for (int i = 0; i < nImages; ++i) {
-        const int64_t nt = (grouped_boxes(counts, i, cap) + kPairTile - 1) / kPairTile;
-        totalItems += nt * nt;
+// The x bin of box g of image i (x of a raw box lies in 0 .. w - 1; clamped
+// into the image's bins all the same).
+__device__ __forceinline__ int64_t x_bin(const int64_t* __restrict__ xOff, int i, int x) {
+    const int64_t nb = xOff[i + 1] - xOff[i];
+    return xOff[i] + min<int64_t>(max(x, 0), nb - 1);
+}
+
+__global__ __launch_bounds__(256) void pair_xhist_kernel(const vj_detection* __restrict__ dets,
+                                                        const int32_t* __restrict__ counts,
+                                                        const int32_t* __restrict__ imgBase, int nImages, int cap,
+                                                        const int64_t* __restrict__ xOff,
+                                                        int32_t* __restrict__ hist) {
+    const int64_t total = all_boxes(counts, nImages, cap);
+    for (int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; g < total;
+         g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int i = image_of(imgBase, nImages, g);
+        if (grouped_boxes(counts, i, cap) == 0) continue;
+        atomicAdd(&hist[x_bin(xOff, i, dets[g].x)], 1);
     }
-    for (int64_t item = blockIdx.x; item < totalItems; item += gridDim.x) {
-        int i = 0, n = 0;
-        int64_t base = 0, nt = 0;
-        for (;; ++i) {
-            n = grouped_boxes(counts, i, cap);
-            nt = (n + kPairTile - 1) / kPairTile;
-            if (item < base + nt * nt) break;
-            base += nt * nt;
-        }
-        const int64_t local = item - base;
-        const int at = static_cast<int>(local / nt), jt = static_cast<int>(local - at * nt);
-        const vj_detection* D = dets + imgBase[i];
-        const int j0 = jt * kPairTile;
-        const int jn = min(kPairTile, n - j0);
-        __syncthreads();  // the previous item's tile is no longer read
-        if (static_cast<int>(threadIdx.x) < jn) sx[threadIdx.x] = D[j0 + threadIdx.x].x;
+}
+
+// One workgroup per image: exclusive scan of its x histogram into start.
+__global__ __launch_bounds__(1024) void pair_xscan_kernel(const int64_t* __restrict__ xOff,
+                                                         const int32_t* __restrict__ hist,
+                                                         int32_t* __restrict__ start) {
+    __shared__ int32_t waveTot[1024 / kWave];
+    __shared__ int32_t carry;
+    const int i = blockIdx.x;
+    const int64_t b0 = xOff[i], nb = xOff[i + 1] - xOff[i];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int64_t c0 = 0; c0 < nb; c0 += blockDim.x) {
+        const int64_t k = c0 + threadIdx.x;
+        const int32_t v = k < nb ? hist[b0 + k] : 0;
+        const int32_t inc = wave_incl_scan(v);
+        if (lane_id() == kWave - 1) waveTot[threadIdx.x / kWave] = inc;
         __syncthreads();
-        const int a = at * kPairTile + static_cast<int>(threadIdx.x);
-        if (a >= n) continue;
-        const int ax = D[a].x;
-        int before = 0;
-        for (int k = 0; k < jn; ++k) {
-            const int bx = sx[k];
-            before += (bx < ax) | ((bx == ax) & (j0 + k < a));
+        int32_t off = carry + inc - v, tot = 0;
+        for (int w = 0; w < static_cast<int>(blockDim.x / kWave); ++w) {
+            if (w < static_cast<int>(threadIdx.x / kWave)) off += waveTot[w];
+            tot += waveTot[w];
         }
-        if (before) atomicAdd(&rank[imgBase[i] + a], before);
+        if (k < nb) start[b0 + k] = off;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void pair_xfill_kernel(const vj_detection* __restrict__ dets,
+                                                        const int32_t* __restrict__ counts,
+                                                        const int32_t* __restrict__ imgBase, int nImages, int cap,
+                                                        const int64_t* __restrict__ xOff,
+                                                        const int32_t* __restrict__ start,
+                                                        int32_t* __restrict__ cursor, int32_t* __restrict__ bucket) {
+    const int64_t total = all_boxes(counts, nImages, cap);
+    for (int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; g < total;
+         g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int i = image_of(imgBase, nImages, g);
+        if (grouped_boxes(counts, i, cap) == 0) continue;
+        const int64_t k = x_bin(xOff, i, dets[g].x);
+        const int32_t pos = atomicAdd(&cursor[k], 1);
+        bucket[imgBase[i] + start[k] + pos] = static_cast<int32_t>(g - imgBase[i]);
+    }
+}
+
+__global__ __launch_bounds__(256) void pair_xrank_kernel(const vj_detection* __restrict__ dets,
+                                                        const int32_t* __restrict__ counts,
+                                                        const int32_t* __restrict__ imgBase, int nImages, int cap,
+                                                        const int64_t* __restrict__ xOff,
+                                                        const int32_t* __restrict__ hist,
+                                                        const int32_t* __restrict__ start,
+                                                        const int32_t* __restrict__ bucket,
+                                                        int32_t* __restrict__ rank) {
+    const int64_t total = all_boxes(counts, nImages, cap);
+    for (int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; g < total;
+         g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int i = image_of(imgBase, nImages, g);
+        if (grouped_boxes(counts, i, cap) == 0) continue;
+        const int64_t k = x_bin(xOff, i, dets[g].x);
+        const int32_t me = static_cast<int32_t>(g - imgBase[i]);
+        const int32_t* bk = bucket + imgBase[i] + start[k];
+        const int n = hist[k];
+        int32_t ties = 0;
+        for (int j = 0; j < n; ++j) ties += bk[j] < me;
+        rank[g] = start[k] + ties;
     }
 }
 
@@ -1245,6 +1298,11 @@ struct vj_engine {
     // grouping help: per raw box its sweep rank, pair count, pair-list offset
     // and a cursor; per image the boxes in sweep order; the pair lists
     DevBuf<int32_t> gRank, gPairCnt, gPairOff, gOrder, gPairs;
+    // sweep ranks: per image x bins (histogram, first rank, fill cursor), the
+    // boxes listed by x bucket, and the bins' offsets per image
+    DevBuf<int32_t> gXHist, gXStart, gXCursor, gBucket;
+    DevBuf<int64_t> gXOff;
+    PinnedBuf<int64_t> hXOff;
     DevBuf<int4> gSorted;
     DevBuf<int64_t> gCtl;
     DevBuf<int64_t> gScanTot;  // pair-count scan: per-block totals
@@ -2074,8 +2132,21 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
     const bool devGroup = applyGrouping != 0;
     if (devGroup) {
         const size_t capTotal = static_cast<size_t>(capPer) * static_cast<size_t>(nImages);
-        if (e->pairCap == 0) e->pairCap = static_cast<int64_t>(std::min<size_t>(capTotal, kGroupDevMax) * 16 + 65536);
+        if (e->pairCap == 0) e->pairCap = static_cast<int64_t>(std::min<size_t>(capTotal, size_t{1} << 20) * 16 + 65536);
         VJ_TRY(e->gRank.ensure(capTotal));
+        VJ_TRY(e->gBucket.ensure(capTotal));
+        // x bins per image: one per column of the image
+        std::vector<int64_t> xOff(static_cast<size_t>(nImages) + 1, 0);
+        for (int i = 0; i < nImages; ++i)
+            xOff[static_cast<size_t>(i) + 1] = xOff[static_cast<size_t>(i)] + imgs[static_cast<size_t>(i)].w + 1;
+        const int64_t xBins = xOff.back();
+        VJ_TRY(e->gXHist.ensure(static_cast<size_t>(xBins)));
+        VJ_TRY(e->gXStart.ensure(static_cast<size_t>(xBins)));
+        VJ_TRY(e->gXCursor.ensure(static_cast<size_t>(xBins)));
+        VJ_TRY(e->gXOff.ensure(xOff.size()));
+        VJ_TRY(e->hXOff.ensure(xOff.size()));
+        std::memcpy(e->hXOff.get(), xOff.data(), sizeof(int64_t) * xOff.size());
+        VJ_HIP(hipMemcpyAsync(e->gXOff.get(), e->hXOff.get(), sizeof(int64_t) * xOff.size(), hipMemcpyHostToDevice, s));
         VJ_TRY(e->gPairCnt.ensure(capTotal));
         VJ_TRY(e->gPairOff.ensure(capTotal));
         VJ_TRY(e->gSorted.ensure(capTotal));
@@ -2084,11 +2155,21 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
         VJ_TRY(e->gCtl.ensure(1));
         VJ_TRY(e->gScanTot.ensure((capTotal + kScanItems - 1) / kScanItems));
         constexpr int kPairGrid = 1024;
-        hipLaunchKernelGGL(pair_zero_kernel, dim3(64), dim3(256), 0, s, e->dCounts.get(), nImages, capPer,
-                           e->gRank.get());
+        hipLaunchKernelGGL(pair_xzero_kernel, dim3(256), dim3(256), 0, s, xBins, e->gXHist.get(), e->gXCursor.get());
         VJ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(pair_rank_kernel, dim3(kPairGrid), dim3(kPairTile), 0, s, e->dOut.get(), e->dCounts.get(),
-                           e->dImgBase.get(), nImages, capPer, e->gRank.get());
+        hipLaunchKernelGGL(pair_xhist_kernel, dim3(kPairGrid), dim3(256), 0, s, e->dOut.get(), e->dCounts.get(),
+                           e->dImgBase.get(), nImages, capPer, e->gXOff.get(), e->gXHist.get());
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(pair_xscan_kernel, dim3(nImages), dim3(1024), 0, s, e->gXOff.get(), e->gXHist.get(),
+                           e->gXStart.get());
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(pair_xfill_kernel, dim3(kPairGrid), dim3(256), 0, s, e->dOut.get(), e->dCounts.get(),
+                           e->dImgBase.get(), nImages, capPer, e->gXOff.get(), e->gXStart.get(), e->gXCursor.get(),
+                           e->gBucket.get());
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(pair_xrank_kernel, dim3(kPairGrid), dim3(256), 0, s, e->dOut.get(), e->dCounts.get(),
+                           e->dImgBase.get(), nImages, capPer, e->gXOff.get(), e->gXHist.get(), e->gXStart.get(),
+                           e->gBucket.get(), e->gRank.get());
         VJ_HIP(hipGetLastError());
         hipLaunchKernelGGL(pair_place_kernel, dim3(1024), dim3(256), 0, s, e->dOut.get(), e->dCounts.get(),
                            e->dImgBase.get(), nImages, capPer, e->gRank.get(), e->gSorted.get(), e->gOrder.get());

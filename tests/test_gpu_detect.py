@@ -261,9 +261,10 @@ def test_pair_mode_margins_near_threshold(cascade7, offset):
 def test_grouping_device_pairs_capacity_and_host_images():
     """Grouping with the device's sweep order and pairs: a dense 16,000-box
     image (about 40 pairs per box) overflows the first call's pair capacity,
-    so that call groups on the host and the next one on the device pairs;
-    a 20,000-box image (above the device limit) always groups on the host.
-    Every result equals the oracle's grouping of the oracle's raw boxes."""
+    so that call groups on the host and the next one on the device pairs
+    (the capacity has grown); a 20,000-box image then groups on the device
+    pairs from its first call.  Every result equals the oracle's grouping of
+    the oracle's raw boxes."""
     feats, stumps, stages = synth.blob_cascade((1,), seed=5, n_sample=200)
     stages = stages.copy()
     stages["threshold"] = np.float32(-1e30)
