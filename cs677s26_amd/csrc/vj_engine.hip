@@ -31,6 +31,11 @@ namespace vj {
 // Kernels
 // ===========================================================================
 
+// Vectors with element alignment only, for multi-word global stores at
+// element-aligned addresses.
+typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef uint64_t u64x2_a8 __attribute__((ext_vector_type(2), aligned(8)));
+
 // Pixel-centre bilinear resize of one pyramid level from its source level,
 // 4 output pixels per thread.  Same float sequence as the CPU path
 // (src/main.cu:565-581): fx = (x + 0.5) * sx - 0.5, clamped floor, lerp in x
@@ -87,41 +92,50 @@ __global__ __launch_bounds__(256) void resize_kernel(const LevelDesc* __restrict
     }
 }
 
-// Loads pixels x..x+3 of a row (zero beyond w).
-__device__ __forceinline__ void load4(const uint8_t* row, int x, int w, uint32_t p[4]) {
-    if (x + 3 < w && ((reinterpret_cast<uintptr_t>(row + x) & 3u) == 0)) {
-        const uint32_t v = *reinterpret_cast<const uint32_t*>(row + x);
-        p[0] = v & 0xFFu;
-        p[1] = (v >> 8) & 0xFFu;
-        p[2] = (v >> 16) & 0xFFu;
-        p[3] = v >> 24;
-    } else {
+// Pixels x..x+3 of a row packed little-endian into one word (zero beyond w).
+__device__ __forceinline__ uint32_t load4_packed(const uint8_t* row, int x, int w) {
+    if (x + 3 < w && ((reinterpret_cast<uintptr_t>(row + x) & 3u) == 0))
+        return *reinterpret_cast<const uint32_t*>(row + x);
+    uint32_t v = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) p[k] = (x + k < w) ? row[x + k] : 0u;
-    }
+    for (int k = 0; k < 4; ++k)
+        if (x + k < w) v |= static_cast<uint32_t>(row[x + k]) << (8 * k);
+    return v;
+}
+
+// Every row of a band, four pixels per lane, loaded before any is used.
+__device__ __forceinline__ void load_band(const LevelDesc& lv, int x, int y0, int y1, uint32_t pv[kBandRows]) {
+#pragma unroll
+    for (int r = 0; r < kBandRows; ++r)
+        pv[r] = (y0 + r < y1) ? load4_packed(lv.pix + static_cast<size_t>(y0 + r) * lv.stride, x, lv.w) : 0u;
 }
 
 // Integral phase 1, one wavefront per (level, band of 32 rows, chunk of 256
 // columns): column sums over the band (pixels and squares) and each row's
 // total over the chunk.
-__global__ __launch_bounds__(64) void ii_colsum_kernel(const LevelDesc* __restrict__ L,
+__global__ __launch_bounds__(64) void ii_colsum_kernel(const LevelDesc* __restrict__ L, int nItems,
                                                        const int32_t* __restrict__ itemOffs, int nLevels,
                                                        uint32_t* __restrict__ colS, uint32_t* __restrict__ colQ,
                                                        uint32_t* __restrict__ rowS, uint32_t* __restrict__ rowQ) {
     const int item = blockIdx.x;
+    if (item >= nItems) return;
     const int li = find_segment(itemOffs, nLevels, item);
     const LevelDesc lv = L[li];
     const int local = item - lv.itemOff;
     const int b = local / lv.nChunks;
     const int c = local - b * lv.nChunks;
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     const int x = c * kChunkCols + lane * 4;
     const int y0 = b * kBandRows;
     const int y1 = min(lv.h, y0 + kBandRows);
     uint32_t cs[4] = {0, 0, 0, 0}, cq[4] = {0, 0, 0, 0};
-    for (int y = y0; y < y1; ++y) {
-        uint32_t p[4];
-        load4(lv.pix + static_cast<size_t>(y) * lv.stride, x, lv.w, p);
+    uint32_t pv[kBandRows];
+    load_band(lv, x, y0, y1, pv);
+#pragma unroll
+    for (int r = 0; r < kBandRows; ++r) {
+        const int y = y0 + r;
+        if (y >= y1) break;
+        const uint32_t p[4] = {pv[r] & 0xFFu, (pv[r] >> 8) & 0xFFu, (pv[r] >> 16) & 0xFFu, pv[r] >> 24};
         uint32_t rs = 0, rq = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -195,7 +209,7 @@ __global__ __launch_bounds__(256) void ii_vscan_kernel(const LevelDesc* __restri
 // the band's top edge comes from the vertical prefixes (scanned across the
 // chunk and the chunks to its left), each row adds its row prefix; writes
 // ii[y+1][x+1] as u32 sum and u64 square sum.
-__global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restrict__ L,
+__global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restrict__ L, int nItems,
                                                       const int32_t* __restrict__ itemOffs, int nLevels,
                                                       const uint32_t* __restrict__ colS,
                                                       const uint32_t* __restrict__ colQ,
@@ -205,12 +219,13 @@ __global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restric
                                                       const uint32_t* __restrict__ rowQ,
                                                       uint32_t* __restrict__ iiS, uint64_t* __restrict__ iiQ) {
     const int item = blockIdx.x;
+    if (item >= nItems) return;
     const int li = find_segment(itemOffs, nLevels, item);
     const LevelDesc lv = L[li];
     const int local = item - lv.itemOff;
     const int b = local / lv.nChunks;
     const int c = local - b * lv.nChunks;
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     const int x = c * kChunkCols + lane * 4;
     const int y0 = b * kBandRows;
     const int y1 = min(lv.h, y0 + kBandRows);
@@ -218,19 +233,27 @@ __global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restric
     uint32_t* outS = iiS + lv.iiOff;
     uint64_t* outQ = iiQ + lv.iiOff;
 
-    // carry from the chunks to the left: band totals and per-row totals
-    uint32_t ccS = 0;
-    uint64_t ccQ = 0;
-    for (int k = 0; k < c; ++k) {
-        ccS += totS[lv.totOff + static_cast<int64_t>(b) * lv.nChunks + k];
-        ccQ += totQ[lv.totOff + static_cast<int64_t>(b) * lv.nChunks + k];
-    }
+    // carry from the chunks to the left (a level has at most 32 chunks):
+    // band totals, one chunk per lane, and per-row totals, loaded four
+    // chunks at a time
+    const int64_t tb = lv.totOff + static_cast<int64_t>(b) * lv.nChunks;
+    const uint32_t ccS = wave_sum(lane < c ? totS[tb + lane] : 0u);
+    const uint64_t ccQ = wave_sum(lane < c ? totQ[tb + lane] : uint64_t{0});
     uint32_t rcS = 0, rcQ = 0;
     if (lane < kBandRows && y0 + lane < lv.h) {
         const int64_t r = lv.rowOff + static_cast<int64_t>(y0 + lane) * lv.nChunks;
-        for (int k = 0; k < c; ++k) {
-            rcS += rowS[r + k];
-            rcQ += rowQ[r + k];
+        for (int k0 = 0; k0 < c; k0 += 4) {
+            uint32_t vs[4], vq[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                vs[u] = k0 + u < c ? rowS[r + k0 + u] : 0u;
+                vq[u] = k0 + u < c ? rowQ[r + k0 + u] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                rcS += vs[u];
+                rcQ += vq[u];
+            }
         }
     }
 
@@ -271,13 +294,18 @@ __global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restric
             outQ[0] = 0u;
         }
     }
-    for (int y = y0; y < y1; ++y) {
-        uint32_t p[4];
-        load4(lv.pix + static_cast<size_t>(y) * lv.stride, x, lv.w, p);
+    uint32_t pv[kBandRows];
+    load_band(lv, x, y0, y1, pv);
+    const bool full = x + 3 < lv.w;
+#pragma unroll
+    for (int r = 0; r < kBandRows; ++r) {
+        const int y = y0 + r;
+        if (y >= y1) break;
+        const uint32_t p[4] = {pv[r] & 0xFFu, (pv[r] >> 8) & 0xFFu, (pv[r] >> 16) & 0xFFu, pv[r] >> 24};
         const uint32_t ls = p[0] + p[1] + p[2] + p[3];
         const uint32_t lq = p[0] * p[0] + p[1] * p[1] + p[2] * p[2] + p[3] * p[3];
-        const uint32_t rowCarryS = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rcS), y - y0));
-        const uint32_t rowCarryQ = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rcQ), y - y0));
+        const uint32_t rowCarryS = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rcS), r));
+        const uint32_t rowCarryQ = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rcQ), r));
         uint32_t rs = wave_incl_scan_dpp(ls) - ls + rowCarryS;
         uint32_t rq = wave_incl_scan_dpp(lq) - lq + rowCarryQ;
         uint32_t* rowOutS = outS + static_cast<int64_t>(y + 1) * iw;
@@ -288,10 +316,20 @@ __global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restric
             rq += p[k] * p[k];
             aS[k] += rs;
             aQ[k] += rq;
-            if (x + k < lv.w) {
-                rowOutS[x + k + 1] = aS[k];
-                rowOutQ[x + k + 1] = aQ[k];
-            }
+        }
+        if (full) {
+            // columns x+1 .. x+4: one 16-byte store of sums, two of squares
+            // (global accesses need only element alignment)
+            *reinterpret_cast<u32x4_a4*>(rowOutS + x + 1) = u32x4_a4{aS[0], aS[1], aS[2], aS[3]};
+            *reinterpret_cast<u64x2_a8*>(rowOutQ + x + 1) = u64x2_a8{aQ[0], aQ[1]};
+            *reinterpret_cast<u64x2_a8*>(rowOutQ + x + 3) = u64x2_a8{aQ[2], aQ[3]};
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (x + k < lv.w) {
+                    rowOutS[x + k + 1] = aS[k];
+                    rowOutQ[x + k + 1] = aQ[k];
+                }
         }
         if (c == 0 && lane == 0) {
             rowOutS[0] = 0u;
@@ -1681,13 +1719,13 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
         if (e->timing) VJ_HIP(hipEventRecord(e->ev[1], ls));
         // 2. integrals of every level
         if (P.nItems > 0) {
-            hipLaunchKernelGGL(ii_colsum_kernel, dim3(P.nItems), dim3(64), 0, ls, dL, m32 + oItem, nLevels,
+            hipLaunchKernelGGL(ii_colsum_kernel, dim3(P.nItems), dim3(64), 0, ls, dL, P.nItems, m32 + oItem, nLevels,
                                e->colS.get(), e->colQ.get(), e->rowS.get(), e->rowQ.get());
             VJ_HIP(hipGetLastError());
             hipLaunchKernelGGL(ii_vscan_kernel, dim3(P.nChunkItems), dim3(256), 0, ls, dL, m32 + oChunk, nLevels,
                                e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get());
             VJ_HIP(hipGetLastError());
-            hipLaunchKernelGGL(ii_write_kernel, dim3(P.nItems), dim3(64), 0, ls, dL, m32 + oItem, nLevels,
+            hipLaunchKernelGGL(ii_write_kernel, dim3(P.nItems), dim3(64), 0, ls, dL, P.nItems, m32 + oItem, nLevels,
                                e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get(), e->rowS.get(),
                                e->rowQ.get(), e->iiS.get(), e->iiQ.get());
             VJ_HIP(hipGetLastError());
@@ -1911,13 +1949,13 @@ extern "C" int vj_engine_integral_batch(vj_engine* e, const uint8_t* dGray, int 
         VJ_TRY(upload(e->dLevels, lv, e->hStage, stOff, s));
         VJ_TRY(upload(e->dMeta32, offs, e->hStage, stOff, s));
         const int items = d.nBands * d.nChunks;
-        hipLaunchKernelGGL(ii_colsum_kernel, dim3(items), dim3(64), 0, s, e->dLevels.get(), e->dMeta32.get(), 1,
+        hipLaunchKernelGGL(ii_colsum_kernel, dim3(items), dim3(64), 0, s, e->dLevels.get(), items, e->dMeta32.get(), 1,
                            e->colS.get(), e->colQ.get(), e->rowS.get(), e->rowQ.get());
         VJ_HIP(hipGetLastError());
         hipLaunchKernelGGL(ii_vscan_kernel, dim3(d.nChunks), dim3(256), 0, s, e->dLevels.get(),
                            e->dMeta32.get() + 1, 1, e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get());
         VJ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(ii_write_kernel, dim3(items), dim3(64), 0, s, e->dLevels.get(), e->dMeta32.get(), 1,
+        hipLaunchKernelGGL(ii_write_kernel, dim3(items), dim3(64), 0, s, e->dLevels.get(), items, e->dMeta32.get(), 1,
                            e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get(), e->rowS.get(),
                            e->rowQ.get(), e->bSum.get(), e->bSq.get());
         VJ_HIP(hipGetLastError());
