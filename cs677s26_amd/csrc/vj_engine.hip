@@ -2097,7 +2097,7 @@ namespace {
 // Cascade tile rows per window step (1, 2, 3) the autotuner tries; each keeps
 // a tile within kTileMaxWin windows and 32 rows.
 constexpr std::array<int, 4> kTileCandidates[] = {
-    {0, 8, 4, 4}, {0, 4, 4, 4}, {0, 8, 8, 8}, {0, 8, 8, 4}, {0, 4, 8, 8}};
+    {0, 8, 8, 8}, {0, 8, 4, 4}, {0, 4, 4, 4}, {0, 8, 8, 4}, {0, 4, 8, 8}};
 
 // Autotuner: the first call of a batch shape (image sizes, scale factor,
 // minimum object size, cascade size) times the cascade kernel for each

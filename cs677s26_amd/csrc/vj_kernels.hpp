@@ -16,18 +16,18 @@ constexpr int kBandRows = 32;
 constexpr int kChunkCols = 256;
 
 // Cascade tiling: a tile is 2^lgcols x rows windows of one level, shaped per
-// window step so that every step's integral footprint stays near 11 KB of LDS
-// (64x8 windows at step 1, 32x4 at step 2, 16x4 at step 3: 88x32, 87x31 and
-// 70x34 integral entries for a 24x24 window) and eight 256-thread workgroups
-// fit on a CU.  A tile row is a 64-, 32- or 16-bit piece of one 64-bit mask
-// word.
+// window step so that every step's integral footprint stays within 14 KB of
+// LDS (64x8 windows at step 1, 32x8 at step 2, 16x8 at step 3: 88x32, 87x39
+// and 70x46 integral entries for a 24x24 window) and eight 256-thread
+// workgroups fit on a CU.  A tile row is a 64-, 32- or 16-bit piece of one
+// 64-bit mask word.  (The autotuner may pick other row counts per batch.)
 constexpr int kTileMaxWin = 512;
 
 __host__ __device__ constexpr int tile_lgcols_for_step(int step) {
     return step == 1 ? 6 : (step == 2 ? 5 : 4);
 }
-__host__ __device__ constexpr int tile_rows_for_step(int step) {
-    return step == 1 ? 8 : 4;
+__host__ __device__ constexpr int tile_rows_for_step(int) {
+    return 8;
 }
 
 // One pyramid level of one image, as every level-parallel kernel sees it.
