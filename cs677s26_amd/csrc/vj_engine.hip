@@ -408,33 +408,40 @@ __device__ __forceinline__ TileGeo tile_geo(const LevelDesc* __restrict__ L, con
     return g;
 }
 
-// Stages the tile's integral footprint into LDS as f32, four loads in
-// flight per thread before their stores.
+// Stages the tile's integral footprint into LDS as f32, row by row: wave k
+// takes rows k, k + 4, ..., four rows per step, each lane columns lane and
+// lane + 64 of a row (a footprint row is at most 128 entries: the window is
+// at most 64 wide), so eight loads are in flight per lane before their
+// stores.
 __device__ __forceinline__ void stage_footprint(const TileGeo& g, const uint32_t* __restrict__ iiS, float* lds,
                                                 int winW, int winH) {
+    constexpr int kWaves = kCascadeBlock / kWave;
     const int FW = (g.TW - 1) * g.step + winW + 1;
     const int FH = (g.TH - 1) * g.step + winH + 1;
     const int ix0 = g.wx0 * g.step, iy0 = g.wy0 * g.step;
     const int iw = g.lv.w + 1;
     const uint32_t* sumL = iiS + g.lv.iiOff;
-    const int total = FW * FH;
-    for (int e0 = threadIdx.x; e0 < total; e0 += 4 * kCascadeBlock) {
-        float v[4];
+    const int lane = lane_id(), wave = static_cast<int>(threadIdx.x) / kWave;
+    const int gx0 = ix0 + lane, gx1 = gx0 + kWave;
+    const bool c0 = lane < FW && gx0 <= g.lv.w;
+    const bool c1 = lane + kWave < FW && gx1 <= g.lv.w;
+    for (int r0 = wave; r0 < FH; r0 += 4 * kWaves) {
+        float v[4][2];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int e = e0 + u * kCascadeBlock;
-            v[u] = 0.0f;
-            if (e < total) {
-                const int r = e / FW, cc = e - r * FW;
-                const int gy = iy0 + r, gx = ix0 + cc;
-                if (gy <= g.lv.h && gx <= g.lv.w)
-                    v[u] = static_cast<float>(static_cast<int32_t>(sumL[static_cast<int64_t>(gy) * iw + gx]));
-            }
+            const int r = r0 + u * kWaves, gy = iy0 + r;
+            const bool rowOk = r < FH && gy <= g.lv.h;
+            const uint32_t* row = sumL + static_cast<int64_t>(gy) * iw;
+            v[u][0] = (rowOk && c0) ? static_cast<float>(static_cast<int32_t>(row[gx0])) : 0.0f;
+            v[u][1] = (rowOk && c1) ? static_cast<float>(static_cast<int32_t>(row[gx1])) : 0.0f;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int e = e0 + u * kCascadeBlock;
-            if (e < total) lds[e] = v[u];
+            const int r = r0 + u * kWaves;
+            if (r < FH) {
+                if (lane < FW) lds[r * FW + lane] = v[u][0];
+                if (lane + kWave < FW) lds[r * FW + lane + kWave] = v[u][1];
+            }
         }
     }
 }
