@@ -1049,126 +1049,72 @@ __device__ __forceinline__ int image_of(const int32_t* __restrict__ imgBase, int
     return lo;
 }
 
-// Sweep ranks without a sort.  A box's rank in its image's (x, index) order
-// is the number of its image's boxes with a smaller x plus the number with
-// the same x and a smaller index.  Per image an x histogram (bins xOff[i] ..
-// xOff[i + 1], one per column 0 .. w) is scanned into each x's first rank;
-// each x's boxes are listed (in any order) in a bucket at that rank, and a
-// box's ties are the bucket entries with a smaller index.
-__global__ __launch_bounds__(256) void pair_xzero_kernel(int64_t nBins, int32_t* __restrict__ hist,
-                                                        int32_t* __restrict__ cursor) {
-    for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; k < nBins;
-         k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        hist[k] = 0;
-        cursor[k] = 0;
-    }
-}
+// Sweep order without a sort, one workgroup per image.  A box's rank in
+// its image's (x, index) order is the number of the image's boxes with a
+// smaller x plus the number with the same x and a smaller index.  An x
+// histogram in LDS (one bin per column 0 .. w) is scanned into each x's
+// first rank; each x's boxes are listed (in any order) in a bucket at that
+// rank, and a box's ties are the bucket entries with a smaller index.  The
+// boxes are then placed in sweep order: sorted[base + rank] = the box's
+// edges, order[base + rank] = its index in its image.
+constexpr int kSweepThreads = 1024;
 
-// The x bin of box g of image i (x of a raw box lies in 0 .. w - 1; clamped
-// into the image's bins all the same).
-__device__ __forceinline__ int64_t x_bin(const int64_t* __restrict__ xOff, int i, int x) {
-    const int64_t nb = xOff[i + 1] - xOff[i];
-    return xOff[i] + min<int64_t>(max(x, 0), nb - 1);
-}
+inline size_t sweep_lds_bytes(int maxW) { return sizeof(int32_t) * (2 * static_cast<size_t>(maxW + 1) + 1); }
 
-__global__ __launch_bounds__(256) void pair_xhist_kernel(const vj_detection* __restrict__ dets,
-                                                        const int32_t* __restrict__ counts,
-                                                        const int32_t* __restrict__ imgBase, int nImages, int cap,
-                                                        const int64_t* __restrict__ xOff,
-                                                        int32_t* __restrict__ hist) {
-    const int64_t total = all_boxes(counts, nImages, cap);
-    for (int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; g < total;
-         g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int i = image_of(imgBase, nImages, g);
-        if (grouped_boxes(counts, i, cap) == 0) continue;
-        atomicAdd(&hist[x_bin(xOff, i, dets[g].x)], 1);
-    }
-}
-
-// One workgroup per image: exclusive scan of its x histogram into start.
-__global__ __launch_bounds__(1024) void pair_xscan_kernel(const int64_t* __restrict__ xOff,
-                                                         const int32_t* __restrict__ hist,
-                                                         int32_t* __restrict__ start) {
-    __shared__ int32_t waveTot[1024 / kWave];
-    __shared__ int32_t carry;
+__global__ __launch_bounds__(kSweepThreads) void pair_sweep_kernel(const vj_detection* __restrict__ dets,
+                                                                  const int32_t* __restrict__ counts,
+                                                                  const int32_t* __restrict__ imgBase,
+                                                                  const int32_t* __restrict__ imgW, int cap,
+                                                                  int32_t* __restrict__ bucket,
+                                                                  int4* __restrict__ sorted,
+                                                                  int32_t* __restrict__ order) {
+    extern __shared__ int32_t xs[];  // cnt[nb] (then the fill cursor), first[nb + 1]
+    __shared__ int32_t waveTot[kSweepThreads / kWave];
     const int i = blockIdx.x;
-    const int64_t b0 = xOff[i], nb = xOff[i + 1] - xOff[i];
-    if (threadIdx.x == 0) carry = 0;
+    const int n = grouped_boxes(counts, i, cap);
+    if (n == 0) return;
+    const int nb = imgW[i] + 1;
+    int32_t* cnt = xs;
+    int32_t* first = xs + nb;
+    const vj_detection* D = dets + imgBase[i];
+    auto bin = [nb](int x) { return min(max(x, 0), nb - 1); };
+    for (int k = threadIdx.x; k < nb; k += kSweepThreads) cnt[k] = 0;
     __syncthreads();
-    for (int64_t c0 = 0; c0 < nb; c0 += blockDim.x) {
-        const int64_t k = c0 + threadIdx.x;
-        const int32_t v = k < nb ? hist[b0 + k] : 0;
-        const int32_t inc = wave_incl_scan(v);
-        if (lane_id() == kWave - 1) waveTot[threadIdx.x / kWave] = inc;
-        __syncthreads();
-        int32_t off = carry + inc - v, tot = 0;
-        for (int w = 0; w < static_cast<int>(blockDim.x / kWave); ++w) {
-            if (w < static_cast<int>(threadIdx.x / kWave)) off += waveTot[w];
-            tot += waveTot[w];
-        }
-        if (k < nb) start[b0 + k] = off;
-        __syncthreads();
-        if (threadIdx.x == 0) carry += tot;
-        __syncthreads();
+    for (int g = threadIdx.x; g < n; g += kSweepThreads) atomicAdd(&cnt[bin(D[g].x)], 1);
+    __syncthreads();
+    // exclusive scan of cnt into first: each thread a run of consecutive bins
+    const int per = (nb + kSweepThreads - 1) / kSweepThreads;
+    const int k0 = min(static_cast<int>(threadIdx.x) * per, nb), k1 = min(k0 + per, nb);
+    int32_t run = 0;
+    for (int k = k0; k < k1; ++k) run += cnt[k];
+    const int32_t inc = wave_incl_scan(run);
+    if (lane_id() == kWave - 1) waveTot[threadIdx.x / kWave] = inc;
+    __syncthreads();
+    int32_t off = inc - run;
+    for (int w = 0; w < static_cast<int>(threadIdx.x / kWave); ++w) off += waveTot[w];
+    for (int k = k0; k < k1; ++k) {
+        first[k] = off;
+        off += cnt[k];
     }
-}
-
-__global__ __launch_bounds__(256) void pair_xfill_kernel(const vj_detection* __restrict__ dets,
-                                                        const int32_t* __restrict__ counts,
-                                                        const int32_t* __restrict__ imgBase, int nImages, int cap,
-                                                        const int64_t* __restrict__ xOff,
-                                                        const int32_t* __restrict__ start,
-                                                        int32_t* __restrict__ cursor, int32_t* __restrict__ bucket) {
-    const int64_t total = all_boxes(counts, nImages, cap);
-    for (int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; g < total;
-         g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int i = image_of(imgBase, nImages, g);
-        if (grouped_boxes(counts, i, cap) == 0) continue;
-        const int64_t k = x_bin(xOff, i, dets[g].x);
-        const int32_t pos = atomicAdd(&cursor[k], 1);
-        bucket[imgBase[i] + start[k] + pos] = static_cast<int32_t>(g - imgBase[i]);
+    if (threadIdx.x == kSweepThreads - 1) first[nb] = n;
+    __syncthreads();
+    for (int k = threadIdx.x; k < nb; k += kSweepThreads) cnt[k] = 0;
+    __syncthreads();
+    int32_t* B = bucket + imgBase[i];
+    for (int g = threadIdx.x; g < n; g += kSweepThreads) {
+        const int x = bin(D[g].x);
+        B[first[x] + atomicAdd(&cnt[x], 1)] = g;
     }
-}
-
-__global__ __launch_bounds__(256) void pair_xrank_kernel(const vj_detection* __restrict__ dets,
-                                                        const int32_t* __restrict__ counts,
-                                                        const int32_t* __restrict__ imgBase, int nImages, int cap,
-                                                        const int64_t* __restrict__ xOff,
-                                                        const int32_t* __restrict__ hist,
-                                                        const int32_t* __restrict__ start,
-                                                        const int32_t* __restrict__ bucket,
-                                                        int32_t* __restrict__ rank) {
-    const int64_t total = all_boxes(counts, nImages, cap);
-    for (int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; g < total;
-         g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int i = image_of(imgBase, nImages, g);
-        if (grouped_boxes(counts, i, cap) == 0) continue;
-        const int64_t k = x_bin(xOff, i, dets[g].x);
-        const int32_t me = static_cast<int32_t>(g - imgBase[i]);
-        const int32_t* bk = bucket + imgBase[i] + start[k];
-        const int n = hist[k];
+    __syncthreads();  // the bucket (global, this workgroup's own) is complete
+    for (int g = threadIdx.x; g < n; g += kSweepThreads) {
+        const vj_detection d = D[g];
+        const int x = bin(d.x);
+        const int b0 = first[x], b1 = first[x + 1];
         int32_t ties = 0;
-        for (int j = 0; j < n; ++j) ties += bk[j] < me;
-        rank[g] = start[k] + ties;
-    }
-}
-
-// The boxes in sweep order: sorted[base + rank] = the box's edges, order[base
-// + rank] = its index in its image.
-__global__ __launch_bounds__(256) void pair_place_kernel(const vj_detection* __restrict__ dets,
-                                                        const int32_t* __restrict__ counts,
-                                                        const int32_t* __restrict__ imgBase, int nImages, int cap,
-                                                        const int32_t* __restrict__ rank, int4* __restrict__ sorted,
-                                                        int32_t* __restrict__ order) {
-    const int64_t total = all_boxes(counts, nImages, cap);
-    for (int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; g < total;
-         g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int i = image_of(imgBase, nImages, g);
-        if (grouped_boxes(counts, i, cap) == 0) continue;
-        const int64_t p = imgBase[i] + rank[g];
-        const vj_detection d = dets[g];
+        for (int j = b0; j < b1; ++j) ties += B[j] < g;
+        const int64_t p = imgBase[i] + b0 + ties;
         sorted[p] = make_int4(d.x, d.y, d.w, d.h);
-        order[p] = static_cast<int32_t>(g - imgBase[i]);
+        order[p] = g;
     }
 }
 
@@ -1342,12 +1288,10 @@ struct vj_engine {
     PinnedBuf<int32_t> hCounts;
     // grouping help: per raw box its sweep rank, pair count, pair-list offset
     // and a cursor; per image the boxes in sweep order; the pair lists
-    DevBuf<int32_t> gRank, gPairCnt, gPairOff, gOrder, gPairs;
-    // sweep ranks: per image x bins (histogram, first rank, fill cursor), the
-    // boxes listed by x bucket, and the bins' offsets per image
-    DevBuf<int32_t> gXHist, gXStart, gXCursor, gBucket;
-    DevBuf<int64_t> gXOff;
-    PinnedBuf<int64_t> hXOff;
+    DevBuf<int32_t> gPairCnt, gPairOff, gOrder, gPairs;
+    // sweep order: per image its width (its x bins), the boxes listed by x
+    DevBuf<int32_t> gImgW, gBucket;
+    PinnedBuf<int32_t> hImgW;
     DevBuf<int4> gSorted;
     DevBuf<int64_t> gCtl;
     DevBuf<int64_t> gScanTot;  // pair-count scan: per-block totals
@@ -1879,6 +1823,16 @@ extern "C" int vj_engine_create(int winW, int winH, int maxSamples, int maxImage
         set_error("vj_engine_create: no HIP device");
         return VJ_DEVICE_ERROR;
     }
+    {
+        // the sweep kernel's x bins of an 8192-wide image take just over 64 KB of LDS
+        static const hipError_t ldsAttr = hipFuncSetAttribute(reinterpret_cast<const void*>(pair_sweep_kernel),
+                                                              hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+        if (ldsAttr != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("vj_engine_create: cannot raise the grouping kernel's LDS limit");
+            return VJ_DEVICE_ERROR;
+        }
+    }
     vj_engine* e = new (std::nothrow) vj_engine();
     if (!e) {
         set_error("vj_engine_create: out of host memory");
@@ -2178,20 +2132,16 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
     if (devGroup) {
         const size_t capTotal = static_cast<size_t>(capPer) * static_cast<size_t>(nImages);
         if (e->pairCap == 0) e->pairCap = static_cast<int64_t>(std::min<size_t>(capTotal, size_t{1} << 20) * 16 + 65536);
-        VJ_TRY(e->gRank.ensure(capTotal));
+        // image widths: one x bin per column 0 .. w of an image (LDS of the sweep)
+        int maxW = 1;
+        VJ_TRY(e->hImgW.ensure(static_cast<size_t>(nImages)));
+        for (int i = 0; i < nImages; ++i) {
+            e->hImgW.get()[i] = imgs[static_cast<size_t>(i)].w;
+            maxW = std::max(maxW, imgs[static_cast<size_t>(i)].w);
+        }
+        VJ_TRY(e->gImgW.ensure(static_cast<size_t>(nImages)));
+        VJ_HIP(hipMemcpyAsync(e->gImgW.get(), e->hImgW.get(), sizeof(int32_t) * nImages, hipMemcpyHostToDevice, s));
         VJ_TRY(e->gBucket.ensure(capTotal));
-        // x bins per image: one per column of the image
-        std::vector<int64_t> xOff(static_cast<size_t>(nImages) + 1, 0);
-        for (int i = 0; i < nImages; ++i)
-            xOff[static_cast<size_t>(i) + 1] = xOff[static_cast<size_t>(i)] + imgs[static_cast<size_t>(i)].w + 1;
-        const int64_t xBins = xOff.back();
-        VJ_TRY(e->gXHist.ensure(static_cast<size_t>(xBins)));
-        VJ_TRY(e->gXStart.ensure(static_cast<size_t>(xBins)));
-        VJ_TRY(e->gXCursor.ensure(static_cast<size_t>(xBins)));
-        VJ_TRY(e->gXOff.ensure(xOff.size()));
-        VJ_TRY(e->hXOff.ensure(xOff.size()));
-        std::memcpy(e->hXOff.get(), xOff.data(), sizeof(int64_t) * xOff.size());
-        VJ_HIP(hipMemcpyAsync(e->gXOff.get(), e->hXOff.get(), sizeof(int64_t) * xOff.size(), hipMemcpyHostToDevice, s));
         VJ_TRY(e->gPairCnt.ensure(capTotal));
         VJ_TRY(e->gPairOff.ensure(capTotal));
         VJ_TRY(e->gSorted.ensure(capTotal));
@@ -2199,25 +2149,9 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
         VJ_TRY(e->gPairs.ensure(static_cast<size_t>(e->pairCap)));
         VJ_TRY(e->gCtl.ensure(1));
         VJ_TRY(e->gScanTot.ensure((capTotal + kScanItems - 1) / kScanItems));
-        constexpr int kPairGrid = 1024;
-        hipLaunchKernelGGL(pair_xzero_kernel, dim3(256), dim3(256), 0, s, xBins, e->gXHist.get(), e->gXCursor.get());
-        VJ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(pair_xhist_kernel, dim3(kPairGrid), dim3(256), 0, s, e->dOut.get(), e->dCounts.get(),
-                           e->dImgBase.get(), nImages, capPer, e->gXOff.get(), e->gXHist.get());
-        VJ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(pair_xscan_kernel, dim3(nImages), dim3(1024), 0, s, e->gXOff.get(), e->gXHist.get(),
-                           e->gXStart.get());
-        VJ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(pair_xfill_kernel, dim3(kPairGrid), dim3(256), 0, s, e->dOut.get(), e->dCounts.get(),
-                           e->dImgBase.get(), nImages, capPer, e->gXOff.get(), e->gXStart.get(), e->gXCursor.get(),
-                           e->gBucket.get());
-        VJ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(pair_xrank_kernel, dim3(kPairGrid), dim3(256), 0, s, e->dOut.get(), e->dCounts.get(),
-                           e->dImgBase.get(), nImages, capPer, e->gXOff.get(), e->gXHist.get(), e->gXStart.get(),
-                           e->gBucket.get(), e->gRank.get());
-        VJ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(pair_place_kernel, dim3(1024), dim3(256), 0, s, e->dOut.get(), e->dCounts.get(),
-                           e->dImgBase.get(), nImages, capPer, e->gRank.get(), e->gSorted.get(), e->gOrder.get());
+        hipLaunchKernelGGL(pair_sweep_kernel, dim3(nImages), dim3(kSweepThreads), sweep_lds_bytes(maxW), s,
+                           e->dOut.get(), e->dCounts.get(), e->dImgBase.get(), e->gImgW.get(), capPer,
+                           e->gBucket.get(), e->gSorted.get(), e->gOrder.get());
         VJ_HIP(hipGetLastError());
         hipLaunchKernelGGL(pair_find_kernel<false>, dim3(2048), dim3(256), 0, s, e->dCounts.get(), e->dImgBase.get(),
                            nImages, capPer, 0.2f, e->gSorted.get(), e->gPairCnt.get(), nullptr, nullptr, int64_t{0});
