@@ -618,15 +618,20 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
                         part += (haar_raw(o, sp) * inv < sp.theta) ? sp.left : sp.right;
                     }
                 }
-            } else {
-                KStump curRec{};
-                if (js0 < nt) curRec = sg[js0];
-                for (int jb = 0; jb < nt; jb += spp) {
-                    const int js = jb + js0;
-                    KStump nextRec{};
-                    if (js + spp < nt) nextRec = sg[js + spp];
-                    if (evalLive && js < nt) part += (haar_raw(o, curRec) * inv < curRec.theta) ? curRec.left : curRec.right;
-                    curRec = nextRec;
+            } else if (nt > 0) {
+                // each lane's records come one pass ahead, from indices clamped
+                // into the stage (always valid; a clamped record is loaded but
+                // not used), in two registers that alternate, so no record is
+                // zeroed or copied
+                const int last = nt - 1;
+                KStump recA = sg[min(js0, last)];
+                for (int jb = 0; jb < nt; jb += 2 * spp) {
+                    const int jsA = jb + js0, jsB = jsA + spp;
+                    const KStump recB = sg[min(jsB, last)];
+                    if (evalLive && jsA < nt) part += (haar_raw(o, recA) * inv < recA.theta) ? recA.left : recA.right;
+                    if (jb + spp >= nt) break;
+                    recA = sg[min(jsB + spp, last)];
+                    if (evalLive && jsB < nt) part += (haar_raw(o, recB) * inv < recB.theta) ? recB.left : recB.right;
                 }
             }
             // lanes l and l ^ d (d >= nwp) of a wave hold the same window

@@ -183,8 +183,8 @@ int vj_engine_phase_ms(vj_engine* e, float* ms);
 /* Batched detect (no reference counterpart: the reference detects one image
  * per call).  nImages images, each through the same pyramid, integral,
  * cascade and compaction kernels as vj_engine_detect but all images' work in
- * the same launches; the launch sequence is captured once per batch shape in
- * a HIP graph and replayed (vj_engine_set_graphs), with tile shapes from the
+ * the same launches; the launch sequence of a batch shape that repeats is
+ * captured in a HIP graph and replayed (vj_engine_set_graphs), with tile shapes from the
  * autotuner when it is on (vj_engine_set_autotune).  Per image the detections are identical to
  * vj_engine_detect on that image.  hOut receives the images' detections
  * concatenated in image order; hCounts[i] is image i's count; the return is
@@ -197,8 +197,9 @@ int vj_engine_detect_batch(vj_engine* e, int nImages, const uint8_t* const* dIma
 
 /* Launch-path switches of the detect pipeline (no reference counterpart).
  * Graphs (default on): the launch sequence of a detect is captured in a HIP
- * graph and replayed while the plan shape and the engine's buffers are
- * unchanged.  Autotune (default off): the first call of each batch shape
+ * graph on the second consecutive call of a plan shape and replayed while the
+ * shape and the engine's buffers are unchanged; a shape seen once is launched
+ * directly.  Autotune (default off): the first call of each batch shape
  * (image sizes, scale factor, minimum object size, cascade) times the
  * cascade kernel over a few candidate tile shapes and keeps the fastest for
  * that shape.  Neither changes any detection.  vj_engine_tile_rows reports
