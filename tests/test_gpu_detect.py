@@ -279,3 +279,29 @@ def test_grouping_device_pairs_capacity_and_host_images():
         for rep in range(2):
             got = eng.detectMultiScale(d, 640, 480, 640, 1.25, 4, 24, max_det, applyGrouping=True)
             assert det_tuples(got) == want, (max_det, rep)
+
+
+def test_grouping_widest_images_and_empty_ones():
+    """Device grouping help at the widest supported image (8192 columns: the
+    sweep kernel's x bins take just over 64 KB of LDS) in a batch that also
+    holds a flat image (no detections) and a narrow one: every image's
+    grouped boxes equal the oracle's grouping of the oracle's raw boxes."""
+    say("grouping: 8192-wide image, flat image, narrow image in one batch")
+    feats, stumps, stages = synth.blob_cascade((1,), seed=5, n_sample=200)
+    stages = stages.copy()
+    stages["threshold"] = np.float32(-1e30)  # every window that passes the variance gate
+    wide = synth.detect_image(8192, 40, 12, 24, 36, seed=61)
+    flat = np.full((120, 200), 77, np.uint8)  # zero variance: no window passes the gate
+    narrow = synth.detect_image(60, 300, 3, 24, 40, seed=62)
+    imgs = [wide, flat, narrow]
+    eng = _engine(8192, 300, (feats, stumps, stages))
+    dev = [DeviceArray.from_host(im) for im in imgs]
+    sizes = [(im.shape[1], im.shape[0], im.shape[1]) for im in imgs]
+    got = eng.detect_batch(dev, sizes, 1.25, 3, 24, 200000, True)
+    for im, g in zip(imgs, got):
+        raw, _ = O.detect_raw(feats, stumps, stages, im)
+        assert det_tuples(g) == det_tuples(O.group(raw, 3)), im.shape
+    assert len(got[1]) == 0
+    # the plain path on the wide image alone
+    single = eng.detectMultiScale(dev[0], 8192, 40, 8192, 1.25, 3, 24, 200000, True)
+    assert det_tuples(single) == det_tuples(got[0])
