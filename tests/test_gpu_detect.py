@@ -305,3 +305,31 @@ def test_grouping_widest_images_and_empty_ones():
     # the plain path on the wide image alone
     single = eng.detectMultiScale(dev[0], 8192, 40, 8192, 1.25, 3, 24, 200000, True)
     assert det_tuples(single) == det_tuples(got[0])
+
+
+def test_batch64_workload_equals_plain_per_image():
+    """The batched form at its full benchmark size (64 seeded images, sizes
+    log-uniform from 320x240 to 4K, the 38-stage / 6,061-stump cascade, graph
+    replay and the autotuner on): every image's detections, grouped and raw,
+    equal the plain per-image path's on that image."""
+    from cs677s26_amd import workloads
+    say("batch of 64 (benchmark workload) vs plain per image")
+    sizes, imgs = workloads.batch64()
+    feats, stumps, stages = workloads.cascade_38()
+    mw, mh = max(s[0] for s in sizes), max(s[1] for s in sizes)
+    eng = _engine(mw, mh, (feats, stumps, stages))
+    dev = [DeviceArray.from_host(im) for im in imgs]
+    spec = [(w, h, w) for w, h in sizes]
+    eng.set_autotune(True)
+    for grouping in (True, False):
+        for rep in range(3):  # direct launch, capture, replay
+            got = eng.detect_batch(dev, spec, 1.25, 4, 24, 200000, grouping)
+            if rep == 0:
+                first = [det_tuples(x) for x in got]
+            else:
+                assert [det_tuples(x) for x in got] == first, (grouping, rep)
+        plain = FaceVisionEngine((24, 24), 1, mw, mh, False, 200000)
+        plain.set_cascade_host(feats, stumps, stages)
+        for i, (d, (w, h)) in enumerate(zip(dev, sizes)):
+            assert det_tuples(plain.detectMultiScale(d, w, h, w, 1.25, 4, 24, 200000, grouping)) == first[i], (grouping, i)
+    say(f"  raw boxes {sum(len(x) for x in first)}")
