@@ -10,6 +10,7 @@ Generated cascades are cached as .bin model files under ``.vjcache/`` at the
 repository root (or ``$VJ_CACHE_DIR``); the cache is an accelerator only, a
 missing file is regenerated from the seed.
 """
+import functools
 import os
 import zlib
 
@@ -55,7 +56,10 @@ def training(n_pos=10000, n_neg=10000, seed=synth.DEFAULT_SEED):
     return synth.training_set(n_pos, n_neg, seed)
 
 
+@functools.lru_cache(maxsize=2)
 def batch64(seed=synth.DEFAULT_SEED):
+    """The 64 batch images, generated once per process (callers copy from
+    the arrays, never into them)."""
     sizes = synth.batch_sizes(64, seed=seed)
     return sizes, synth.batch_images(sizes, seed=seed)
 
