@@ -341,8 +341,6 @@ __device__ __forceinline__ int64_t wave_incl_scan_dpp64(int64_t v) {
 
 constexpr int kScanPer = 8;  // sorted entries per lane per step (one 16-byte load)
 
-// Exact threshold scan of one feature's sorted order by one wavefront; returns
-// the feature's best (error, split, parity) in every lane (split -1: none).
 // A search weight as the scan reads it from LDS: a packed word, or (when the
 // samples' int64 weights fit the workgroup's LDS) the int64 value itself.
 __device__ __forceinline__ int64_t weight_value(uint32_t w) { return unpack_weight(w); }
@@ -350,7 +348,25 @@ __device__ __forceinline__ int64_t weight_value(int64_t w) { return w; }
 __device__ __forceinline__ void store_weight(uint32_t* lsw, int s, uint32_t packed) { lsw[s] = packed; }
 __device__ __forceinline__ void store_weight(int64_t* lsw, int s, uint32_t packed) { lsw[s] = unpack_weight(packed); }
 
-template <typename W>
+// Exact threshold scan of one feature's sorted order by one wavefront; returns
+// the feature's best (error, split, parity) in every lane (split -1: none).
+// With kSplit false only the error is exact: the scan keeps the largest and
+// smallest prefix but not where they occur, and fK is 0 when the feature has
+// a split at all (the error does not depend on which split or parity wins a
+// tie, so the search runs this form and rescans only its winners).
+// v ? b : a for each lane v of the wave's lane mask (v_cndmask_b32 on both
+// halves; VALU only).
+__device__ __forceinline__ int64_t select_lanes(int64_t a, int64_t b, uint64_t mask) {
+    uint32_t lo, hi;
+    asm("v_cndmask_b32 %0, %1, %2, %3" : "=v"(lo) : "v"(static_cast<uint32_t>(a)), "v"(static_cast<uint32_t>(b)), "s"(mask));
+    asm("v_cndmask_b32 %0, %1, %2, %3"
+        : "=v"(hi)
+        : "v"(static_cast<uint32_t>(static_cast<uint64_t>(a) >> 32)), "v"(static_cast<uint32_t>(static_cast<uint64_t>(b) >> 32)),
+          "s"(mask));
+    return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+template <bool kSplit, typename W>
 __device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, int n, const W* lsw,
                                              int64_t totPos, int64_t totNeg, int lane,
                                              int64_t& fE, int32_t& fK, int32_t& fP) {
@@ -382,13 +398,24 @@ __device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, i
             D += q[j];
             const uint32_t e = (ew[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
             // branch-free: the two selects chains are independent
-            const bool valid = (e & 0x8000u) && (!kTail || k0 + j < n);
-            const bool up = valid && D > hiD;
-            const bool down = valid && D < loD;
-            hiD = up ? D : hiD;
-            hiK = up ? k0 + j : hiK;
-            loD = down ? D : loD;
-            loK = down ? k0 + j : loK;
+            if constexpr (kSplit) {
+                const bool valid = (e & 0x8000u) && (!kTail || k0 + j < n);
+                const bool up = valid && D > hiD;
+                const bool down = valid && D < loD;
+                hiD = up ? D : hiD;
+                loD = down ? D : loD;
+                hiK = up ? k0 + j : hiK;
+                loK = down ? k0 + j : loK;
+            } else {
+                // two selects per chain under lane masks formed straight from
+                // the compares (written as plain selects, the compiler turns
+                // them into a 64-bit max followed by a select on valid: four
+                // per chain)
+                uint64_t vm = __builtin_amdgcn_uicmp(e & 0x8000u, 0u, 33 /* ne */);
+                if constexpr (kTail) vm &= __builtin_amdgcn_sicmp(k0 + j, n, 40 /* slt */);
+                hiD = select_lanes(hiD, D, vm & __builtin_amdgcn_sicmpl(D, hiD, 38 /* sgt */));
+                loD = select_lanes(loD, D, vm & __builtin_amdgcn_sicmpl(D, loD, 40 /* slt */));
+            }
         }
         const uint64_t last = static_cast<uint64_t>(inc);
         carry += static_cast<int64_t>(
@@ -396,23 +423,53 @@ __device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, i
              << 32) |
             static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(last), 63)));
     };
-    // whole steps with the next step's entries loaded one step ahead, then
-    // the partial last step (rows are padded to 256 entries, so a lane's
-    // 16-byte load starting below n stays inside its row)
+    // whole steps, each step's entries loaded kAhead steps before it is
+    // scanned (with one workgroup per CU the scan waits on HBM latency unless
+    // several 1 KB loads per wave are in flight), then the partial last step
+    // (rows are padded to 256 entries, so a lane's 16-byte load starting
+    // below n stays inside its row)
     constexpr int kStep = kWave * kScanPer;
+    constexpr int kAhead = 4;
     auto load8 = [&](int base) { return *reinterpret_cast<const uint4*>(row + base + lane * kScanPer); };
-    int base = 0;
-    if (n >= kStep) {
-        uint4 cur = load8(0);
-        for (; base + kStep <= n; base += kStep) {
-            const uint4 nxt = (base + 2 * kStep <= n) ? load8(base + kStep) : make_uint4(0u, 0u, 0u, 0u);
-            step(base, cur, std::false_type{});
-            cur = nxt;
+    const int nFull = n / kStep;
+    uint4 buf[kAhead];
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u) buf[u] = (u < nFull) ? load8(u * kStep) : make_uint4(0u, 0u, 0u, 0u);
+    int s0 = 0;
+    for (; s0 + kAhead <= nFull; s0 += kAhead) {
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) {
+            const uint4 cur = buf[u];
+            const int nx = s0 + u + kAhead;
+            buf[u] = (nx < nFull) ? load8(nx * kStep) : make_uint4(0u, 0u, 0u, 0u);
+            step((s0 + u) * kStep, cur, std::false_type{});
         }
     }
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u)
+        if (s0 + u < nFull) step((s0 + u) * kStep, buf[u], std::false_type{});
+    const int base = nFull * kStep;
     if (base < n) {
         const uint4 e8 = (base + lane * kScanPer < n) ? load8(base) : make_uint4(0u, 0u, 0u, 0u);
         step(base, e8, std::true_type{});
+    }
+    if constexpr (!kSplit) {
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) {
+            const int64_t oH = __shfl_xor(hiD, d, kWave);
+            const int64_t oL = __shfl_xor(loD, d, kWave);
+            hiD = oH > hiD ? oH : hiD;
+            loD = oL < loD ? oL : loD;
+        }
+        fE = INT64_MAX;
+        fK = -1;
+        fP = +1;
+        if (loD != INT64_MAX) {
+            const int64_t eN = totNeg + loD, eP = totPos - hiD;
+            fE = eP < eN ? eP : eN;
+            fK = 0;
+        }
+        return;
     }
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) {
@@ -493,13 +550,16 @@ __global__ __launch_bounds__(kSearchThreads) __attribute__((amdgpu_waves_per_eu(
         if (used && used[f]) continue;
         int64_t fE;
         int32_t fK, fP;
-        scan_feature(ord + static_cast<int64_t>(f) * pitch, n, lsw, totPos, totNeg, lane, fE, fK, fP);
+        scan_feature<false>(ord + static_cast<int64_t>(f) * pitch, n, lsw, totPos, totNeg, lane, fE, fK, fP);
         if (fK >= 0 && (fE < bE || (fE == bE && f < bF))) {
             bE = fE;
-            bK = fK;
-            bP = fP;
             bF = f;
         }
+    }
+    // the wave's winner again, now with its split and parity (same error)
+    if (bF != INT_MAX) {
+        int64_t fE;
+        scan_feature<true>(ord + static_cast<int64_t>(bF) * pitch, n, lsw, totPos, totNeg, lane, fE, bK, bP);
     }
     if (lane == 0) wb[wave] = {bE, bK, bP, bF, 0};
     __syncthreads();
@@ -858,7 +918,7 @@ __global__ __launch_bounds__(kSearchThreads) void search_each_kernel(const uint1
         const uint16_t* row = ord + static_cast<int64_t>(f) * pitch;
         int64_t fE;
         int32_t fK, fP;
-        scan_feature(row, n, lsw, totPos, totNeg, lane, fE, fK, fP);
+        scan_feature<true>(row, n, lsw, totPos, totNeg, lane, fE, fK, fP);
         if (lane == 0) {
             FeatureBestSplit r{};
             if (fK >= 0 && fK + 1 < n) {
