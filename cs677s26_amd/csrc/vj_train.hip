@@ -411,7 +411,9 @@ __device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, i
                 // the compares (written as plain selects, the compiler turns
                 // them into a 64-bit max followed by a select on valid: four
                 // per chain)
-                uint64_t vm = __builtin_amdgcn_uicmp(e & 0x8000u, 0u, 33 /* ne */);
+                const int32_t ev = (j & 1) ? static_cast<int32_t>(ew[j >> 1])
+                                           : static_cast<int32_t>(static_cast<int16_t>(ew[j >> 1] & 0xFFFFu));
+                uint64_t vm = __builtin_amdgcn_sicmp(ev, 0, 40 /* slt: bit 15 of the entry */);
                 if constexpr (kTail) vm &= __builtin_amdgcn_sicmp(k0 + j, n, 40 /* slt */);
                 hiD = select_lanes(hiD, D, vm & __builtin_amdgcn_sicmpl(D, hiD, 38 /* sgt */));
                 loD = select_lanes(loD, D, vm & __builtin_amdgcn_sicmpl(D, loD, 40 /* slt */));
