@@ -82,6 +82,28 @@ def test_rounds_bit_exact_and_f64_errors():
     assert remain == int((strong[1000:] >= thr).sum())
 
 
+def test_rounds_with_duplicate_features_take_the_smaller_index():
+    """Every feature twice (the second copy 200 indices later, so copies sit in
+    different waves and workgroups): every round two features tie on the
+    error, and the smaller index must win.  The scan that finds each feature's error does not track the split; the
+    split and parity come from the rescan of each wave's winner."""
+    say("duplicate-feature rounds 2000 samples")
+    patches, labels = synth.training_set(1000, 1000, seed=11)
+    sub = _subset_pool()[:200]
+    pool = np.concatenate([sub, sub])
+    tr = AdaBoostTrainer((24, 24), len(labels), 1024)
+    d_p, d_f = DeviceArray.from_host(patches), DeviceArray.from_host(pool)
+    tr.stage_begin(d_p, labels, 1000, d_f, len(pool))
+    recs, w_or, _, _ = O.train_stage(patches, labels, 1000, pool, 6)
+    for i, rec in enumerate(recs):
+        r = tr.round(0.99, 0.40)
+        assert (r["feature"], r["split"], r["parity"]) == (rec["feature"], rec["split"], rec["parity"]), i
+        assert r["err"] == rec["err"], i
+        assert np.float32(r["theta"]).view(np.uint32) == rec["theta_bits"], i
+    w, _, _, _ = tr.read_state()
+    assert np.array_equal(w.view(np.uint64), w_or.view(np.uint64))
+
+
 def test_per_op_interface_matches_oracle():
     say("per-op trainer interface")
     patches, labels = synth.training_set(700, 900, seed=21)
