@@ -4,6 +4,7 @@ Imported only by tests/, the smoke entry point and bench.py's cpu_baseline
 leg, where it is the checker.  The product (cs677s26_amd) never imports it.
 Builds the library on first use when it is missing and gcc is present.
 """
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -20,40 +21,67 @@ LEVEL = np.dtype([("scaleIdx", "<i4"), ("w", "<i4"), ("h", "<i4"), ("step", "<i4
 FEATURE_BEST = np.dtype([("errQ", "<i8"), ("split", "<i4"), ("parity", "<i4")])
 
 
-def build():
-    subprocess.check_call(["make", "-s", "-C", _HERE])
+def build(target="all"):
+    subprocess.check_call(["make", "-s", "-C", _HERE, target])
+
+
+def _bind(L):
+    P = C.c_void_p
+    i, f = C.c_int, C.c_float
+    L.or_haar_pool.argtypes = [i, i, i, P, i]
+    L.or_integral.argtypes = [P, i, i, i, P, P]
+    L.or_train_invnorm.argtypes = [P, P, i, i]
+    L.or_train_invnorm.restype = f
+    L.or_resize.argtypes = [P, i, i, i, P, i, i]
+    L.or_crop_resize.argtypes = [P, i, i, i, i, i, i, i, i, i, P]
+    L.or_levels.argtypes = [i, i, i, i, f, i, P, i]
+    L.or_detect_raw.argtypes = [P, i, P, i, P, i, i, i, P, i, i, i, f, i, i, P, i, P]
+    L.or_group.argtypes = [P, i, i, f, P]
+    L.or_train_integrals.argtypes = [P, i, i, i, P, P, P]
+    L.or_train_response.argtypes = [P, P, P, i, i]
+    L.or_train_response.restype = f
+    L.or_train_responses.argtypes = [P, P, P, i, P]
+    L.or_presort.argtypes = [P, i, i, P, P, i, P, P]
+    L.or_fixed_weight.argtypes = [f]
+    L.or_fixed_weight.restype = C.c_int64
+    L.or_search.argtypes = [P, P, i, i, P, P]
+    L.or_select.argtypes = [P, i, P, P]
+    L.or_exact_sum.argtypes = [P, i]
+    L.or_exact_sum.restype = C.c_double
+    return L
 
 
 def lib():
+    """The oracle library: build/libvjoracle.so, or the file $VJ_ORACLE_LIB
+    names (the sanitizer build, for instance)."""
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+        path = os.environ.get("VJ_ORACLE_LIB") or _LIB_PATH
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(_LIB_PATH)
-        P = C.c_void_p
-        i, f = C.c_int, C.c_float
-        L.or_haar_pool.argtypes = [i, i, i, P, i]
-        L.or_integral.argtypes = [P, i, i, i, P, P]
-        L.or_train_invnorm.argtypes = [P, P, i, i]
-        L.or_train_invnorm.restype = f
-        L.or_resize.argtypes = [P, i, i, i, P, i, i]
-        L.or_crop_resize.argtypes = [P, i, i, i, i, i, i, i, i, i, P]
-        L.or_levels.argtypes = [i, i, i, i, f, i, P, i]
-        L.or_detect_raw.argtypes = [P, i, P, i, P, i, i, i, P, i, i, i, f, i, i, P, i, P]
-        L.or_group.argtypes = [P, i, i, f, P]
-        L.or_train_integrals.argtypes = [P, i, i, i, P, P, P]
-        L.or_train_response.argtypes = [P, P, P, i, i]
-        L.or_train_response.restype = f
-        L.or_train_responses.argtypes = [P, P, P, i, P]
-        L.or_presort.argtypes = [P, i, i, P, P, i, P, P]
-        L.or_fixed_weight.argtypes = [f]
-        L.or_fixed_weight.restype = C.c_int64
-        L.or_search.argtypes = [P, P, i, i, P, P]
-        L.or_select.argtypes = [P, i, P, P]
-        L.or_exact_sum.argtypes = [P, i]
-        L.or_exact_sum.restype = C.c_double
-        _lib = L
+        _lib = _bind(C.CDLL(path))
     return _lib
+
+
+@contextlib.contextmanager
+def use_library(path):
+    """Runs the block with every wrapper of this module bound to the oracle
+    build at `path` (the planted-mistake builds of `make mutants`)."""
+    global _lib
+    saved = lib()
+    _lib = _bind(C.CDLL(path))
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
+def mutant_path(k):
+    """Path of the build with planted mistake k (built on first use)."""
+    path = os.path.join(_HERE, "build", f"libvjoracle_m{k}.so")
+    if not os.path.exists(path):
+        build("mutants")
+    return path
 
 
 def _p(a):

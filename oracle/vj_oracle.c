@@ -18,6 +18,18 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* OR_MUTANT (0 in every normal build) plants one known mistake, so the tests
+ * can show that their brute-force checks catch it (oracle/Makefile target
+ * `mutants`; tests/test_oracle.py::test_mutants_fail_the_brute_force_checks):
+ *   1  detect: the third corner of a rectangle is added, not subtracted
+ *   2  detect: a stage's stumps are summed in reverse order
+ *   3  training: the second rectangle takes the first one's weight
+ *   4  training: the last corner of the first rectangle is read one column left
+ *   5  detect: the window normaliser uses the inset rect's p1 corner twice */
+#ifndef OR_MUTANT
+#define OR_MUTANT 0
+#endif
+
 #define MAXI(a, b) ((a) > (b) ? (a) : (b))
 #define MINI(a, b) ((a) < (b) ? (a) : (b))
 
@@ -251,7 +263,11 @@ static float det_rect(const or_rect* r, const uint32_t* sum, int iiW, int base, 
     const int dy3 = r->p3 / mW, dx3 = r->p3 - dy3 * mW;
     const int p0 = base + dy0 * iiW + dx0, p1 = base + dy1 * iiW + dx1;
     const int p2 = base + dy2 * iiW + dx2, p3 = base + dy3 * iiW + dx3;
+#if OR_MUTANT == 1
+    return (ffi(sum, p0) - ffi(sum, p1) + ffi(sum, p2) + ffi(sum, p3)) * r->w;
+#else
     return (ffi(sum, p0) - ffi(sum, p1) - ffi(sum, p2) + ffi(sum, p3)) * r->w;
+#endif
 }
 
 int or_detect_raw(const or_feature* features, int nFeatures,
@@ -316,8 +332,13 @@ int or_detect_raw(const or_feature* features, int nFeatures,
             for (int x = 0; x < workW; x += step) {
                 const int base = y * iiW + x;
                 ++nvis;
+#if OR_MUTANT == 5
+                const double normSum = (double)(int32_t)(sum[base + np0] - sum[base + np1] -
+                                                         sum[base + np1] + sum[base + np3]);
+#else
                 const double normSum = (double)(int32_t)(sum[base + np0] - sum[base + np1] -
                                                          sum[base + np2] + sum[base + np3]);
+#endif
                 const uint32_t normSqU = (uint32_t)sq[base + np0] - (uint32_t)sq[base + np1] -
                                          (uint32_t)sq[base + np2] + (uint32_t)sq[base + np3];
                 const double nf2 = (double)narea * (double)normSqU - normSum * normSum;
@@ -328,7 +349,12 @@ int or_detect_raw(const or_feature* features, int nFeatures,
                 float last = 0.0f;
                 for (int si = 0; si < nStages; ++si) {
                     float stageSum = 0.0f;
-                    for (int wi = 0; wi < stages[si].ntrees; ++wi) {
+                    for (int wj = 0; wj < stages[si].ntrees; ++wj) {
+#if OR_MUTANT == 2
+                        const int wi = stages[si].ntrees - 1 - wj;
+#else
+                        const int wi = wj;
+#endif
                         const or_stump* st = &stumps[stages[si].first + wi];
                         const or_feature* f = &features[st->feature];
                         float raw = det_rect(&f->r[0], sum, iiW, base, mW) +
@@ -536,9 +562,17 @@ void or_train_integrals(const uint8_t* patches, int w, int h, int n,
 float or_train_response(const or_feature* f, const uint32_t* sumT, const float* invNorm,
                         int n, int s) {
 #define RS(P) ((float)(int32_t)sumT[(size_t)(P) * n + s])
+#if OR_MUTANT == 4
+    const float s0 = RS(f->r[0].p0) - RS(f->r[0].p1) - RS(f->r[0].p2) + RS(f->r[0].p3 - 1);
+#else
     const float s0 = RS(f->r[0].p0) - RS(f->r[0].p1) - RS(f->r[0].p2) + RS(f->r[0].p3);
+#endif
     const float s1 = RS(f->r[1].p0) - RS(f->r[1].p1) - RS(f->r[1].p2) + RS(f->r[1].p3);
+#if OR_MUTANT == 3
+    float resp = f->r[0].w * s0 + f->r[0].w * s1;
+#else
     float resp = f->r[0].w * s0 + f->r[1].w * s1;
+#endif
     if (f->rectCount == 3) {
         const float s2 = RS(f->r[2].p0) - RS(f->r[2].p1) - RS(f->r[2].p2) + RS(f->r[2].p3);
         resp += f->r[2].w * s2;

@@ -202,6 +202,242 @@ def test_golden_detect_fixtures():
     assert det_tuples(O.group(raw, 4)) == det_tuples(g["grouped"])
 
 
+# ---- Haar responses and the stage loop, pinned from raw pixels ---------------
+
+def _rect_of(r, iiw=25):
+    """(x, y, w, h) of an upright rectangle record, after checking that its
+    four corners are those of that rectangle in a window integral of width iiw."""
+    x, y = int(r["p0"]) % iiw, int(r["p0"]) // iiw
+    x3, y3 = int(r["p3"]) % iiw, int(r["p3"]) // iiw
+    w, h = x3 - x, y3 - y
+    assert w > 0 and h > 0
+    assert int(r["p1"]) == x + w + iiw * y and int(r["p2"]) == x + iiw * (y + h)
+    return x, y, w, h
+
+
+def _haar_kind(f):
+    """Haar type of an upright feature from its geometry: rect count, the
+    first rectangle's size and the second's offset in units of the second's
+    size, and the second's weight."""
+    x0, y0, w0, h0 = _rect_of(f["r"][0])
+    x1, y1, w1, h1 = _rect_of(f["r"][1])
+    return (int(f["rectCount"]), Fraction(w0, w1), Fraction(h0, h1), Fraction(x1 - x0, w1),
+            Fraction(y1 - y0, h1), float(f["r"][1]["w"]))
+
+
+def _exact_feature_sums(f, patches):
+    """sum_r w_r * S_r per patch in exact integers, with S_r the plain sum of
+    the rectangle's raw pixels (upright) or of the four corner points' pixel
+    blocks pix[:y, :x] with signs + - - + (tilted records, read from the
+    upright integral as the reference does)."""
+    a = patches.astype(np.int64)
+    tot = np.zeros(len(patches), np.int64)
+    for k in range(int(f["rectCount"])):
+        r = f["r"][k]
+        if f["tilted"]:
+            pts = [divmod(int(r[c]), 25) for c in ("p0", "p1", "p2", "p3")]
+            blk = [a[:, :py, :px].sum(axis=(1, 2)) for py, px in pts]
+            S = blk[0] - blk[1] - blk[2] + blk[3]
+        else:
+            x, y, w, h = _rect_of(r)
+            S = a[:, y:y + h, x:x + w].sum(axis=(1, 2))
+        wt = float(r["w"])
+        assert wt == int(wt)
+        tot += int(wt) * S
+    return tot
+
+
+def _check_train_responses(pool, patches):
+    """Every response is fl(exact integer * invNorm): the integer part is the
+    exact weighted rectangle sum (below 2^24, so converting it to float is
+    exact), and the product rounds once."""
+    sumT, _, inv = O.train_integrals(patches)
+    for f in pool:
+        tot = _exact_feature_sums(f, patches)
+        assert np.abs(tot).max() < 2 ** 24
+        want = tot.astype(np.float32) * inv
+        got = O.feature_responses(f, sumT, inv)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    one = O.lib().or_train_response  # the single-sample entry point agrees
+    f = np.ascontiguousarray(pool[-1:])
+    for s in (0, len(patches) // 2, len(patches) - 1):
+        v = one(f.ctypes.data, sumT.ctypes.data, inv.ctypes.data, sumT.shape[1], s)
+        assert np.float32(v) == (_exact_feature_sums(pool[-1], patches[s:s + 1]).astype(np.float32) * inv[s])[0]
+
+
+def _response_patches():
+    patches, _ = synth.training_set(40, 40, seed=31)
+    flat = np.full((1, 24, 24), 90, np.uint8)  # invNorm 0: responses are signed zeros
+    return np.concatenate([patches, flat])
+
+
+def test_train_responses_match_raw_pixel_sums():
+    core = O.haar_pool(24, 1)
+    pool = core[::53]
+    kinds = {_haar_kind(f) for f in pool}
+    F = Fraction
+    assert kinds == {(2, F(2), F(1), F(1), F(0), 2.0),     # x2
+                     (2, F(1), F(2), F(0), F(1), 2.0),     # y2
+                     (2, F(3), F(1), F(1), F(0), 2.0),     # x3
+                     (2, F(1), F(3), F(0), F(1), 2.0),     # y3
+                     (3, F(2), F(2), F(0), F(0), 2.0),     # x2y2
+                     (2, F(2), F(1), F(1, 2), F(0), 2.0),  # x4
+                     (2, F(1), F(2), F(0), F(1, 2), 2.0),  # y4
+                     (2, F(3), F(3), F(1), F(1), 9.0)}     # centre-surround
+    _check_train_responses(pool, _response_patches())
+
+
+def test_train_responses_tilted_records():
+    full = O.haar_pool(24, 2)
+    tilted = full[full["tilted"] == 1]
+    assert len(tilted) > 0
+    _check_train_responses(tilted[::41], _response_patches())
+
+
+def _brute_detect(feats, stumps, stages, img, sf=1.25, min_obj=24, exact=True):
+    """Raw detections recomputed in numpy: summed-area tables of each level
+    (numpy cumsum), the window normaliser on the inset 22x22 box in exact
+    integers and f64, every rectangle sum either from the raw pixels (exact;
+    valid while every integral value of the level is below 2^24, where the
+    reference's float corner arithmetic is exact) or from float32 corners
+    ((p0 - p1) - p2) + p3 of the absolute integral values, and stage sums
+    accumulated in float32 in stump order.  A stage threshold that is NaN is
+    set to the float32 median of the stage sums of the windows reaching that
+    stage (so about half of them pass); returns (detections, thresholds)."""
+    W, H = img.shape[1], img.shape[0]
+    out = []
+    thr_used = [float(g["threshold"]) for g in stages]
+    prev = img
+    for l in O.levels(W, H, sf, min_obj):
+        lvl = img if l["scaleIdx"] == 0 else O.resize(prev, int(l["w"]), int(l["h"]))
+        prev = lvl
+        a = lvl.astype(np.int64)
+        s = np.zeros((a.shape[0] + 1, a.shape[1] + 1), np.int64)
+        q = np.zeros_like(s)
+        s[1:, 1:] = a.cumsum(0).cumsum(1)
+        q[1:, 1:] = (a * a).cumsum(0).cumsum(1)
+        if exact:
+            assert s.max() < 2 ** 24
+        sf32 = s.astype(np.float32)
+        st = int(l["step"])
+        Y, X = np.meshgrid(np.arange(0, a.shape[0] - 23, st), np.arange(0, a.shape[1] - 23, st), indexing="ij")
+
+        def box(t, x, y, w, h):
+            return t[Y + y + h, X + x + w] - t[Y + y, X + x + w] - t[Y + y + h, X + x] + t[Y + y, X + x]
+
+        nsum, nsq = box(s, 1, 1, 22, 22), box(q, 1, 1, 22, 22)
+        nf2 = 484.0 * nsq.astype(np.float64) - nsum.astype(np.float64) ** 2
+        ok = nf2 > 0
+        inv = np.zeros(Y.shape, np.float32)
+        inv[ok] = (1.0 / np.sqrt(nf2[ok])).astype(np.float32)
+        alive = ok & (484.0 * inv.astype(np.float64) < 0.1)
+        score = np.zeros(Y.shape, np.float32)
+        for gi, g in enumerate(stages):
+            acc = np.zeros(Y.shape, np.float32)
+            for j in range(int(g["first"]), int(g["first"]) + int(g["ntrees"])):
+                sp = stumps[j]
+                f = feats[int(sp["feature"])]
+                raw = None
+                for k in range(int(f["rectCount"])):
+                    r = f["r"][k]
+                    if exact:
+                        v = box(s, *_rect_of(r)).astype(np.float32)
+                    else:
+                        (dy0, dx0), (dy1, dx1), (dy2, dx2), (dy3, dx3) = [divmod(int(r[c]), 25)
+                                                                          for c in ("p0", "p1", "p2", "p3")]
+                        v = ((sf32[Y + dy0, X + dx0] - sf32[Y + dy1, X + dx1]) - sf32[Y + dy2, X + dx2]) + \
+                            sf32[Y + dy3, X + dx3]
+                    v = v * np.float32(r["w"])
+                    raw = v if raw is None else raw + v
+                resp = raw * inv
+                acc = acc + np.where(resp < np.float32(sp["theta"]), np.float32(sp["left"]),
+                                     np.float32(sp["right"]))
+            if math.isnan(thr_used[gi]) and alive.any():
+                thr_used[gi] = float(np.float32(np.median(acc[alive])))
+            alive &= ~(acc < np.float32(thr_used[gi]))
+            score = np.where(alive, acc, score)
+        scale = np.float32(l["scale"])
+        for y, x in zip(*np.nonzero(alive)):
+            out.append((int(np.float32(X[y, x]) * scale + np.float32(0.5)),
+                        int(np.float32(Y[y, x]) * scale + np.float32(0.5)),
+                        int(np.float32(24) * scale + np.float32(0.5)), int(np.float32(24) * scale + np.float32(0.5)),
+                        int(l["scaleIdx"]), int(score[y, x].view(np.uint32))))
+    return out, thr_used
+
+
+def _mixed_leaf_cascade(sizes, seed):
+    """The blob cascade's features, stumps and thetas regrouped into stages of
+    the given sizes, with leaves of log-uniform magnitude (1e-3..1e3) and
+    random sign, so the float32 stage sums depend on the stump order."""
+    feats, stumps, _ = workloads.cascade((1, 10, 25))
+    rng = np.random.default_rng(seed)
+    st = np.array(stumps[:sum(sizes)])
+    mag = 10.0 ** rng.uniform(-3, 3, size=(len(st), 2))
+    sgn = rng.choice([-1.0, 1.0], size=(len(st), 2))
+    st["left"], st["right"] = (mag * sgn)[:, 0].astype(np.float32), (mag * sgn)[:, 1].astype(np.float32)
+    stages = np.zeros(len(sizes), STAGE)
+    stages["first"] = np.cumsum((0,) + tuple(sizes[:-1]))
+    stages["ntrees"] = sizes
+    stages["threshold"] = np.nan
+    return feats, st, stages
+
+
+def _det_rows(d):
+    return [(int(r["x"]), int(r["y"]), int(r["w"]), int(r["h"]), int(r["scaleIdx"]),
+             int(np.float32(r["score"]).view(np.uint32))) for r in d]
+
+
+def _check_detect_brute(sizes, seed, img, exact):
+    feats, st, stages = _mixed_leaf_cascade(sizes, seed)
+    want, thr = _brute_detect(feats, st, stages, img, exact=exact)
+    stages["threshold"] = np.array(thr, np.float32)
+    got, _ = O.detect_raw(feats, st, stages, img, max_det=10 ** 6)
+    assert _det_rows(got) == want
+    return want
+
+
+def _small_detect_image():
+    return synth.detect_image(240, 200, 8, 24, 60, seed=5)
+
+
+@pytest.mark.parametrize("sizes,seed", [((1,), 1), ((4, 6), 2), ((3, 8, 12), 3)])
+def test_detect_stage_loop_matches_raw_pixel_brute_force(sizes, seed):
+    """Finite thetas and stage thresholds (the median stage sum, so about half
+    the windows pass each stage): pass sets, boxes and score bits equal a
+    brute force that takes every rectangle sum from the raw level pixels."""
+    want = _check_detect_brute(sizes, seed, _small_detect_image(), exact=True)
+    assert len(want) > 20 and len({r[4] for r in want}) > 3
+
+
+def test_detect_float_corners_above_2_24():
+    """At 640x480 the integrals pass 2^24, where the reference rounds each
+    absolute corner value to float before the rectangle arithmetic; the brute
+    force does the same, and the pass sets and score bits agree.  The rounding
+    is live: some rectangle sums differ from the exact pixel sums."""
+    img, _ = workloads.detect_640()
+    want = _check_detect_brute((2, 5), 4, img, exact=False)
+    assert len(want) > 20
+    s = np.zeros((481, 641), np.int64)
+    s[1:, 1:] = img.astype(np.int64).cumsum(0).cumsum(1)
+    f = s.astype(np.float32)
+    exact = s[24:, 24:] - s[24:, :-24] - s[:-24, 24:] + s[:-24, :-24]
+    rounded = ((f[:-24, :-24] - f[:-24, 24:]) - f[24:, :-24]) + f[24:, 24:]
+    assert (rounded.astype(np.int64) != exact).any()
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+def test_mutants_fail_the_brute_force_checks(k):
+    """Each planted-mistake build of the oracle (OR_MUTANT k: a corner sign, the
+    stump order of a stage, a rectangle weight, a corner index, the window
+    normaliser) fails the brute-force checks above."""
+    with O.use_library(O.mutant_path(k)):
+        with pytest.raises(AssertionError):
+            if k in (3, 4):
+                _check_train_responses(O.haar_pool(24, 1)[::53], _response_patches())
+            else:
+                _check_detect_brute((3, 8, 12), 3, _small_detect_image(), exact=True)
+
+
 # ---- grouping ----------------------------------------------------------------
 
 def _dets(rows):
