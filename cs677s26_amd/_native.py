@@ -102,6 +102,7 @@ _PROTOS = {
     "vj_engine_feature_pool": (I, [P, PP, C.POINTER(I)]),
     "vj_engine_integral_batch": (I, [P, P, I, I, I, I, I, P]),
     "vj_engine_integral_view": (I, [P, C.POINTER(IntegralView)]),
+    "vj_integral_image_noalloc": (I, [P, I, I, I, P, P, P]),
     "vj_engine_set_cascade": (I, [P, P, I, P, I, P, I, P]),
     "vj_engine_clear_cascade": (I, [P, P]),
     "vj_engine_detect": (I, [P, P, I, I, I, F, I, I, I, I, P, I, C.POINTER(I), P]),

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -1950,6 +1951,109 @@ extern "C" int vj_engine_integral_view(vj_engine* e, vj_integral_view* out) {
     out->sumT = e->bSum.get();
     out->sqsumT = e->bSq.get();
     out->invNorm = e->bInv.get();
+    return VJ_OK;
+}
+
+// ---- one image's integral without allocating ------------------------------
+// Scratch of vj_integral_image_noalloc: the level descriptor and item table
+// of the integral kernels, then the column band sums, row-chunk totals and
+// band totals of one image.  Each array is sized for the largest image the
+// integral kernels accept (sides up to 8192, w * h * 255 < 2^31): at most
+// 270,878 column sums, 40,960 row-chunk totals and 1,280 band totals.  Every
+// element the kernels read was written earlier in the same call, so nothing
+// needs clearing.  One copy per GPU, held until the process ends.
+constexpr int64_t kNaCols = 270878, kNaRows = 40960, kNaTots = 1280;
+constexpr size_t na_align(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kNaDesc = 0;
+constexpr size_t kNaOffs = na_align(sizeof(LevelDesc));
+constexpr size_t kNaColS = kNaOffs + 256;
+constexpr size_t kNaColQ = kNaColS + na_align(4 * kNaCols);
+constexpr size_t kNaRowS = kNaColQ + na_align(4 * kNaCols);
+constexpr size_t kNaRowQ = kNaRowS + na_align(4 * kNaRows);
+constexpr size_t kNaTotS = kNaRowQ + na_align(4 * kNaRows);
+constexpr size_t kNaTotQ = kNaTotS + na_align(4 * kNaTots);
+constexpr size_t kNaBytes = kNaTotQ + na_align(8 * kNaTots);
+static_assert(kNaBytes < (size_t{3} << 20), "integral scratch stays under 3 MiB");
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+// Writes the level descriptor and the one-level item table into the scratch
+// (the kernels read them from device memory; no host copy is recorded).
+__global__ __launch_bounds__(64) void na_setup_kernel(LevelDesc d, unsigned char* base) {
+    if (threadIdx.x == 0) {
+        *reinterpret_cast<LevelDesc*>(base + kNaDesc) = d;
+        int32_t* offs = reinterpret_cast<int32_t*>(base + kNaOffs);
+        offs[0] = 0;
+        offs[1] = 0;
+    }
+}
+
+namespace {
+// The scratch's device address on each GPU, looked up once per device with
+// that device current (host code never passes the array's own name).
+constexpr int kNaMaxDevices = 64;
+std::mutex gNaMutex;
+std::atomic<unsigned char*> gNaPtr[kNaMaxDevices];
+
+int na_scratch(unsigned char** out) {
+    int dev = 0;
+    VJ_HIP(hipGetDevice(&dev));
+    VJ_CHECK(dev >= 0 && dev < kNaMaxDevices, "integral_image_noalloc: device index above 63");
+    unsigned char* p = gNaPtr[dev].load(std::memory_order_acquire);
+    if (!p) {
+        std::lock_guard<std::mutex> lock(gNaMutex);
+        p = gNaPtr[dev].load(std::memory_order_relaxed);
+        if (!p) {
+            void* a = nullptr;
+            VJ_HIP(hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)));
+            VJ_CHECK(a != nullptr, "integral_image_noalloc: no device address for the scratch");
+            p = static_cast<unsigned char*>(a);
+            gNaPtr[dev].store(p, std::memory_order_release);
+        }
+    }
+    *out = p;
+    return VJ_OK;
+}
+}  // namespace
+
+extern "C" int vj_integral_image_noalloc(const uint8_t* dGray, int w, int h, int stride, uint32_t* dSum,
+                                         uint64_t* dSq, vj_stream sv) {
+    VJ_CHECK(dGray && dSum && dSq && w > 0 && h > 0 && stride >= w, "integral_image_noalloc: invalid arguments");
+    VJ_CHECK(w <= 8192 && h <= 8192, "integral_image_noalloc: sides above 8192 exceed the scratch");
+    VJ_CHECK(static_cast<int64_t>(w) * h * 255 < (int64_t{1} << 31),
+             "integral_image_noalloc: image too large for exact 32-bit integrals (w*h*255 must be < 2^31)");
+    LevelDesc d{};
+    d.pix = dGray;
+    d.w = w;
+    d.h = h;
+    d.stride = stride;
+    d.step = 1;
+    d.nBands = (h + kBandRows - 1) / kBandRows;
+    d.nChunks = (w + kChunkCols - 1) / kChunkCols;
+    VJ_CHECK(static_cast<int64_t>(d.nBands) * w <= kNaCols && static_cast<int64_t>(h) * d.nChunks <= kNaRows &&
+                 static_cast<int64_t>(d.nBands) * d.nChunks <= kNaTots,
+             "integral_image_noalloc: image exceeds the scratch");
+    unsigned char* base = nullptr;
+    VJ_TRY(na_scratch(&base));
+    hipStream_t s = static_cast<hipStream_t>(sv);
+    const LevelDesc* L = reinterpret_cast<const LevelDesc*>(base + kNaDesc);
+    const int32_t* offs = reinterpret_cast<const int32_t*>(base + kNaOffs);
+    uint32_t* colS = reinterpret_cast<uint32_t*>(base + kNaColS);
+    uint32_t* colQ = reinterpret_cast<uint32_t*>(base + kNaColQ);
+    uint32_t* rowS = reinterpret_cast<uint32_t*>(base + kNaRowS);
+    uint32_t* rowQ = reinterpret_cast<uint32_t*>(base + kNaRowQ);
+    uint32_t* totS = reinterpret_cast<uint32_t*>(base + kNaTotS);
+    uint64_t* totQ = reinterpret_cast<uint64_t*>(base + kNaTotQ);
+    const int items = d.nBands * d.nChunks;
+    hipLaunchKernelGGL(na_setup_kernel, dim3(1), dim3(64), 0, s, d, base);
+    VJ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(ii_colsum_kernel, dim3(items), dim3(64), 0, s, L, items, offs, 1, colS, colQ, rowS, rowQ);
+    VJ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(ii_vscan_kernel, dim3(d.nChunks), dim3(256), 0, s, L, offs + 1, 1, colS, colQ, totS, totQ);
+    VJ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(ii_write_kernel, dim3(items), dim3(64), 0, s, L, items, offs, 1, colS, colQ, totS, totQ, rowS,
+                       rowQ, dSum, dSq);
+    VJ_HIP(hipGetLastError());
     return VJ_OK;
 }
 

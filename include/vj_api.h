@@ -149,6 +149,21 @@ int vj_engine_integral_batch(vj_engine* e, const uint8_t* dGray, int w, int h, i
                              int nSamples, int computeInvNorm, vj_stream s);
 /* integralSetView, src/face_vision_engine.cu:1150-1162 */
 int vj_engine_integral_view(vj_engine* e, vj_integral_view* out);
+/* One gray image's integral without allocating.  No counterpart in the
+ * reference: it is the single-image form of vj_engine_integral_batch
+ * (computeIntegralBatchTransposed with n = 1) for callers that cannot
+ * allocate device memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation.  It calls no hipMalloc, hipFree
+ * or other allocating routine: the caller passes the image and the outputs,
+ * dSum (u32) and dSq (u64), (h+1) x (w+1) each, row-major with row pitch
+ * w+1, row 0 and column 0 zero.  The intermediate sums live in a fixed
+ * device array of the library (under 3 MiB per GPU, held until the process
+ * ends), sized for sides up to 8192 with w * h * 255 < 2^31; larger images
+ * are refused with VJ_INVALID_ARG.  Every call on a GPU shares that array,
+ * so calls that overlap on one GPU (other streams or host threads) race.
+ * Asynchronous on s. */
+int vj_integral_image_noalloc(const uint8_t* dGray, int w, int h, int stride, uint32_t* dSum, uint64_t* dSq,
+                              vj_stream s);
 /* setCascadeModel, src/face_vision_engine.cu:1176-1322 (copies its inputs). */
 int vj_engine_set_cascade(vj_engine* e, const vj_feature* dFeatures, int nFeatures,
                           const vj_stump* dStumps, int nStumps,
