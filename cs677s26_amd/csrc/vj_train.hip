@@ -36,6 +36,7 @@ constexpr int kSortMaxPerThread = 20;              // N <= 20480
 constexpr int kSortMaxN = kSortThreads * kSortMaxPerThread;
 constexpr int kSearchThreads = 1024;               // 16 waves, one feature each at a time
 constexpr size_t kSearchWideLds = 160 * 1024 - 2048; // dynamic LDS limit of the search kernels
+constexpr int kPackedMaxN = 32767;                 // bit 15 of an order entry is its split flag
 
 struct RoundBest {
     int64_t errQ;
@@ -119,6 +120,23 @@ __device__ __forceinline__ float response(const vj_feature& f, const uint32_t* _
 // semantics).
 // ---------------------------------------------------------------------------
 constexpr int kSortWaves = kSortThreads / kWave;
+
+// The lanes of a full wavefront whose 8-bit digit equals this lane's: a lane
+// differs from this one where some bit differs, so OR over the bits of
+// (lanes with the bit set) XOR (all ones if this lane has it), then invert.
+__device__ __forceinline__ uint64_t same_digit_lanes(uint32_t dg) {
+    uint32_t xlo = 0u, xhi = 0u;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const uint32_t ext = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(dg), b, 1));
+        // the ballot of ext != 0, compared on ext itself
+        uint64_t bb;
+        asm("v_cmp_ne_u32_e64 %0, 0, %1" : "=s"(bb) : "v"(ext));
+        xlo = __builtin_amdgcn_bitop3_b32(ext, xlo, static_cast<uint32_t>(bb), 0xDE);   // xlo | (bb ^ ext)
+        xhi = __builtin_amdgcn_bitop3_b32(ext, xhi, static_cast<uint32_t>(bb >> 32), 0xDE);
+    }
+    return ~((static_cast<uint64_t>(xhi) << 32) | xlo);
+}
 // Digit counters: one row of 256 per wave, rows kCntPitch words apart, so a
 // wave's lanes reading the counters of distinct digits hit distinct banks.
 constexpr int kCntPitch = 257;
@@ -209,20 +227,7 @@ __global__ __launch_bounds__(kSortThreads) void presort_kernel(const vj_feature*
         for (int i = 0; i < kSortMaxPerThread; ++i) {
             if (i < nb) {
                 const uint32_t dg = ((kr[i] - kmin) >> shift) & 0xFFu;
-                // lanes whose digit differs from this lane's in some bit: OR
-                // over the bits of (lanes with the bit set) XOR (all ones if
-                // this lane has it); the other lanes hold the same digit
-                uint32_t xlo = 0u, xhi = 0u;
-#pragma unroll
-                for (int b = 0; b < 8; ++b) {
-                    const uint32_t ext = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(dg), b, 1));
-                    // the ballot of ext != 0, compared on ext itself
-                    uint64_t bb;
-                    asm("v_cmp_ne_u32_e64 %0, 0, %1" : "=s"(bb) : "v"(ext));
-                    xlo = __builtin_amdgcn_bitop3_b32(ext, xlo, static_cast<uint32_t>(bb), 0xDE);   // xlo | (bb ^ ext)
-                    xhi = __builtin_amdgcn_bitop3_b32(ext, xhi, static_cast<uint32_t>(bb >> 32), 0xDE);
-                }
-                const uint64_t m = ~((static_cast<uint64_t>(xhi) << 32) | xlo);
+                const uint64_t m = same_digit_lanes(dg);
                 const uint32_t below = static_cast<uint32_t>(mask_rank(m));
                 const uint32_t c = wcnt[dg];
                 rk[i] = c + below;
@@ -273,6 +278,194 @@ __global__ __launch_bounds__(kSortThreads) void presort_kernel(const vj_feature*
             if (v0 + 1e-7f < v1) e |= 0x8000u;
         }
         o[k] = static_cast<uint16_t>(e);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Presort above the LDS limit (kSortMaxN < n <= 65,535): the same stable LSD
+// radix sort of rebased keys, 8 bits per pass, with the keys and indices in
+// two global ping-pong buffers of the workgroup's slot, streamed through in
+// tiles of kWideTile positions.  Each pass counts the digit histogram (the
+// counts do not depend on the order), scans it into each digit's first
+// position, then tile by tile every wave ranks its 256 consecutive positions
+// with the same ballot ranks as the LDS sort, a scan over the waves of each
+// digit places the tile's elements after the earlier tiles' elements of that
+// digit, and the elements are written to the other buffer.  Persistent
+// workgroups take features blockIdx.x, blockIdx.x + gridDim.x, ...  Output:
+// mode 0 writes, per position k < outPitch, the sample index (0 from n on,
+// so that every entry is a valid sample) and bit k of the feature's split
+// mask row, set when fl(v_k + 1e-7f) < v_{k+1}; mode 1 writes plain indices
+// for k < n (sortSamplesPerFeature).
+// ---------------------------------------------------------------------------
+constexpr int kWidePer = 4;                           // 64-position batches per wave per tile
+constexpr int kWideTile = kSortThreads * kWidePer;    // 4096 positions
+constexpr int kTrainMaxN = 65535;                     // u16 sample indices (src/boosting_core.cu:704-706)
+
+__global__ __launch_bounds__(kSortThreads) void presort_wide_kernel(
+        const vj_feature* __restrict__ feats, int fBegin, const uint32_t* __restrict__ sumT,
+        const float* __restrict__ inv, const float* __restrict__ resp, int n, int nFeatures,
+        uint16_t* __restrict__ out, int64_t outPitch, uint64_t* __restrict__ mask, int64_t maskPitch, int mode,
+        uint32_t* __restrict__ scrKeys, uint16_t* __restrict__ scrIdx, int64_t scrPitch) {
+    __shared__ uint32_t cnt[kSortWaves * kCntPitch];
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t grp[4][256];
+    __shared__ uint32_t redMin[kSortWaves], redMax[kSortWaves], wtot[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nPad = (n + kWideTile - 1) / kWideTile * kWideTile;
+    uint32_t* const kA = scrKeys + static_cast<int64_t>(blockIdx.x) * 2 * scrPitch;
+    uint32_t* const kB = kA + scrPitch;
+    uint16_t* const iA = scrIdx + static_cast<int64_t>(blockIdx.x) * 2 * scrPitch;
+    uint16_t* const iB = iA + scrPitch;
+    uint32_t* const wcnt = cnt + wave * kCntPitch;
+    for (int f = blockIdx.x; f < nFeatures; f += gridDim.x) {
+        uint32_t kmin = 0xFFFFFFFFu, kmax = 0u;
+        if (mode == 0) {
+            const vj_feature ft = feats[fBegin + f];
+            for (int s = tid; s < n; s += kSortThreads) {
+                const uint32_t k = float_key(response(ft, sumT, inv, n, s));
+                kA[s] = k;
+                iA[s] = static_cast<uint16_t>(s);
+                kmin = min(kmin, k);
+                kmax = max(kmax, k);
+            }
+        } else {
+            const float* row = resp + static_cast<size_t>(f) * n;
+            for (int s = tid; s < n; s += kSortThreads) {
+                const uint32_t k = float_key(row[s]);
+                kA[s] = k;
+                iA[s] = static_cast<uint16_t>(s);
+                kmin = min(kmin, k);
+                kmax = max(kmax, k);
+            }
+        }
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) {
+            kmin = min(kmin, static_cast<uint32_t>(__shfl_xor(static_cast<int>(kmin), d, kWave)));
+            kmax = max(kmax, static_cast<uint32_t>(__shfl_xor(static_cast<int>(kmax), d, kWave)));
+        }
+        if (lane == 0) {
+            redMin[wave] = kmin;
+            redMax[wave] = kmax;
+        }
+        __syncthreads();
+        kmin = redMin[0];
+        kmax = redMax[0];
+        for (int w = 1; w < kSortWaves; ++w) {
+            kmin = min(kmin, redMin[w]);
+            kmax = max(kmax, redMax[w]);
+        }
+        // pads take the largest key: stability keeps them after every sample
+        for (int s = n + tid; s < nPad; s += kSortThreads) {
+            kA[s] = kmax;
+            iA[s] = 0xFFFFu;
+        }
+        __syncthreads();
+        const uint32_t span = kmax - kmin;
+        const int bits = span ? 32 - __clz(static_cast<int>(span)) : 0;
+        uint32_t* sk = kA;
+        uint32_t* dk = kB;
+        uint16_t* si = iA;
+        uint16_t* di = iB;
+        for (int shift = 0; shift < bits; shift += 8) {
+            // digit histogram, each wave into its own counter row
+            for (int d = lane; d < 256; d += kWave) wcnt[d] = 0u;
+            for (int p = wave * kWave; p < nPad; p += kSortThreads) {
+                const uint32_t dg = ((sk[p + lane] - kmin) >> shift) & 0xFFu;
+                const uint64_t m = same_digit_lanes(dg);
+                const uint32_t c = wcnt[dg];
+                if (mask_rank(m) == 0) wcnt[dg] = c + static_cast<uint32_t>(__popcll(m));
+            }
+            __syncthreads();
+            // each digit's first position: an exclusive scan of the totals
+            uint32_t t = 0, inc = 0;
+            if (tid < 256) {
+                for (int w = 0; w < kSortWaves; ++w) t += cnt[w * kCntPitch + tid];
+                inc = wave_incl_scan_dpp(t);
+                if (lane == 63) wtot[wave] = inc;
+            }
+            __syncthreads();
+            if (tid < 256) {
+                uint32_t off = inc - t;
+                for (int w = 0; w < wave; ++w) off += wtot[w];
+                run[tid] = off;
+            }
+            __syncthreads();
+            for (int t0 = 0; t0 < nPad; t0 += kWideTile) {
+                for (int d = lane; d < 256; d += kWave) wcnt[d] = 0u;
+                const int base = t0 + wave * (kWidePer * kWave);
+                uint32_t kr[kWidePer], rk[kWidePer], dgs[kWidePer];
+                uint16_t ir[kWidePer];
+#pragma unroll
+                for (int i = 0; i < kWidePer; ++i) {
+                    kr[i] = sk[base + i * kWave + lane];
+                    ir[i] = si[base + i * kWave + lane];
+                }
+#pragma unroll
+                for (int i = 0; i < kWidePer; ++i) {
+                    dgs[i] = ((kr[i] - kmin) >> shift) & 0xFFu;
+                    const uint64_t m = same_digit_lanes(dgs[i]);
+                    const uint32_t below = static_cast<uint32_t>(mask_rank(m));
+                    const uint32_t c = wcnt[dgs[i]];
+                    rk[i] = c + below;
+                    if (below == 0) wcnt[dgs[i]] = c + static_cast<uint32_t>(__popcll(m));
+                }
+                __syncthreads();
+                // per digit, four threads of four waves each: the tile's
+                // elements of the earlier waves, after the earlier tiles'
+                {
+                    const int d = tid & 255, g = tid >> 8;
+                    uint32_t v[4], gt = 0;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        v[k] = cnt[(4 * g + k) * kCntPitch + d];
+                        gt += v[k];
+                    }
+                    grp[g][d] = gt;
+                    __syncthreads();
+                    uint32_t off = run[d];
+                    for (int gg = 0; gg < g; ++gg) off += grp[gg][d];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        cnt[(4 * g + k) * kCntPitch + d] = off;
+                        off += v[k];
+                    }
+                    __syncthreads();
+                    if (g == 3) run[d] = off;
+                }
+#pragma unroll
+                for (int i = 0; i < kWidePer; ++i) {
+                    const uint32_t pos = wcnt[dgs[i]] + rk[i];
+                    dk[pos] = kr[i];
+                    di[pos] = ir[i];
+                }
+                __syncthreads();  // counters, run and the written tile are complete
+            }
+            uint32_t* tk = sk;
+            sk = dk;
+            dk = tk;
+            uint16_t* ti = si;
+            si = di;
+            di = ti;
+        }
+        uint16_t* o = out + static_cast<int64_t>(f) * outPitch;
+        if (mode == 0) {
+            uint64_t* mrow = mask + static_cast<int64_t>(f) * maskPitch;
+            for (int k0 = wave * kWave; k0 < outPitch; k0 += kSortThreads) {
+                const int k = k0 + lane;
+                bool valid = false;
+                uint16_t e = 0;
+                if (k < n) {
+                    e = si[k];
+                    if (k + 1 < n) valid = key_float(sk[k]) + 1e-7f < key_float(sk[k + 1]);
+                }
+                o[k] = e;
+                const uint64_t word = __ballot(valid);
+                if (lane == 0) mrow[k0 >> 6] = word;
+            }
+        } else {
+            for (int k = tid; k < n; k += kSortThreads) o[k] = si[k];
+        }
+        __syncthreads();  // the slot's buffers and the LDS serve the next feature
     }
 }
 
@@ -366,24 +559,28 @@ __device__ __forceinline__ int64_t select_lanes(int64_t a, int64_t b, uint64_t m
     return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
 }
 
-template <bool kSplit, typename W>
-__device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, int n, const W* lsw,
-                                             int64_t totPos, int64_t totNeg, int lane,
+// kWide: entries are plain 16-bit sample indices and the split flags are the
+// bits of the feature's mask row (byte k/8, bit k%8 for position k), read one
+// byte per lane per step; otherwise bit 15 of an entry is its flag.
+template <bool kSplit, bool kWide, typename W>
+__device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, const uint8_t* __restrict__ mrow,
+                                             int n, const W* lsw, int64_t totPos, int64_t totNeg, int lane,
                                              int64_t& fE, int32_t& fK, int32_t& fP) {
+    constexpr uint32_t kIdxMask = kWide ? 0xFFFFu : 0x7FFFu;
     int64_t carry = 0;
     int64_t hiD = INT64_MIN, loD = INT64_MAX;
     int32_t hiK = INT_MAX, loK = INT_MAX;
     // one step of 64 * kScanPer sorted entries; kTail adds the bounds checks
     // that only the last, partial step needs (without them the eight LDS
     // reads of a lane issue together)
-    auto step = [&](int base, const uint4 e8, auto tailTag) {
+    auto step = [&](int base, const uint4 e8, const uint32_t mb, auto tailTag) {
         constexpr bool kTail = decltype(tailTag)::value;
         const int k0 = base + lane * kScanPer;
         const uint32_t ew[4] = {e8.x, e8.y, e8.z, e8.w};
         // all eight weight words are read before any is used
         W pw[kScanPer];
 #pragma unroll
-        for (int j = 0; j < kScanPer; ++j) pw[j] = lsw[(ew[j >> 1] >> (16 * (j & 1))) & 0x7FFFu];
+        for (int j = 0; j < kScanPer; ++j) pw[j] = lsw[(ew[j >> 1] >> (16 * (j & 1))) & kIdxMask];
         int64_t q[kScanPer];
         int64_t c = 0;
 #pragma unroll
@@ -399,7 +596,8 @@ __device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, i
             const uint32_t e = (ew[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
             // branch-free: the two selects chains are independent
             if constexpr (kSplit) {
-                const bool valid = (e & 0x8000u) && (!kTail || k0 + j < n);
+                const bool flag = kWide ? ((mb >> j) & 1u) != 0u : (e & 0x8000u) != 0u;
+                const bool valid = flag && (!kTail || k0 + j < n);
                 const bool up = valid && D > hiD;
                 const bool down = valid && D < loD;
                 hiD = up ? D : hiD;
@@ -411,9 +609,14 @@ __device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, i
                 // the compares (written as plain selects, the compiler turns
                 // them into a 64-bit max followed by a select on valid: four
                 // per chain)
-                const int32_t ev = (j & 1) ? static_cast<int32_t>(ew[j >> 1])
-                                           : static_cast<int32_t>(static_cast<int16_t>(ew[j >> 1] & 0xFFFFu));
-                uint64_t vm = __builtin_amdgcn_sicmp(ev, 0, 40 /* slt: bit 15 of the entry */);
+                uint64_t vm;
+                if constexpr (kWide) {
+                    vm = __builtin_amdgcn_sicmp(static_cast<int32_t>((mb >> j) & 1u), 0, 33 /* ne */);
+                } else {
+                    const int32_t ev = (j & 1) ? static_cast<int32_t>(ew[j >> 1])
+                                               : static_cast<int32_t>(static_cast<int16_t>(ew[j >> 1] & 0xFFFFu));
+                    vm = __builtin_amdgcn_sicmp(ev, 0, 40 /* slt: bit 15 of the entry */);
+                }
                 if constexpr (kTail) vm &= __builtin_amdgcn_sicmp(k0 + j, n, 40 /* slt */);
                 hiD = select_lanes(hiD, D, vm & __builtin_amdgcn_sicmpl(D, hiD, 38 /* sgt */));
                 loD = select_lanes(loD, D, vm & __builtin_amdgcn_sicmpl(D, loD, 40 /* slt */));
@@ -433,27 +636,39 @@ __device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, i
     constexpr int kStep = kWave * kScanPer;
     constexpr int kAhead = 4;
     auto load8 = [&](int base) { return *reinterpret_cast<const uint4*>(row + base + lane * kScanPer); };
+    auto loadm = [&](int base) -> uint32_t {
+        if constexpr (kWide) return mrow[(base >> 3) + lane];
+        else return 0u;
+    };
     const int nFull = n / kStep;
     uint4 buf[kAhead];
+    uint32_t mbuf[kAhead];
 #pragma unroll
-    for (int u = 0; u < kAhead; ++u) buf[u] = (u < nFull) ? load8(u * kStep) : make_uint4(0u, 0u, 0u, 0u);
+    for (int u = 0; u < kAhead; ++u) {
+        buf[u] = (u < nFull) ? load8(u * kStep) : make_uint4(0u, 0u, 0u, 0u);
+        mbuf[u] = (u < nFull) ? loadm(u * kStep) : 0u;
+    }
     int s0 = 0;
     for (; s0 + kAhead <= nFull; s0 += kAhead) {
 #pragma unroll
         for (int u = 0; u < kAhead; ++u) {
             const uint4 cur = buf[u];
+            const uint32_t curm = mbuf[u];
             const int nx = s0 + u + kAhead;
             buf[u] = (nx < nFull) ? load8(nx * kStep) : make_uint4(0u, 0u, 0u, 0u);
-            step((s0 + u) * kStep, cur, std::false_type{});
+            mbuf[u] = (nx < nFull) ? loadm(nx * kStep) : 0u;
+            step((s0 + u) * kStep, cur, curm, std::false_type{});
         }
     }
 #pragma unroll
     for (int u = 0; u < kAhead; ++u)
-        if (s0 + u < nFull) step((s0 + u) * kStep, buf[u], std::false_type{});
+        if (s0 + u < nFull) step((s0 + u) * kStep, buf[u], mbuf[u], std::false_type{});
     const int base = nFull * kStep;
     if (base < n) {
-        const uint4 e8 = (base + lane * kScanPer < n) ? load8(base) : make_uint4(0u, 0u, 0u, 0u);
-        step(base, e8, std::true_type{});
+        const bool in = base + lane * kScanPer < n;
+        const uint4 e8 = in ? load8(base) : make_uint4(0u, 0u, 0u, 0u);
+        const uint32_t m8 = in ? loadm(base) : 0u;
+        step(base, e8, m8, std::true_type{});
     }
     if constexpr (!kSplit) {
 #pragma unroll
@@ -552,7 +767,7 @@ __global__ __launch_bounds__(kSearchThreads) __attribute__((amdgpu_waves_per_eu(
         if (used && used[f]) continue;
         int64_t fE;
         int32_t fK, fP;
-        scan_feature<false>(ord + static_cast<int64_t>(f) * pitch, n, lsw, totPos, totNeg, lane, fE, fK, fP);
+        scan_feature<false, false>(ord + static_cast<int64_t>(f) * pitch, nullptr, n, lsw, totPos, totNeg, lane, fE, fK, fP);
         if (fK >= 0 && (fE < bE || (fE == bE && f < bF))) {
             bE = fE;
             bF = f;
@@ -561,7 +776,87 @@ __global__ __launch_bounds__(kSearchThreads) __attribute__((amdgpu_waves_per_eu(
     // the wave's winner again, now with its split and parity (same error)
     if (bF != INT_MAX) {
         int64_t fE;
-        scan_feature<true>(ord + static_cast<int64_t>(bF) * pitch, n, lsw, totPos, totNeg, lane, fE, bK, bP);
+        scan_feature<true, false>(ord + static_cast<int64_t>(bF) * pitch, nullptr, n, lsw, totPos, totNeg, lane, fE, bK, bP);
+    }
+    if (lane == 0) wb[wave] = {bE, bK, bP, bF, 0};
+    __syncthreads();
+    if (tid == 0) {
+        WaveBest r = wb[0];
+        for (int k = 1; k < 16; ++k) {
+            const WaveBest& o = wb[k];
+            if (o.split >= 0 && (r.split < 0 || o.errQ < r.errQ || (o.errQ == r.errQ && o.feature < r.feature))) r = o;
+        }
+        blockBest[blockIdx.x] = r;
+    }
+}
+
+// Search weights of the wide layout: each sample's exact fixed-point weight
+// (the value of its packed word) in global memory, where the scan gathers it
+// (the array stays in L2), and the exact positive and negative totals.  One
+// workgroup.
+__global__ __launch_bounds__(1024) void wide_weights_kernel(const float* __restrict__ sw32, int n,
+                                                            int64_t* __restrict__ q, int64_t* __restrict__ tot) {
+    __shared__ int64_t tp[16], tn[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int64_t a = 0, b = 0;
+    for (int s = threadIdx.x; s < n; s += blockDim.x) {
+        const int64_t v = unpack_weight(pack_weight(sw32[s]));
+        q[s] = v;
+        if (v > 0) a += v;
+        else b -= v;
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if (lane == 0) {
+        tp[wave] = a;
+        tn[wave] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t A = 0, B = 0;
+        for (int k = 0; k < static_cast<int>(blockDim.x >> 6); ++k) {
+            A += tp[k];
+            B += tn[k];
+        }
+        tot[0] = A;
+        tot[1] = B;
+    }
+}
+
+// search_kernel over the wide layout (plain indices, split mask rows, weights
+// gathered from global memory).
+__global__ __launch_bounds__(kSearchThreads) void search_wide_kernel(const uint16_t* __restrict__ ord,
+                                                                    const uint64_t* __restrict__ mask, int64_t pitch,
+                                                                    int n, int nFeatures,
+                                                                    const int64_t* __restrict__ q,
+                                                                    const int64_t* __restrict__ tot,
+                                                                    const uint8_t* __restrict__ used,
+                                                                    WaveBest* __restrict__ blockBest) {
+    __shared__ WaveBest wb[16];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int64_t totPos = tot[0], totNeg = tot[1];
+    const int64_t mPitch = pitch / 64;
+    int64_t bE = INT64_MAX;
+    int32_t bK = -1, bP = +1, bF = INT_MAX;
+    const int nWavesTotal = gridDim.x * (blockDim.x >> 6);
+    for (int f = blockIdx.x * (blockDim.x >> 6) + wave; f < nFeatures; f += nWavesTotal) {
+        if (used && used[f]) continue;
+        int64_t fE;
+        int32_t fK, fP;
+        scan_feature<false, true>(ord + static_cast<int64_t>(f) * pitch,
+                                  reinterpret_cast<const uint8_t*>(mask + static_cast<int64_t>(f) * mPitch), n, q,
+                                  totPos, totNeg, lane, fE, fK, fP);
+        if (fK >= 0 && (fE < bE || (fE == bE && f < bF))) {
+            bE = fE;
+            bF = f;
+        }
+    }
+    if (bF != INT_MAX) {
+        int64_t fE;
+        scan_feature<true, true>(ord + static_cast<int64_t>(bF) * pitch,
+                                 reinterpret_cast<const uint8_t*>(mask + static_cast<int64_t>(bF) * mPitch), n, q,
+                                 totPos, totNeg, lane, fE, bK, bP);
     }
     if (lane == 0) wb[wave] = {bE, bK, bP, bF, 0};
     __syncthreads();
@@ -582,7 +877,7 @@ __global__ __launch_bounds__(256) void reduce_best_kernel(const WaveBest* __rest
                                                           const vj_feature* __restrict__ feats,
                                                           const uint32_t* __restrict__ sumT,
                                                           const float* __restrict__ inv, int n,
-                                                          RoundBest* __restrict__ out) {
+                                                          uint32_t idxMask, RoundBest* __restrict__ out) {
     __shared__ WaveBest sb[256];
     WaveBest r{INT64_MAX, -1, +1, INT_MAX, 0};
     for (int i = threadIdx.x; i < nb; i += blockDim.x) {
@@ -609,7 +904,7 @@ __global__ __launch_bounds__(256) void reduce_best_kernel(const WaveBest* __rest
         rb.theta = 0.0f;
         if (b.split >= 0 && b.split + 1 < n) {
             const uint16_t* row = ord + static_cast<int64_t>(b.feature) * pitch;
-            const int s0 = row[b.split] & 0x7FFF, s1 = row[b.split + 1] & 0x7FFF;
+            const int s0 = row[b.split] & idxMask, s1 = row[b.split + 1] & idxMask;
             const vj_feature f = feats[b.feature];
             const float v0 = response(f, sumT, inv, n, s0);
             const float v1 = response(f, sumT, inv, n, s1);
@@ -920,7 +1215,7 @@ __global__ __launch_bounds__(kSearchThreads) void search_each_kernel(const uint1
         const uint16_t* row = ord + static_cast<int64_t>(f) * pitch;
         int64_t fE;
         int32_t fK, fP;
-        scan_feature<true>(row, n, lsw, totPos, totNeg, lane, fE, fK, fP);
+        scan_feature<true, false>(row, nullptr, n, lsw, totPos, totNeg, lane, fE, fK, fP);
         if (lane == 0) {
             FeatureBestSplit r{};
             if (fK >= 0 && fK + 1 < n) {
@@ -929,6 +1224,60 @@ __global__ __launch_bounds__(kSearchThreads) void search_each_kernel(const uint1
                 r.bestParity = static_cast<int8_t>(fP);
                 const float* rr = resp + static_cast<size_t>(f) * n;
                 r.bestTheta = 0.5f * (rr[row[fK] & 0x7FFF] + rr[row[fK + 1] & 0x7FFF]);
+            } else {
+                r.bestErr = FLT_MAX;
+                r.bestTheta = 0.0f;
+                r.bestSplitPos = -1;
+                r.bestParity = +1;
+            }
+            out[f] = r;
+        }
+    }
+}
+
+// mark_valid_kernel for the wide layout: rows padded to pitch (entries 0 past
+// n), split flags as mask bits.  The grid covers pitch positions per row.
+__global__ __launch_bounds__(256) void mark_valid_wide_kernel(const float* __restrict__ resp,
+                                                              const uint16_t* __restrict__ sorted, int n,
+                                                              uint16_t* __restrict__ out, uint64_t* __restrict__ mask,
+                                                              int64_t pitch) {
+    const int f = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const float* row = resp + static_cast<size_t>(f) * n;
+    const uint16_t* sr = sorted + static_cast<size_t>(f) * n;
+    uint16_t e = 0;
+    bool valid = false;
+    if (k < n) {
+        e = sr[k];
+        if (k + 1 < n) valid = row[sr[k]] + 1e-7f < row[sr[k + 1]];
+    }
+    out[static_cast<int64_t>(f) * pitch + k] = e;
+    const uint64_t word = __ballot(valid);
+    if ((threadIdx.x & 63) == 0) mask[static_cast<int64_t>(f) * (pitch / 64) + k / 64] = word;
+}
+
+// search_each_kernel over the wide layout.
+__global__ __launch_bounds__(kSearchThreads) void search_each_wide_kernel(
+        const uint16_t* __restrict__ ord, const uint64_t* __restrict__ mask, int64_t pitch, int n, int nFeatures,
+        const int64_t* __restrict__ q, const int64_t* __restrict__ tot, const float* __restrict__ resp,
+        FeatureBestSplit* __restrict__ out) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t totPos = tot[0], totNeg = tot[1];
+    const int nWavesTotal = gridDim.x * (blockDim.x >> 6);
+    for (int f = blockIdx.x * (blockDim.x >> 6) + wave; f < nFeatures; f += nWavesTotal) {
+        const uint16_t* row = ord + static_cast<int64_t>(f) * pitch;
+        int64_t fE;
+        int32_t fK, fP;
+        scan_feature<true, true>(row, reinterpret_cast<const uint8_t*>(mask + static_cast<int64_t>(f) * (pitch / 64)),
+                                 n, q, totPos, totNeg, lane, fE, fK, fP);
+        if (lane == 0) {
+            FeatureBestSplit r{};
+            if (fK >= 0 && fK + 1 < n) {
+                r.bestErr = static_cast<float>(ldexp(static_cast<double>(fE), -62));
+                r.bestSplitPos = fK;
+                r.bestParity = static_cast<int8_t>(fP);
+                const float* rr = resp + static_cast<size_t>(f) * n;
+                r.bestTheta = 0.5f * (rr[row[fK]] + rr[row[fK + 1]]);
             } else {
                 r.bestErr = FLT_MAX;
                 r.bestTheta = 0.0f;
@@ -1003,6 +1352,7 @@ struct vj_trainer {
     int bN = 0;
     DevBuf<float> opSw;
     DevBuf<uint16_t> opOrd;
+    DevBuf<uint64_t> opMask;
 
     // fused stage state
     int n = 0, numPos = 0, nFeat = 0, stageNegBase = 0;
@@ -1012,6 +1362,14 @@ struct vj_trainer {
     DevBuf<uint64_t> sqT;
     DevBuf<float> inv;
     DevBuf<uint16_t> ord;
+    // wide layout (n above the LDS sort limit): plain indices in ord, split
+    // flags in mask rows of pitch / 64 words, exact weights and totals for the
+    // scan, and the presort's per-workgroup ping-pong buffers
+    bool wide = false;
+    DevBuf<uint64_t> mask;
+    DevBuf<int64_t> wq, wtot;
+    DevBuf<uint32_t> scrKeys;
+    DevBuf<uint16_t> scrIdx;
     DevBuf<int8_t> label;
     DevBuf<uint8_t> active;
     DevBuf<double> w;
@@ -1038,6 +1396,21 @@ int search_blocks() {
     }
     (void)hipGetLastError();
     return 2 * cus;
+}
+
+// The wide presort over count features into out (rows outPitch apart) and
+// mask (rows outPitch / 64 words apart; mode 0 only).
+int launch_presort_wide(vj_trainer* t, const vj_feature* feats, const uint32_t* sumT, const float* inv,
+                        const float* resp, int n, int count, uint16_t* out, int64_t outPitch, uint64_t* mask, int mode,
+                        hipStream_t s) {
+    const int grid = std::max(1, std::min(count, t->searchBlocks));
+    const int64_t nPad = (n + kWideTile - 1) / kWideTile * kWideTile;
+    VJ_TRY(t->scrKeys.ensure(static_cast<size_t>(grid) * 2 * static_cast<size_t>(nPad)));
+    VJ_TRY(t->scrIdx.ensure(static_cast<size_t>(grid) * 2 * static_cast<size_t>(nPad)));
+    hipLaunchKernelGGL(presort_wide_kernel, dim3(grid), dim3(kSortThreads), 0, s, feats, 0, sumT, inv, resp, n, count,
+                       out, outPitch, mask, outPitch / 64, mode, t->scrKeys.get(), t->scrIdx.get(), nPad);
+    VJ_HIP(hipGetLastError());
+    return VJ_OK;
 }
 
 size_t presort_lds(int n) {
@@ -1145,7 +1518,10 @@ extern "C" int vj_trainer_sort_per_feature(vj_trainer* t, const float* dResp, in
                                            vj_stream sv) {
     VJ_CHECK(t && dResp && dOut && count > 0 && count <= t->maxTile, "sortSamplesPerFeature: bad arguments");
     const int n = t->bN;
-    VJ_CHECK(n > 0 && n <= kSortMaxN, "sortSamplesPerFeature: sample count above the LDS sort limit (20480)");
+    VJ_CHECK(n > 0 && n <= kTrainMaxN, "sortSamplesPerFeature: sample count must be 1..65535");
+    if (n > kSortMaxN)
+        return launch_presort_wide(t, nullptr, nullptr, nullptr, dResp, n, count, dOut, static_cast<int64_t>(n),
+                                   nullptr, 1, static_cast<hipStream_t>(sv));
     hipLaunchKernelGGL(presort_kernel, dim3(count), dim3(kSortThreads), presort_lds(n), static_cast<hipStream_t>(sv),
                        nullptr, 0, nullptr, nullptr, dResp, n, dOut, static_cast<int64_t>(n), 1);
     VJ_HIP(hipGetLastError());
@@ -1157,7 +1533,7 @@ extern "C" int vj_trainer_find_threshold(vj_trainer* t, const float* dResp, cons
     VJ_CHECK(t && dResp && dSorted && dOutBest && t->bLabel && t->bWeight, "evaluateAndFindThreshold: not bound");
     VJ_CHECK(count > 0 && count <= t->maxTile, "evaluateAndFindThreshold: bad count");
     const int n = t->bN;
-    VJ_CHECK(n > 1 && n <= 32767, "evaluateAndFindThreshold: sample count must be 2..32767");
+    VJ_CHECK(n > 1 && n <= kTrainMaxN, "evaluateAndFindThreshold: sample count must be 2..65535");
     hipStream_t s = static_cast<hipStream_t>(sv);
     // exact sums count 2^-62 units in int64: the weights must be normalised
     // (every boosting round leaves them summing to 1)
@@ -1178,6 +1554,24 @@ extern "C" int vj_trainer_find_threshold(vj_trainer* t, const float* dResp, cons
     hipLaunchKernelGGL(search_weights_kernel, dim3((n + 255) / 256), dim3(256), 0, s, t->bWeight, t->bLabel,
                        t->bActive, n, t->opSw.get());
     VJ_HIP(hipGetLastError());
+    if (n > kPackedMaxN) {
+        // indices need all 16 bits: the wide layout
+        VJ_TRY(t->opMask.ensure(static_cast<size_t>(pitch / 64) * count));
+        VJ_TRY(t->wq.ensure(static_cast<size_t>(n)));
+        VJ_TRY(t->wtot.ensure(2));
+        hipLaunchKernelGGL(wide_weights_kernel, dim3(1), dim3(1024), 0, s, t->opSw.get(), n, t->wq.get(),
+                           t->wtot.get());
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(mark_valid_wide_kernel, dim3(static_cast<unsigned>(pitch / 256), count), dim3(256), 0, s,
+                           dResp, dSorted, n, t->opOrd.get(), t->opMask.get(), pitch);
+        VJ_HIP(hipGetLastError());
+        const int blocks = std::min((count + 15) / 16, t->searchBlocks);
+        hipLaunchKernelGGL(search_each_wide_kernel, dim3(blocks), dim3(kSearchThreads), 0, s, t->opOrd.get(),
+                           t->opMask.get(), pitch, n, count, t->wq.get(), t->wtot.get(), dResp,
+                           static_cast<FeatureBestSplit*>(dOutBest));
+        VJ_HIP(hipGetLastError());
+        return VJ_OK;
+    }
     hipLaunchKernelGGL(mark_valid_kernel, dim3((n + 255) / 256, count), dim3(256), 0, s, dResp, dSorted, n,
                        t->opOrd.get(), pitch);
     VJ_HIP(hipGetLastError());
@@ -1203,8 +1597,8 @@ extern "C" int vj_trainer_stage_begin(vj_trainer* t, const uint8_t* dPatches, co
                                       const uint8_t* hActive, int n, int numPos, const vj_feature* dFeatures,
                                       int nFeatures, vj_stream sv) {
     VJ_CHECK(t && dPatches && hLabels && dFeatures && nFeatures > 0, "stage_begin: bad arguments");
-    VJ_CHECK(n > 1 && n <= t->maxSamples && n <= kSortMaxN,
-             "stage_begin: sample count must be 2..min(maxSamples, 20480)");
+    VJ_CHECK(n > 1 && n <= t->maxSamples && n <= kTrainMaxN,
+             "stage_begin: sample count must be 2..min(maxSamples, 65535)");
     VJ_CHECK(numPos > 0 && numPos < n, "stage_begin: need positives and negatives");
     VJ_CHECK(static_cast<int64_t>(t->winW) * t->winH * 255 < (1 << 24),
              "stage_begin: window too large for exact float rectangle sums");
@@ -1213,6 +1607,12 @@ extern "C" int vj_trainer_stage_begin(vj_trainer* t, const uint8_t* dPatches, co
     const int iw = t->winW + 1, ih = t->winH + 1;
     const size_t area = static_cast<size_t>(iw) * ih;
     t->pitch = (n + 255) / 256 * 256;
+    t->wide = n > kSortMaxN;
+    if (t->wide) {
+        VJ_TRY(t->mask.ensure(static_cast<size_t>(t->pitch / 64) * nFeatures));
+        VJ_TRY(t->wq.ensure(static_cast<size_t>(n)));
+        VJ_TRY(t->wtot.ensure(2));
+    }
     VJ_TRY(t->sumT.ensure(area * n));
     VJ_TRY(t->sqT.ensure(area * n));
     VJ_TRY(t->inv.ensure(static_cast<size_t>(n)));
@@ -1241,9 +1641,14 @@ extern "C" int vj_trainer_stage_begin(vj_trainer* t, const uint8_t* dPatches, co
                        t->winH, n, t->sumT.get(), t->sqT.get(), t->inv.get());
     VJ_HIP(hipGetLastError());
     // sample order of every feature, once for the whole stage
-    hipLaunchKernelGGL(presort_kernel, dim3(nFeatures), dim3(kSortThreads), presort_lds(n), s, dFeatures, 0,
-                       t->sumT.get(), t->inv.get(), nullptr, n, t->ord.get(), t->pitch, 0);
-    VJ_HIP(hipGetLastError());
+    if (t->wide) {
+        VJ_TRY(launch_presort_wide(t, dFeatures, t->sumT.get(), t->inv.get(), nullptr, n, nFeatures, t->ord.get(),
+                                   t->pitch, t->mask.get(), 0, s));
+    } else {
+        hipLaunchKernelGGL(presort_kernel, dim3(nFeatures), dim3(kSortThreads), presort_lds(n), s, dFeatures, 0,
+                           t->sumT.get(), t->inv.get(), nullptr, n, t->ord.get(), t->pitch, 0);
+        VJ_HIP(hipGetLastError());
+    }
     // weights 0.5/numPos and 0.5/activeNeg, normalised (src/main.cu:1955-1962)
     hipLaunchKernelGGL(reset_kernel, dim3((n + 255) / 256), dim3(256), 0, s, t->w.get(), t->strong.get(),
                        t->active.get(), n, numPos, 0.5 / static_cast<double>(numPos), 0.5 / static_cast<double>(activeNeg));
@@ -1268,8 +1673,14 @@ extern "C" int vj_trainer_round(vj_trainer* t, const vj_round_config* cfg, vj_we
     std::memset(out, 0, sizeof *out);
     out->feature = -1;
     const int blocks = std::min((t->nFeat + 15) / 16, t->searchBlocks);
-    // int64 weights in LDS when they fit beside the kernel's static LDS
-    if (static_cast<size_t>(t->n) * 8 <= kSearchWideLds) {
+    if (t->wide) {
+        hipLaunchKernelGGL(wide_weights_kernel, dim3(1), dim3(1024), 0, s, t->sw32.get(), t->n, t->wq.get(),
+                           t->wtot.get());
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(search_wide_kernel, dim3(blocks), dim3(kSearchThreads), 0, s, t->ord.get(), t->mask.get(),
+                           t->pitch, t->n, t->nFeat, t->wq.get(), t->wtot.get(), t->used.get(), t->blockBest.get());
+    } else if (static_cast<size_t>(t->n) * 8 <= kSearchWideLds) {
+        // int64 weights in LDS when they fit beside the kernel's static LDS
         hipLaunchKernelGGL(search_kernel<int64_t>, dim3(blocks), dim3(kSearchThreads), static_cast<size_t>(t->n) * 8,
                            s, t->ord.get(), t->pitch, t->n, t->nFeat, t->sw32.get(), t->used.get(),
                            t->blockBest.get());
@@ -1280,7 +1691,8 @@ extern "C" int vj_trainer_round(vj_trainer* t, const vj_round_config* cfg, vj_we
     }
     VJ_HIP(hipGetLastError());
     hipLaunchKernelGGL(reduce_best_kernel, dim3(1), dim3(256), 0, s, t->blockBest.get(), blocks, t->ord.get(), t->pitch,
-                       t->feats, t->sumT.get(), t->inv.get(), t->n, t->roundBest.get());
+                       t->feats, t->sumT.get(), t->inv.get(), t->n, t->wide ? 0xFFFFu : 0x7FFFu,
+                       t->roundBest.get());
     VJ_HIP(hipGetLastError());
     VJ_HIP(hipMemcpyAsync(t->hRound.get(), t->roundBest.get(), sizeof(RoundBest), hipMemcpyDeviceToHost, s));
     VJ_HIP(hipStreamSynchronize(s));
@@ -1356,5 +1768,23 @@ extern "C" int vj_trainer_read_orders(vj_trainer* t, int begin, int count, uint1
     VJ_HIP(hipMemcpy2D(hOut, static_cast<size_t>(t->n) * 2, t->ord.get() + static_cast<size_t>(begin) * t->pitch,
                        static_cast<size_t>(t->pitch) * 2, static_cast<size_t>(t->n) * 2, static_cast<size_t>(count),
                        hipMemcpyDeviceToHost));
+    return VJ_OK;
+}
+
+extern "C" int vj_trainer_read_splits(vj_trainer* t, int begin, int count, uint8_t* hOut) {
+    VJ_CHECK(t && hOut && begin >= 0 && count > 0 && begin + count <= t->nFeat, "read_splits: bad range");
+    const size_t n = static_cast<size_t>(t->n);
+    if (t->wide) {
+        const size_t words = static_cast<size_t>(t->pitch / 64);
+        std::vector<uint64_t> m(words * static_cast<size_t>(count));
+        VJ_HIP(hipMemcpy(m.data(), t->mask.get() + static_cast<size_t>(begin) * words, m.size() * 8,
+                         hipMemcpyDeviceToHost));
+        for (size_t f = 0; f < static_cast<size_t>(count); ++f)
+            for (size_t k = 0; k < n; ++k) hOut[f * n + k] = static_cast<uint8_t>((m[f * words + k / 64] >> (k % 64)) & 1u);
+    } else {
+        std::vector<uint16_t> o(n * static_cast<size_t>(count));
+        VJ_TRY(vj_trainer_read_orders(t, begin, count, o.data()));
+        for (size_t i = 0; i < o.size(); ++i) hOut[i] = static_cast<uint8_t>(o[i] >> 15);
+    }
     return VJ_OK;
 }

@@ -98,6 +98,12 @@ class AdaBoostTrainer:
         check(lib().vj_trainer_read_orders(self.handle, int(begin), int(count), out.ctypes.data), "read_orders")
         return out
 
+    def read_splits(self, begin, count):
+        """Valid-split flags (0/1) of the presorted orders, per position."""
+        out = np.empty((count, self.n), np.uint8)
+        check(lib().vj_trainer_read_splits(self.handle, int(begin), int(count), out.ctypes.data), "read_splits")
+        return out
+
 
 def cpu_best_weak(patches, labels, weights, features, active=None, win=(24, 24)):
     """The host leg of train_compare (runCpuWeakSearch, src/main.cu:587-682),

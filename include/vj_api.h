@@ -286,11 +286,13 @@ int vj_trainer_bind(vj_trainer* t, const vj_integral_view* ii, const int8_t* dLa
 int vj_trainer_evaluate_responses(vj_trainer* t, const vj_feature* dFeatures, int begin, int count,
                                   float* dOut, vj_stream s);
 /* sortSamplesPerFeature, src/boosting_core.cu:829-892: per-feature ascending
- * (value bits, index) order of dResp rows, written as plain u16 indices. */
+ * (value bits, index) order of dResp rows, written as plain u16 indices
+ * (any bound sample count up to 65,535). */
 int vj_trainer_sort_per_feature(vj_trainer* t, const float* dResp, int count, uint16_t* dOutSorted,
                                 vj_stream s);
 /* evaluateAndFindThreshold, src/boosting_core.cu:789-827 (FeatureBestSplit
- * records, 16 B each: err f32, theta f32, split i32, parity i8, pad). */
+ * records, 16 B each: err f32, theta f32, split i32, parity i8, pad); up to
+ * 65,535 samples. */
 int vj_trainer_find_threshold(vj_trainer* t, const float* dResp, const uint16_t* dSorted, int count,
                               void* dOutBest, vj_stream s);
 /* selectBestInTile, src/boosting_core.cu:894-925 (TileBestCandidate, 16 B). */
@@ -307,7 +309,7 @@ typedef struct vj_round_config {
 } vj_round_config;
 
 /* Stage start (src/main.cu:1936-1997): integrals and invNorm of the n
- * 24x24 patches (positives first), presort of every feature of dFeatures,
+ * 24x24 patches (positives first, n <= 65,535), presort of every feature of dFeatures,
  * weights 0.5/numPos and 0.5/activeNeg normalised, strong scores 0, used
  * mask cleared.  hActive may be NULL (all active). */
 int vj_trainer_stage_begin(vj_trainer* t, const uint8_t* dPatches, const int8_t* hLabels,
@@ -322,9 +324,14 @@ int vj_trainer_stage_end(vj_trainer* t, float stageThreshold, int* remainNeg, vj
 /* Copies of the trainer's per-sample state (for checks and debugging). */
 int vj_trainer_read_state(vj_trainer* t, double* hWeights, float* hStrong, uint8_t* hActive,
                           float* hInvNorm);
-/* Presorted orders of features [begin, begin+count) (u16 entries, bit 15 =
- * valid split after this position) for checking the presort. */
+/* Presorted orders of features [begin, begin+count), n entries each, for
+ * checking the presort.  Up to 20,480 samples an entry is the sample index
+ * with bit 15 set when a valid split follows the position; above that the
+ * entries are plain 16-bit indices and the flags are kept apart. */
 int vj_trainer_read_orders(vj_trainer* t, int begin, int count, uint16_t* hOut);
+/* The valid-split flags of the same orders, one byte (0 or 1) per position:
+ * fl(v_k + 1e-7f) < v_{k+1} (src/boosting_core.cu:395-405). */
+int vj_trainer_read_splits(vj_trainer* t, int begin, int count, uint8_t* hOut);
 
 /* TileBestCandidate, include/vj/boosting_core.cuh (16 B): what
  * vj_trainer_select_best writes.  featureIdx is INT_MAX only when every

@@ -264,7 +264,11 @@ def train_stage(patches, labels, num_pos, features, rounds, min_hit=0.99, max_fa
     act = np.ones(n, np.uint8) if active is None else np.array(active, np.uint8)
     act[:num_pos] = 1
     sumT, _sq, inv = train_integrals(patches)
-    ords = presort(features, sumT, inv)
+    # above 32,767 samples a 16-bit entry has no room for the split flag: the
+    # 32-bit layout (index in bits 0-30, flag in bit 31)
+    wide = n > 32767
+    idx_mask = 0x7FFFFFFF if wide else 0x7FFF
+    ords = presort(features, sumT, inv, wide=wide)
     active_neg = int(act[num_pos:].sum())
     w = np.where(np.arange(n) < num_pos, 0.5 / num_pos, np.where(act != 0, 0.5 / active_neg, 0.0))
     w = _normalise(w, act)
@@ -281,7 +285,7 @@ def train_stage(patches, labels, num_pos, features, rounds, min_hit=0.99, max_fa
             break
         k, parity = int(best[fi]["split"]), int(best[fi]["parity"])
         resp = feature_responses(features[fi], sumT, inv)
-        s0, s1 = int(ords[fi, k]) & 0x7FFF, int(ords[fi, k + 1]) & 0x7FFF
+        s0, s1 = int(ords[fi, k]) & idx_mask, int(ords[fi, k + 1]) & idx_mask
         theta = np.float32(0.5) * (resp[s0] + resp[s1])
         err = math.ldexp(float(eq), -62)
         ec = min(max(err, 1e-6), 1.0 - 1e-6)
