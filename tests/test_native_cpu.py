@@ -166,6 +166,24 @@ def test_no_device_is_reported_not_faked():
     assert ei.value.status == _native.VJ_DEVICE_ERROR
 
 
+def test_facade_compiles_and_constructors_throw_without_a_device():
+    """include/vj_facade.hpp (the reference's FaceVisionEngine / AdaBoostTrainer
+    over the C ABI) compiles with -Wall -Wextra -Werror in reference-style call
+    sites (tests/facade/reference_callers.cpp) and links with libvj_hip.so;
+    without a device both constructors throw std::runtime_error with the
+    library's message, as the reference's constructors throw."""
+    subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "tests", "facade")])
+    exe = os.path.join(ROOT, "tests", "facade", "build", "reference_callers")
+    out = subprocess.run([exe, "nodevice"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "pool 162336" in out.stdout
+    if _native.device_count() == 0:
+        assert "FaceVisionEngine threw: FaceVisionEngine: vj_engine_create: no HIP device" in out.stdout
+        assert "AdaBoostTrainer threw: AdaBoostTrainer: vj_trainer_create: no HIP device" in out.stdout
+    usage = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert usage.returncode == 2 and "usage" in usage.stderr
+
+
 def test_cli_usage_and_errors(tmp_path):
     exe = os.path.join(ROOT, "cs677s26_amd", "bin", "cuda_hello")
     r = subprocess.run([exe], capture_output=True, text=True)
