@@ -78,3 +78,18 @@ def served(n=240, qps=100.0, seed=synth.DEFAULT_SEED):
         y0 = int(rng.integers(0, 2160 - h + 1))
         imgs.append(base[y0:y0 + h, x0:x0 + w])
     return t, imgs
+
+
+def train_cli_inputs(n_pos=400, stages=3, seed=synth.DEFAULT_SEED):
+    """Inputs of a `cuda_hello train` run, all from seeds: a positive pool of
+    n_pos face patches, the stage negative files (10000 patches for stage 0,
+    30000 for each later stage -- enough for any top-up), and four mining
+    images (blurred noise with eye-banded blobs, 320x240 to 640x480)."""
+    rng = np.random.default_rng(seed + 11)
+    pos = synth.face_patches(n_pos, rng)
+    negs = {0: synth.nonface_patches(10000, rng)}
+    for s in range(1, stages):
+        negs[s] = synth.nonface_patches(30000, rng)
+    sizes = [(640, 480), (480, 360), (400, 300), (320, 240)]
+    images = [synth.detect_image(w, h, 12, 24, 90, seed=seed + 20 + k) for k, (w, h) in enumerate(sizes)]
+    return pos, negs, images

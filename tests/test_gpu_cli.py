@@ -82,6 +82,49 @@ def test_cli_train_mines_hard_negatives(tmp_path):
     assert out.read_bytes() == out2.read_bytes()
 
 
+def test_cli_train_equals_oracle_driver(tmp_path):
+    """`cuda_hello train` for three stages against the oracle's restatement of
+    the reference's training loop (oracle/pyoracle.py train_cascade,
+    src/main.cu:1523-2392), fixed in tests/golden/train_cli.json: the same
+    stumps (feature, theta, leaves, bit for bit) and stage records
+    (first, ntrees, threshold - 1e-5f), so the same stage-0 negatives, the
+    same hard negatives mined in the same order (std::shuffle of the images,
+    the 256 best of each by std::partial_sort, cropResizeTo24), the same
+    top-up picks and the same stage ends.  The model holds the Core pool."""
+    import json
+    from cs677s26_amd import workloads
+    say("cli train vs the oracle's training driver (golden)")
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "train_cli.json")))
+    pos, negs, images = workloads.train_cli_inputs()
+    pos.tofile(tmp_path / "faces_u8.bin")
+    cache = tmp_path / "cache"
+    (cache / "stage_negatives").mkdir(parents=True)
+    for s, arr in negs.items():
+        arr.tofile(cache / "stage_negatives" / f"non_faces_stage{s:02d}.bin")
+    imgs = tmp_path / "neg"
+    imgs.mkdir()
+    for i, im in enumerate(images):
+        write_pgm(imgs / f"img{i}.pgm", im)
+    out = tmp_path / "model.bin"
+    a = gold["args"]
+    args = [EXE, "train", str(tmp_path / "faces_u8.bin"), "10000", str(a["maxWeakPerStage"]), "200000", "4",
+            str(a["scaleFactor"]), str(a["maxStages"]), str(a["minHitRate"]), str(a["maxFalseAlarm"]), "24",
+            str(cache), str(imgs), str(a["seed"]), str(a["hardNegCandidateMultiplier"]), str(out)]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    say(r.stdout[-1500:])
+    assert r.returncode == 0, r.stderr
+    for inf in gold["stage_info"][1:]:
+        assert f"[TRAIN] stage={inf['stage']} hardNegSelected={inf['mined']}/10000" in r.stdout
+    win, f, s, g = loadCascadeModel(out)
+    assert win == (24, 24)
+    got_stumps = [[int(x["feature"]), int(np.float32(x["theta"]).view(np.uint32)),
+                   int(np.float32(x["left"]).view(np.uint32)), int(np.float32(x["right"]).view(np.uint32))] for x in s]
+    got_stages = [[int(x["first"]), int(x["ntrees"]), int(np.float32(x["threshold"]).view(np.uint32))] for x in g]
+    assert got_stumps == gold["stumps"]
+    assert got_stages == gold["stages"]
+    assert np.array_equal(np.asarray(f).view(np.uint8), O.haar_pool(24, 1).view(np.uint8))
+
+
 def test_cli_detect_compare_legs_agree(tmp_path):
     """detect_compare (src/main.cu:1216-1368): the host leg and the GPU leg
     report the oracle's raw detection count and agree box for box; the PPM

@@ -585,3 +585,54 @@ def test_train_stage_calibration_counts():
         up = np.nextafter(cut, np.float32(np.inf))
         assert thr in (cut, up)
         assert (thr == up) == (int((pos > cut).sum()) >= min_pass)
+
+
+# ---- the training driver -------------------------------------------------------
+
+def test_std_helpers():
+    """The standard-library steps: a shuffle is a permutation and depends on
+    the seed; uniform draws stay in [lo, hi] and cover it; the partial sort
+    puts the keep largest scores first in descending order (ties in the heap
+    order the algorithm leaves) and keeps every record."""
+    p = O.std_shuffle(1000, 677)
+    assert sorted(p.tolist()) == list(range(1000)) and not np.array_equal(p, O.std_shuffle(1000, 678))
+    u = O.std_uniform_ints(5000, 3, 9, 11)
+    assert u.min() == 3 and u.max() == 9 and len(set(u.tolist())) == 7
+    rng = np.random.default_rng(3)
+    sc = rng.integers(0, 20, size=500).astype(np.float32)
+    o = O.std_partial_sort_desc(sc, 37)
+    assert sorted(o.tolist()) == list(range(500))
+    top = sc[o[:37]]
+    assert (np.diff(top) <= 0).all() and np.array_equal(np.sort(top), np.sort(sc)[-37:])
+
+
+def test_mining_step_takes_each_images_best_boxes():
+    """The driver's mining step with the stage-0 cascade of the train-CLI
+    fixture (tests/golden/train_cli.json): images in shuffle order, from each
+    the 256 highest-scoring raw boxes (a multiset check against a plain sort),
+    each crop equal to cropResizeTo24 of its box, 1024 negatives in all, as
+    the fixture records for stage 1."""
+    import json
+    from cs677s26_amd.vjtypes import STAGE, STUMP
+    gold = json.load(open(os.path.join(HERE, "golden", "train_cli.json")))
+    _, _, images = workloads.train_cli_inputs(n_pos=1, stages=1)
+    feats = O.haar_pool(24, 1)
+    st = np.zeros(1, STUMP)
+    f, th, lb, rb = gold["stumps"][0]
+    st[0] = (f, np.uint32(th).view(np.float32), np.uint32(lb).view(np.float32), np.uint32(rb).view(np.float32))
+    first, nt, thr = gold["stages"][0]
+    sg = np.zeros(1, STAGE)
+    sg[0] = (first, nt, np.uint32(thr).view(np.float32))
+    neg = np.zeros((10000, 24, 24), np.uint8)
+    seed1 = 677 + 1 * 1000003
+    hard, used = O.mine_hard_negatives(feats, st, sg, images, seed1, neg)
+    assert hard == gold["stage_info"][1]["mined"] == 1024
+    order = [i for i, _ in used]
+    assert order == sorted(order, key=lambda i: list(O.std_shuffle(len(images), seed1)).index(i))
+    for i in set(order):
+        raw, _ = O.detect_raw(feats, st, sg, images[i], max_det=20000)
+        mine = np.array([d["score"] for k, d in used if k == i], np.float32)
+        assert np.array_equal(np.sort(mine), np.sort(raw["score"])[-256:])
+    for k in (0, 300, 777, 1023):
+        i, d = used[k]
+        assert np.array_equal(neg[k], O.crop_resize(images[i], (d["x"], d["y"], d["w"], d["h"])))
