@@ -37,7 +37,8 @@ class IntegralView(C.Structure):
 
 class DetectStats(C.Structure):
     _fields_ = [("levels", C.c_int32), ("windows", C.c_int64), ("rawDetections", C.c_int64),
-                ("pyramidPixels", C.c_int64), ("integralBytes", C.c_int64), ("cascadeBytes", C.c_int64)]
+                ("pyramidPixels", C.c_int64), ("integralBytes", C.c_int64), ("cascadeBytes", C.c_int64),
+                ("imagesGroupedOnHost", C.c_int32)]
 
 
 class Weak(C.Structure):

@@ -119,21 +119,9 @@ private:
 // Host-side helpers implemented in vj_host.cpp and used by the engine.
 int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps,
                      vj_detection* out, int* count);
-// A list's sweep order and similar pairs computed elsewhere (on the device):
-// order[r] is the box of sweep rank r, its pairs are the ascending sweep
-// ranks pairs[pairOff[r] .. pairOff[r] + pairCnt[r]).  order == nullptr:
-// none (the host finds them).
-struct GroupRanked {
-    const int32_t* order = nullptr;
-    const int32_t* pairOff = nullptr;
-    const int32_t* pairCnt = nullptr;
-    const int32_t* pairs = nullptr;
-};
-
 // The same for many images at once (in[i], n[i] boxes -> out[i], count[i]),
-// their steps spread over the host thread pool; ranked (optional, one per
-// image) supplies device-computed sweep orders and pairs.
+// their steps spread over the host thread pool.
 int group_rectangles_many(const vj_detection* const* in, const int* n, int nImages, int minNeighbors, float eps,
-                          vj_detection* const* out, int* count, const GroupRanked* ranked);
+                          vj_detection* const* out, int* count);
 
 }  // namespace vj

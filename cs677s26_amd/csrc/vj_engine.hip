@@ -25,6 +25,7 @@
 #include "vj_common.hpp"
 #include "vj_kernels.hpp"
 #include "vj_integral_dev.hpp"
+#include "vj_pair_plan.hpp"
 
 namespace vj {
 
@@ -1033,6 +1034,65 @@ using namespace vj;
 // ---------------------------------------------------------------------------
 constexpr int kGroupDevMax = 1 << 24;
 
+// Grouping on the device.  The components of the similarity graph do not
+// depend on the order of the unions, so the pair search unions every similar
+// pair at once in a lock-free union-find over sweep positions (the larger
+// root is linked under the smaller, so a component's representative is its
+// smallest sweep position).  Per component: box sums, count, last member and
+// the best score (the smallest raw index among the members of the highest
+// score, as the reference's strict > in input order keeps the first).  Only
+// the reference's cluster order needs its exact union-by-rank replay: the
+// output lists clusters by the raw index of the DSU root that
+// groupRectanglesCpu's unions leave, so each kept component's own pairs are
+// replayed in the reference's order (src/face_vision_engine.cu:1430-1445).
+struct GrpCount {
+    int32_t cnt, last;  // members; largest member position
+};
+// A kept component's cluster: integer-mean box, member count, the raw index
+// of the reference's DSU root, the best score and its scale index.
+struct GrpCluster {
+    int32_t x, y, w, h, n, rootRaw, scaleIdx;
+    float score;
+};
+
+// Union-find over sweep positions.  A label changes once, from itself to a
+// smaller position (a root linked under a smaller one), so the labels never
+// form a cycle and a find over values that are not yet the latest still ends
+// at a node of the same component.
+__device__ __forceinline__ int32_t uf_find(const int32_t* label, int32_t x) {
+    for (;;) {
+        const int32_t px = label[x];
+        if (px == x) return x;
+        x = px;
+    }
+}
+
+// Links the components of a and b, the larger root under the smaller.  A
+// failed compare-and-swap returns b's new parent, the next candidate root.
+__device__ __forceinline__ void uf_union(int32_t* label, int32_t a, int32_t b) {
+    a = uf_find(label, a);
+    b = uf_find(label, b);
+    while (a != b) {
+        if (a > b) {
+            const int32_t t = a;
+            a = b;
+            b = t;
+        }
+        const int32_t old = atomicCAS(label + b, b, a);
+        if (old == b) return;
+        b = uf_find(label, old);
+        a = uf_find(label, a);
+    }
+}
+
+// Ordered key of a cluster score candidate: 0 for NaN and -inf (they never
+// beat the initial -inf), and -0 keyed as +0 (the two compare equal).
+__device__ __forceinline__ uint32_t best_key(float s) {
+    if (!(s > -INFINITY)) return 0u;
+    const uint32_t b = __float_as_uint(s + 0.0f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
 __device__ __forceinline__ int grouped_boxes(const int32_t* __restrict__ counts, int i, int cap) {
     const int n = min(counts[i], cap);
     return n <= kGroupDevMax ? n : 0;
@@ -1073,10 +1133,19 @@ __global__ __launch_bounds__(kSweepThreads) void pair_sweep_kernel(const vj_dete
                                                                   const int32_t* __restrict__ imgW, int cap,
                                                                   int32_t* __restrict__ bucket,
                                                                   int4* __restrict__ sorted,
-                                                                  int32_t* __restrict__ order) {
+                                                                  int32_t* __restrict__ order,
+                                                                  int32_t* __restrict__ grpImg,
+                                                                  int64_t* __restrict__ ctl) {
     extern __shared__ int32_t xs[];  // cnt[nb] (then the fill cursor), first[nb + 1]
     __shared__ int32_t waveTot[kSweepThreads / kWave];
     const int i = blockIdx.x;
+    if (threadIdx.x == 0) {
+        // this call's grouping counters: kept clusters, output count, host flag
+        grpImg[3 * i] = 0;
+        grpImg[3 * i + 1] = 0;
+        grpImg[3 * i + 2] = 0;
+        if (i == 0) ctl[1] = 0;
+    }
     const int n = grouped_boxes(counts, i, cap);
     if (n == 0) return;
     const int nb = imgW[i] + 1;
@@ -1133,8 +1202,9 @@ __global__ __launch_bounds__(256) void pair_find_kernel(const int32_t* __restric
                                                        const int32_t* __restrict__ imgBase, int nImages, int cap,
                                                        float eps, const int4* __restrict__ sorted,
                                                        int32_t* __restrict__ pairCnt,
-                                                       const int32_t* __restrict__ pairOff,
-                                                       int32_t* __restrict__ pairs, int64_t pairCap) {
+                                                       const int64_t* __restrict__ pairOff,
+                                                       int32_t* __restrict__ pairs, int64_t pairCap,
+                                                       int32_t* __restrict__ label, GrpCount* __restrict__ gc) {
     const int64_t total = all_boxes(counts, nImages, cap);
     const int lane = lane_id();
     const int64_t wavesTotal = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
@@ -1142,6 +1212,13 @@ __global__ __launch_bounds__(256) void pair_find_kernel(const int32_t* __restric
          p += wavesTotal) {
         const int i = image_of(imgBase, nImages, p);
         const int n = grouped_boxes(counts, i, cap);
+        if constexpr (!kWrite) {
+            // every position starts as its own component, with empty sums
+            if (lane == 0) {
+                label[p] = static_cast<int32_t>(p);
+                gc[p] = GrpCount{0, 0};
+            }
+        }
         if (n == 0) {  // an image left to the host: no pairs here
             if constexpr (!kWrite) {
                 if (lane == 0) pairCnt[p] = 0;
@@ -1153,9 +1230,10 @@ __global__ __launch_bounds__(256) void pair_find_kernel(const int32_t* __restric
         const int ar = A.x + A.z, ab = A.y + A.w;
         const int reach = static_cast<int>(eps * static_cast<float>(A.z + A.w) * 0.5f);
         int64_t o = 0;
+        bool store = true;
         if constexpr (kWrite) {
             o = pairOff[p];
-            if (o + pairCnt[p] > pairCap) continue;
+            store = o + pairCnt[p] <= pairCap;
         }
         int found = 0;
         for (int64_t q0 = p + 1; q0 < end; q0 += kWave) {
@@ -1171,7 +1249,10 @@ __global__ __launch_bounds__(256) void pair_find_kernel(const int32_t* __restric
             }
             const uint64_t m = __ballot(sim);
             if constexpr (kWrite) {
-                if (sim) pairs[o + found + mask_rank(m)] = static_cast<int32_t>(q - base);
+                if (sim) {
+                    if (store) pairs[o + found + mask_rank(m)] = static_cast<int32_t>(q - base);
+                    uf_union(label, static_cast<int32_t>(p), static_cast<int32_t>(q));
+                }
             }
             found += __popcll(m);
             if (__ballot(valid && !inReach)) break;  // sweep order: the rest is out of reach too
@@ -1185,15 +1266,14 @@ __global__ __launch_bounds__(256) void pair_find_kernel(const int32_t* __restric
 // Exclusive scan of the pair counts of all sweep positions, in two launches
 // over blocks of kScanItems counts (each thread kScanPer consecutive ones):
 // pair_scan_local_kernel writes each block's own exclusive offsets and its
-// total; pair_scan_add_kernel adds the totals of the blocks before (offsets
-// saturate at INT32_MAX, past any pair capacity) and block 0 of the last
-// block writes the grand total into ctl[0].
+// total; pair_scan_add_kernel adds the totals of the blocks before (64-bit
+// offsets) and block 0 of the last block writes the grand total into ctl[0].
 constexpr int kScanThreads = 256, kScanPer = 16, kScanItems = kScanThreads * kScanPer;
 
 __global__ __launch_bounds__(kScanThreads) void pair_scan_local_kernel(const int32_t* __restrict__ counts,
                                                                       int nImages, int cap,
                                                                       const int32_t* __restrict__ pairCnt,
-                                                                      int32_t* __restrict__ pairOff,
+                                                                      int64_t* __restrict__ pairOff,
                                                                       int64_t* __restrict__ blockTot) {
     __shared__ int64_t waveTot[kScanThreads / kWave];
     const int64_t total = all_boxes(counts, nImages, cap);
@@ -1218,7 +1298,7 @@ __global__ __launch_bounds__(kScanThreads) void pair_scan_local_kernel(const int
         }
 #pragma unroll
         for (int k = 0; k < kScanPer; ++k) {
-            if (g0 + k < total) pairOff[g0 + k] = static_cast<int32_t>(off);
+            if (g0 + k < total) pairOff[g0 + k] = off;
             off += v[k];
         }
         if (threadIdx.x == 0) blockTot[b] = blk;
@@ -1229,7 +1309,7 @@ __global__ __launch_bounds__(kScanThreads) void pair_scan_local_kernel(const int
 __global__ __launch_bounds__(kScanThreads) void pair_scan_add_kernel(const int32_t* __restrict__ counts,
                                                                     int nImages, int cap,
                                                                     const int64_t* __restrict__ blockTot,
-                                                                    int32_t* __restrict__ pairOff,
+                                                                    int64_t* __restrict__ pairOff,
                                                                     int64_t* __restrict__ ctl) {
     __shared__ int64_t part[kScanThreads / kWave];
     const int64_t total = all_boxes(counts, nImages, cap);
@@ -1247,10 +1327,350 @@ __global__ __launch_bounds__(kScanThreads) void pair_scan_add_kernel(const int32
         const int64_t g0 = b * kScanItems + static_cast<int64_t>(threadIdx.x) * kScanPer;
 #pragma unroll
         for (int k = 0; k < kScanPer; ++k)
-            if (g0 + k < total) pairOff[g0 + k] = static_cast<int32_t>(min<int64_t>(pre + pairOff[g0 + k], INT32_MAX));
+            if (g0 + k < total) pairOff[g0 + k] += pre;
         if (b == nb - 1 && threadIdx.x == 0) ctl[0] = pre + blockTot[b];
         __syncthreads();  // part is rewritten by the next block
     }
+}
+
+// Per sweep position: its component (the root of the union-find, written
+// back into label, which from here on names each position's component), and
+// the component's member count and last member.  Lanes of a wave that share
+// a component add up before one atomic each.  Nothing runs when the pair
+// buffer overflowed (the call is then grouped on the host).
+__global__ __launch_bounds__(256) void grp_comp_kernel(const int32_t* __restrict__ counts,
+                                                       const int32_t* __restrict__ imgBase, int nImages, int cap,
+                                                       int32_t* __restrict__ label, GrpCount* __restrict__ gc,
+                                                       const int64_t* __restrict__ ctl, int64_t pairCap) {
+    if (ctl[0] > pairCap) return;
+    const int64_t total = all_boxes(counts, nImages, cap);
+    const int lane = lane_id();
+    for (int64_t g0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) & ~int64_t{kWave - 1}; g0 < total;
+         g0 += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t g = g0 + lane;
+        bool valid = g < total;
+        int32_t r = -1;
+        if (valid) {
+            valid = grouped_boxes(counts, image_of(imgBase, nImages, g), cap) > 0;
+            if (valid) {
+                r = uf_find(label, static_cast<int32_t>(g));
+                label[g] = r;
+            }
+        }
+        uint64_t act = __ballot(valid);
+        while (act) {
+            const int leader = __ffsll(static_cast<long long>(act)) - 1;
+            const int32_t rl = __builtin_amdgcn_readlane(r, leader);
+            const uint64_t m = __ballot(valid && r == rl);
+            if (lane == leader) {
+                atomicAdd(&gc[rl].cnt, __popcll(m));
+                atomicMax(&gc[rl].last, static_cast<int32_t>(g0 + 63 - __clzll(static_cast<long long>(m))));
+            }
+            act &= ~m;
+        }
+    }
+}
+
+// Each component root whose count passes the minNeighbors rule (count >
+// max(1, minNeighbors)) gets the next cluster slot of its image (grpImg[3 i]
+// counts them) and a place in the list of all kept components (ctl[1]).
+__global__ __launch_bounds__(256) void grp_list_kernel(const int32_t* __restrict__ counts,
+                                                       const int32_t* __restrict__ imgBase, int nImages, int cap,
+                                                       int minNeighbors, const int32_t* __restrict__ label,
+                                                       const GrpCount* __restrict__ gc, int32_t* __restrict__ keptAll,
+                                                       int32_t* __restrict__ keptSlot, int32_t* __restrict__ grpImg,
+                                                       int64_t* __restrict__ ctl, int64_t pairCap) {
+    if (ctl[0] > pairCap) return;
+    const int64_t total = all_boxes(counts, nImages, cap);
+    const int mn = max(1, minNeighbors);
+    for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+         g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        if (label[g] != g || gc[g].cnt <= mn) continue;
+        const int i = image_of(imgBase, nImages, g);
+        if (grouped_boxes(counts, i, cap) == 0) continue;
+        const int64_t k = static_cast<int64_t>(atomicAdd(reinterpret_cast<unsigned long long*>(&ctl[1]), 1ull));
+        keptAll[k] = static_cast<int32_t>(g);
+        keptSlot[k] = atomicAdd(&grpImg[3 * i], 1);
+    }
+}
+
+// One kept component per wavefront.  First the span between its root (its
+// smallest sweep position) and its last member is staged in LDS, all loads
+// in parallel: the member bits and each position's pair-list offset; the
+// cluster sums and the best score are reduced on the way.  Then the
+// reference's union by rank (ties to the first argument's root) is replayed
+// over the members' pairs in the reference's order -- members by ascending
+// sweep position, each one's partners in ascending sweep order
+// (src/face_vision_engine.cu:1430-1445).  Pair lists are stored in sweep
+// order, so the span's lists form one range of pair entries, and that range
+// in order is exactly this order (entries of other components are masked
+// out).  It is read 64 entries at a time through an LDS window of
+// kReplayEdges entries (each entry with its owner position), reloaded by all
+// lanes together.  Per chunk the lanes find the roots and ranks of both ends
+// of their entries at once; the unions the chunk needs are then applied one
+// by one in order from registers (every lane whose root lost takes the
+// winner and its new rank), and each union writes only the loser's parent
+// and the winner's rank to LDS for the chunks after.  The root's raw index
+// orders the clusters.  A span of more than kReplaySpan positions leaves its
+// image to the host.
+constexpr int kReplaySpan = 4096;
+constexpr int kReplayEdges = 8192;
+struct ReplayLds {
+    int32_t off[kReplaySpan + 1];                 // pair-list offsets, relative to the span's first
+    unsigned long long member[kReplaySpan / 64];  // member bits
+    int16_t par[kReplaySpan];                     // union-find parents (span-local)
+    int16_t edge[kReplayEdges];                   // window: partner (span-local; -1 outside)
+    int16_t owner[kReplayEdges];                  // window: the position whose list holds the entry
+    uint8_t rnk[kReplaySpan];                     // union-by-rank ranks
+};
+
+__global__ __launch_bounds__(64) void grp_replay_kernel(
+    const int32_t* __restrict__ imgBase, int nImages, const vj_detection* __restrict__ dets,
+    const int4* __restrict__ sorted, const int32_t* __restrict__ order, const int32_t* __restrict__ label,
+    const GrpCount* __restrict__ gc, const int32_t* __restrict__ keptAll, const int32_t* __restrict__ keptSlot,
+    const int64_t* __restrict__ ctl, int64_t pairCap, const int64_t* __restrict__ pairOff,
+    const int32_t* __restrict__ pairCnt, const int32_t* __restrict__ pairs, int32_t* __restrict__ grpImg,
+    GrpCluster* __restrict__ clusters) {
+    __shared__ ReplayLds S;
+    if (ctl[0] > pairCap) return;
+    const int64_t nKept = ctl[1];
+    const int lane = lane_id();
+    auto find = [&](int x) {
+        for (;;) {
+            const int p = S.par[x];
+            if (p == x) return x;
+            const int gp = S.par[p];
+            if (gp == p) return p;
+            S.par[x] = static_cast<int16_t>(gp);  // path halving: the root is unchanged
+            x = gp;
+        }
+    };
+    for (int64_t k = blockIdx.x; k < nKept; k += gridDim.x) {
+        const int32_t r = keptAll[k];
+        const int i = image_of(imgBase, nImages, r);
+        const int32_t base = imgBase[i];
+        const int32_t last = gc[r].last;
+        const int span = last - r + 1;
+        if (span > kReplaySpan) {
+            if (lane == 0) grpImg[3 * i + 2] = 1;
+            continue;
+        }
+        const int64_t e0 = pairOff[r];
+        const int64_t E = pairOff[last] + pairCnt[last] - e0;
+        unsigned long long sx = 0, sy = 0, sw = 0, sh = 0;
+        int n = 0;
+        uint32_t bestKey = 0;
+        int32_t bestRaw = INT_MAX;
+        for (int j0 = 0; j0 < span; j0 += kWave) {
+            const int j = j0 + lane;
+            const bool in = j < span;
+            const int32_t p = r + j;
+            const bool mem = in && label[p] == r;
+            const uint64_t mb = __ballot(mem);
+            if (lane == 0) S.member[j0 >> 6] = mb;
+            if (in) {
+                S.off[j] = static_cast<int32_t>(pairOff[p] - e0);
+                S.par[j] = static_cast<int16_t>(j);
+                S.rnk[j] = 0;
+            }
+            if (mem) {
+                const int4 B = sorted[p];
+                sx += static_cast<unsigned long long>(B.x);
+                sy += static_cast<unsigned long long>(B.y);
+                sw += static_cast<unsigned long long>(B.z);
+                sh += static_cast<unsigned long long>(B.w);
+                ++n;
+                const int32_t raw = order[p];
+                const uint32_t kk = best_key(dets[base + raw].score);
+                if (kk && (kk > bestKey || (kk == bestKey && raw < bestRaw))) {
+                    bestKey = kk;
+                    bestRaw = raw;
+                }
+            }
+        }
+        if (lane == 0) S.off[span] = static_cast<int32_t>(E);
+        sx = wave_sum(sx);
+        sy = wave_sum(sy);
+        sw = wave_sum(sw);
+        sh = wave_sum(sh);
+        n = wave_sum(n);
+        const uint32_t topKey = wave_max(bestKey);
+        int32_t topRaw = (bestKey == topKey && topKey) ? bestRaw : INT_MAX;
+#pragma unroll
+        for (int d = kWave / 2; d > 0; d >>= 1) topRaw = min(topRaw, __shfl_xor(topRaw, d, kWave));
+
+        // the window of pair entries [winLo, winHi): partners and owners
+        int64_t winLo = 0, winHi = 0;
+        auto load_window = [&](int64_t from) {
+            __syncthreads();  // the staged span is complete; the old window is no longer read
+            winLo = from;
+            winHi = min(E, from + kReplayEdges);
+            for (int64_t t = from + lane; t < winHi; t += kWave) {
+                const int32_t q = base + pairs[e0 + t] - r;
+                S.edge[t - from] = static_cast<int16_t>(q >= 0 && q < span ? q : -1);
+            }
+            int lo = 0, hi = span - 1;  // the last position whose list starts at or before `from`
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (S.off[mid] <= from) lo = mid;
+                else hi = mid - 1;
+            }
+            for (int j = lo + lane; j < span && S.off[j] < winHi; j += kWave) {
+                const int64_t a0 = max(static_cast<int64_t>(S.off[j]), from);
+                const int64_t a1 = min(static_cast<int64_t>(S.off[j + 1]), winHi);
+                for (int64_t t = a0; t < a1; ++t) S.owner[t - from] = static_cast<int16_t>(j);
+            }
+            __syncthreads();
+        };
+        load_window(0);
+        for (int64_t t0 = 0; t0 < E; t0 += kWave) {
+            if (t0 + kWave > winHi && winHi < E) load_window(t0);
+            const int64_t t = t0 + lane;
+            int src = 0, dst = -1;
+            if (t < winHi) {
+                src = S.owner[t - winLo];
+                dst = S.edge[t - winLo];
+            }
+            // (a member's partners are members, so inside the span)
+            const bool valid = dst >= 0 && ((S.member[src >> 6] >> (src & 63)) & 1ull);
+            int rs = 0, rq = 0, ks = 0, kq = 0;
+            if (valid) {
+                rs = find(src);
+                rq = find(dst);
+                ks = S.rnk[rs];
+                kq = S.rnk[rq];
+            }
+            uint64_t nm = __ballot(valid && rs != rq);
+            while (nm) {
+                const int f = __ffsll(static_cast<long long>(nm)) - 1;  // uniform: read the lane directly
+                const int a = __builtin_amdgcn_readlane(rs, f), b = __builtin_amdgcn_readlane(rq, f);
+                const int ra = __builtin_amdgcn_readlane(ks, f), rb = __builtin_amdgcn_readlane(kq, f);
+                const int win = ra < rb ? b : a, lose = ra < rb ? a : b;
+                const int rw = ra == rb ? ra + 1 : max(ra, rb);
+                if (lane == 0) {
+                    S.par[lose] = static_cast<int16_t>(win);
+                    S.rnk[win] = static_cast<uint8_t>(rw);
+                }
+                rs = rs == lose ? win : rs;
+                rq = rq == lose ? win : rq;
+                ks = rs == win ? rw : ks;
+                kq = rq == win ? rw : kq;
+                nm = __ballot(valid && rs != rq);
+            }
+        }
+        const int root = find(0);
+        if (lane == 0) {
+            GrpCluster c;
+            c.x = static_cast<int32_t>(sx / static_cast<unsigned long long>(n));
+            c.y = static_cast<int32_t>(sy / static_cast<unsigned long long>(n));
+            c.w = static_cast<int32_t>(sw / static_cast<unsigned long long>(n));
+            c.h = static_cast<int32_t>(sh / static_cast<unsigned long long>(n));
+            c.n = n;
+            c.rootRaw = order[r + root];
+            if (topRaw != INT_MAX) {
+                const vj_detection& d = dets[base + topRaw];
+                c.scaleIdx = d.scaleIdx;
+                c.score = d.score;
+            } else {
+                c.scaleIdx = 0;
+                c.score = -INFINITY;
+            }
+            clusters[base + keptSlot[k]] = c;
+        }
+        __syncthreads();  // the LDS serves the next component
+    }
+}
+
+// Clusters of one image (one workgroup per image): the kept components in
+// ascending order of their DSU root's raw index, minus those inside a
+// stronger cluster (src/face_vision_engine.cu:1483-1536: r1 is dropped when
+// some other r2 with (int)leftBound <= r2.x <= (int)rightBound contains it
+// within (int)(r2.w * eps), (int)(r2.h * eps) and n2 > max(3, n1) || n1 < 3).
+// One wavefront per cluster compares it with 64 others at a time.  More
+// than kFinishMax clusters leave the image to the host.  The clusters of
+// image i are written from position sum_{j<i} (kept components of image j)
+// of the output, so one copy of that prefix brings every image's.
+constexpr int kFinishMax = 2048;
+constexpr int kFinishThreads = 1024;
+
+__global__ __launch_bounds__(kFinishThreads) void grp_finish_kernel(
+    const int32_t* __restrict__ counts, const int32_t* __restrict__ imgBase, int cap, float eps,
+    const GrpCluster* __restrict__ clusters, const int64_t* __restrict__ ctl, int64_t pairCap,
+    int32_t* __restrict__ grpImg, vj_detection* __restrict__ grouped) {
+    __shared__ int32_t X[kFinishMax], Y[kFinishMax], W[kFinishMax], H[kFinishMax], N[kFinishMax], R[kFinishMax];
+    __shared__ int32_t byRank[kFinishMax];
+    __shared__ uint8_t keepFlag[kFinishMax];
+    __shared__ int32_t waveTot[kFinishThreads / kWave];
+    __shared__ int32_t sMaxW;
+    const int i = blockIdx.x;
+    const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+    if (ctl[0] > pairCap || grouped_boxes(counts, i, cap) == 0 || grpImg[3 * i + 2]) return;
+    const int K = grpImg[3 * i];
+    if (K > kFinishMax) {
+        if (tid == 0) grpImg[3 * i + 2] = 1;
+        return;
+    }
+    const int32_t base = imgBase[i];
+    int64_t outBase = 0;
+    for (int j = 0; j < i; ++j) outBase += grpImg[3 * j];
+    if (tid == 0) sMaxW = 0;
+    __syncthreads();
+    int mw = 0;
+    for (int c = tid; c < K; c += kFinishThreads) {
+        const GrpCluster g = clusters[base + c];
+        X[c] = g.x;
+        Y[c] = g.y;
+        W[c] = g.w;
+        H[c] = g.h;
+        N[c] = g.n;
+        R[c] = g.rootRaw;
+        mw = max(mw, g.w);
+    }
+    mw = wave_max(mw);
+    if (lane == 0) atomicMax(&sMaxW, mw);
+    __syncthreads();
+    const float maxW = static_cast<float>(sMaxW);
+    for (int c = wave; c < K; c += kFinishThreads / kWave) {
+        const int x1 = X[c], y1 = Y[c], w1 = W[c], h1 = H[c], n1 = N[c], root1 = R[c];
+        const int lo = static_cast<int>(static_cast<float>(x1 + w1) - (1.0f + eps) * maxW);
+        const int hi = static_cast<int>(static_cast<float>(x1) + eps * maxW);
+        int rank = 0;
+        bool inside = false;
+        for (int o = lane; o < K; o += kWave) {
+            rank += R[o] < root1;
+            const int x2 = X[o];
+            if (o == c || x2 < lo || x2 > hi) continue;
+            const int w2 = W[o], h2 = H[o], y2 = Y[o], n2 = N[o];
+            const int dx = static_cast<int>(static_cast<float>(w2) * eps);
+            const int dy = static_cast<int>(static_cast<float>(h2) * eps);
+            inside = inside || (x1 >= x2 - dx && y1 >= y2 - dy && x1 + w1 <= x2 + w2 + dx &&
+                                y1 + h1 <= y2 + h2 + dy && (n2 > max(3, n1) || n1 < 3));
+        }
+        rank = wave_sum(rank);
+        const bool drop = __ballot(inside) != 0;
+        if (lane == 0) {
+            byRank[rank] = c;
+            keepFlag[rank] = drop ? 0 : 1;
+        }
+    }
+    __syncthreads();
+    // survivors in rank order: an exclusive scan of the flags, two per thread
+    const int r0 = 2 * tid, r1 = r0 + 1;
+    const int f0 = r0 < K ? keepFlag[r0] : 0, f1 = r1 < K ? keepFlag[r1] : 0;
+    const int t = f0 + f1;
+    const int inc = wave_incl_scan(t);
+    if (lane == kWave - 1) waveTot[wave] = inc;
+    __syncthreads();
+    int off = inc - t;
+    for (int w = 0; w < wave; ++w) off += waveTot[w];
+    auto put = [&](int rk, int pos) {
+        const int c = byRank[rk];
+        const GrpCluster& g = clusters[base + c];
+        grouped[outBase + pos] = vj_detection{X[c], Y[c], W[c], H[c], g.scaleIdx, g.score};
+    };
+    if (f0) put(r0, off);
+    if (f1) put(r1, off + f0);
+    if (tid == kFinishThreads - 1) grpImg[3 * i + 1] = off + t;
 }
 
 struct vj_engine {
@@ -1292,16 +1712,24 @@ struct vj_engine {
     DevBuf<vj_detection> dOut;
     PinnedBuf<vj_detection> hOut;
     PinnedBuf<int32_t> hCounts;
-    // grouping help: per raw box its sweep rank, pair count, pair-list offset
-    // and a cursor; per image the boxes in sweep order; the pair lists
-    DevBuf<int32_t> gPairCnt, gPairOff, gOrder, gPairs;
+    // device grouping: per raw box its pair count and pair-list offset, the
+    // boxes in sweep order and their raw indices, the pair lists, the
+    // union-find labels (then components) and per-component sums; the kept
+    // components per image and over all images; per image its kept count,
+    // output count and host flag; the clusters of all images
+    DevBuf<int32_t> gPairCnt, gOrder, gPairs, gLabel, gKeptAll, gKeptSlot, gGrpImg;
+    DevBuf<int64_t> gPairOff;
+    DevBuf<GrpCount> gCount;
+    DevBuf<GrpCluster> gClusters;
+    DevBuf<vj_detection> gOutG;
+    PinnedBuf<int32_t> hGrpImg;
+    PinnedBuf<vj_detection> hOutG;
     // sweep order: per image its width (its x bins), the boxes listed by x
     DevBuf<int32_t> gImgW, gBucket;
     PinnedBuf<int32_t> hImgW;
     DevBuf<int4> gSorted;
     DevBuf<int64_t> gCtl;
     DevBuf<int64_t> gScanTot;  // pair-count scan: per-block totals
-    PinnedBuf<int32_t> hOrder, hPairOff, hPairCnt, hPairs;
     PinnedBuf<int64_t> hCtl;
     int64_t pairCap = 0;  // capacity of gPairs (grows after an overflow)
     PinnedBuf<uint8_t> hStage;  // staging for plan uploads
@@ -2236,10 +2664,12 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
     VJ_TRY(build_plan(e, imgs, sf, minObj, P));
     const int nImages = static_cast<int>(imgs.size());
     VJ_TRY(run_pipeline(e, P, capPer, s));
-    // grouping help on the device: sweep ranks and similar pairs per image
-    const bool devGroup = applyGrouping != 0;
+    // grouping on the device, every image of the call in the same launches:
+    // sweep order, similar pairs (and their unions), component statistics,
+    // the reference's DSU roots of the kept components, the clusters
+    const size_t capTotal = static_cast<size_t>(capPer) * static_cast<size_t>(nImages);
+    const bool devGroup = applyGrouping != 0 && capTotal < (size_t{1} << 31);
     if (devGroup) {
-        const size_t capTotal = static_cast<size_t>(capPer) * static_cast<size_t>(nImages);
         if (e->pairCap == 0) e->pairCap = static_cast<int64_t>(std::min<size_t>(capTotal, size_t{1} << 20) * 16 + 65536);
         // image widths: one x bin per column 0 .. w of an image (LDS of the sweep)
         int maxW = 1;
@@ -2256,14 +2686,22 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
         VJ_TRY(e->gSorted.ensure(capTotal));
         VJ_TRY(e->gOrder.ensure(capTotal));
         VJ_TRY(e->gPairs.ensure(static_cast<size_t>(e->pairCap)));
-        VJ_TRY(e->gCtl.ensure(1));
+        VJ_TRY(e->gLabel.ensure(capTotal));
+        VJ_TRY(e->gCount.ensure(capTotal));
+        VJ_TRY(e->gClusters.ensure(capTotal));
+        VJ_TRY(e->gKeptAll.ensure(capTotal));
+        VJ_TRY(e->gKeptSlot.ensure(capTotal));
+        VJ_TRY(e->gOutG.ensure(capTotal));
+        VJ_TRY(e->gGrpImg.ensure(3 * static_cast<size_t>(nImages)));
+        VJ_TRY(e->gCtl.ensure(2));
         VJ_TRY(e->gScanTot.ensure((capTotal + kScanItems - 1) / kScanItems));
         hipLaunchKernelGGL(pair_sweep_kernel, dim3(nImages), dim3(kSweepThreads), sweep_lds_bytes(maxW), s,
                            e->dOut.get(), e->dCounts.get(), e->dImgBase.get(), e->gImgW.get(), capPer,
-                           e->gBucket.get(), e->gSorted.get(), e->gOrder.get());
+                           e->gBucket.get(), e->gSorted.get(), e->gOrder.get(), e->gGrpImg.get(), e->gCtl.get());
         VJ_HIP(hipGetLastError());
         hipLaunchKernelGGL(pair_find_kernel<false>, dim3(2048), dim3(256), 0, s, e->dCounts.get(), e->dImgBase.get(),
-                           nImages, capPer, 0.2f, e->gSorted.get(), e->gPairCnt.get(), nullptr, nullptr, int64_t{0});
+                           nImages, capPer, 0.2f, e->gSorted.get(), e->gPairCnt.get(), nullptr, nullptr, int64_t{0},
+                           e->gLabel.get(), e->gCount.get());
         VJ_HIP(hipGetLastError());
         const int scanGrid = static_cast<int>(std::min<size_t>((capTotal + kScanItems - 1) / kScanItems, 1024));
         hipLaunchKernelGGL(pair_scan_local_kernel, dim3(scanGrid), dim3(kScanThreads), 0, s, e->dCounts.get(),
@@ -2274,10 +2712,29 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
         VJ_HIP(hipGetLastError());
         hipLaunchKernelGGL(pair_find_kernel<true>, dim3(2048), dim3(256), 0, s, e->dCounts.get(), e->dImgBase.get(),
                            nImages, capPer, 0.2f, e->gSorted.get(), e->gPairCnt.get(), e->gPairOff.get(),
-                           e->gPairs.get(), e->pairCap);
+                           e->gPairs.get(), e->pairCap, e->gLabel.get(), nullptr);
         VJ_HIP(hipGetLastError());
-        VJ_TRY(e->hCtl.ensure(1));
-        VJ_HIP(hipMemcpyAsync(e->hCtl.get(), e->gCtl.get(), sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(grp_comp_kernel, dim3(1024), dim3(256), 0, s, e->dCounts.get(), e->dImgBase.get(), nImages,
+                           capPer, e->gLabel.get(), e->gCount.get(), e->gCtl.get(), e->pairCap);
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(grp_list_kernel, dim3(1024), dim3(256), 0, s, e->dCounts.get(), e->dImgBase.get(), nImages,
+                           capPer, minNeighbors, e->gLabel.get(), e->gCount.get(), e->gKeptAll.get(),
+                           e->gKeptSlot.get(), e->gGrpImg.get(), e->gCtl.get(), e->pairCap);
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(grp_replay_kernel, dim3(2048), dim3(kWave), 0, s, e->dImgBase.get(), nImages, e->dOut.get(),
+                           e->gSorted.get(), e->gOrder.get(), e->gLabel.get(), e->gCount.get(), e->gKeptAll.get(),
+                           e->gKeptSlot.get(), e->gCtl.get(), e->pairCap, e->gPairOff.get(), e->gPairCnt.get(),
+                           e->gPairs.get(), e->gGrpImg.get(), e->gClusters.get());
+        VJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(grp_finish_kernel, dim3(nImages), dim3(kFinishThreads), 0, s, e->dCounts.get(),
+                           e->dImgBase.get(), capPer, 0.2f, e->gClusters.get(), e->gCtl.get(), e->pairCap,
+                           e->gGrpImg.get(), e->gOutG.get());
+        VJ_HIP(hipGetLastError());
+        VJ_TRY(e->hCtl.ensure(2));
+        VJ_TRY(e->hGrpImg.ensure(3 * static_cast<size_t>(nImages)));
+        VJ_HIP(hipMemcpyAsync(e->hCtl.get(), e->gCtl.get(), 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        VJ_HIP(hipMemcpyAsync(e->hGrpImg.get(), e->gGrpImg.get(), 3 * sizeof(int32_t) * nImages,
+                              hipMemcpyDeviceToHost, s));
     }
     VJ_TRY(e->hCounts.ensure(static_cast<size_t>(nImages)));
     VJ_HIP(hipMemcpyAsync(e->hCounts.get(), e->dCounts.get(), sizeof(int32_t) * nImages, hipMemcpyDeviceToHost, s));
@@ -2290,37 +2747,6 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
         raw += e->hCounts.get()[i];
         total += std::min(e->hCounts.get()[i], capPer);
     }
-    VJ_TRY(e->hOut.ensure(static_cast<size_t>(std::max<int64_t>(total, 1))));
-    // the device pairs are used when they all fit; otherwise the host finds
-    // them, and the next call has room
-    bool usePairs = false;
-    int64_t nPairs = 0;
-    if (devGroup && total > 0) {
-        nPairs = e->hCtl.get()[0];
-        if (nPairs <= e->pairCap) {
-            usePairs = true;
-        } else {
-            e->pairCap = nPairs + nPairs / 4 + 65536;
-        }
-    }
-    if (total > 0) {
-        VJ_HIP(hipMemcpyAsync(e->hOut.get(), e->dOut.get(), sizeof(vj_detection) * total, hipMemcpyDeviceToHost, s));
-        if (usePairs) {
-            VJ_TRY(e->hOrder.ensure(static_cast<size_t>(total)));
-            VJ_TRY(e->hPairOff.ensure(static_cast<size_t>(total)));
-            VJ_TRY(e->hPairCnt.ensure(static_cast<size_t>(total)));
-            VJ_TRY(e->hPairs.ensure(static_cast<size_t>(std::max<int64_t>(nPairs, 1))));
-            VJ_HIP(hipMemcpyAsync(e->hOrder.get(), e->gOrder.get(), sizeof(int32_t) * total, hipMemcpyDeviceToHost, s));
-            VJ_HIP(hipMemcpyAsync(e->hPairOff.get(), e->gPairOff.get(), sizeof(int32_t) * total, hipMemcpyDeviceToHost,
-                                  s));
-            VJ_HIP(hipMemcpyAsync(e->hPairCnt.get(), e->gPairCnt.get(), sizeof(int32_t) * total, hipMemcpyDeviceToHost,
-                                  s));
-            if (nPairs > 0)
-                VJ_HIP(hipMemcpyAsync(e->hPairs.get(), e->gPairs.get(), sizeof(int32_t) * nPairs,
-                                      hipMemcpyDeviceToHost, s));
-        }
-        VJ_HIP(hipStreamSynchronize(s));
-    }
     e->stats.levels = 0;
     for (int32_t n : P.imgLevels) e->stats.levels += n;
     e->stats.windows = P.windows;
@@ -2328,48 +2754,88 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
     e->stats.pyramidPixels = P.pyramidPixels;
     e->stats.integralBytes = P.integralBytes;
     e->stats.cascadeBytes = P.cascadeBytes + static_cast<int64_t>(e->nStumps) * 64 + 24 * total;
-
-    // per image: optional grouping (every image's steps spread over the host
-    // thread pool, largest images first), then copy out in image order
     std::vector<int64_t> rawOff(static_cast<size_t>(nImages) + 1, 0);
     for (int i = 0; i < nImages; ++i)
         rawOff[static_cast<size_t>(i) + 1] = rawOff[static_cast<size_t>(i)] + std::min(e->hCounts.get()[i], capPer);
-    std::vector<std::vector<vj_detection>> grouped(applyGrouping ? static_cast<size_t>(nImages) : 0);
+    auto rawCount = [&](int i) { return static_cast<int>(rawOff[static_cast<size_t>(i) + 1] - rawOff[static_cast<size_t>(i)]); };
+    VJ_TRY(e->hOut.ensure(static_cast<size_t>(std::max<int64_t>(total, 1))));
+
+    // per image: where its result comes from.  Grouped on the device unless
+    // the pair buffer overflowed (it grows for the next call), the image has
+    // more boxes than the device sweep takes, or a component or the cluster
+    // list passed the device limits; those images are grouped on the host
+    // (every image's steps spread over the host thread pool).
+    std::vector<char> onDev(static_cast<size_t>(nImages), 0);
+    std::vector<int64_t> devOff(static_cast<size_t>(nImages), 0);
+    std::vector<int> hostImgs;
+    int64_t devTotal = 0;
     if (applyGrouping) {
-        std::vector<const vj_detection*> ins(static_cast<size_t>(nImages));
-        std::vector<vj_detection*> outs(static_cast<size_t>(nImages));
-        std::vector<int> ns(static_cast<size_t>(nImages)), cnt(static_cast<size_t>(nImages));
-        for (int i = 0; i < nImages; ++i) {
-            const size_t k = static_cast<size_t>(i);
-            ns[k] = static_cast<int>(rawOff[k + 1] - rawOff[k]);
-            grouped[k].resize(static_cast<size_t>(std::max(ns[k], 1)));
-            ins[k] = e->hOut.get() + rawOff[k];
-            outs[k] = grouped[k].data();
+        bool usable = false;
+        if (devGroup) {
+            int64_t nextCap = e->pairCap;
+            usable = pair_plan(e->hCtl.get()[0], e->pairCap, &nextCap);
+            e->pairCap = nextCap;
         }
-        std::vector<GroupRanked> ranked(static_cast<size_t>(nImages));
-        if (usePairs) {
-            for (int i = 0; i < nImages; ++i) {
-                const size_t k = static_cast<size_t>(i);
-                if (ns[k] == 0 || ns[k] > kGroupDevMax) continue;
-                ranked[k].order = e->hOrder.get() + rawOff[k];
-                ranked[k].pairOff = e->hPairOff.get() + rawOff[k];
-                ranked[k].pairCnt = e->hPairCnt.get() + rawOff[k];
-                ranked[k].pairs = e->hPairs.get();
+        for (int i = 0; i < nImages; ++i) {
+            const int n = rawCount(i);
+            devOff[static_cast<size_t>(i)] = devTotal;
+            if (devGroup) devTotal += e->hGrpImg.get()[3 * i];
+            if (n == 0) continue;
+            if (usable && n <= kGroupDevMax && !e->hGrpImg.get()[3 * i + 2]) onDev[static_cast<size_t>(i)] = 1;
+            else hostImgs.push_back(i);
+        }
+        if (!hostImgs.empty()) {
+            if (static_cast<int>(hostImgs.size()) == nImages) {
+                VJ_HIP(hipMemcpyAsync(e->hOut.get(), e->dOut.get(), sizeof(vj_detection) * total, hipMemcpyDeviceToHost, s));
+            } else {
+                for (int i : hostImgs)
+                    VJ_HIP(hipMemcpyAsync(e->hOut.get() + rawOff[static_cast<size_t>(i)],
+                                          e->dOut.get() + rawOff[static_cast<size_t>(i)],
+                                          sizeof(vj_detection) * rawCount(i), hipMemcpyDeviceToHost, s));
             }
         }
-        VJ_CHECK(group_rectangles_many(ins.data(), ns.data(), nImages, minNeighbors, 0.2f, outs.data(), cnt.data(),
-                                       ranked.data()) == VJ_OK,
+        const bool anyDev = std::find(onDev.begin(), onDev.end(), 1) != onDev.end();
+        if (anyDev && devTotal > 0) {
+            VJ_TRY(e->hOutG.ensure(static_cast<size_t>(devTotal)));
+            VJ_HIP(hipMemcpyAsync(e->hOutG.get(), e->gOutG.get(), sizeof(vj_detection) * devTotal,
+                                  hipMemcpyDeviceToHost, s));
+        }
+        VJ_HIP(hipStreamSynchronize(s));
+    } else if (total > 0) {
+        VJ_HIP(hipMemcpyAsync(e->hOut.get(), e->dOut.get(), sizeof(vj_detection) * total, hipMemcpyDeviceToHost, s));
+        VJ_HIP(hipStreamSynchronize(s));
+    }
+    e->stats.imagesGroupedOnHost = static_cast<int32_t>(hostImgs.size());
+    std::vector<std::vector<vj_detection>> grouped(static_cast<size_t>(nImages));
+    if (!hostImgs.empty()) {
+        const size_t m = hostImgs.size();
+        std::vector<const vj_detection*> ins(m);
+        std::vector<vj_detection*> outs(m);
+        std::vector<int> ns(m), cnt(m);
+        for (size_t k = 0; k < m; ++k) {
+            const int i = hostImgs[k];
+            ns[k] = rawCount(i);
+            grouped[static_cast<size_t>(i)].resize(static_cast<size_t>(ns[k]));
+            ins[k] = e->hOut.get() + rawOff[static_cast<size_t>(i)];
+            outs[k] = grouped[static_cast<size_t>(i)].data();
+        }
+        VJ_CHECK(group_rectangles_many(ins.data(), ns.data(), static_cast<int>(m), minNeighbors, 0.2f, outs.data(),
+                                       cnt.data()) == VJ_OK,
                  "groupRectangles failed (host memory)");
-        for (int i = 0; i < nImages; ++i) grouped[static_cast<size_t>(i)].resize(static_cast<size_t>(cnt[static_cast<size_t>(i)]));
+        for (size_t k = 0; k < m; ++k) grouped[static_cast<size_t>(hostImgs[k])].resize(static_cast<size_t>(cnt[k]));
     }
     int64_t outPos = 0;
     for (int i = 0; i < nImages; ++i) {
-        const int n = static_cast<int>(rawOff[static_cast<size_t>(i) + 1] - rawOff[static_cast<size_t>(i)]);
         const vj_detection* src = e->hOut.get() + rawOff[static_cast<size_t>(i)];
-        int m = n;
+        int m = rawCount(i);
         if (applyGrouping) {
-            src = grouped[static_cast<size_t>(i)].data();
-            m = static_cast<int>(grouped[static_cast<size_t>(i)].size());
+            if (onDev[static_cast<size_t>(i)]) {
+                m = e->hGrpImg.get()[3 * i + 1];
+                src = m > 0 ? e->hOutG.get() + devOff[static_cast<size_t>(i)] : nullptr;
+            } else {
+                src = grouped[static_cast<size_t>(i)].data();
+                m = static_cast<int>(grouped[static_cast<size_t>(i)].size());
+            }
         }
         if (hCounts) hCounts[i] = m;
         if (batch) {

@@ -403,17 +403,6 @@ void sweep_order(const vj_detection* d, int n, GroupScratch& g) {
     for (int k = 0; k < n; ++k) g.idx[static_cast<size_t>(k)] = static_cast<int>(g.key[static_cast<size_t>(k)] & 0xffffffffu);
 }
 
-// Union-find reset for n boxes (the sweep order comes from elsewhere).
-void group_reset(GroupScratch& g, const vj_detection* in, int n, float eps) {
-    g.in = in;
-    g.n = n;
-    g.eps = eps;
-    g.parent.resize(static_cast<size_t>(n));
-    g.rank.assign(static_cast<size_t>(n), 0);
-    std::iota(g.parent.begin(), g.parent.end(), 0);
-    g.nBlocks = 0;
-}
-
 // Grouping in three steps: prepare (sweep order, the boxes' edges in that
 // order as flat arrays, union-find reset), the pair search in blocks of
 // sweep positions (independent, any thread), and finish (the unions, the
@@ -582,21 +571,6 @@ int group_finish(GroupScratch& g, int minNeighbors, vj_detection* out, int* coun
     return group_clusters(g, minNeighbors, out, count);
 }
 
-// The same unions from a sweep order and pair lists computed elsewhere:
-// order[r] is the box of sweep rank r; its pairs are the sweep ranks
-// pairs[pairOff[r] .. pairOff[r] + pairCnt[r]), ascending.
-int group_finish_ranked(GroupScratch& g, const GroupRanked& rk, int minNeighbors, vj_detection* out, int* count) {
-    UnionFind uf{g.parent.data(), g.rank.data()};
-    for (int r = 0; r < g.n; ++r) {
-        const int a = rk.order[r];
-        const int32_t* L = rk.pairs + rk.pairOff[r];
-        const int m = rk.pairCnt[r];
-        int ra = -1;
-        for (int k = 0; k < m; ++k) ra = uf.join_root(ra < 0 ? uf.root(a) : ra, rk.order[L[k]]);
-    }
-    return group_clusters(g, minNeighbors, out, count);
-}
-
 }  // namespace
 
 int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps, vj_detection* out,
@@ -625,7 +599,7 @@ int group_rectangles(const vj_detection* in, int n, int minNeighbors, float eps,
 }
 
 int group_rectangles_many(const vj_detection* const* in, const int* n, int nImages, int minNeighbors, float eps,
-                          vj_detection* const* out, int* count, const GroupRanked* ranked) {
+                          vj_detection* const* out, int* count) {
     // every image's steps on the host pool: prepares, then all pair-search
     // blocks of all images, then finishes, largest images first
     // (a named reference, so that the pool's workers reach this thread's
@@ -646,10 +620,7 @@ int group_rectangles_many(const vj_detection* const* in, const int* n, int nImag
     pool.run(m, [&](int k) {
         const int i = order[static_cast<size_t>(k)];
         try {
-            if (ranked && ranked[i].order)
-                group_reset(scratch[static_cast<size_t>(i)], in[i], n[i], eps);
-            else
-                group_prepare(scratch[static_cast<size_t>(i)], in[i], n[i], eps);
+            group_prepare(scratch[static_cast<size_t>(i)], in[i], n[i], eps);
         } catch (const std::exception&) {
             failed[static_cast<size_t>(k)] = 1;
         }
@@ -675,9 +646,7 @@ int group_rectangles_many(const vj_detection* const* in, const int* n, int nImag
         const int i = order[static_cast<size_t>(k)];
         try {
             GroupScratch& g = scratch[static_cast<size_t>(i)];
-            status[static_cast<size_t>(i)] = (ranked && ranked[i].order)
-                                                 ? group_finish_ranked(g, ranked[i], minNeighbors, out[i], &count[i])
-                                                 : group_finish(g, minNeighbors, out[i], &count[i]);
+            status[static_cast<size_t>(i)] = group_finish(g, minNeighbors, out[i], &count[i]);
         } catch (const std::exception&) {
             status[static_cast<size_t>(i)] = VJ_OUT_OF_MEMORY;
         }
@@ -695,7 +664,7 @@ extern "C" int vj_group_rectangles_many(const vj_detection* const* in, const int
     for (int i = 0; i < nImages; ++i)
         VJ_CHECK(n[i] >= 0 && (n[i] == 0 || (in[i] && out[i])), "vj_group_rectangles_many: bad list");
     try {
-        return vj::group_rectangles_many(in, n, nImages, minNeighbors, eps, out, count, nullptr);
+        return vj::group_rectangles_many(in, n, nImages, minNeighbors, eps, out, count);
     } catch (const std::exception& ex) {
         vj::set_error(std::string("vj_group_rectangles_many: ") + ex.what());
         return VJ_OUT_OF_MEMORY;

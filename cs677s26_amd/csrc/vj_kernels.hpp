@@ -117,6 +117,16 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        const T o = __shfl_xor(v, d, kWave);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
 // Index of the last element of sorted offsets[0..n) that is <= key.
 __device__ __forceinline__ int find_segment(const int32_t* offsets, int n, int key) {
     int lo = 0, hi = n - 1;

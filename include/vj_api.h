@@ -185,6 +185,8 @@ typedef struct vj_detect_stats {
     int64_t pyramidPixels;   /* sum of level W*H                          */
     int64_t integralBytes;   /* algorithmic bytes of the integral kernels */
     int64_t cascadeBytes;    /* algorithmic bytes of the cascade kernel   */
+    int32_t imagesGroupedOnHost; /* images whose grouping fell back to the
+                                    host (pair buffer overflow, device limits) */
 } vj_detect_stats;
 /* Statistics of the engine's last detect (no reference counterpart). */
 int vj_engine_last_stats(vj_engine* e, vj_detect_stats* out);

@@ -85,6 +85,7 @@ def test_detect_config1_raw_and_grouped(cascade3):
     assert det_tuples(raw) == det_tuples(want)
     grouped = eng.detectMultiScale(d, 640, 480, 640, 1.25, 4, 24, 200000, applyGrouping=True)
     assert det_tuples(grouped) == det_tuples(O.group(want, 4))
+    assert eng.last_stats()["imagesGroupedOnHost"] == 0  # grouped on the device
     st = eng.last_stats()
     assert st["windows"] == 653485 and st["levels"] == 14
 
@@ -167,6 +168,7 @@ def test_detect_config3_golden():
     assert eng.last_stats()["windows"] == int(g["windows"]) == 4729998
     grouped = eng.detectMultiScale(d, 1920, 1080, 1920, 1.25, 4, 24, 200000, applyGrouping=True)
     assert det_tuples(grouped) == det_tuples(g["grouped"])
+    assert eng.last_stats()["imagesGroupedOnHost"] == 0  # grouped on the device
 
 
 def test_graph_replays_and_autotune_match_eager(cascade3):
@@ -259,12 +261,12 @@ def test_pair_mode_margins_near_threshold(cascade7, offset):
 
 
 def test_grouping_device_pairs_capacity_and_host_images():
-    """Grouping with the device's sweep order and pairs: a dense 16,000-box
-    image (about 40 pairs per box) overflows the first call's pair capacity,
-    so that call groups on the host and the next one on the device pairs
-    (the capacity has grown); a 20,000-box image then groups on the device
-    pairs from its first call.  Every result equals the oracle's grouping of
-    the oracle's raw boxes."""
+    """Grouping of dense images: a 16,000-box image (about 40 pairs per box)
+    overflows the first call's pair capacity, so that call groups on the host
+    and the capacity grows; the next calls, and a 20,000-box image, group on
+    the device unless a component passes the device limits (then on the
+    host).  Every result equals the oracle's grouping of the oracle's raw
+    boxes."""
     feats, stumps, stages = synth.blob_cascade((1,), seed=5, n_sample=200)
     stages = stages.copy()
     stages["threshold"] = np.float32(-1e30)
@@ -279,6 +281,7 @@ def test_grouping_device_pairs_capacity_and_host_images():
         for rep in range(2):
             got = eng.detectMultiScale(d, 640, 480, 640, 1.25, 4, 24, max_det, applyGrouping=True)
             assert det_tuples(got) == want, (max_det, rep)
+            say(f"    call {rep}: images grouped on the host {eng.last_stats()['imagesGroupedOnHost']}")
 
 
 def test_grouping_widest_images_and_empty_ones():

@@ -184,6 +184,43 @@ def test_facade_compiles_and_constructors_throw_without_a_device():
     assert usage.returncode == 2 and "usage" in usage.stderr
 
 
+def test_pair_capacity_rule(tmp_path):
+    """The device grouping's pair-buffer rule (csrc/vj_pair_plan.hpp): the
+    device result is used only when every pair was stored; an overflow sends
+    the call to the host and grows the buffer for the next call by a quarter
+    plus 64K entries, never past the 2^30-entry cap (beyond it such calls
+    stay on the host) and never shrinking; a pair count of 2^31 or more
+    (which 32-bit offsets could not address) is an overflow like any other."""
+    src = tmp_path / "plan.cpp"
+    src.write_text(r"""
+#include <cstdio>
+#include "vj_pair_plan.hpp"
+int main() {
+    using vj::pair_plan;
+    int64_t next = 0;
+    const struct { int64_t n, cap; int use; int64_t next; } cases[] = {
+        {0, 100, 1, 100}, {100, 100, 1, 100}, {101, 100, 0, 101 + 25 + 65536},
+        {1000000, 2000000, 1, 2000000}, {int64_t{1} << 31, 1000, 0, vj::kPairCapMax},
+        {int64_t{3} << 30, vj::kPairCapMax, 0, vj::kPairCapMax}, {int64_t{1} << 29, int64_t{1} << 29, 1, int64_t{1} << 29},
+    };
+    for (const auto& c : cases) {
+        const int use = pair_plan(c.n, c.cap, &next) ? 1 : 0;
+        if (use != c.use || next != c.next) {
+            std::printf("fail n=%lld cap=%lld use=%d next=%lld\n", (long long)c.n, (long long)c.cap, use, (long long)next);
+            return 1;
+        }
+    }
+    std::printf("ok\n");
+    return 0;
+}
+""")
+    exe = tmp_path / "plan"
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "cs677s26_amd", "csrc"), str(src), "-o",
+                           str(exe)])
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
+
+
 def test_cli_usage_and_errors(tmp_path):
     exe = os.path.join(ROOT, "cs677s26_amd", "bin", "cuda_hello")
     r = subprocess.run([exe], capture_output=True, text=True)
