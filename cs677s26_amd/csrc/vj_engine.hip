@@ -376,6 +376,17 @@ constexpr int kDeepSlices = kTileMaxWin / kDeepSlice;
 // persistent grid (eight workgroups on each of 256 CUs).
 constexpr int kSplitStumps = 128;
 constexpr int kDeepGrid = 2048;
+// The deep pass stops once it has run about kDeepStumps more stumps; its
+// survivors go on to a tail pass in slices of kTailSlice windows, so a stage
+// of the tail runs 256 / kTailSlice stumps of each window in parallel.  By
+// then few windows are left, and slices of kDeepSlice would leave a few
+// workgroups with long sequential stage sums after all others are done.
+constexpr int kDeepStumps = 1600;
+constexpr int kTailSlice = 8;
+constexpr int kTailPerDeep = kDeepSlice / kTailSlice;
+// Floats of LDS the deep and tail passes keep for pair-mode leaf values
+// (after the staged footprint): eight windows of a 181-stump stage fit.
+constexpr int kLeafCap = 2048;
 
 // LDS of one cascade workgroup besides the staged footprint.
 struct CascadeSmem {
@@ -384,6 +395,7 @@ struct CascadeSmem {
     float red[kCascadeBlock];                    // pair mode: partial stage sums
     int listCnt[2];
     int item;                                    // deep pass: current work item
+    int tailBase;                                // deep pass: first tail item of the slice
     unsigned long long maskWords[32];            // final bits per tile row
 };
 
@@ -506,10 +518,13 @@ __device__ __forceinline__ void gate_windows(const TileGeo& g, const uint32_t* _
 // first reads the entries it needs into registers, then (after a barrier)
 // appends its survivors; the two counters alternate between "read" and
 // "append".  Returns the counter slot holding the survivors of stage si1-1.
+// leaf (LDS, leafCap floats, or null) holds a pair-mode stage's leaf values
+// when they fit, so a sequential stage sum adds them up again instead of
+// re-evaluating the stumps.
 // Stage rule and window normaliser: src/main.cu:752-804.
 __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, const KStump* __restrict__ stumps,
                           const KStage* __restrict__ stages, int si0, int si1, int nStages, int winW,
-                          float* __restrict__ scores) {
+                          float* __restrict__ scores, float* leaf = nullptr, int leafCap = 0) {
     const int lane = lane_id();
     const int FW = (g.TW - 1) * g.step + winW + 1;
     auto settle = [&](bool pass, bool last, uint32_t widx, float stageSum, int nxt) {
@@ -594,7 +609,9 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
             // sum), so S < lo = thr - B fails and S > hi = thr + B passes
             // exactly as the sequential sum would.  Windows in between, and
             // every window the cascade's last stage may pass (its score is
-            // the sequential sum), are summed again sequentially.
+            // the sequential sum), are summed again sequentially: from the
+            // leaf values kept in LDS (row w, ntp entries per row) when they
+            // fit, else by evaluating the stumps again.
             int lgw = 0;
             while ((1 << lgw) < n) ++lgw;
             const int nwp = 1 << lgw;
@@ -602,6 +619,8 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
             const int w = tid & (nwp - 1);
             const int js0 = tid >> lgw;
             const bool evalLive = w < n;
+            const int ntp = (nt + 3) & ~3;
+            float* const leafRow = (leaf && n * ntp <= leafCap) ? leaf + w * ntp : nullptr;
             const uint32_t ew = evalLive ? sm.listIdx[w] : 0u;
             const bool mine = tid < n;
             const uint32_t myIdx = mine ? sm.listIdx[tid] : 0u;
@@ -617,7 +636,9 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
                     const int js = __builtin_amdgcn_readfirstlane(jb + js0);
                     if (js < nt && evalLive) {
                         const KStump sp = sg[js];
-                        part += (haar_raw(o, sp) * inv < sp.theta) ? sp.left : sp.right;
+                        const float v = (haar_raw(o, sp) * inv < sp.theta) ? sp.left : sp.right;
+                        part += v;
+                        if (leafRow) leafRow[js] = v;
                     }
                 }
             } else if (nt > 0) {
@@ -630,10 +651,18 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
                 for (int jb = 0; jb < nt; jb += 2 * spp) {
                     const int jsA = jb + js0, jsB = jsA + spp;
                     const KStump recB = sg[min(jsB, last)];
-                    if (evalLive && jsA < nt) part += (haar_raw(o, recA) * inv < recA.theta) ? recA.left : recA.right;
+                    if (evalLive && jsA < nt) {
+                        const float v = (haar_raw(o, recA) * inv < recA.theta) ? recA.left : recA.right;
+                        part += v;
+                        if (leafRow) leafRow[jsA] = v;
+                    }
                     if (jb + spp >= nt) break;
                     recA = sg[min(jsB + spp, last)];
-                    if (evalLive && jsB < nt) part += (haar_raw(o, recB) * inv < recB.theta) ? recB.left : recB.right;
+                    if (evalLive && jsB < nt) {
+                        const float v = (haar_raw(o, recB) * inv < recB.theta) ? recB.left : recB.right;
+                        part += v;
+                        if (leafRow) leafRow[jsB] = v;
+                    }
                 }
             }
             // lanes l and l ^ d (d >= nwp) of a wave hold the same window
@@ -651,6 +680,19 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
                     pass = false;
                 } else if (S > st.hi && !last) {
                     pass = true;
+                } else if (leafRow) {
+                    // leaf values of window tid, in stump order
+                    const float4* lr = reinterpret_cast<const float4*>(leaf + tid * ntp);
+                    int k = 0;
+                    for (; k + 4 <= nt; k += 4) {
+                        const float4 v = lr[k >> 2];
+                        stageSum += v.x;
+                        stageSum += v.y;
+                        stageSum += v.z;
+                        stageSum += v.w;
+                    }
+                    for (; k < nt; ++k) stageSum += leaf[tid * ntp + k];
+                    pass = !(stageSum < st.threshold);
                 } else {
                     const char* om = origin(myIdx);
                     const float invm = sm.invTab[myIdx];
@@ -760,8 +802,10 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(
 // total and a zeroed work counter, then every tile's items at its count's
 // offset (tiles of one count in any order).  ctl layout: [0] survivor total,
 // [1] items, [2] work counter, then per count its tile histogram (replaced
-// by its first item) and its cursor.
-constexpr int kDeepHist = 4, kDeepCursor = kDeepHist + kTileMaxWin + 1, kDeepCtlInts = kDeepCursor + kTileMaxWin + 1;
+// by its first item) and its cursor, then the tail pass's item total and
+// work counter.
+constexpr int kDeepHist = 4, kDeepCursor = kDeepHist + kTileMaxWin + 1, kTailCtl = kDeepCursor + kTileMaxWin + 1;
+constexpr int kDeepCtlInts = kTailCtl + 2;
 
 __global__ __launch_bounds__(256) void deep_reset_kernel(int32_t* __restrict__ ctl) {
     for (int i = threadIdx.x; i < kDeepCtlInts; i += blockDim.x) ctl[i] = 0;
@@ -826,20 +870,24 @@ __global__ __launch_bounds__(256) void deep_scatter_kernel(const int32_t* __rest
 
 // Cascade evaluation, deep pass: persistent workgroups take work items
 // (a slice of up to kDeepSlice first-pass survivors of one tile) from a
-// counter until none are left, restage the tile's footprint, run stages
-// [stageSplit, nStages) over the slice and OR the final bits into the mask
-// words (several slices of a tile share words).
+// counter until none are left, restage the tile's footprint and run stages
+// [stageSplit, stageTail) over the slice.  When stageTail is the cascade's
+// end, the final bits are ORed into the mask words (several slices of a tile
+// share words).  Otherwise the slice's survivors are written back over its
+// own entries of survIdx / survInv (no other item reads them) and appended
+// to the tail pass's work list in items of up to kTailSlice windows.
 __global__ __launch_bounds__(kCascadeBlock) void cascade_deep_kernel(
     const LevelDesc* __restrict__ L, const int32_t* __restrict__ lvlIdx, const int32_t* __restrict__ tileOffs,
     int nLv, const uint32_t* __restrict__ iiS, const KStump* __restrict__ stumpsAll, int nStumps,
-    const KStage* __restrict__ stages, int nStages, int stageSplit, int winW, int winH,
-    unsigned long long* __restrict__ words, float* __restrict__ scores, const uint16_t* __restrict__ survIdx,
-    const float* __restrict__ survInv, const int32_t* __restrict__ tileSurvOff,
-    const int32_t* __restrict__ tileCount, const int32_t* __restrict__ items, const int32_t* __restrict__ nItems,
-    int32_t* __restrict__ workCounter) {
+    const KStage* __restrict__ stages, int nStages, int stageSplit, int stageTail, int winW, int winH, int leafOff,
+    unsigned long long* __restrict__ words, float* __restrict__ scores, uint16_t* __restrict__ survIdx,
+    float* __restrict__ survInv, const int32_t* __restrict__ tileSurvOff, const int32_t* __restrict__ tileCount,
+    const int32_t* __restrict__ items, const int32_t* __restrict__ nItems, int32_t* __restrict__ workCounter,
+    int4* __restrict__ tailItems, int32_t* __restrict__ tailTotal) {
     extern __shared__ float lds[];
     __shared__ CascadeSmem sm;
     const int total = *nItems;
+    const bool tail = stageTail < nStages;
     for (;;) {
         if (threadIdx.x == 0) sm.item = atomicAdd(workCounter, 1);
         __syncthreads();
@@ -863,7 +911,71 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_deep_kernel(
         }
         __syncthreads();
         const KStump* __restrict__ stumps = stumpsAll + static_cast<size_t>(g.step - 1) * nStumps;
-        run_stages(g, sm, lds, stumps, stages, stageSplit, nStages, nStages, winW, scores);
+        const int slot = run_stages(g, sm, lds, stumps, stages, stageSplit, stageTail, nStages, winW, scores,
+                                    lds + leafOff, kLeafCap);
+        if (tail) {
+            const int n = sm.listCnt[slot];
+            const int nt = (n + kTailSlice - 1) / kTailSlice;
+            if (static_cast<int>(threadIdx.x) < n) {
+                const uint32_t widx = sm.listIdx[threadIdx.x];
+                survIdx[off + threadIdx.x] = static_cast<uint16_t>(widx);
+                survInv[off + threadIdx.x] = sm.invTab[widx];
+            }
+            if (threadIdx.x == 0) sm.tailBase = nt ? atomicAdd(tailTotal, nt) : 0;
+            __syncthreads();
+            if (static_cast<int>(threadIdx.x) < nt) {
+                const int a = static_cast<int>(threadIdx.x) * kTailSlice;
+                tailItems[sm.tailBase + threadIdx.x] = make_int4(t, off + a, min(kTailSlice, n - a), 0);
+            }
+        } else {
+            for (int r = threadIdx.x; r < g.TH; r += kCascadeBlock) {
+                const unsigned long long bits = sm.maskWords[r];
+                if (bits) {
+                    const int64_t word =
+                        g.lv.wordOff + static_cast<int64_t>(g.wy0 + r) * g.lv.rowWords + (g.wx0 >> 6);
+                    atomicOr(&words[word], bits << (g.wx0 & 63));
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Cascade evaluation, tail pass: persistent workgroups take the deep pass's
+// tail items (up to kTailSlice windows of one tile: tile, first survIdx
+// entry, count) from a counter, restage the tile's footprint, run stages
+// [stageTail, nStages) and OR the final bits into the mask words.
+__global__ __launch_bounds__(kCascadeBlock) void cascade_tail_kernel(
+    const LevelDesc* __restrict__ L, const int32_t* __restrict__ lvlIdx, const int32_t* __restrict__ tileOffs,
+    int nLv, const uint32_t* __restrict__ iiS, const KStump* __restrict__ stumpsAll, int nStumps,
+    const KStage* __restrict__ stages, int nStages, int stageTail, int winW, int winH, int leafOff,
+    unsigned long long* __restrict__ words, float* __restrict__ scores, const uint16_t* __restrict__ survIdx,
+    const float* __restrict__ survInv, const int4* __restrict__ tailItems, const int32_t* __restrict__ tailTotal,
+    int32_t* __restrict__ workCounter) {
+    extern __shared__ float lds[];
+    __shared__ CascadeSmem sm;
+    const int total = *tailTotal;
+    for (;;) {
+        if (threadIdx.x == 0) sm.item = atomicAdd(workCounter, 1);
+        __syncthreads();
+        const int item = sm.item;
+        if (item >= total) break;
+        const int4 it = tailItems[item];
+        const TileGeo g = tile_geo(L, lvlIdx, tileOffs, nLv, it.x);
+        stage_footprint(g, iiS, lds, winW, winH);
+        if (threadIdx.x < 32) sm.maskWords[threadIdx.x] = 0ull;
+        if (static_cast<int>(threadIdx.x) < it.z) {
+            const uint32_t widx = survIdx[it.y + threadIdx.x];
+            sm.listIdx[threadIdx.x] = static_cast<uint16_t>(widx);
+            sm.invTab[widx] = survInv[it.y + threadIdx.x];
+        }
+        if (threadIdx.x == 0) {
+            sm.listCnt[0] = it.z;
+            sm.listCnt[1] = 0;
+        }
+        __syncthreads();
+        const KStump* __restrict__ stumps = stumpsAll + static_cast<size_t>(g.step - 1) * nStumps;
+        run_stages(g, sm, lds, stumps, stages, stageTail, nStages, nStages, winW, scores, lds + leafOff, kLeafCap);
         for (int r = threadIdx.x; r < g.TH; r += kCascadeBlock) {
             const unsigned long long bits = sm.maskWords[r];
             if (bits) {
@@ -1706,9 +1818,11 @@ struct vj_engine {
     DevBuf<int32_t> dCounts, dBlockCount, dBlockOff, dImgBase;
     // two-pass cascade: first-pass survivors and the deep pass's work list
     int stageSplit = 0;                        // stages run by the first pass
+    int stageTail = 0;                         // first stage of the tail pass (nStages: none)
     DevBuf<uint16_t> survIdx;
     DevBuf<float> survInv;
     DevBuf<int32_t> tileSurvOff, tileCount, deepItems, deepCtl;  // deepCtl: total, items, counter, list scratch
+    DevBuf<int4> tailItems;                    // tail pass: tile, first survivor entry, count
     DevBuf<vj_detection> dOut;
     PinnedBuf<vj_detection> hOut;
     PinnedBuf<int32_t> hCounts;
@@ -2021,6 +2135,8 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
         VJ_TRY(e->tileCount.ensure(static_cast<size_t>(P.nTiles)));
         VJ_TRY(e->deepItems.ensure(static_cast<size_t>(P.nTiles) * kDeepSlices));
         VJ_TRY(e->deepCtl.ensure(kDeepCtlInts));
+        if (e->stageTail < e->nStages)
+            VJ_TRY(e->tailItems.ensure(static_cast<size_t>(P.nTiles) * kDeepSlices * kTailPerDeep));
     }
     const int nImages = static_cast<int>(P.imgWordOff.size()) - 1;
     VJ_TRY(e->dCounts.ensure(static_cast<size_t>(nImages)));
@@ -2088,8 +2204,13 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     size_t ldsBytes = 0;
     for (int st = 1; st <= P.maxStep; ++st)
         ldsBytes = std::max(ldsBytes, cascade_footprint(st, e->tileRows[st], e->winW, e->winH) * sizeof(float));
+    // deep and tail passes: the leaf values after the footprint, 16-byte aligned
+    const int leafOff = static_cast<int>((ldsBytes / sizeof(float) + 3) & ~size_t(3));
+    const size_t deepLds = (static_cast<size_t>(leafOff) + kLeafCap) * sizeof(float);
     const int nBlocks = static_cast<int>(blockImg.size());
     const int deepGrid = deep ? std::min(P.nTiles * kDeepSlices, kDeepGrid) : 0;
+    const bool tail = deep && e->stageTail < e->nStages;
+    const int stageTail = deep ? e->stageTail : e->nStages;
 
     auto launch_all = [&](hipStream_t ls) -> int {
         if (e->timing) VJ_HIP(hipEventRecord(e->ev[0], ls));
@@ -2139,12 +2260,22 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
                 hipLaunchKernelGGL(deep_scatter_kernel, dim3(listGrid), dim3(256), 0, ls, e->tileCount.get(),
                                    P.nTiles, e->deepCtl.get(), e->deepItems.get());
                 VJ_HIP(hipGetLastError());
-                hipLaunchKernelGGL(cascade_deep_kernel, dim3(deepGrid), dim3(kCascadeBlock), ldsBytes, ls, dL,
+                hipLaunchKernelGGL(cascade_deep_kernel, dim3(deepGrid), dim3(kCascadeBlock), deepLds, ls, dL,
                                    m32 + oLvl, m32 + oTile, static_cast<int>(P.tileLvl.size()), e->iiS.get(),
-                                   e->kst.get(), e->nStumps, e->kstage.get(), e->nStages, e->stageSplit, e->winW,
-                                   e->winH, reinterpret_cast<unsigned long long*>(e->words.get()), e->scores.get(),
-                                   e->survIdx.get(), e->survInv.get(), e->tileSurvOff.get(), e->tileCount.get(),
-                                   e->deepItems.get(), e->deepCtl.get() + 1, e->deepCtl.get() + 2);
+                                   e->kst.get(), e->nStumps, e->kstage.get(), e->nStages, e->stageSplit, stageTail,
+                                   e->winW, e->winH, leafOff, reinterpret_cast<unsigned long long*>(e->words.get()),
+                                   e->scores.get(), e->survIdx.get(), e->survInv.get(), e->tileSurvOff.get(),
+                                   e->tileCount.get(), e->deepItems.get(), e->deepCtl.get() + 1,
+                                   e->deepCtl.get() + 2, e->tailItems.get(), e->deepCtl.get() + kTailCtl);
+                VJ_HIP(hipGetLastError());
+            }
+            if (tail) {
+                hipLaunchKernelGGL(cascade_tail_kernel, dim3(deepGrid), dim3(kCascadeBlock), deepLds, ls, dL,
+                                   m32 + oLvl, m32 + oTile, static_cast<int>(P.tileLvl.size()), e->iiS.get(),
+                                   e->kst.get(), e->nStumps, e->kstage.get(), e->nStages, stageTail, e->winW,
+                                   e->winH, leafOff, reinterpret_cast<unsigned long long*>(e->words.get()), e->scores.get(),
+                                   e->survIdx.get(), e->survInv.get(), e->tailItems.get(),
+                                   e->deepCtl.get() + kTailCtl, e->deepCtl.get() + kTailCtl + 1);
                 VJ_HIP(hipGetLastError());
             }
         }
@@ -2179,7 +2310,7 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
         k(P.resizeItems[dpt]);
         k(static_cast<int64_t>(jobStart[dpt]));
     }
-    for (int64_t v : {int64_t(e->stageSplit), int64_t(deepGrid), int64_t(nLevels), int64_t(P.nItems), int64_t(P.nChunkItems), int64_t(P.nTiles),
+    for (int64_t v : {int64_t(e->stageSplit), int64_t(stageTail), int64_t(deepGrid), int64_t(nLevels), int64_t(P.nItems), int64_t(P.nChunkItems), int64_t(P.nTiles),
                       int64_t(P.tileLvl.size()), int64_t(ldsBytes), int64_t(nBlocks), int64_t(nImages), int64_t(cap),
                       int64_t(e->nStumps), int64_t(e->nStages), int64_t(e->winW), int64_t(e->winH),
                       int64_t(oSrc), int64_t(oItem), int64_t(oChunk), int64_t(oLvl), int64_t(oTile), int64_t(oLvWord),
@@ -2198,7 +2329,7 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
                             static_cast<const void*>(e->dOut.get()), static_cast<const void*>(e->survIdx.get()),
                             static_cast<const void*>(e->survInv.get()), static_cast<const void*>(e->tileSurvOff.get()),
                             static_cast<const void*>(e->tileCount.get()), static_cast<const void*>(e->deepItems.get()),
-                            static_cast<const void*>(e->deepCtl.get())})
+                            static_cast<const void*>(e->deepCtl.get()), static_cast<const void*>(e->tailItems.get())})
         k(static_cast<int64_t>(reinterpret_cast<uintptr_t>(ptr)));
     if (e->graphExec && key == e->graphKey) {
         VJ_HIP(hipGraphLaunch(e->graphExec, s));
@@ -2577,6 +2708,11 @@ extern "C" int vj_engine_set_cascade(vj_engine* e, const vj_feature* dFeatures, 
         int k = 0;
         while (k < nSt && (k == 0 || cum + hStages[k].ntrees <= kSplitStumps)) cum += hStages[k++].ntrees;
         e->stageSplit = k;
+        // the deep pass runs stages until it has about kDeepStumps stumps
+        // (at least one stage); the tail pass runs the rest
+        cum = 0;
+        while (k < nSt && cum < kDeepStumps) cum += hStages[k++].ntrees;
+        e->stageTail = k;
     }
     return VJ_OK;
 }
