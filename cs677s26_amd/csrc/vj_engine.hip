@@ -1522,9 +1522,9 @@ __global__ __launch_bounds__(256) void grp_list_kernel(const int32_t* __restrict
 // of their entries at once; the unions the chunk needs are then applied one
 // by one in order from registers (every lane whose root lost takes the
 // winner and its new rank), and each union writes only the loser's parent
-// and the winner's rank to LDS for the chunks after.  The root's raw index
-// orders the clusters.  A span of more than kReplaySpan positions leaves its
-// image to the host.
+// and the winner's rank to LDS for the chunks after.  The replay stops once
+// the members form one set.  The root's raw index orders the clusters.  A
+// span of more than kReplaySpan positions leaves its image to the host.
 constexpr int kReplaySpan = 4096;
 constexpr int kReplayEdges = 8192;
 struct ReplayLds {
@@ -1573,30 +1573,51 @@ __global__ __launch_bounds__(64) void grp_replay_kernel(
         int n = 0;
         uint32_t bestKey = 0;
         int32_t bestRaw = INT_MAX;
-        for (int j0 = 0; j0 < span; j0 += kWave) {
-            const int j = j0 + lane;
-            const bool in = j < span;
-            const int32_t p = r + j;
-            const bool mem = in && label[p] == r;
-            const uint64_t mb = __ballot(mem);
-            if (lane == 0) S.member[j0 >> 6] = mb;
-            if (in) {
-                S.off[j] = static_cast<int32_t>(pairOff[p] - e0);
-                S.par[j] = static_cast<int16_t>(j);
-                S.rnk[j] = 0;
+        // four positions per lane per step: every independent load of the
+        // step is issued before the first is used
+        constexpr int kU = 4;
+        for (int j0 = 0; j0 < span; j0 += kU * kWave) {
+            int32_t lab[kU], raw[kU];
+            int64_t po[kU];
+            int4 B[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int j = j0 + u * kWave + lane;
+                const int32_t p = r + min(j, span - 1);
+                lab[u] = label[p];
+                po[u] = pairOff[p];
+                B[u] = sorted[p];
+                raw[u] = order[p];
             }
-            if (mem) {
-                const int4 B = sorted[p];
-                sx += static_cast<unsigned long long>(B.x);
-                sy += static_cast<unsigned long long>(B.y);
-                sw += static_cast<unsigned long long>(B.z);
-                sh += static_cast<unsigned long long>(B.w);
-                ++n;
-                const int32_t raw = order[p];
-                const uint32_t kk = best_key(dets[base + raw].score);
-                if (kk && (kk > bestKey || (kk == bestKey && raw < bestRaw))) {
-                    bestKey = kk;
-                    bestRaw = raw;
+            float sc[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int j = j0 + u * kWave + lane;
+                sc[u] = (j < span && lab[u] == r) ? dets[base + raw[u]].score : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int j = j0 + u * kWave + lane;
+                const bool in = j < span;
+                const bool mem = in && lab[u] == r;
+                const uint64_t mb = __ballot(mem);
+                if (lane == 0 && j0 + u * kWave < span) S.member[(j0 + u * kWave) >> 6] = mb;
+                if (in) {
+                    S.off[j] = static_cast<int32_t>(po[u] - e0);
+                    S.par[j] = static_cast<int16_t>(j);
+                    S.rnk[j] = 0;
+                }
+                if (mem) {
+                    sx += static_cast<unsigned long long>(B[u].x);
+                    sy += static_cast<unsigned long long>(B[u].y);
+                    sw += static_cast<unsigned long long>(B[u].z);
+                    sh += static_cast<unsigned long long>(B[u].w);
+                    ++n;
+                    const uint32_t kk = best_key(sc[u]);
+                    if (kk && (kk > bestKey || (kk == bestKey && raw[u] < bestRaw))) {
+                        bestKey = kk;
+                        bestRaw = raw[u];
+                    }
                 }
             }
         }
@@ -1617,9 +1638,19 @@ __global__ __launch_bounds__(64) void grp_replay_kernel(
             __syncthreads();  // the staged span is complete; the old window is no longer read
             winLo = from;
             winHi = min(E, from + kReplayEdges);
-            for (int64_t t = from + lane; t < winHi; t += kWave) {
-                const int32_t q = base + pairs[e0 + t] - r;
-                S.edge[t - from] = static_cast<int16_t>(q >= 0 && q < span ? q : -1);
+            for (int64_t t0 = from; t0 < winHi; t0 += 8 * kWave) {
+                int32_t pv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int64_t t = t0 + u * kWave + lane;
+                    pv[u] = t < winHi ? pairs[e0 + t] : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int64_t t = t0 + u * kWave + lane;
+                    const int32_t q = base + pv[u] - r;
+                    if (t < winHi) S.edge[t - from] = static_cast<int16_t>(q >= 0 && q < span ? q : -1);
+                }
             }
             int lo = 0, hi = span - 1;  // the last position whose list starts at or before `from`
             while (lo < hi) {
@@ -1635,7 +1666,9 @@ __global__ __launch_bounds__(64) void grp_replay_kernel(
             __syncthreads();
         };
         load_window(0);
-        for (int64_t t0 = 0; t0 < E; t0 += kWave) {
+        // once the members form one set no later pair changes a root
+        int sets = n;
+        for (int64_t t0 = 0; t0 < E && sets > 1; t0 += kWave) {
             if (t0 + kWave > winHi && winHi < E) load_window(t0);
             const int64_t t = t0 + lane;
             int src = 0, dst = -1;
@@ -1668,6 +1701,7 @@ __global__ __launch_bounds__(64) void grp_replay_kernel(
                 ks = rs == win ? rw : ks;
                 kq = rq == win ? rw : kq;
                 nm = __ballot(valid && rs != rq);
+                --sets;
             }
         }
         const int root = find(0);
@@ -1815,6 +1849,7 @@ struct vj_engine {
     DevBuf<int32_t> dMeta32;
     DevBuf<int64_t> dMeta64;
     DevBuf<ResizeJob> dJobs;
+    DevBuf<uint8_t> dPlan;                     // a detect call's tables (levels, scales, offsets, jobs)
     DevBuf<int32_t> dCounts, dBlockCount, dBlockOff, dImgBase;
     // two-pass cascade: first-pass survivors and the deep pass's work list
     int stageSplit = 0;                        // stages run by the first pass
@@ -2185,20 +2220,31 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
         jobStart.push_back(jobs.size());
         jobs.insert(jobs.end(), jv.begin(), jv.end());
     }
+    // plan upload: every table into one staging buffer, then one copy
     const size_t stageBytes = P.levels.size() * sizeof(LevelDesc) + P.rs.size() * 4 + meta32.size() * 4 +
                               meta64.size() * 8 + jobs.size() * sizeof(ResizeJob) + 6 * 256;
     VJ_TRY(e->hStage.ensure(stageBytes));
-    size_t stOff = 0;
     VJ_HIP(hipStreamSynchronize(s));  // staging buffer reuse across calls
-    VJ_TRY(upload(e->dLevels, P.levels, e->hStage, stOff, s));
-    VJ_TRY(upload(e->dRs, P.rs, e->hStage, stOff, s));
-    VJ_TRY(upload(e->dMeta32, meta32, e->hStage, stOff, s));
-    VJ_TRY(upload(e->dMeta64, meta64, e->hStage, stOff, s));
-    if (!jobs.empty()) VJ_TRY(upload(e->dJobs, jobs, e->hStage, stOff, s));
+    size_t stOff = 0;
+    auto place = [&](const void* src, size_t bytes) {
+        const size_t at = stOff;
+        if (bytes) std::memcpy(e->hStage.get() + at, src, bytes);
+        stOff += static_cast<size_t>(align_up(static_cast<int64_t>(bytes), 256));
+        return at;
+    };
+    const size_t aLv = place(P.levels.data(), P.levels.size() * sizeof(LevelDesc));
+    const size_t aRs = place(P.rs.data(), P.rs.size() * sizeof(float));
+    const size_t a32 = place(meta32.data(), meta32.size() * sizeof(int32_t));
+    const size_t a64 = place(meta64.data(), meta64.size() * sizeof(int64_t));
+    const size_t aJobs = place(jobs.data(), jobs.size() * sizeof(ResizeJob));
+    VJ_TRY(e->dPlan.ensure(std::max<size_t>(stOff, 256)));
+    VJ_HIP(hipMemcpyAsync(e->dPlan.get(), e->hStage.get(), stOff, hipMemcpyHostToDevice, s));
 
-    const LevelDesc* dL = e->dLevels.get();
-    const int32_t* m32 = e->dMeta32.get();
-    const int64_t* m64 = e->dMeta64.get();
+    const LevelDesc* dL = reinterpret_cast<const LevelDesc*>(e->dPlan.get() + aLv);
+    const float* dRs = reinterpret_cast<const float*>(e->dPlan.get() + aRs);
+    const int32_t* m32 = reinterpret_cast<const int32_t*>(e->dPlan.get() + a32);
+    const int64_t* m64 = reinterpret_cast<const int64_t*>(e->dPlan.get() + a64);
+    const ResizeJob* dJobs = reinterpret_cast<const ResizeJob*>(e->dPlan.get() + aJobs);
     const int nLevels = static_cast<int>(P.levels.size());
 
     size_t ldsBytes = 0;
@@ -2219,7 +2265,7 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
             const int n = static_cast<int>(P.resizeJobs[dpt].size());
             if (!n || !P.resizeItems[dpt]) continue;
             hipLaunchKernelGGL(resize_kernel, dim3(P.resizeItems[dpt]), dim3(256), 0, ls, dL,
-                               e->dJobs.get() + jobStart[dpt], n, e->dRs.get(), m32 + oSrc);
+                               dJobs + jobStart[dpt], n, dRs, m32 + oSrc);
             VJ_HIP(hipGetLastError());
         }
         if (e->timing) VJ_HIP(hipEventRecord(e->ev[1], ls));
@@ -2317,7 +2363,7 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
                       int64_t(oBlockWord), int64_t(oBlockImg), int64_t(oImgBlock)})
         k(v);
     for (const void* ptr : {static_cast<const void*>(dL), static_cast<const void*>(m32), static_cast<const void*>(m64),
-                            static_cast<const void*>(e->dJobs.get()), static_cast<const void*>(e->dRs.get()),
+                            static_cast<const void*>(dJobs), static_cast<const void*>(dRs),
                             static_cast<const void*>(e->colS.get()), static_cast<const void*>(e->colQ.get()),
                             static_cast<const void*>(e->rowS.get()), static_cast<const void*>(e->rowQ.get()),
                             static_cast<const void*>(e->totS.get()), static_cast<const void*>(e->totQ.get()),
@@ -2785,6 +2831,9 @@ int autotune(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int minOb
     return VJ_OK;
 }
 
+// Grouped boxes copied back together with the per-image counts.
+constexpr size_t kOutPrefetch = 4096;
+
 int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int minNeighbors, int minObj,
                   int maxDet, int applyGrouping, vj_detection* hOut, int cap, int* hCounts, bool batch,
                   hipStream_t s) {
@@ -2805,6 +2854,7 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
     // the reference's DSU roots of the kept components, the clusters
     const size_t capTotal = static_cast<size_t>(capPer) * static_cast<size_t>(nImages);
     const bool devGroup = applyGrouping != 0 && capTotal < (size_t{1} << 31);
+    size_t outPrefetch = 0;
     if (devGroup) {
         if (e->pairCap == 0) e->pairCap = static_cast<int64_t>(std::min<size_t>(capTotal, size_t{1} << 20) * 16 + 65536);
         // image widths: one x bin per column 0 .. w of an image (LDS of the sweep)
@@ -2871,6 +2921,12 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
         VJ_HIP(hipMemcpyAsync(e->hCtl.get(), e->gCtl.get(), 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
         VJ_HIP(hipMemcpyAsync(e->hGrpImg.get(), e->gGrpImg.get(), 3 * sizeof(int32_t) * nImages,
                               hipMemcpyDeviceToHost, s));
+        // the first grouped boxes come back with the counts, so a call with
+        // few of them needs no second round trip
+        outPrefetch = std::min<size_t>(capTotal, kOutPrefetch);
+        VJ_TRY(e->hOutG.ensure(outPrefetch));
+        VJ_HIP(hipMemcpyAsync(e->hOutG.get(), e->gOutG.get(), sizeof(vj_detection) * outPrefetch,
+                              hipMemcpyDeviceToHost, s));
     }
     VJ_TRY(e->hCounts.ensure(static_cast<size_t>(nImages)));
     VJ_HIP(hipMemcpyAsync(e->hCounts.get(), e->dCounts.get(), sizeof(int32_t) * nImages, hipMemcpyDeviceToHost, s));
@@ -2920,7 +2976,9 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
             if (usable && n <= kGroupDevMax && !e->hGrpImg.get()[3 * i + 2]) onDev[static_cast<size_t>(i)] = 1;
             else hostImgs.push_back(i);
         }
+        bool copied = false;
         if (!hostImgs.empty()) {
+            copied = true;
             if (static_cast<int>(hostImgs.size()) == nImages) {
                 VJ_HIP(hipMemcpyAsync(e->hOut.get(), e->dOut.get(), sizeof(vj_detection) * total, hipMemcpyDeviceToHost, s));
             } else {
@@ -2931,12 +2989,13 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
             }
         }
         const bool anyDev = std::find(onDev.begin(), onDev.end(), 1) != onDev.end();
-        if (anyDev && devTotal > 0) {
+        if (anyDev && devTotal > static_cast<int64_t>(outPrefetch)) {
+            copied = true;
             VJ_TRY(e->hOutG.ensure(static_cast<size_t>(devTotal)));
             VJ_HIP(hipMemcpyAsync(e->hOutG.get(), e->gOutG.get(), sizeof(vj_detection) * devTotal,
                                   hipMemcpyDeviceToHost, s));
         }
-        VJ_HIP(hipStreamSynchronize(s));
+        if (copied) VJ_HIP(hipStreamSynchronize(s));
     } else if (total > 0) {
         VJ_HIP(hipMemcpyAsync(e->hOut.get(), e->dOut.get(), sizeof(vj_detection) * total, hipMemcpyDeviceToHost, s));
         VJ_HIP(hipStreamSynchronize(s));
