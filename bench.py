@@ -9,11 +9,13 @@ fields below).  Extra legs, each optional:
 * ``stump``    -- one AdaBoost stage: presort of 162,336 features x 20,000
   samples, then 50 boosting rounds (seconds per stage);
 * ``cpu``      -- the oracle (C restatement of the reference CPU path) on the
-  same 1080p frame, one thread (the reference CPU path is single-threaded).
+  same 1080p frame, one thread (the reference CPU path is single-threaded)
+  and every allowed core; one warm-up, then the median of three runs.
 
 ``roofline`` is for the cascade phase (the dominant one: ``cascade_kernel``,
-the first pass over every tile, then the deep work-list kernels and the persistent
-``cascade_deep_kernel`` over the survivors): algorithmic bytes per frame
+the first pass over every tile, then the deep work-list kernels, the persistent
+``cascade_deep_kernel`` over the survivors and ``cascade_tail_kernel`` over the
+last stages' few windows): algorithmic bytes per frame
 (12*(W+1)*(H+1) summed over levels, plus model and output bytes) over the
 phase's average duration from HIP events on the detect stream.
 """
@@ -250,49 +252,103 @@ def stump_leg(args):
                                 "traffic": measured_traffic_search()}}
 
 
+def cpu_info():
+    """The host CPU model, the cores this process may run on, and the thread
+    count of the all-core runs (the job's OpenMP share when it sets one)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = len(os.sched_getaffinity(0))
+    threads = nproc
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        threads = min(nproc, int(env))
+    return {"cpu_model": model, "nproc": nproc, "all_core_threads": threads}
+
+
+def timed_median(fn, runs=3):
+    """One untimed warm-up call, then the median of ``runs`` timed calls
+    (monotonic clock), as the reference's compare modes repeat their runs."""
+    import statistics
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts), ts
+
+
 def cpu_leg(args):
+    """The oracle's detect + grouping of the 1080p frame: one thread (the
+    reference CPU path is single-threaded), then every allowed core."""
     from cs677s26_amd import workloads
     from oracle import pyoracle as O
 
     img, (feats, stumps, stages) = workloads.detect_1080()
     h, w = img.shape
-    O.lib()
-    t0 = time.perf_counter()
-    raw, _ = O.detect_raw(feats, stumps, stages, img)
-    O.group(raw, 4)
-    dt = time.perf_counter() - t0
-    return {"value": w * h / 1e6 / dt, "unit": "Mpix/s", "cores": 1, "kind": "port",
-            "sample": "one 1920x1080 frame, 38-stage cascade, detect + grouping, oracle C restatement, 1 thread",
-            "seconds": dt}
+    info = cpu_info()
+    mpix = w * h / 1e6
+
+    def run():
+        raw, _ = O.detect_raw(feats, stumps, stages, img)
+        O.group(raw, 4)
+
+    O.set_threads(1)
+    t1, runs1 = timed_median(run)
+    O.set_threads(info["all_core_threads"])
+    tn, runsn = timed_median(run)
+    O.set_threads(1)
+    return {"value": mpix / t1, "unit": "Mpix/s", "cores": 1, "kind": "port",
+            "sample": "one 1920x1080 frame, 38-stage cascade, detect + grouping, oracle C restatement; "
+                      "1 warm-up, median of 3",
+            "seconds": t1, "runs_s": runs1, **info,
+            "all_cores": {"value": mpix / tn, "unit": "Mpix/s", "cores": info["all_core_threads"], "seconds": tn,
+                          "runs_s": runsn}}
 
 
 def cpu_batched_leg(args):
-    """The oracle's detect + grouping, one thread, over the batch's images in
-    order, skipping any that would take the sample past 3 Mpix, until 2.5 Mpix
-    are covered (about 10 s of CPU work): the CPU path's Mpix/s on the batched
-    workload's image mix."""
+    """The oracle's detect + grouping over the batch's images in order,
+    skipping any that would take the sample past 1.6 Mpix, until 1.2 Mpix are
+    covered: the CPU path's Mpix/s on the batched workload's image mix, one
+    thread and every allowed core."""
     from cs677s26_amd import workloads
     from oracle import pyoracle as O
 
     sizes, imgs = workloads.batch64()
     feats, stumps, stages = workloads.cascade_38()
-    O.lib()
-    mpix, dt, used = 0.0, 0.0, 0
+    info = cpu_info()
+    pick, mpix = [], 0.0
     for (w, h), im in zip(sizes, imgs):
-        if mpix + w * h / 1e6 > 3.0:
+        if mpix + w * h / 1e6 > 1.6:
             continue
-        t0 = time.perf_counter()
-        raw, _ = O.detect_raw(feats, stumps, stages, im)
-        O.group(raw, 4)
-        dt += time.perf_counter() - t0
+        pick.append(im)
         mpix += w * h / 1e6
-        used += 1
-        if mpix >= 2.5:
+        if mpix >= 1.2:
             break
-    return {"value": mpix / dt, "unit": "Mpix/s", "cores": 1, "kind": "port",
-            "sample": f"{used} of the 64 batch images ({mpix:.2f} Mpix, in order, none past a 3 Mpix total), "
-                      "38-stage cascade, detect + "
-                      "grouping per image, oracle C restatement, 1 thread", "seconds": dt}
+
+    def run():
+        for im in pick:
+            raw, _ = O.detect_raw(feats, stumps, stages, im)
+            O.group(raw, 4)
+
+    O.set_threads(1)
+    t1, runs1 = timed_median(run)
+    O.set_threads(info["all_core_threads"])
+    tn, runsn = timed_median(run)
+    O.set_threads(1)
+    return {"value": mpix / t1, "unit": "Mpix/s", "cores": 1, "kind": "port",
+            "sample": f"{len(pick)} of the 64 batch images ({mpix:.2f} Mpix, in order, none past a 1.6 Mpix "
+                      "total), 38-stage cascade, detect + grouping per image, oracle C restatement; 1 warm-up, "
+                      "median of 3", "seconds": t1, "runs_s": runs1, **info,
+            "all_cores": {"value": mpix / tn, "unit": "Mpix/s", "cores": info["all_core_threads"], "seconds": tn,
+                          "runs_s": runsn}}
 
 
 def measured_traffic(name):
@@ -326,35 +382,51 @@ def measured_traffic_search():
 
 def cpu_stump_leg(args):
     """The oracle's stump search (presort + exact sorted sweep, the
-    reference's runCpuWeakSearch semantics) on one thread over the first
-    2,048 features of the 162,336-feature pool and all 20,000 samples, timed
-    for one presort and one round and scaled to the full pool and 50 rounds
-    (extrapolated, as the reference's own compare mode samples features)."""
+    reference's runCpuWeakSearch semantics) over the first features of the
+    162,336-feature pool and all 20,000 samples, timed for one presort and
+    one round and scaled to the full pool and 50 rounds (extrapolated, as the
+    reference's own compare mode samples features): 2,048 features on one
+    thread, 16,384 on every allowed core."""
     import numpy as np
 
     from cs677s26_amd import workloads
     from oracle import pyoracle as O
 
     patches, labels = workloads.training()
-    O.set_threads(1)
+    info = cpu_info()
     sumT, _, inv = O.train_integrals(patches)
     pool = O.haar_pool(24, 0)
-    sub = pool[:2048]
     n = len(labels)
     w = np.full(n, 1.0 / n)
     sw = np.where(labels > 0, w, -w).astype(np.float32)
-    t0 = time.perf_counter()
-    ords = O.presort(sub, sumT, inv)
-    t1 = time.perf_counter()
-    best = O.search(ords, sw)
-    O.select(best)
-    t2 = time.perf_counter()
-    scale = len(pool) / len(sub)
-    per_stage = (t1 - t0) * scale + args.rounds * (t2 - t1) * scale
-    return {"value": per_stage, "unit": "s/stage (extrapolated)", "cores": 1, "kind": "port",
-            "sample": f"{len(sub)} of {len(pool)} features x {n} samples: one presort ({t1 - t0:.3f} s) and one "
-                      f"round ({t2 - t1:.3f} s), scaled to the full pool and {args.rounds} rounds",
-            "higher_is_better": False}
+
+    def measure(nfeat, threads):
+        O.set_threads(threads)
+        sub = pool[:nfeat]
+        box = {}
+
+        def presort():
+            box["ords"] = O.presort(sub, sumT, inv)
+
+        def round_():
+            O.select(O.search(box["ords"], sw))
+
+        tp, runs_p = timed_median(presort)
+        tr, runs_r = timed_median(round_)
+        scale = len(pool) / nfeat
+        return tp * scale + args.rounds * tr * scale, tp, tr, runs_p, runs_r
+
+    v1, tp1, tr1, rp1, rr1 = measure(2048, 1)
+    vn, tpn, trn, rpn, rrn = measure(16384, info["all_core_threads"])
+    O.set_threads(1)
+    return {"value": v1, "unit": "s/stage (extrapolated)", "cores": 1, "kind": "port",
+            "sample": f"2048 of {len(pool)} features x {n} samples: presort {tp1:.3f} s and one round "
+                      f"{tr1:.4f} s (each 1 warm-up, median of 3), scaled to the full pool and {args.rounds} "
+                      "rounds",
+            "higher_is_better": False, "presort_runs_s": rp1, "round_runs_s": rr1, **info,
+            "all_cores": {"value": vn, "unit": "s/stage (extrapolated)", "cores": info["all_core_threads"],
+                          "sample": f"16384 of {len(pool)} features: presort {tpn:.3f} s, one round {trn:.4f} s",
+                          "presort_runs_s": rpn, "round_runs_s": rrn}}
 
 
 def detect_fields(args):
@@ -379,7 +451,7 @@ def detect_fields(args):
     casc_ms = d["phase_ms"][2]
     casc_bytes = d["stats"]["cascadeBytes"]
     achieved = casc_bytes / (casc_ms * 1e-3) / 1e9 if casc_ms > 0 else 0.0
-    out["roofline"] = {"bound": "hbm", "kernel": "cascade_kernel+deep_hist/base/scatter_kernel+cascade_deep_kernel",
+    out["roofline"] = {"bound": "hbm", "kernel": "cascade_kernel+deep_hist/base/scatter_kernel+cascade_deep_kernel+cascade_tail_kernel",
                        "achieved": achieved, "peak": HBM_PEAK_GBS,
                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                        "traffic": measured_traffic("cascade_kernel_detect1080"),
@@ -433,11 +505,11 @@ def main():
         out["served"] = v
         out["p99_latency_ms"] = v["p99_ms"]
     if "cpu" not in skip:
-        log("cpu baseline (oracle, 1 thread)")
+        log("cpu baseline (oracle, 1 thread and all cores, warm-up + median of 3)")
         out["cpu_baseline"] = cpu_leg(args)
-        log("cpu baseline: stump search (oracle, 1 thread, 2,048-feature sample)")
+        log("cpu baseline: stump search (oracle, 2,048 features on 1 thread, 16,384 on all cores)")
         out["cpu_baseline_stump"] = cpu_stump_leg(args)
-        log("cpu baseline: batched workload images (oracle, 1 thread, about 2.5 Mpix)")
+        log("cpu baseline: batched workload images (oracle, about 1.2 Mpix, 1 thread and all cores)")
         out["cpu_baseline_batched"] = cpu_batched_leg(args)
     else:
         out["cpu_baseline"] = None

@@ -9,7 +9,8 @@ import os
 import re
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libvj_hip.so")
+# $VJ_LIB names another build of the same library (the sanitizer build)
+LIB_PATH = os.environ.get("VJ_LIB") or os.path.join(_HERE, "libvj_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "vj_api.h")
 
 VJ_OK, VJ_INVALID_ARG, VJ_DEVICE_ERROR, VJ_OUT_OF_MEMORY, VJ_RUNTIME_ERROR = range(5)
@@ -95,6 +96,9 @@ _PROTOS = {
     "vj_group_rectangles_many": (I, [P, P, I, I, F, P, P]),
     "vj_pgm_peek": (I, [C.c_char_p, C.POINTER(I), C.POINTER(I)]),
     "vj_pgm_load": (I, [C.c_char_p, P, SZ]),
+    "vj_samples_count": (I, [C.c_char_p, I, C.POINTER(I)]),
+    "vj_samples_load": (I, [C.c_char_p, I, P, SZ, C.POINTER(I)]),
+    "vj_pgm_list": (I, [C.c_char_p, C.c_char_p, SZ, C.POINTER(SZ), C.POINTER(I)]),
     "vj_ppm_save_gray_boxes": (I, [C.c_char_p, P, I, I, P, I]),
     "vj_crop_resize": (I, [P, I, I, I, P, I, I, P]),
     "vj_engine_create": (I, [I, I, I, I, I, I, I, PP]),

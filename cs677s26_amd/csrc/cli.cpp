@@ -209,43 +209,26 @@ int main(int argc, char** argv) {
 // ---------------------------------------------------------------------------
 namespace {
 
+// A raw sample file through the library (count patches of px bytes); the
+// library's message names the failure.
 bool load_bin(const std::string& path, int px, std::vector<uint8_t>& out, int& count, bool quiet = false) {
-    std::ifstream ifs(path, std::ios::binary);
-    if (!ifs) {
-        if (!quiet) std::cerr << "[ERR] Cannot open: " << path << "\n";
-        return false;
+    int n = 0;
+    if (vj_samples_count(path.c_str(), px, &n) == VJ_OK) {
+        out.resize(static_cast<size_t>(n) * static_cast<size_t>(px));
+        if (vj_samples_load(path.c_str(), px, out.data(), out.size(), &count) == VJ_OK) return true;
     }
-    ifs.seekg(0, std::ios::end);
-    const std::streamoff sz = ifs.tellg();
-    ifs.seekg(0, std::ios::beg);
-    if (sz <= 0) {
-        if (!quiet) std::cerr << "[ERR] Empty file: " << path << "\n";
-        return false;
-    }
-    if (sz % px != 0) {
-        if (!quiet) std::cerr << "[ERR] File size is not divisible by samplePixels: " << path << "\n";
-        return false;
-    }
-    out.resize(static_cast<size_t>(sz));
-    if (!ifs.read(reinterpret_cast<char*>(out.data()), sz)) return false;
-    count = static_cast<int>(sz / px);
-    return true;
+    if (!quiet) std::cerr << "[ERR] " << vj_last_error() << "\n";
+    return false;
 }
 
-// Every .pgm file under dir (any depth, extension in any case), sorted so a
-// seed gives the same order on every file system.
-std::vector<std::string> collect_pgm_paths(const std::string& dir) {
-    namespace fs = std::filesystem;
+std::vector<std::string> pgm_paths(const std::string& dir) {
+    size_t bytes = 0;
+    int n = 0;
     std::vector<std::string> out;
-    std::error_code ec;
-    if (dir.empty() || !fs::exists(dir, ec)) return out;
-    for (fs::recursive_directory_iterator it(dir, ec), end; !ec && it != end; it.increment(ec)) {
-        if (!it->is_regular_file(ec)) continue;
-        std::string ext = it->path().extension().string();
-        std::transform(ext.begin(), ext.end(), ext.begin(), [](unsigned char c) { return static_cast<char>(std::tolower(c)); });
-        if (ext == ".pgm") out.push_back(it->path().string());
-    }
-    std::sort(out.begin(), out.end());
+    if (vj_pgm_list(dir.c_str(), nullptr, 0, &bytes, &n) != VJ_OK || n == 0) return out;
+    std::vector<char> buf(bytes);
+    if (vj_pgm_list(dir.c_str(), buf.data(), buf.size(), &bytes, &n) != VJ_OK) return out;
+    for (size_t at = 0; at < bytes; at += out.back().size() + 1) out.emplace_back(buf.data() + at);
     return out;
 }
 
@@ -403,7 +386,7 @@ int run_train(int argc, char** argv) {
     std::vector<vj_feature> pool(static_cast<size_t>(poolN));
     vj_haar_pool(kWin, kWin, 1, pool.data(), poolN, &poolN);
     std::cout << "[INFO] Feature count: " << poolN << "\n";
-    const std::vector<std::string> images = collect_pgm_paths(imagesDir);
+    const std::vector<std::string> images = pgm_paths(imagesDir);
 
     vj_trainer* tr = nullptr;
     if (vj_trainer_create(kWin, kWin, n, 1024, &tr) != VJ_OK) return fail("AdaBoostTrainer");

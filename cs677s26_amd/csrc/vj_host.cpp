@@ -11,6 +11,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <dirent.h>
+#include <sys/stat.h>
 #include <fstream>
 #include <limits>
 #include <numeric>
@@ -746,6 +748,95 @@ extern "C" int vj_pgm_load(const char* path, uint8_t* out, size_t cap) {
     VJ_CHECK(cap >= n, "vj_pgm_load: output buffer too small");
     VJ_CHECK(ifs.read(reinterpret_cast<char*>(out), static_cast<std::streamsize>(n)),
              "failed to read PGM pixel payload");
+    return VJ_OK;
+}
+
+namespace {
+
+// Size of a raw sample file in whole patches (src/main.cu:122-162).
+int samples_open(const char* path, int px, FILE** f, long* bytes) {
+    VJ_CHECK(path && px > 0, "vj_samples: bad arguments");
+    *f = std::fopen(path, "rb");
+    VJ_CHECK(*f, std::string("Cannot open: ") + path);
+    long sz = -1;
+    if (std::fseek(*f, 0, SEEK_END) == 0) sz = std::ftell(*f);
+    std::rewind(*f);
+    if (sz <= 0 || sz % px != 0) {
+        std::fclose(*f);
+        *f = nullptr;
+        VJ_CHECK(sz > 0, std::string("Empty file: ") + path);
+        VJ_CHECK(false, std::string("File size is not divisible by samplePixels: ") + path);
+    }
+    *bytes = sz;
+    return VJ_OK;
+}
+
+// Regular *.pgm files under dir, depth first (order fixed by the caller).
+void pgm_walk(const std::string& dir, std::vector<std::string>& out) {
+    DIR* d = opendir(dir.c_str());
+    if (!d) return;
+    while (const dirent* ent = readdir(d)) {
+        const std::string name = ent->d_name;
+        if (name == "." || name == "..") continue;
+        const std::string path = dir + "/" + name;
+        struct stat st;
+        if (stat(path.c_str(), &st) != 0) continue;
+        if (S_ISDIR(st.st_mode)) {
+            pgm_walk(path, out);
+        } else if (S_ISREG(st.st_mode) && name.size() > 4) {
+            std::string ext = name.substr(name.size() - 4);
+            for (char& c : ext) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
+            if (ext == ".pgm") out.push_back(path);
+        }
+    }
+    closedir(d);
+}
+
+}  // namespace
+
+extern "C" int vj_samples_count(const char* path, int samplePixels, int* count) {
+    VJ_CHECK(count, "vj_samples_count: bad arguments");
+    FILE* f = nullptr;
+    long sz = 0;
+    VJ_TRY(samples_open(path, samplePixels, &f, &sz));
+    std::fclose(f);
+    *count = static_cast<int>(sz / samplePixels);
+    return VJ_OK;
+}
+
+extern "C" int vj_samples_load(const char* path, int samplePixels, uint8_t* hOut, size_t cap, int* count) {
+    VJ_CHECK(hOut && count, "vj_samples_load: bad arguments");
+    FILE* f = nullptr;
+    long sz = 0;
+    VJ_TRY(samples_open(path, samplePixels, &f, &sz));
+    const size_t n = static_cast<size_t>(sz);
+    const bool fits = cap >= n;
+    const bool ok = fits && std::fread(hOut, 1, n, f) == n;
+    std::fclose(f);
+    VJ_CHECK(fits, "vj_samples_load: output buffer too small");
+    VJ_CHECK(ok, std::string("Failed to read: ") + path);
+    *count = static_cast<int>(sz / samplePixels);
+    return VJ_OK;
+}
+
+extern "C" int vj_pgm_list(const char* dir, char* buf, size_t cap, size_t* bytes, int* count) {
+    VJ_CHECK(dir && bytes && count, "vj_pgm_list: bad arguments");
+    std::vector<std::string> paths;
+    std::string root = dir;
+    while (root.size() > 1 && root.back() == '/') root.pop_back();
+    if (!root.empty()) pgm_walk(root, paths);
+    std::sort(paths.begin(), paths.end());
+    size_t need = 0;
+    for (const std::string& q : paths) need += q.size() + 1;
+    *bytes = need;
+    *count = static_cast<int>(paths.size());
+    if (buf && cap >= need) {
+        size_t at = 0;
+        for (const std::string& q : paths) {
+            std::memcpy(buf + at, q.c_str(), q.size() + 1);
+            at += q.size() + 1;
+        }
+    }
     return VJ_OK;
 }
 

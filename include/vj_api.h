@@ -119,6 +119,25 @@ int vj_pgm_load(const char* path, uint8_t* hOut, size_t cap);
 int vj_ppm_save_gray_boxes(const char* path, const uint8_t* hGray, int w, int h,
                            const vj_detection* hBoxes, int n);
 
+/* ---- raw sample files: loadRawSampleBin, src/main.cu:122-162 ------------
+ * A file of patches of samplePixels bytes each, back to back.  *count is the
+ * number of patches.  An unreadable or empty file, or one whose size is not
+ * a whole number of patches, gives VJ_INVALID_ARG and the reference's
+ * message ("Cannot open: <path>", "Empty file: <path>", "File size is not
+ * divisible by samplePixels: <path>").  vj_samples_load needs cap >= the
+ * file size. */
+int vj_samples_count(const char* path, int samplePixels, int* count);
+int vj_samples_load(const char* path, int samplePixels, uint8_t* hOut, size_t cap, int* count);
+
+/* ---- PGM files under a directory: collectPgmPaths, src/main.cu:226-244 --
+ * Every regular file named *.pgm (extension in any case) under dir, at any
+ * depth, as NUL-terminated paths packed into buf in ascending byte order
+ * (the reference keeps directory order; sorted, a seed picks the same
+ * images on every file system).  *bytes is the size the whole list needs,
+ * *count the number of paths; buf is filled only when cap >= *bytes.  A
+ * missing directory gives an empty list. */
+int vj_pgm_list(const char* dir, char* buf, size_t cap, size_t* bytes, int* count);
+
 /* ---- hard-negative crops: cropResizeTo24, src/main.cu:245-284 (host) -----
  * The box (x, y, w, h) of a gray image (row pitch stride) resampled to
  * outW x outH by pixel-centre bilinear interpolation with clamped taps (the

@@ -270,6 +270,48 @@ static float det_rect(const or_rect* r, const uint32_t* sum, int iiW, int base, 
 #endif
 }
 
+/* One window at integral offset base: the variance gate, then every stage
+ * in order, each stage sum in stump order; returns whether it passes all of
+ * them, with the last stage sum in *last (src/main.cu:752-804). */
+static int det_window(const or_feature* features, const or_stump* stumps, const or_stage* stages, int nStages,
+                      const uint32_t* sum, const uint64_t* sq, int iiW, int base, int mW, int narea,
+                      int np0, int np1, int np2, int np3, float* last) {
+#if OR_MUTANT == 5
+    const double normSum = (double)(int32_t)(sum[base + np0] - sum[base + np1] -
+                                             sum[base + np1] + sum[base + np3]);
+#else
+    const double normSum = (double)(int32_t)(sum[base + np0] - sum[base + np1] -
+                                             sum[base + np2] + sum[base + np3]);
+#endif
+    const uint32_t normSqU = (uint32_t)sq[base + np0] - (uint32_t)sq[base + np1] -
+                             (uint32_t)sq[base + np2] + (uint32_t)sq[base + np3];
+    const double nf2 = (double)narea * (double)normSqU - normSum * normSum;
+    if (!(nf2 > 0.0)) return 0;
+    const float invNorm = (float)(1.0 / sqrt(nf2));
+    if (!((double)narea * (double)invNorm < 1e-1)) return 0;
+    float lastSum = 0.0f;
+    for (int si = 0; si < nStages; ++si) {
+        float stageSum = 0.0f;
+        for (int wj = 0; wj < stages[si].ntrees; ++wj) {
+#if OR_MUTANT == 2
+            const int wi = stages[si].ntrees - 1 - wj;
+#else
+            const int wi = wj;
+#endif
+            const or_stump* st = &stumps[stages[si].first + wi];
+            const or_feature* f = &features[st->feature];
+            float raw = det_rect(&f->r[0], sum, iiW, base, mW) + det_rect(&f->r[1], sum, iiW, base, mW);
+            if (f->rectCount == 3) raw += det_rect(&f->r[2], sum, iiW, base, mW);
+            const float resp = raw * invNorm;
+            stageSum += (resp < st->theta) ? st->left : st->right;
+        }
+        if (stageSum < stages[si].threshold) return 0;
+        lastSum = stageSum;
+    }
+    *last = lastSum;
+    return 1;
+}
+
 int or_detect_raw(const or_feature* features, int nFeatures,
                   const or_stump* stumps, int nStumps,
                   const or_stage* stages, int nStages,
@@ -328,61 +370,42 @@ int or_detect_raw(const or_feature* features, int nFeatures,
         const int nw = winW - 2, nh = winH - 2, narea = nw * nh;
         const int np0 = 1 + iiW * 1, np1 = (1 + nw) + iiW * 1;
         const int np2 = 1 + iiW * (1 + nh), np3 = (1 + nw) + iiW * (1 + nh);
-        for (int y = 0; y < workH; y += step) {
-            for (int x = 0; x < workW; x += step) {
-                const int base = y * iiW + x;
+        /* every window of the level (rows spread over the OpenMP threads),
+         * then the passing ones emitted in (y, x) order */
+        const int nRy = (workH + step - 1) / step, nRx = (workW + step - 1) / step;
+        uint8_t* pf = (uint8_t*)malloc((size_t)nRy * nRx + 1);
+        float* lastSum = (float*)malloc(sizeof(float) * ((size_t)nRy * nRx + 1));
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int ry = 0; ry < nRy; ++ry) {
+            for (int rx = 0; rx < nRx; ++rx) {
+                const int base = ry * step * iiW + rx * step;
+                pf[(size_t)ry * nRx + rx] = (uint8_t)det_window(features, stumps, stages, nStages, sum, sq, iiW,
+                                                                base, mW, narea, np0, np1, np2, np3,
+                                                                &lastSum[(size_t)ry * nRx + rx]);
+            }
+        }
+        for (int ry = 0; ry < nRy; ++ry) {
+            for (int rx = 0; rx < nRx; ++rx) {
                 ++nvis;
-#if OR_MUTANT == 5
-                const double normSum = (double)(int32_t)(sum[base + np0] - sum[base + np1] -
-                                                         sum[base + np1] + sum[base + np3]);
-#else
-                const double normSum = (double)(int32_t)(sum[base + np0] - sum[base + np1] -
-                                                         sum[base + np2] + sum[base + np3]);
-#endif
-                const uint32_t normSqU = (uint32_t)sq[base + np0] - (uint32_t)sq[base + np1] -
-                                         (uint32_t)sq[base + np2] + (uint32_t)sq[base + np3];
-                const double nf2 = (double)narea * (double)normSqU - normSum * normSum;
-                if (!(nf2 > 0.0)) continue;
-                const float invNorm = (float)(1.0 / sqrt(nf2));
-                if (!((double)narea * (double)invNorm < 1e-1)) continue;
-                int pass = 1;
-                float last = 0.0f;
-                for (int si = 0; si < nStages; ++si) {
-                    float stageSum = 0.0f;
-                    for (int wj = 0; wj < stages[si].ntrees; ++wj) {
-#if OR_MUTANT == 2
-                        const int wi = stages[si].ntrees - 1 - wj;
-#else
-                        const int wi = wj;
-#endif
-                        const or_stump* st = &stumps[stages[si].first + wi];
-                        const or_feature* f = &features[st->feature];
-                        float raw = det_rect(&f->r[0], sum, iiW, base, mW) +
-                                    det_rect(&f->r[1], sum, iiW, base, mW);
-                        if (f->rectCount == 3) raw += det_rect(&f->r[2], sum, iiW, base, mW);
-                        const float resp = raw * invNorm;
-                        stageSum += (resp < st->theta) ? st->left : st->right;
-                    }
-                    if (stageSum < stages[si].threshold) {
-                        pass = 0;
-                        break;
-                    }
-                    last = stageSum;
-                }
-                if (pass) {
-                    or_det d;
-                    d.x = (int)((float)x * scale + 0.5f);
-                    d.y = (int)((float)y * scale + 0.5f);
-                    d.w = (int)((float)winW * scale + 0.5f);
-                    d.h = (int)((float)winH * scale + 0.5f);
-                    d.scaleIdx = scaleIdx;
-                    d.score = last;
-                    if (count < cap) out[count] = d;
-                    ++count;
-                    if (count >= maxDet) goto done;
+                if (!pf[(size_t)ry * nRx + rx]) continue;
+                or_det d;
+                d.x = (int)((float)(rx * step) * scale + 0.5f);
+                d.y = (int)((float)(ry * step) * scale + 0.5f);
+                d.w = (int)((float)winW * scale + 0.5f);
+                d.h = (int)((float)winH * scale + 0.5f);
+                d.scaleIdx = scaleIdx;
+                d.score = lastSum[(size_t)ry * nRx + rx];
+                if (count < cap) out[count] = d;
+                ++count;
+                if (count >= maxDet) {
+                    free(pf);
+                    free(lastSum);
+                    goto done;
                 }
             }
         }
+        free(pf);
+        free(lastSum);
         scale *= sf;
         ++scaleIdx;
     }

@@ -158,6 +158,46 @@ def test_pgm_reader(tmp_path):
         assert lib().vj_pgm_peek(str(q).encode(), C.byref(w), C.byref(h)) == _native.VJ_INVALID_ARG
 
 
+def test_sample_files_and_pgm_list(tmp_path):
+    """loadRawSampleBin (src/main.cu:122-162) and collectPgmPaths
+    (src/main.cu:226-244) through the library: patch counts, the reference's
+    error messages, and every *.pgm under a tree in sorted order."""
+    from cs677s26_amd._native import lib
+    import ctypes as C
+    px = 24 * 24
+    data = np.random.default_rng(3).integers(0, 256, 3 * px, dtype=np.uint8)
+    good = tmp_path / "s.bin"
+    good.write_bytes(data.tobytes())
+    n = C.c_int()
+    assert lib().vj_samples_count(str(good).encode(), px, C.byref(n)) == 0 and n.value == 3
+    out = np.zeros(3 * px, np.uint8)
+    assert lib().vj_samples_load(str(good).encode(), px, out.ctypes.data, out.nbytes, C.byref(n)) == 0
+    assert n.value == 3 and np.array_equal(out, data)
+    assert lib().vj_samples_load(str(good).encode(), px, out.ctypes.data, out.nbytes - 1,
+                                 C.byref(n)) == _native.VJ_INVALID_ARG
+    (tmp_path / "empty.bin").write_bytes(b"")
+    (tmp_path / "ragged.bin").write_bytes(bytes(px + 1))
+    for name, msg in (("missing.bin", "Cannot open: "), ("empty.bin", "Empty file: "),
+                      ("ragged.bin", "File size is not divisible by samplePixels: ")):
+        path = str(tmp_path / name)
+        assert lib().vj_samples_count(path.encode(), px, C.byref(n)) == _native.VJ_INVALID_ARG
+        assert lib().vj_last_error().decode() == msg + path
+
+    root = tmp_path / "imgs"
+    for rel in ("b.pgm", "a/x.PGM", "a/y.txt", "a/deep/z.pgm", "c.pgm.bak", "a/deep/.pgm"):
+        (root / rel).parent.mkdir(parents=True, exist_ok=True)
+        (root / rel).write_bytes(b"P5\n1 1\n255\n\0")
+    need, cnt = C.c_size_t(), C.c_int()
+    assert lib().vj_pgm_list(str(root).encode(), None, 0, C.byref(need), C.byref(cnt)) == 0
+    buf = C.create_string_buffer(need.value)
+    assert lib().vj_pgm_list(str(root).encode(), buf, need.value, C.byref(need), C.byref(cnt)) == 0
+    got = buf.raw[:need.value].split(b"\0")[:-1]
+    want = sorted(str(root / r).encode() for r in ("b.pgm", "a/x.PGM", "a/deep/z.pgm"))
+    assert cnt.value == 3 and got == want
+    assert lib().vj_pgm_list(str(tmp_path / "nope").encode(), None, 0, C.byref(need), C.byref(cnt)) == 0
+    assert cnt.value == 0 and need.value == 0
+
+
 def test_no_device_is_reported_not_faked():
     if _native.device_count() > 0:
         pytest.skip("a GPU is present")
