@@ -396,6 +396,7 @@ struct CascadeSmem {
     int listCnt[2];
     int item;                                    // deep pass: current work item
     int tailBase;                                // deep pass: first tail item of the slice
+    int box[4];                                  // deep / tail item: window columns and rows it covers
     unsigned long long maskWords[32];            // final bits per tile row
 };
 
@@ -422,42 +423,52 @@ __device__ __forceinline__ TileGeo tile_geo(const LevelDesc* __restrict__ L, con
     return g;
 }
 
-// Stages the tile's integral footprint into LDS as f32, row by row: wave k
-// takes rows k, k + 4, ..., four rows per step, each lane columns lane and
-// lane + 64 of a row (a footprint row is at most 128 entries: the window is
-// at most 64 wide), so eight loads are in flight per lane before their
-// stores.
+// Stages rows [ra, rb) x columns [ca, cb) of the tile's integral footprint
+// (FH x FW entries, pitch FW) into LDS as f32, row by row: wave k takes rows
+// ra + k, ra + k + 4, ..., four rows per step, each lane columns ca + lane
+// and ca + lane + 64 of a row (a footprint row is at most 128 entries: the
+// window is at most 64 wide), so eight loads are in flight per lane before
+// their stores.  The first pass stages the whole footprint; a deep or tail
+// item only the box its windows cover.
 __device__ __forceinline__ void stage_footprint(const TileGeo& g, const uint32_t* __restrict__ iiS, float* lds,
-                                                int winW, int winH) {
+                                                int winW, int ra, int rb, int ca, int cb) {
     constexpr int kWaves = kCascadeBlock / kWave;
     const int FW = (g.TW - 1) * g.step + winW + 1;
-    const int FH = (g.TH - 1) * g.step + winH + 1;
     const int ix0 = g.wx0 * g.step, iy0 = g.wy0 * g.step;
     const int iw = g.lv.w + 1;
     const uint32_t* sumL = iiS + g.lv.iiOff;
     const int lane = lane_id(), wave = static_cast<int>(threadIdx.x) / kWave;
-    const int gx0 = ix0 + lane, gx1 = gx0 + kWave;
-    const bool c0 = lane < FW && gx0 <= g.lv.w;
-    const bool c1 = lane + kWave < FW && gx1 <= g.lv.w;
-    for (int r0 = wave; r0 < FH; r0 += 4 * kWaves) {
+    const int cx0 = ca + lane, cx1 = cx0 + kWave;
+    const bool c0 = cx0 < cb && ix0 + cx0 <= g.lv.w;
+    const bool c1 = cx1 < cb && ix0 + cx1 <= g.lv.w;
+    for (int r0 = ra + wave; r0 < rb; r0 += 4 * kWaves) {
         float v[4][2];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int r = r0 + u * kWaves, gy = iy0 + r;
-            const bool rowOk = r < FH && gy <= g.lv.h;
-            const uint32_t* row = sumL + static_cast<int64_t>(gy) * iw;
-            v[u][0] = (rowOk && c0) ? static_cast<float>(static_cast<int32_t>(row[gx0])) : 0.0f;
-            v[u][1] = (rowOk && c1) ? static_cast<float>(static_cast<int32_t>(row[gx1])) : 0.0f;
+            const bool rowOk = r < rb && gy <= g.lv.h;
+            const uint32_t* row = sumL + static_cast<int64_t>(gy) * iw + ix0;
+            v[u][0] = (rowOk && c0) ? static_cast<float>(static_cast<int32_t>(row[cx0])) : 0.0f;
+            v[u][1] = (rowOk && c1) ? static_cast<float>(static_cast<int32_t>(row[cx1])) : 0.0f;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int r = r0 + u * kWaves;
-            if (r < FH) {
-                if (lane < FW) lds[r * FW + lane] = v[u][0];
-                if (lane + kWave < FW) lds[r * FW + lane + kWave] = v[u][1];
+            if (r < rb) {
+                if (cx0 < cb) lds[r * FW + cx0] = v[u][0];
+                if (cx1 < cb) lds[r * FW + cx1] = v[u][1];
             }
         }
     }
+}
+
+// The footprint box of the windows in sm.box (tile-local window columns
+// [box[0], box[1]] and rows [box[2], box[3]]).
+__device__ __forceinline__ void stage_window_box(const TileGeo& g, const uint32_t* __restrict__ iiS, float* lds,
+                                                 const int* box, int winW, int winH) {
+    if (box[1] < box[0]) return;  // no windows
+    stage_footprint(g, iiS, lds, winW, box[2] * g.step, box[3] * g.step + winH + 1, box[0] * g.step,
+                    box[1] * g.step + winW + 1);
 }
 
 // Variance gate (exact int sum, u32 square-sum delta, f64 normaliser) into
@@ -764,7 +775,7 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(
         return;
     }
 
-    stage_footprint(g, iiS, lds, winW, winH);
+    stage_footprint(g, iiS, lds, winW, 0, (g.TH - 1) * g.step + winH + 1, 0, (g.TW - 1) * g.step + winW + 1);
     if (threadIdx.x < 32) sm.maskWords[threadIdx.x] = 0ull;
     if (threadIdx.x == 0) {
         sm.listCnt[0] = 0;
@@ -889,13 +900,16 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_deep_kernel(
     const int total = *nItems;
     const bool tail = stageTail < nStages;
     for (;;) {
-        if (threadIdx.x == 0) sm.item = atomicAdd(workCounter, 1);
+        if (threadIdx.x == 0) {
+            sm.item = atomicAdd(workCounter, 1);
+            sm.box[0] = sm.box[2] = INT_MAX;
+            sm.box[1] = sm.box[3] = -1;
+        }
         __syncthreads();
         const int item = sm.item;
         if (item >= total) break;
         const int t = items[item] / kDeepSlices, slice = items[item] % kDeepSlices;
         const TileGeo g = tile_geo(L, lvlIdx, tileOffs, nLv, t);
-        stage_footprint(g, iiS, lds, winW, winH);
         const int first = slice * kDeepSlice;
         const int m = min(kDeepSlice, tileCount[t] - first);
         const int off = tileSurvOff[t] + first;
@@ -904,11 +918,18 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_deep_kernel(
             const uint32_t widx = survIdx[off + threadIdx.x];
             sm.listIdx[threadIdx.x] = static_cast<uint16_t>(widx);
             sm.invTab[widx] = survInv[off + threadIdx.x];
+            const int wx = static_cast<int>(widx & static_cast<uint32_t>(g.TW - 1)), wy = static_cast<int>(widx >> g.lgc);
+            atomicMin(&sm.box[0], wx);
+            atomicMax(&sm.box[1], wx);
+            atomicMin(&sm.box[2], wy);
+            atomicMax(&sm.box[3], wy);
         }
         if (threadIdx.x == 0) {
             sm.listCnt[0] = m;
             sm.listCnt[1] = 0;
         }
+        __syncthreads();
+        stage_window_box(g, iiS, lds, sm.box, winW, winH);
         __syncthreads();
         const KStump* __restrict__ stumps = stumpsAll + static_cast<size_t>(g.step - 1) * nStumps;
         const int slot = run_stages(g, sm, lds, stumps, stages, stageSplit, stageTail, nStages, winW, scores,
@@ -956,23 +977,33 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_tail_kernel(
     __shared__ CascadeSmem sm;
     const int total = *tailTotal;
     for (;;) {
-        if (threadIdx.x == 0) sm.item = atomicAdd(workCounter, 1);
+        if (threadIdx.x == 0) {
+            sm.item = atomicAdd(workCounter, 1);
+            sm.box[0] = sm.box[2] = INT_MAX;
+            sm.box[1] = sm.box[3] = -1;
+        }
         __syncthreads();
         const int item = sm.item;
         if (item >= total) break;
         const int4 it = tailItems[item];
         const TileGeo g = tile_geo(L, lvlIdx, tileOffs, nLv, it.x);
-        stage_footprint(g, iiS, lds, winW, winH);
         if (threadIdx.x < 32) sm.maskWords[threadIdx.x] = 0ull;
         if (static_cast<int>(threadIdx.x) < it.z) {
             const uint32_t widx = survIdx[it.y + threadIdx.x];
             sm.listIdx[threadIdx.x] = static_cast<uint16_t>(widx);
             sm.invTab[widx] = survInv[it.y + threadIdx.x];
+            const int wx = static_cast<int>(widx & static_cast<uint32_t>(g.TW - 1)), wy = static_cast<int>(widx >> g.lgc);
+            atomicMin(&sm.box[0], wx);
+            atomicMax(&sm.box[1], wx);
+            atomicMin(&sm.box[2], wy);
+            atomicMax(&sm.box[3], wy);
         }
         if (threadIdx.x == 0) {
             sm.listCnt[0] = it.z;
             sm.listCnt[1] = 0;
         }
+        __syncthreads();
+        stage_window_box(g, iiS, lds, sm.box, winW, winH);
         __syncthreads();
         const KStump* __restrict__ stumps = stumpsAll + static_cast<size_t>(g.step - 1) * nStumps;
         run_stages(g, sm, lds, stumps, stages, stageTail, nStages, nStages, winW, scores, lds + leafOff, kLeafCap);
