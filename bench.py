@@ -18,6 +18,10 @@ the first pass over every tile, then the deep work-list kernels, the persistent
 last stages' few windows): algorithmic bytes per frame
 (12*(W+1)*(H+1) summed over levels, plus model and output bytes) over the
 phase's average duration from HIP events on the detect stream.
+``lds_roofline`` is the same phase against the LDS, where its stump
+evaluations gather their corners: four f32 reads per rectangle per (window,
+stump) evaluation, with the windows entering each stage counted by the engine
+in its counting mode (one extra call), over 256 CUs x 128 B per clock x 2.4 GHz.
 """
 import argparse
 import json
@@ -29,6 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+LDS_PEAK_GBS = 256 * 128 * 2.4
 METRIC = "detect Mpix/s (1080p, 38-stage); stump-search s/stage; p99 latency at QPS"
 
 
@@ -65,7 +70,16 @@ def detect_leg(args):
         eng.detectMultiScale(d_img, w, h, w, 1.25, 4, 24, 200000, True)
         phases += np.array(eng.phase_ms())
     eng.set_timing(False)
+    # one more call in counting mode: the windows entering each stage
+    eng.count_stages(True)
+    eng.detectMultiScale(d_img, w, h, w, 1.25, 4, 24, 200000, True)
+    enter = np.array(eng.stage_windows(), np.int64)
+    eng.count_stages(False)
     phases /= args.steps
+    # LDS gather bytes of the cascade: every (window, stump) evaluation reads
+    # four f32 corners per rectangle of its feature from the staged footprint
+    rects = feats["rectCount"][stumps["feature"]].astype(np.int64)
+    stage_bytes = np.array([16 * int(rects[g["first"]:g["first"] + g["ntrees"]].sum()) for g in stages], np.int64)
     return {
         "mpix": w * h / 1e6,
         "wall_s": wall,
@@ -73,6 +87,9 @@ def detect_leg(args):
         "detections": int(len(dets)),
         "stats": stats,
         "phase_ms": phases.tolist(),
+        "stage_windows": enter.tolist(),
+        "evals": int((enter * stages["ntrees"].astype(np.int64)).sum()),
+        "gather_bytes": int((enter * stage_bytes).sum()),
     }
 
 
@@ -456,6 +473,12 @@ def detect_fields(args):
                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                        "traffic": measured_traffic("cascade_kernel_detect1080"),
                        "algorithmic_bytes": casc_bytes, "avg_ms": casc_ms}
+    # the same phase against the LDS: its algorithmic gather bytes over the
+    # chip's LDS bandwidth (256 CUs x 128 B per clock x 2.4 GHz)
+    lds_gbs = d["gather_bytes"] / (casc_ms * 1e-3) / 1e9 if casc_ms > 0 else 0.0
+    out["lds_roofline"] = {"bound": "lds", "achieved": lds_gbs, "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                           "frac": lds_gbs / LDS_PEAK_GBS, "gather_bytes": d["gather_bytes"],
+                           "evals": d["evals"], "stage_windows": d["stage_windows"], "avg_ms": casc_ms}
     int_ms = d["phase_ms"][1] + d["phase_ms"][0]
     int_bytes = d["stats"]["integralBytes"]
     out["phases_ms"] = {"pyramid": d["phase_ms"][0], "integral": d["phase_ms"][1], "cascade": d["phase_ms"][2],

@@ -114,6 +114,8 @@ _PROTOS = {
     "vj_engine_last_stats": (I, [P, C.POINTER(DetectStats)]),
     "vj_engine_set_timing": (I, [P, I]),
     "vj_engine_phase_ms": (I, [P, C.POINTER(F)]),
+    "vj_engine_count_stages": (I, [P, I]),
+    "vj_engine_stage_windows": (I, [P, P, I, C.POINTER(I)]),
     "vj_engine_detect_batch": (I, [P, I, P, P, P, P, F, I, I, I, I, P, I, P, P]),
     "vj_engine_detect_footprint": (I, [P, I, I, F, I, I, C.POINTER(I64)]),
     "vj_engine_set_graphs": (I, [P, I]),

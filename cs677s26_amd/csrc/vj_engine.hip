@@ -531,11 +531,15 @@ __device__ __forceinline__ void gate_windows(const TileGeo& g, const uint32_t* _
 // "append".  Returns the counter slot holding the survivors of stage si1-1.
 // leaf (LDS, leafCap floats, or null) holds a pair-mode stage's leaf values
 // when they fit, so a sequential stage sum adds them up again instead of
-// re-evaluating the stumps.
+// re-evaluating the stumps.  With kCount, enter counts the windows entering
+// each stage (the engine's counting mode; the kernels of ordinary calls are
+// built without it).
 // Stage rule and window normaliser: src/main.cu:752-804.
+template <bool kCount>
 __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, const KStump* __restrict__ stumps,
                           const KStage* __restrict__ stages, int si0, int si1, int nStages, int winW,
-                          float* __restrict__ scores, float* leaf = nullptr, int leafCap = 0) {
+                          float* __restrict__ scores, unsigned long long* __restrict__ enter, float* leaf = nullptr,
+                          int leafCap = 0) {
     const int lane = lane_id();
     const int FW = (g.TW - 1) * g.step + winW + 1;
     auto settle = [&](bool pass, bool last, uint32_t widx, float stageSum, int nxt) {
@@ -564,6 +568,9 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
     for (int si = si0; si < si1; ++si) {
         const int n = sm.listCnt[cur];
         if (n == 0) break;
+        if constexpr (kCount) {
+            if (threadIdx.x == 0) atomicAdd(&enter[si], static_cast<unsigned long long>(n));
+        }
         const int nxt = cur ^ 1;
         const KStage st = stages[si];
         const bool last = (si == nStages - 1);
@@ -733,13 +740,14 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
 // survivors' mask bits are written here; otherwise the tile's mask pieces
 // are written as zero and its survivors (tile-local index and normaliser)
 // are appended to a global list for the deep pass.
+template <bool kCount>
 __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(
     const LevelDesc* __restrict__ L, const int32_t* __restrict__ lvlIdx, const int32_t* __restrict__ tileOffs,
     int nLv, int nTiles, const uint32_t* __restrict__ iiS, const uint64_t* __restrict__ iiQ,
     const KStump* __restrict__ stumpsAll, int nStumps, const KStage* __restrict__ stages, int nStages,
     int stageSplit, int winW, int winH, uint64_t* __restrict__ words, float* __restrict__ scores,
     int32_t* __restrict__ survTotal, uint16_t* __restrict__ survIdx, float* __restrict__ survInv,
-    int32_t* __restrict__ tileSurvOff, int32_t* __restrict__ tileCount) {
+    int32_t* __restrict__ tileSurvOff, int32_t* __restrict__ tileCount, unsigned long long* __restrict__ enter) {
     extern __shared__ float lds[];
     __shared__ CascadeSmem sm;
     __shared__ int survBase;
@@ -785,7 +793,8 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(
     gate_windows(g, iiS, iiQ, sm, winW, winH);
     __syncthreads();
     const KStump* __restrict__ stumps = stumpsAll + static_cast<size_t>(g.step - 1) * nStumps;
-    const int slot = run_stages(g, sm, lds, stumps, stages, 0, min(stageSplit, nStages), nStages, winW, scores);
+    const int slot =
+        run_stages<kCount>(g, sm, lds, stumps, stages, 0, min(stageSplit, nStages), nStages, winW, scores, enter);
 
     for (int r = threadIdx.x; r < g.TH; r += kCascadeBlock)
         if (g.wy0 + r < g.lv.scanH) write_mask(r, deep ? 0ull : sm.maskWords[r]);
@@ -817,6 +826,10 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(
 // work counter.
 constexpr int kDeepHist = 4, kDeepCursor = kDeepHist + kTileMaxWin + 1, kTailCtl = kDeepCursor + kTileMaxWin + 1;
 constexpr int kDeepCtlInts = kTailCtl + 2;
+
+__global__ __launch_bounds__(256) void zero_u64_kernel(unsigned long long* __restrict__ a, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = 0ull;
+}
 
 __global__ __launch_bounds__(256) void deep_reset_kernel(int32_t* __restrict__ ctl) {
     for (int i = threadIdx.x; i < kDeepCtlInts; i += blockDim.x) ctl[i] = 0;
@@ -887,6 +900,7 @@ __global__ __launch_bounds__(256) void deep_scatter_kernel(const int32_t* __rest
 // share words).  Otherwise the slice's survivors are written back over its
 // own entries of survIdx / survInv (no other item reads them) and appended
 // to the tail pass's work list in items of up to kTailSlice windows.
+template <bool kCount>
 __global__ __launch_bounds__(kCascadeBlock) void cascade_deep_kernel(
     const LevelDesc* __restrict__ L, const int32_t* __restrict__ lvlIdx, const int32_t* __restrict__ tileOffs,
     int nLv, const uint32_t* __restrict__ iiS, const KStump* __restrict__ stumpsAll, int nStumps,
@@ -894,7 +908,7 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_deep_kernel(
     unsigned long long* __restrict__ words, float* __restrict__ scores, uint16_t* __restrict__ survIdx,
     float* __restrict__ survInv, const int32_t* __restrict__ tileSurvOff, const int32_t* __restrict__ tileCount,
     const int32_t* __restrict__ items, const int32_t* __restrict__ nItems, int32_t* __restrict__ workCounter,
-    int4* __restrict__ tailItems, int32_t* __restrict__ tailTotal) {
+    int4* __restrict__ tailItems, int32_t* __restrict__ tailTotal, unsigned long long* __restrict__ enter) {
     extern __shared__ float lds[];
     __shared__ CascadeSmem sm;
     const int total = *nItems;
@@ -932,8 +946,8 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_deep_kernel(
         stage_window_box(g, iiS, lds, sm.box, winW, winH);
         __syncthreads();
         const KStump* __restrict__ stumps = stumpsAll + static_cast<size_t>(g.step - 1) * nStumps;
-        const int slot = run_stages(g, sm, lds, stumps, stages, stageSplit, stageTail, nStages, winW, scores,
-                                    lds + leafOff, kLeafCap);
+        const int slot = run_stages<kCount>(g, sm, lds, stumps, stages, stageSplit, stageTail, nStages, winW,
+                                            scores, enter, lds + leafOff, kLeafCap);
         if (tail) {
             const int n = sm.listCnt[slot];
             const int nt = (n + kTailSlice - 1) / kTailSlice;
@@ -966,13 +980,14 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_deep_kernel(
 // tail items (up to kTailSlice windows of one tile: tile, first survIdx
 // entry, count) from a counter, restage the tile's footprint, run stages
 // [stageTail, nStages) and OR the final bits into the mask words.
+template <bool kCount>
 __global__ __launch_bounds__(kCascadeBlock) void cascade_tail_kernel(
     const LevelDesc* __restrict__ L, const int32_t* __restrict__ lvlIdx, const int32_t* __restrict__ tileOffs,
     int nLv, const uint32_t* __restrict__ iiS, const KStump* __restrict__ stumpsAll, int nStumps,
     const KStage* __restrict__ stages, int nStages, int stageTail, int winW, int winH, int leafOff,
     unsigned long long* __restrict__ words, float* __restrict__ scores, const uint16_t* __restrict__ survIdx,
     const float* __restrict__ survInv, const int4* __restrict__ tailItems, const int32_t* __restrict__ tailTotal,
-    int32_t* __restrict__ workCounter) {
+    int32_t* __restrict__ workCounter, unsigned long long* __restrict__ enter) {
     extern __shared__ float lds[];
     __shared__ CascadeSmem sm;
     const int total = *tailTotal;
@@ -1006,7 +1021,8 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_tail_kernel(
         stage_window_box(g, iiS, lds, sm.box, winW, winH);
         __syncthreads();
         const KStump* __restrict__ stumps = stumpsAll + static_cast<size_t>(g.step - 1) * nStumps;
-        run_stages(g, sm, lds, stumps, stages, stageTail, nStages, nStages, winW, scores, lds + leafOff, kLeafCap);
+        run_stages<kCount>(g, sm, lds, stumps, stages, stageTail, nStages, nStages, winW, scores, enter,
+                           lds + leafOff, kLeafCap);
         for (int r = threadIdx.x; r < g.TH; r += kCascadeBlock) {
             const unsigned long long bits = sm.maskWords[r];
             if (bits) {
@@ -1889,6 +1905,9 @@ struct vj_engine {
     DevBuf<float> survInv;
     DevBuf<int32_t> tileSurvOff, tileCount, deepItems, deepCtl;  // deepCtl: total, items, counter, list scratch
     DevBuf<int4> tailItems;                    // tail pass: tile, first survivor entry, count
+    bool countStages = false;                  // counting mode (vj_engine_count_stages)
+    DevBuf<unsigned long long> stageEnter;     // counting mode: windows entering each stage
+    std::vector<int64_t> stageWindows;         // the same, of the last counted call
     DevBuf<vj_detection> dOut;
     PinnedBuf<vj_detection> hOut;
     PinnedBuf<int32_t> hCounts;
@@ -2289,7 +2308,17 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     const bool tail = deep && e->stageTail < e->nStages;
     const int stageTail = deep ? e->stageTail : e->nStages;
 
+    // counting mode: the windows entering each stage
+    unsigned long long* enter = nullptr;
+    if (e->countStages) {
+        VJ_TRY(e->stageEnter.ensure(static_cast<size_t>(e->nStages)));
+        enter = e->stageEnter.get();
+    }
     auto launch_all = [&](hipStream_t ls) -> int {
+        if (enter) {
+            hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, ls, enter, e->nStages);
+            VJ_HIP(hipGetLastError());
+        }
         if (e->timing) VJ_HIP(hipEventRecord(e->ev[0], ls));
         // 1. pyramid: one launch per resize depth (each depth needs the previous)
         for (size_t dpt = 0; dpt < P.resizeJobs.size(); ++dpt) {
@@ -2321,11 +2350,11 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
                 hipLaunchKernelGGL(deep_reset_kernel, dim3(1), dim3(256), 0, ls, e->deepCtl.get());
                 VJ_HIP(hipGetLastError());
             }
-            hipLaunchKernelGGL(cascade_kernel, dim3(P.nTiles), dim3(kCascadeBlock), ldsBytes, ls, dL, m32 + oLvl,
+            hipLaunchKernelGGL(enter ? cascade_kernel<true> : cascade_kernel<false>, dim3(P.nTiles), dim3(kCascadeBlock), ldsBytes, ls, dL, m32 + oLvl,
                                m32 + oTile, static_cast<int>(P.tileLvl.size()), P.nTiles, e->iiS.get(), e->iiQ.get(),
                                e->kst.get(), e->nStumps, e->kstage.get(), e->nStages, e->stageSplit, e->winW, e->winH,
                                e->words.get(), e->scores.get(), e->deepCtl.get(), e->survIdx.get(), e->survInv.get(),
-                               e->tileSurvOff.get(), e->tileCount.get());
+                               e->tileSurvOff.get(), e->tileCount.get(), enter);
             VJ_HIP(hipGetLastError());
             if (deep) {
                 const int listGrid = std::min((P.nTiles + 255) / 256, 1024);
@@ -2337,22 +2366,22 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
                 hipLaunchKernelGGL(deep_scatter_kernel, dim3(listGrid), dim3(256), 0, ls, e->tileCount.get(),
                                    P.nTiles, e->deepCtl.get(), e->deepItems.get());
                 VJ_HIP(hipGetLastError());
-                hipLaunchKernelGGL(cascade_deep_kernel, dim3(deepGrid), dim3(kCascadeBlock), deepLds, ls, dL,
+                hipLaunchKernelGGL(enter ? cascade_deep_kernel<true> : cascade_deep_kernel<false>, dim3(deepGrid), dim3(kCascadeBlock), deepLds, ls, dL,
                                    m32 + oLvl, m32 + oTile, static_cast<int>(P.tileLvl.size()), e->iiS.get(),
                                    e->kst.get(), e->nStumps, e->kstage.get(), e->nStages, e->stageSplit, stageTail,
                                    e->winW, e->winH, leafOff, reinterpret_cast<unsigned long long*>(e->words.get()),
                                    e->scores.get(), e->survIdx.get(), e->survInv.get(), e->tileSurvOff.get(),
                                    e->tileCount.get(), e->deepItems.get(), e->deepCtl.get() + 1,
-                                   e->deepCtl.get() + 2, e->tailItems.get(), e->deepCtl.get() + kTailCtl);
+                                   e->deepCtl.get() + 2, e->tailItems.get(), e->deepCtl.get() + kTailCtl, enter);
                 VJ_HIP(hipGetLastError());
             }
             if (tail) {
-                hipLaunchKernelGGL(cascade_tail_kernel, dim3(deepGrid), dim3(kCascadeBlock), deepLds, ls, dL,
+                hipLaunchKernelGGL(enter ? cascade_tail_kernel<true> : cascade_tail_kernel<false>, dim3(deepGrid), dim3(kCascadeBlock), deepLds, ls, dL,
                                    m32 + oLvl, m32 + oTile, static_cast<int>(P.tileLvl.size()), e->iiS.get(),
                                    e->kst.get(), e->nStumps, e->kstage.get(), e->nStages, stageTail, e->winW,
                                    e->winH, leafOff, reinterpret_cast<unsigned long long*>(e->words.get()), e->scores.get(),
                                    e->survIdx.get(), e->survInv.get(), e->tailItems.get(),
-                                   e->deepCtl.get() + kTailCtl, e->deepCtl.get() + kTailCtl + 1);
+                                   e->deepCtl.get() + kTailCtl, e->deepCtl.get() + kTailCtl + 1, enter);
                 VJ_HIP(hipGetLastError());
             }
         }
@@ -2375,7 +2404,7 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
         if (e->timing) VJ_HIP(hipEventRecord(e->ev[4], ls));
         return VJ_OK;
     };
-    if (e->timing || !e->graphs) return launch_all(s);
+    if (e->timing || e->countStages || !e->graphs) return launch_all(s);
 
     // Graph replay: the captured kernels read their per-call tables (levels,
     // jobs, offsets) from the device buffers uploaded above, so the graph is
@@ -2964,6 +2993,13 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
     VJ_HIP(hipStreamSynchronize(s));
     if (e->timing)
         for (int k = 0; k < 4; ++k) VJ_HIP(hipEventElapsedTime(&e->phaseMs[k], e->ev[k], e->ev[k + 1]));
+    if (e->countStages) {
+        std::vector<unsigned long long> cnt(static_cast<size_t>(e->nStages), 0ull);
+        if (e->stageEnter.get())
+            VJ_HIP(hipMemcpy(cnt.data(), e->stageEnter.get(), sizeof(unsigned long long) * cnt.size(),
+                             hipMemcpyDeviceToHost));
+        e->stageWindows.assign(cnt.begin(), cnt.end());
+    }
     int64_t total = 0;
     int64_t raw = 0;
     for (int i = 0; i < nImages; ++i) {
@@ -3163,6 +3199,19 @@ extern "C" int vj_engine_set_timing(vj_engine* e, int on) {
 extern "C" int vj_engine_phase_ms(vj_engine* e, float* ms) {
     VJ_CHECK(e && ms, "vj_engine_phase_ms: null");
     for (int k = 0; k < 4; ++k) ms[k] = e->phaseMs[k];
+    return VJ_OK;
+}
+
+extern "C" int vj_engine_count_stages(vj_engine* e, int on) {
+    VJ_CHECK(e, "vj_engine_count_stages: null");
+    e->countStages = on != 0;
+    return VJ_OK;
+}
+
+extern "C" int vj_engine_stage_windows(vj_engine* e, int64_t* out, int cap, int* nStages) {
+    VJ_CHECK(e && nStages && (out || cap == 0), "vj_engine_stage_windows: null");
+    *nStages = static_cast<int>(e->stageWindows.size());
+    for (int k = 0; k < std::min(cap, *nStages); ++k) out[k] = e->stageWindows[static_cast<size_t>(k)];
     return VJ_OK;
 }
 

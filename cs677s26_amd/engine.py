@@ -175,6 +175,18 @@ class FaceVisionEngine:
         check(lib().vj_engine_phase_ms(self.handle, a), "phase_ms")
         return list(a)
 
+    def count_stages(self, on=True):
+        """Counting mode: detect calls also count the windows entering each stage."""
+        check(lib().vj_engine_count_stages(self.handle, int(bool(on))), "count_stages")
+
+    def stage_windows(self):
+        """Windows that entered each cascade stage in the last counted detect."""
+        n = C.c_int()
+        check(lib().vj_engine_stage_windows(self.handle, None, 0, C.byref(n)), "stage_windows")
+        a = (C.c_int64 * max(1, n.value))()
+        check(lib().vj_engine_stage_windows(self.handle, a, n.value, C.byref(n)), "stage_windows")
+        return list(a)[:n.value]
+
     def last_stats(self):
         s = DetectStats()
         check(lib().vj_engine_last_stats(self.handle, C.byref(s)), "last_stats")

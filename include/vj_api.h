@@ -215,6 +215,15 @@ int vj_engine_last_stats(vj_engine* e, vj_detect_stats* out);
  * whole call, src/main.cu:2471-2537). */
 int vj_engine_set_timing(vj_engine* e, int on);
 int vj_engine_phase_ms(vj_engine* e, float* ms);
+/* Counting mode (no reference counterpart; instrumentation of the
+ * cascade's work per stage, for its LDS-gather roofline): while on, detect
+ * calls run cascade kernels that also count the windows entering each stage
+ * (slower; never captured in a graph).  vj_engine_stage_windows gives those
+ * counts for the last counted call (entry 0: the windows that pass the
+ * variance gate), summed over the call's images; *nStages is the cascade's
+ * stage count, out receives up to cap entries. */
+int vj_engine_count_stages(vj_engine* e, int on);
+int vj_engine_stage_windows(vj_engine* e, int64_t* out, int cap, int* nStages);
 
 /* Batched detect (no reference counterpart: the reference detects one image
  * per call).  nImages images, each through the same pyramid, integral,

@@ -35,7 +35,7 @@ def _bind(L):
     L.or_resize.argtypes = [P, i, i, i, P, i, i]
     L.or_crop_resize.argtypes = [P, i, i, i, i, i, i, i, i, i, P]
     L.or_levels.argtypes = [i, i, i, i, f, i, P, i]
-    L.or_detect_raw.argtypes = [P, i, P, i, P, i, i, i, P, i, i, i, f, i, i, P, i, P]
+    L.or_detect_raw.argtypes = [P, i, P, i, P, i, i, i, P, i, i, i, f, i, i, P, i, P, P]
     L.or_group.argtypes = [P, i, i, f, P]
     L.or_train_integrals.argtypes = [P, i, i, i, P, P, P]
     L.or_train_response.argtypes = [P, P, P, i, i]
@@ -170,8 +170,10 @@ def levels(w, h, sf=1.25, min_obj=24, win=24):
     return out
 
 
-def detect_raw(features, stumps, stages, img, sf=1.25, min_obj=24, max_det=200000, win=24):
-    """Raw detections in CPU order (src/main.cu:684-827) and windows visited."""
+def detect_raw(features, stumps, stages, img, sf=1.25, min_obj=24, max_det=200000, win=24, stage_enter=None):
+    """Raw detections in CPU order (src/main.cu:684-827) and windows visited;
+    stage_enter (an int64 array of len(stages), or None) receives the windows
+    that enter each stage."""
     img = np.ascontiguousarray(img, np.uint8)
     h, w = img.shape
     features = np.ascontiguousarray(features, FEATURE)
@@ -180,9 +182,13 @@ def detect_raw(features, stumps, stages, img, sf=1.25, min_obj=24, max_det=20000
     cap = max_det
     out = np.zeros(cap, DETECTION)
     vis = C.c_int64(0)
+    if stage_enter is not None:
+        assert stage_enter.dtype == np.int64 and len(stage_enter) == len(stages)
+        stage_enter[:] = 0
     n = lib().or_detect_raw(_p(features), len(features), _p(stumps), len(stumps), _p(stages),
                             len(stages), win, win, _p(img), w, h, w, C.c_float(sf), min_obj,
-                            max_det, _p(out), cap, C.byref(vis))
+                            max_det, _p(out), cap, C.byref(vis),
+                            None if stage_enter is None else _p(stage_enter))
     return out[:n].copy(), vis.value
 
 

@@ -272,10 +272,11 @@ static float det_rect(const or_rect* r, const uint32_t* sum, int iiW, int base, 
 
 /* One window at integral offset base: the variance gate, then every stage
  * in order, each stage sum in stump order; returns whether it passes all of
- * them, with the last stage sum in *last (src/main.cu:752-804). */
+ * them, with the last stage sum in *last (src/main.cu:752-804).  enter (or
+ * NULL) counts the stages the window enters. */
 static int det_window(const or_feature* features, const or_stump* stumps, const or_stage* stages, int nStages,
                       const uint32_t* sum, const uint64_t* sq, int iiW, int base, int mW, int narea,
-                      int np0, int np1, int np2, int np3, float* last) {
+                      int np0, int np1, int np2, int np3, float* last, int64_t* enter) {
 #if OR_MUTANT == 5
     const double normSum = (double)(int32_t)(sum[base + np0] - sum[base + np1] -
                                              sum[base + np1] + sum[base + np3]);
@@ -291,6 +292,10 @@ static int det_window(const or_feature* features, const or_stump* stumps, const 
     if (!((double)narea * (double)invNorm < 1e-1)) return 0;
     float lastSum = 0.0f;
     for (int si = 0; si < nStages; ++si) {
+        if (enter) {
+#pragma omp atomic
+            enter[si] += 1;
+        }
         float stageSum = 0.0f;
         for (int wj = 0; wj < stages[si].ntrees; ++wj) {
 #if OR_MUTANT == 2
@@ -318,7 +323,7 @@ int or_detect_raw(const or_feature* features, int nFeatures,
                   int winW, int winH,
                   const uint8_t* img, int w, int h, int stride,
                   float sf, int minObj, int maxDet,
-                  or_det* out, int cap, int64_t* visited) {
+                  or_det* out, int cap, int64_t* visited, int64_t* stageEnter) {
     (void)nFeatures;
     (void)nStumps;
     int count = 0;
@@ -381,7 +386,7 @@ int or_detect_raw(const or_feature* features, int nFeatures,
                 const int base = ry * step * iiW + rx * step;
                 pf[(size_t)ry * nRx + rx] = (uint8_t)det_window(features, stumps, stages, nStages, sum, sq, iiW,
                                                                 base, mW, narea, np0, np1, np2, np3,
-                                                                &lastSum[(size_t)ry * nRx + rx]);
+                                                                &lastSum[(size_t)ry * nRx + rx], stageEnter);
             }
         }
         for (int ry = 0; ry < nRy; ++ry) {

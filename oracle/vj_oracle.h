@@ -63,7 +63,9 @@ int or_levels(int imageW, int imageH, int winW, int winH, float scaleFactor,
 
 /* Raw multi-scale detection in (level, y, x) order, returning at maxDetections.
  * src/main.cu:684-827 (detectMultiScaleCpu) with evalDetectionFeatureHost
- * src/main.cu:504-543.  Returns the detection count (<= cap). */
+ * src/main.cu:504-543.  Returns the detection count (<= cap).  stageEnter
+ * (nStages entries, zeroed by the caller, or NULL) receives the windows that
+ * enter each stage, over the levels scanned. */
 int or_detect_raw(const or_feature* features, int nFeatures,
                   const or_stump* stumps, int nStumps,
                   const or_stage* stages, int nStages,
@@ -71,7 +73,7 @@ int or_detect_raw(const or_feature* features, int nFeatures,
                   const uint8_t* img, int w, int h, int stride,
                   float scaleFactor, int minObjectSize, int maxDetections,
                   or_det* out, int cap,
-                  int64_t* windowsVisited);
+                  int64_t* windowsVisited, int64_t* stageEnter);
 
 /* OpenCV-style grouping.  src/face_vision_engine.cu:585-616 (DisjointSet,
  * similarRects), :1401-1539 (groupRectanglesCpu).  Returns the output count. */

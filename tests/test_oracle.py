@@ -636,3 +636,23 @@ def test_mining_step_takes_each_images_best_boxes():
     for k in (0, 300, 777, 1023):
         i, d = used[k]
         assert np.array_equal(neg[k], O.crop_resize(images[i], (d["x"], d["y"], d["w"], d["h"])))
+
+
+def test_stage_enter_counts_match_prefix_cascades():
+    """The oracle's per-stage window counts: stage 0 is entered by the
+    windows that pass the variance gate, and stage s + 1 by exactly the
+    detections of the cascade cut after stage s (a window enters a stage iff
+    it passed every earlier one)."""
+    feats, stumps, stages = synth.blob_cascade((1, 10, 25, 25), seed=4)
+    img = synth.detect_image(320, 240, 8, 24, 96, seed=77)
+    cnt = np.zeros(len(stages), np.int64)
+    raw, _ = O.detect_raw(feats, stumps, stages, img, stage_enter=cnt)
+    gate = stages[:1].copy()
+    gate["ntrees"] = 0
+    gate["threshold"] = -np.inf
+    g, _ = O.detect_raw(feats, stumps, gate, img)
+    assert cnt[0] == len(g)
+    for s in range(len(stages) - 1):
+        pre, _ = O.detect_raw(feats, stumps, stages[:s + 1], img)
+        assert cnt[s + 1] == len(pre), s
+    assert len(raw) <= cnt[-1] and np.all(np.diff(cnt) <= 0)
