@@ -164,13 +164,15 @@ __global__ __launch_bounds__(64) void ii_colsum_kernel(const LevelDesc* __restri
 
 // Integral phase 2, one workgroup per (level, chunk of 256 columns): turns
 // the band column sums into exclusive vertical prefixes (column sums above
-// each band, in place) and records each band's total of them over the chunk.
+// each band, in place) and records each band's total of them over the chunk
+// (per-wave partial totals in LDS, 256 bands at a time).
 __global__ __launch_bounds__(256) void ii_vscan_kernel(const LevelDesc* __restrict__ L,
                                                        const int32_t* __restrict__ chunkOffs, int nLevels,
                                                        uint32_t* __restrict__ colS, uint32_t* __restrict__ colQ,
                                                        uint32_t* __restrict__ totS, uint64_t* __restrict__ totQ) {
-    __shared__ uint32_t partS[256][4];
-    __shared__ uint64_t partQ[256][4];
+    constexpr int kBandBlock = 256;
+    __shared__ uint32_t partS[kBandBlock][4];
+    __shared__ uint64_t partQ[kBandBlock][4];
     const int item = blockIdx.x;
     const int li = find_segment(chunkOffs, nLevels, item);
     const LevelDesc lv = L[li];
@@ -178,32 +180,36 @@ __global__ __launch_bounds__(256) void ii_vscan_kernel(const LevelDesc* __restri
     const int x = c * kChunkCols + threadIdx.x;
     const int wave = threadIdx.x >> 6;
     uint32_t s = 0, q = 0;
-    for (int b = 0; b < lv.nBands; ++b) {
-        uint32_t vs = s, vq = q;
-        if (x < lv.w) {
-            const int64_t i = lv.colOff + static_cast<int64_t>(b) * lv.w + x;
-            const uint32_t a = colS[i], aq = colQ[i];
-            colS[i] = s;
-            colQ[i] = q;
-            s += a;
-            q += aq;
-        } else {
-            vs = 0;
-            vq = 0;
+    for (int b0 = 0; b0 < lv.nBands; b0 += kBandBlock) {
+        const int b1 = min(lv.nBands, b0 + kBandBlock);
+        for (int b = b0; b < b1; ++b) {
+            uint32_t vs = s, vq = q;
+            if (x < lv.w) {
+                const int64_t i = lv.colOff + static_cast<int64_t>(b) * lv.w + x;
+                const uint32_t a = colS[i], aq = colQ[i];
+                colS[i] = s;
+                colQ[i] = q;
+                s += a;
+                q += aq;
+            } else {
+                vs = 0;
+                vq = 0;
+            }
+            const uint32_t ws = wave_sum(vs);
+            const uint64_t wq = wave_sum(static_cast<uint64_t>(vq));
+            if ((threadIdx.x & 63) == 0) {
+                partS[b - b0][wave] = ws;
+                partQ[b - b0][wave] = wq;
+            }
         }
-        const uint32_t ws = wave_sum(vs);
-        const uint64_t wq = wave_sum(static_cast<uint64_t>(vq));
-        if ((threadIdx.x & 63) == 0) {
-            partS[b][wave] = ws;
-            partQ[b][wave] = wq;
+        __syncthreads();
+        for (int b = b0 + static_cast<int>(threadIdx.x); b < b1; b += blockDim.x) {
+            totS[lv.totOff + static_cast<int64_t>(b) * lv.nChunks + c] =
+                partS[b - b0][0] + partS[b - b0][1] + partS[b - b0][2] + partS[b - b0][3];
+            totQ[lv.totOff + static_cast<int64_t>(b) * lv.nChunks + c] =
+                partQ[b - b0][0] + partQ[b - b0][1] + partQ[b - b0][2] + partQ[b - b0][3];
         }
-    }
-    __syncthreads();
-    for (int b = threadIdx.x; b < lv.nBands; b += blockDim.x) {
-        totS[lv.totOff + static_cast<int64_t>(b) * lv.nChunks + c] =
-            partS[b][0] + partS[b][1] + partS[b][2] + partS[b][3];
-        totQ[lv.totOff + static_cast<int64_t>(b) * lv.nChunks + c] =
-            partQ[b][0] + partQ[b][1] + partQ[b][2] + partQ[b][3];
+        __syncthreads();  // the partial totals are rewritten by the next block
     }
 }
 
@@ -235,12 +241,15 @@ __global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restric
     uint32_t* outS = iiS + lv.iiOff;
     uint64_t* outQ = iiQ + lv.iiOff;
 
-    // carry from the chunks to the left (a level has at most 32 chunks):
-    // band totals, one chunk per lane, and per-row totals, loaded four
-    // chunks at a time
+    // carry from the chunks to the left: band totals, one chunk per lane (64
+    // chunks at a time), and per-row totals, loaded four chunks at a time
     const int64_t tb = lv.totOff + static_cast<int64_t>(b) * lv.nChunks;
-    const uint32_t ccS = wave_sum(lane < c ? totS[tb + lane] : 0u);
-    const uint64_t ccQ = wave_sum(lane < c ? totQ[tb + lane] : uint64_t{0});
+    uint32_t ccS = 0;
+    uint64_t ccQ = 0;
+    for (int k0 = 0; k0 < c; k0 += kWave) {
+        ccS += wave_sum(k0 + lane < c ? totS[tb + k0 + lane] : 0u);
+        ccQ += wave_sum(k0 + lane < c ? totQ[tb + k0 + lane] : uint64_t{0});
+    }
     uint32_t rcS = 0, rcQ = 0;
     if (lane < kBandRows && y0 + lane < lv.h) {
         const int64_t r = lv.rowOff + static_cast<int64_t>(y0 + lane) * lv.nChunks;
@@ -1277,14 +1286,26 @@ __device__ __forceinline__ int image_of(const int32_t* __restrict__ imgBase, int
 // Sweep order without a sort, one workgroup per image.  A box's rank in
 // its image's (x, index) order is the number of the image's boxes with a
 // smaller x plus the number with the same x and a smaller index.  An x
-// histogram in LDS (one bin per column 0 .. w) is scanned into each x's
-// first rank; each x's boxes are listed (in any order) in a bucket at that
-// rank, and a box's ties are the bucket entries with a smaller index.  The
+// histogram in LDS (one bin per column 0 .. w, or per 2^s columns for an
+// image wider than kSweepMaxBins - 1) is scanned into each bin's first rank;
+// each bin's boxes are listed (in any order) in a bucket at that rank, and a
+// box's ties are the bucket entries before it in (x, index) order.  The
 // boxes are then placed in sweep order: sorted[base + rank] = the box's
 // edges, order[base + rank] = its index in its image.
 constexpr int kSweepThreads = 1024;
+constexpr int kSweepMaxBins = 8193;
 
-inline size_t sweep_lds_bytes(int maxW) { return sizeof(int32_t) * (2 * static_cast<size_t>(maxW + 1) + 1); }
+// Column shift of the x bins of an image w wide.
+__host__ __device__ inline int sweep_shift(int w) {
+    int sh = 0;
+    while ((w >> sh) + 1 > kSweepMaxBins) ++sh;
+    return sh;
+}
+
+inline size_t sweep_lds_bytes(int maxW) {
+    const size_t nb = static_cast<size_t>(maxW >> sweep_shift(maxW)) + 1;
+    return sizeof(int32_t) * (2 * nb + 1);
+}
 
 __global__ __launch_bounds__(kSweepThreads) void pair_sweep_kernel(const vj_detection* __restrict__ dets,
                                                                   const int32_t* __restrict__ counts,
@@ -1307,11 +1328,12 @@ __global__ __launch_bounds__(kSweepThreads) void pair_sweep_kernel(const vj_dete
     }
     const int n = grouped_boxes(counts, i, cap);
     if (n == 0) return;
-    const int nb = imgW[i] + 1;
+    const int W = imgW[i], sh = sweep_shift(W);
+    const int nb = (W >> sh) + 1;
     int32_t* cnt = xs;
     int32_t* first = xs + nb;
     const vj_detection* D = dets + imgBase[i];
-    auto bin = [nb](int x) { return min(max(x, 0), nb - 1); };
+    auto bin = [W, sh](int x) { return min(max(x, 0), W) >> sh; };
     for (int k = threadIdx.x; k < nb; k += kSweepThreads) cnt[k] = 0;
     __syncthreads();
     for (int g = threadIdx.x; g < n; g += kSweepThreads) atomicAdd(&cnt[bin(D[g].x)], 1);
@@ -1345,7 +1367,15 @@ __global__ __launch_bounds__(kSweepThreads) void pair_sweep_kernel(const vj_dete
         const int x = bin(d.x);
         const int b0 = first[x], b1 = first[x + 1];
         int32_t ties = 0;
-        for (int j = b0; j < b1; ++j) ties += B[j] < g;
+        if (sh == 0) {
+            for (int j = b0; j < b1; ++j) ties += B[j] < g;
+        } else {
+            for (int j = b0; j < b1; ++j) {
+                const int o = B[j];
+                const int ox = D[o].x;
+                ties += ox < d.x || (ox == d.x && o < g);
+            }
+        }
         const int64_t p = imgBase[i] + b0 + ties;
         sorted[p] = make_int4(d.x, d.y, d.w, d.h);
         order[p] = g;
@@ -2176,7 +2206,6 @@ int check_image_limits(const vj_engine* e, int w, int h) {
     VJ_CHECK(w <= e->maxW && h <= e->maxH, "image larger than the engine's maxImageW x maxImageH");
     VJ_CHECK(static_cast<int64_t>(w) * h * 255 < (int64_t{1} << 31),
              "image too large for exact 32-bit integrals (w*h*255 must be < 2^31)");
-    VJ_CHECK(h <= kBandRows * 256 && w <= 8192, "image dimensions above 8192 are not supported");
     return VJ_OK;
 }
 
@@ -2495,7 +2524,7 @@ extern "C" int vj_engine_create(int winW, int winH, int maxSamples, int maxImage
         return VJ_DEVICE_ERROR;
     }
     {
-        // the sweep kernel's x bins of an 8192-wide image take just over 64 KB of LDS
+        // the sweep kernel's x bins (at most kSweepMaxBins) take just over 64 KB of LDS
         static const hipError_t ldsAttr = hipFuncSetAttribute(reinterpret_cast<const void*>(pair_sweep_kernel),
                                                               hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
         if (ldsAttr != hipSuccess) {

@@ -32,7 +32,8 @@ def _engine(w, h, cascade, max_det=200000):
     return eng
 
 
-@pytest.mark.parametrize("w,h", [(24, 24), (65, 70), (641, 479), (1920, 1080), (257, 1031)])
+@pytest.mark.parametrize("w,h", [(24, 24), (65, 70), (641, 479), (1920, 1080), (257, 1031), (16500, 130),
+                                 (130, 9000)])
 def test_integral_single_image(w, h):
     say(f"integral single {w}x{h}")
     rng = np.random.default_rng(w * 7 + h)
@@ -88,6 +89,33 @@ def test_detect_config1_raw_and_grouped(cascade3):
     assert eng.last_stats()["imagesGroupedOnHost"] == 0  # grouped on the device
     st = eng.last_stats()
     assert st["windows"] == 653485 and st["levels"] == 14
+
+
+@pytest.mark.parametrize("w,h", [(20000, 200), (200, 12000)])
+def test_detect_sides_above_8192(cascade3, w, h):
+    """Images wider or taller than 8192 (the integral's chunk carries and
+    band prefixes in blocks; the sweep's x bins cover 4 columns each at
+    20,000 wide): raw and grouped boxes equal the oracle's."""
+    say(f"detect {w}x{h} 3-stage raw+grouped")
+    img = synth.detect_image(w, h, 60, 24, 96, seed=w + h)
+    eng = _engine(w, h, cascade3)
+    d = DeviceArray.from_host(img)
+    raw = eng.detectMultiScale(d, w, h, w, 1.25, 4, 24, 200000, applyGrouping=False)
+    want, _ = O.detect_raw(*cascade3, img)
+    assert len(want) > 100
+    assert det_tuples(raw) == det_tuples(want)
+    grouped = eng.detectMultiScale(d, w, h, w, 1.25, 4, 24, 200000, applyGrouping=True)
+    assert det_tuples(grouped) == det_tuples(O.group(want, 4))
+
+
+def test_detect_refuses_inexact_integral_sizes(cascade3):
+    """w * h * 255 must stay below 2^31 (exact 32-bit integral sums)."""
+    from cs677s26_amd._native import VJError
+    w, h = 30000, 300  # 30000 * 300 * 255 > 2^31
+    eng = _engine(w, h, cascade3)
+    d = DeviceArray.from_host(np.zeros((h, w), np.uint8))
+    with pytest.raises(VJError):
+        eng.detectMultiScale(d, w, h, w, 1.25, 4, 24, 200000, applyGrouping=False)
 
 
 def test_detect_truncates_in_scan_order(cascade3):
