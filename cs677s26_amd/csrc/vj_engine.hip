@@ -38,19 +38,85 @@ namespace vj {
 typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
 typedef uint64_t u64x2_a8 __attribute__((ext_vector_type(2), aligned(8)));
 
-// Pixel-centre bilinear resize of one pyramid level from its source level,
-// 4 output pixels per thread.  Same float sequence as the CPU path
+// Pixel-centre bilinear resize, the CPU path's float sequence
 // (src/main.cu:565-581): fx = (x + 0.5) * sx - 0.5, clamped floor, lerp in x
-// then y, (u8) clamp((int)(v + 0.5), 0, 255).
+// then y, (u8) clamp((int)(v + 0.5), 0, 255).  The taps of an output
+// coordinate c along one axis (source length n): i0 and i1 = min(i0 + 1,
+// n - 1) with the weight of i1.
+struct Taps {
+    int i0, i1;
+    float a;
+};
+__host__ __device__ inline Taps bilinear_taps(int c, float scale, int n) {
+    const float f = (static_cast<float>(c) + 0.5f) * scale - 0.5f;
+    int i0 = static_cast<int>(floorf(f));
+    i0 = i0 < 0 ? 0 : (i0 > n - 1 ? n - 1 : i0);
+    return Taps{i0, i0 + 1 < n - 1 ? i0 + 1 : n - 1, f - static_cast<float>(i0)};
+}
+__device__ __forceinline__ uint8_t bilinear_pixel(float p00, float p01, float p10, float p11, float ax, float ay) {
+    const float top = p00 + (p01 - p00) * ax;
+    const float bot = p10 + (p11 - p10) * ax;
+    const float val = top + (bot - top) * ay;
+    const int v = static_cast<int>(val + 0.5f);
+    return static_cast<uint8_t>(v < 0 ? 0 : (v > 255 ? 255 : v));
+}
+
+// The pyramid in a few launches: a resize group is up to kResizeDepth
+// consecutive levels of one image's resize chain (each level resized from
+// the one before, as the CPU path does), computed from the level before the
+// group.  The group's last level is cut into a grid of gx x gy tiles, and
+// every level of the group into the same grid of proportional pieces
+// [i * W / gx, (i + 1) * W / gx), so each workgroup owns one piece of every
+// level.  It computes, level by level into LDS, the region of each level
+// that its own pieces need (its piece plus the taps of the next level's
+// region), so no workgroup waits for another; it writes only its own pieces.
+// Every pixel is the same function of the same source pixels however often it
+// is recomputed, so the levels equal a level-by-level resize.
+constexpr int kResizeDepth = 6;
+constexpr int kResizeCap = 9216;  // pixels of one level's region in LDS
+constexpr int kResizeThreads = 256;
+
 struct ResizeJob {
-    int32_t level;    // destination level
-    int32_t itemOff;  // first work item of this job
+    int32_t src;                  // level the group resizes from
+    int32_t lv[kResizeDepth];     // the group's levels in chain order
+    int32_t n;                    // levels in the group
+    int32_t gx, gy;               // tile grid
+    int32_t itemOff;              // first work item (tile) of this job
 };
 
-__global__ __launch_bounds__(256) void resize_kernel(const LevelDesc* __restrict__ L,
-                                                     const ResizeJob* __restrict__ jobs, int nJobs,
-                                                     const float* __restrict__ rs,
-                                                     const int32_t* __restrict__ src) {
+// Half-open pixel ranges of one axis of every level of a group for tile i:
+// own[j] the tile's piece of level j (1-based; 0 is the source), need[j] what
+// the tile computes of level j, need[0] the source range it reads.
+struct AxisRanges {
+    int own0[kResizeDepth + 1], own1[kResizeDepth + 1];
+    int need0[kResizeDepth + 1], need1[kResizeDepth + 1];
+};
+// len[j] is level j's length on the axis, sc[j] the scale level j is resized
+// with (source over destination length), g the tile count, i the tile.
+// (Loops run over the fixed depth so that the arrays stay in registers.)
+__host__ __device__ inline void axis_ranges(int n, const int* len, const float* sc, int g, int i, AxisRanges& r) {
+#pragma unroll
+    for (int j = 1; j <= kResizeDepth; ++j) {
+        r.own0[j] = j <= n ? static_cast<int>(static_cast<int64_t>(i) * len[j] / g) : 0;
+        r.own1[j] = j <= n ? static_cast<int>(static_cast<int64_t>(i + 1) * len[j] / g) : 0;
+        r.need0[j] = r.own0[j];
+        r.need1[j] = r.own1[j];
+    }
+#pragma unroll
+    for (int j = kResizeDepth; j >= 1; --j) {
+        if (j > n) continue;
+        // taps of need[j] in level j - 1 (monotone in the coordinate)
+        const int lo = bilinear_taps(r.need0[j], sc[j], len[j - 1]).i0;
+        const int hi = bilinear_taps(r.need1[j] - 1, sc[j], len[j - 1]).i1 + 1;
+        r.need0[j - 1] = (j - 1 >= 1 && r.own0[j - 1] < lo) ? r.own0[j - 1] : lo;
+        r.need1[j - 1] = (j - 1 >= 1 && r.own1[j - 1] > hi) ? r.own1[j - 1] : hi;
+    }
+}
+
+__global__ __launch_bounds__(kResizeThreads) void resize_group_kernel(const LevelDesc* __restrict__ L,
+                                                                      const ResizeJob* __restrict__ jobs,
+                                                                      int nJobs, const float* __restrict__ rs) {
+    __shared__ uint8_t buf[2][kResizeCap];
     const int item = blockIdx.x;
     int lo = 0, hi = nJobs - 1;
     while (lo < hi) {
@@ -58,39 +124,69 @@ __global__ __launch_bounds__(256) void resize_kernel(const LevelDesc* __restrict
         if (jobs[mid].itemOff <= item) lo = mid;
         else hi = mid - 1;
     }
-    const int li = jobs[lo].level;
-    const LevelDesc dst = L[li];
-    const LevelDesc sl = L[src[li]];
-    const int segs = (dst.w + 1023) / 1024;
-    const int local = item - jobs[lo].itemOff;
-    const int y = local / segs;
-    const int x0 = (local - y * segs) * 1024 + threadIdx.x * 4;
-    if (y >= dst.h) return;
-    const float sx = rs[2 * li], sy = rs[2 * li + 1];
-    const float fy = (static_cast<float>(y) + 0.5f) * sy - 0.5f;
-    const int sy0 = max(0, min(static_cast<int>(floorf(fy)), sl.h - 1));
-    const int sy1 = min(sy0 + 1, sl.h - 1);
-    const float ay = fy - static_cast<float>(sy0);
-    const uint8_t* r0 = sl.pix + static_cast<size_t>(sy0) * sl.stride;
-    const uint8_t* r1 = sl.pix + static_cast<size_t>(sy1) * sl.stride;
-    uint8_t* out = const_cast<uint8_t*>(dst.pix) + static_cast<size_t>(y) * dst.stride;
+    const ResizeJob jb = jobs[lo];
+    const int local = item - jb.itemOff;
+    const int ty = local / jb.gx, tx = local - ty * jb.gx;
+    const int n = jb.n;
+    int W[kResizeDepth + 1], H[kResizeDepth + 1];
+    float sx[kResizeDepth + 1], sy[kResizeDepth + 1];
+    const LevelDesc s0 = L[jb.src];
+    W[0] = s0.w;
+    H[0] = s0.h;
+    sx[0] = sy[0] = 1.0f;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int x = x0 + k;
-        if (x >= dst.w) break;
-        const float fx = (static_cast<float>(x) + 0.5f) * sx - 0.5f;
-        const int sx0 = max(0, min(static_cast<int>(floorf(fx)), sl.w - 1));
-        const int sx1 = min(sx0 + 1, sl.w - 1);
-        const float ax = fx - static_cast<float>(sx0);
-        const float p00 = static_cast<float>(r0[sx0]);
-        const float p01 = static_cast<float>(r0[sx1]);
-        const float p10 = static_cast<float>(r1[sx0]);
-        const float p11 = static_cast<float>(r1[sx1]);
-        const float top = p00 + (p01 - p00) * ax;
-        const float bot = p10 + (p11 - p10) * ax;
-        const float val = top + (bot - top) * ay;
-        const int v = static_cast<int>(val + 0.5f);
-        out[x] = static_cast<uint8_t>(v < 0 ? 0 : (v > 255 ? 255 : v));
+    for (int j = 1; j <= kResizeDepth; ++j) {
+        const int li = jb.lv[j <= n ? j - 1 : 0];
+        W[j] = L[li].w;
+        H[j] = L[li].h;
+        sx[j] = rs[2 * li];
+        sy[j] = rs[2 * li + 1];
+    }
+    AxisRanges rx, ry;
+    axis_ranges(n, W, sx, jb.gx, tx, rx);
+    axis_ranges(n, H, sy, jb.gy, ty, ry);
+    const int lane = lane_id(), wave = static_cast<int>(threadIdx.x) / kWave;
+    constexpr int kWaves = kResizeThreads / kWave;
+#pragma unroll
+    for (int j = 1; j <= kResizeDepth; ++j) {
+        if (j > n) break;
+        const int x0 = rx.need0[j], nx = rx.need1[j] - x0;
+        const int y0 = ry.need0[j], ny = ry.need1[j] - y0;
+        if (nx * ny > kResizeCap) return;  // the plan sizes tiles so this never happens
+        const LevelDesc d = L[jb.lv[j - 1]];
+        uint8_t* out = const_cast<uint8_t*>(d.pix);
+        uint8_t* cur = buf[j & 1];
+        const uint8_t* prev = buf[(j - 1) & 1];
+        const int px0 = rx.need0[j - 1], pitch = rx.need1[j - 1] - px0, py0 = ry.need0[j - 1];
+        for (int yy = wave; yy < ny; yy += kWaves) {
+            const int y = y0 + yy;
+            const Taps ty2 = bilinear_taps(y, sy[j], H[j - 1]);
+            const bool ownRow = y >= ry.own0[j] && y < ry.own1[j];
+            for (int xx = lane; xx < nx; xx += kWave) {
+                const int x = x0 + xx;
+                const Taps tx2 = bilinear_taps(x, sx[j], W[j - 1]);
+                float p00, p01, p10, p11;
+                if (j == 1) {
+                    const uint8_t* r0 = s0.pix + static_cast<size_t>(ty2.i0) * s0.stride;
+                    const uint8_t* r1 = s0.pix + static_cast<size_t>(ty2.i1) * s0.stride;
+                    p00 = static_cast<float>(r0[tx2.i0]);
+                    p01 = static_cast<float>(r0[tx2.i1]);
+                    p10 = static_cast<float>(r1[tx2.i0]);
+                    p11 = static_cast<float>(r1[tx2.i1]);
+                } else {
+                    const uint8_t* r0 = prev + (ty2.i0 - py0) * pitch - px0;
+                    const uint8_t* r1 = prev + (ty2.i1 - py0) * pitch - px0;
+                    p00 = static_cast<float>(r0[tx2.i0]);
+                    p01 = static_cast<float>(r0[tx2.i1]);
+                    p10 = static_cast<float>(r1[tx2.i0]);
+                    p11 = static_cast<float>(r1[tx2.i1]);
+                }
+                const uint8_t v = bilinear_pixel(p00, p01, p10, p11, tx2.a, ty2.a);
+                if (j < n) cur[yy * nx + xx] = v;
+                if (ownRow && x >= rx.own0[j] && x < rx.own1[j]) out[static_cast<size_t>(y) * d.stride + x] = v;
+            }
+        }
+        __syncthreads();  // level j's region is complete before level j + 1 reads it
     }
 }
 
@@ -1172,8 +1268,8 @@ struct Plan {
     std::vector<int32_t> itemOffs, chunkOffs;
     std::vector<int64_t> lvWordOff;
     std::vector<int64_t> imgWordOff;
-    std::vector<std::vector<ResizeJob>> resizeJobs;  // per depth
-    std::vector<int32_t> resizeItems;                // per depth
+    std::vector<std::vector<ResizeJob>> resizeJobs;  // per resize group
+    std::vector<int32_t> resizeItems;                // per resize group: tiles
     std::vector<int32_t> tileLvl, tileOff;  // scanned levels and their first cascade tile
     int32_t nTiles = 0;
     int32_t maxStep = 0;
@@ -2044,6 +2140,55 @@ struct ImageIn {
     int w, h, stride;
 };
 
+// Tile grid of a resize group: the largest tile side (on the group's last
+// level) whose every tile's level regions fit in LDS and that gives at least
+// kResizeTiles tiles, else the smallest side that fits (most tiles: the
+// small trailing levels have little work per launch, so parallelism counts
+// more than the recomputed margins).  The regions are computed with the
+// kernel's own arithmetic.
+constexpr int kResizeTiles = 256;
+void resize_grid(const Plan& P, ResizeJob& jb) {
+    int W[kResizeDepth + 1], H[kResizeDepth + 1];
+    float sx[kResizeDepth + 1], sy[kResizeDepth + 1];
+    const LevelDesc& s0 = P.levels[static_cast<size_t>(jb.src)];
+    W[0] = s0.w;
+    H[0] = s0.h;
+    sx[0] = sy[0] = 1.0f;
+    for (int j = 1; j <= kResizeDepth; ++j) {
+        const int li = jb.lv[j <= jb.n ? j - 1 : 0];
+        W[j] = P.levels[static_cast<size_t>(li)].w;
+        H[j] = P.levels[static_cast<size_t>(li)].h;
+        sx[j] = P.rs[2 * static_cast<size_t>(li)];
+        sy[j] = P.rs[2 * static_cast<size_t>(li) + 1];
+    }
+    // widest region of every level over the tiles of one axis
+    auto widest = [&](const int* len, const float* sc, int g, int* out) {
+        for (int j = 0; j <= kResizeDepth; ++j) out[j] = 0;
+        AxisRanges r;
+        for (int i = 0; i < g; ++i) {
+            axis_ranges(jb.n, len, sc, g, i, r);
+            for (int j = 1; j <= jb.n; ++j) out[j] = std::max(out[j], r.need1[j] - r.need0[j]);
+        }
+    };
+    static constexpr int kSides[] = {64, 48, 32, 24, 16, 12, 8, 6, 4, 3, 2, 1};
+    const int Wn = W[jb.n], Hn = H[jb.n];
+    int pick = -1;
+    for (int side : kSides) {
+        const int gx = (Wn + side - 1) / side, gy = (Hn + side - 1) / side;
+        int wx[kResizeDepth + 1], wy[kResizeDepth + 1];
+        widest(W, sx, gx, wx);
+        widest(H, sy, gy, wy);
+        bool fits = true;
+        for (int j = 1; j <= jb.n; ++j) fits = fits && static_cast<int64_t>(wx[j]) * wy[j] <= kResizeCap;
+        if (!fits) continue;
+        pick = side;
+        if (side <= 4 || static_cast<int64_t>(gx) * gy >= kResizeTiles) break;
+    }
+    // a one-pixel tile needs at most 2^n source pixels per axis, so one fits
+    jb.gx = (Wn + pick - 1) / pick;
+    jb.gy = (Hn + pick - 1) / pick;
+}
+
 // Builds the level table for a set of images.  Level 0 of an image is the
 // input itself (stride as given); each further level is resized from the
 // previous scanned level (or from the input when level 0 is skipped).
@@ -2189,15 +2334,24 @@ int build_plan(const vj_engine* e, const std::vector<ImageIn>& imgs, float sf, i
     P.totElems = totOff;
     P.rowElems = rowOff;
     P.nWords = wordOff;
-    P.resizeJobs.assign(static_cast<size_t>(maxDepth), {});
-    P.resizeItems.assign(static_cast<size_t>(maxDepth), 0);
-    for (const Pending& r : resized) {
-        const LevelDesc& d = P.levels[static_cast<size_t>(r.level)];
-        const int segs = (d.w + 1023) / 1024;
-        auto& jobs = P.resizeJobs[static_cast<size_t>(r.depth - 1)];
-        int32_t& n = P.resizeItems[static_cast<size_t>(r.depth - 1)];
-        jobs.push_back({r.level, n});
-        n += segs * d.h;
+    // resize groups: depths [g * kResizeDepth + 1, (g + 1) * kResizeDepth] of
+    // every image's chain; consecutive levels of one image in one group form
+    // a job (resized levels are listed image by image, depth by depth)
+    const int nGroups = (maxDepth + kResizeDepth - 1) / kResizeDepth;
+    P.resizeJobs.assign(static_cast<size_t>(nGroups), {});
+    P.resizeItems.assign(static_cast<size_t>(nGroups), 0);
+    for (size_t k = 0; k < resized.size();) {
+        const int img = P.levels[static_cast<size_t>(resized[k].level)].image;
+        const int g = (resized[k].depth - 1) / kResizeDepth;
+        ResizeJob jb{};
+        jb.src = P.src[static_cast<size_t>(resized[k].level)];
+        while (k < resized.size() && P.levels[static_cast<size_t>(resized[k].level)].image == img &&
+               (resized[k].depth - 1) / kResizeDepth == g)
+            jb.lv[jb.n++] = resized[k++].level;
+        resize_grid(P, jb);
+        jb.itemOff = P.resizeItems[static_cast<size_t>(g)];
+        P.resizeItems[static_cast<size_t>(g)] += jb.gx * jb.gy;
+        P.resizeJobs[static_cast<size_t>(g)].push_back(jb);
     }
     return VJ_OK;
 }
@@ -2349,12 +2503,12 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
             VJ_HIP(hipGetLastError());
         }
         if (e->timing) VJ_HIP(hipEventRecord(e->ev[0], ls));
-        // 1. pyramid: one launch per resize depth (each depth needs the previous)
-        for (size_t dpt = 0; dpt < P.resizeJobs.size(); ++dpt) {
-            const int n = static_cast<int>(P.resizeJobs[dpt].size());
-            if (!n || !P.resizeItems[dpt]) continue;
-            hipLaunchKernelGGL(resize_kernel, dim3(P.resizeItems[dpt]), dim3(256), 0, ls, dL,
-                               dJobs + jobStart[dpt], n, dRs, m32 + oSrc);
+        // 1. pyramid: one launch per resize group (each group needs the one before)
+        for (size_t grp = 0; grp < P.resizeJobs.size(); ++grp) {
+            const int n = static_cast<int>(P.resizeJobs[grp].size());
+            if (!n || !P.resizeItems[grp]) continue;
+            hipLaunchKernelGGL(resize_group_kernel, dim3(P.resizeItems[grp]), dim3(kResizeThreads), 0, ls, dL,
+                               dJobs + jobStart[grp], n, dRs);
             VJ_HIP(hipGetLastError());
         }
         if (e->timing) VJ_HIP(hipEventRecord(e->ev[1], ls));
@@ -2440,10 +2594,10 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     // valid while every grid size, scalar argument and buffer address matches.
     std::vector<int64_t> key;
     auto k = [&key](int64_t v) { key.push_back(v); };
-    for (size_t dpt = 0; dpt < P.resizeJobs.size(); ++dpt) {
-        k(static_cast<int64_t>(P.resizeJobs[dpt].size()));
-        k(P.resizeItems[dpt]);
-        k(static_cast<int64_t>(jobStart[dpt]));
+    for (size_t grp = 0; grp < P.resizeJobs.size(); ++grp) {
+        k(static_cast<int64_t>(P.resizeJobs[grp].size()));
+        k(P.resizeItems[grp]);
+        k(static_cast<int64_t>(jobStart[grp]));
     }
     for (int64_t v : {int64_t(e->stageSplit), int64_t(stageTail), int64_t(deepGrid), int64_t(nLevels), int64_t(P.nItems), int64_t(P.nChunkItems), int64_t(P.nTiles),
                       int64_t(P.tileLvl.size()), int64_t(ldsBytes), int64_t(nBlocks), int64_t(nImages), int64_t(cap),
