@@ -20,7 +20,12 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <array>
+#include <chrono>
 #include <cstring>
+#include <iterator>
+#include <map>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -562,7 +567,7 @@ __device__ __forceinline__ int64_t select_lanes(int64_t a, int64_t b, uint64_t m
 // kWide: entries are plain 16-bit sample indices and the split flags are the
 // bits of the feature's mask row (byte k/8, bit k%8 for position k), read one
 // byte per lane per step; otherwise bit 15 of an entry is its flag.
-template <bool kSplit, bool kWide, typename W>
+template <bool kSplit, bool kWide, int kAhead = 4, typename W>
 __device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, const uint8_t* __restrict__ mrow,
                                              int n, const W* lsw, int64_t totPos, int64_t totNeg, int lane,
                                              int64_t& fE, int32_t& fK, int32_t& fP) {
@@ -634,7 +639,6 @@ __device__ __forceinline__ void scan_feature(const uint16_t* __restrict__ row, c
     // (rows are padded to 256 entries, so a lane's 16-byte load starting
     // below n stays inside its row)
     constexpr int kStep = kWave * kScanPer;
-    constexpr int kAhead = 4;
     auto load8 = [&](int base) { return *reinterpret_cast<const uint4*>(row + base + lane * kScanPer); };
     auto loadm = [&](int base) -> uint32_t {
         if constexpr (kWide) return mrow[(base >> 3) + lane];
@@ -746,7 +750,7 @@ __device__ __forceinline__ void load_weights(const float* __restrict__ sw32, int
     }
 }
 
-template <typename W>
+template <typename W, int kAhead>
 __global__ __launch_bounds__(kSearchThreads) __attribute__((amdgpu_waves_per_eu(1, 4))) void search_kernel(const uint16_t* __restrict__ ord, int64_t pitch,
                                                                int n, int nFeatures,
                                                                const float* __restrict__ sw32,
@@ -767,7 +771,8 @@ __global__ __launch_bounds__(kSearchThreads) __attribute__((amdgpu_waves_per_eu(
         if (used && used[f]) continue;
         int64_t fE;
         int32_t fK, fP;
-        scan_feature<false, false>(ord + static_cast<int64_t>(f) * pitch, nullptr, n, lsw, totPos, totNeg, lane, fE, fK, fP);
+        scan_feature<false, false, kAhead>(ord + static_cast<int64_t>(f) * pitch, nullptr, n, lsw, totPos, totNeg, lane,
+                                           fE, fK, fP);
         if (fK >= 0 && (fE < bE || (fE == bE && f < bF))) {
             bE = fE;
             bF = f;
@@ -776,7 +781,8 @@ __global__ __launch_bounds__(kSearchThreads) __attribute__((amdgpu_waves_per_eu(
     // the wave's winner again, now with its split and parity (same error)
     if (bF != INT_MAX) {
         int64_t fE;
-        scan_feature<true, false>(ord + static_cast<int64_t>(bF) * pitch, nullptr, n, lsw, totPos, totNeg, lane, fE, bK, bP);
+        scan_feature<true, false, kAhead>(ord + static_cast<int64_t>(bF) * pitch, nullptr, n, lsw, totPos, totNeg, lane,
+                                          fE, bK, bP);
     }
     if (lane == 0) wb[wave] = {bE, bK, bP, bF, 0};
     __syncthreads();
@@ -825,6 +831,7 @@ __global__ __launch_bounds__(1024) void wide_weights_kernel(const float* __restr
 
 // search_kernel over the wide layout (plain indices, split mask rows, weights
 // gathered from global memory).
+template <int kAhead>
 __global__ __launch_bounds__(kSearchThreads) void search_wide_kernel(const uint16_t* __restrict__ ord,
                                                                     const uint64_t* __restrict__ mask, int64_t pitch,
                                                                     int n, int nFeatures,
@@ -844,7 +851,7 @@ __global__ __launch_bounds__(kSearchThreads) void search_wide_kernel(const uint1
         if (used && used[f]) continue;
         int64_t fE;
         int32_t fK, fP;
-        scan_feature<false, true>(ord + static_cast<int64_t>(f) * pitch,
+        scan_feature<false, true, kAhead>(ord + static_cast<int64_t>(f) * pitch,
                                   reinterpret_cast<const uint8_t*>(mask + static_cast<int64_t>(f) * mPitch), n, q,
                                   totPos, totNeg, lane, fE, fK, fP);
         if (fK >= 0 && (fE < bE || (fE == bE && f < bF))) {
@@ -854,7 +861,7 @@ __global__ __launch_bounds__(kSearchThreads) void search_wide_kernel(const uint1
     }
     if (bF != INT_MAX) {
         int64_t fE;
-        scan_feature<true, true>(ord + static_cast<int64_t>(bF) * pitch,
+        scan_feature<true, true, kAhead>(ord + static_cast<int64_t>(bF) * pitch,
                                  reinterpret_cast<const uint8_t*>(mask + static_cast<int64_t>(bF) * mPitch), n, q,
                                  totPos, totNeg, lane, fE, bK, bP);
     }
@@ -1342,6 +1349,16 @@ __global__ __launch_bounds__(256) void select_kernel(const FeatureBestSplit* __r
 
 using namespace vj;
 
+// How a stage sorts and scans; every schedule gives the same rounds (the
+// sums are exact integers and the sort is a stable total order), only the
+// speed differs.
+struct TrainSchedule {
+    int wide = 0;    // global-memory sort and split-mask layout (always above the LDS sort limit)
+    int ahead = 4;   // search: steps of 16-byte order loads per lane in flight (2, 4 or 8)
+    int blocks = 2;  // search: workgroups per CU (1, 2 or 4)
+    int packed = 0;  // search: packed 32-bit weights in LDS even where int64 weights fit
+};
+
 struct vj_trainer {
     int winW = 24, winH = 24, maxSamples = 0, maxTile = 0;
     // per-op binding
@@ -1384,19 +1401,28 @@ struct vj_trainer {
     PinnedBuf<Calib> hCalib;
     PinnedBuf<uint32_t> hCounter;
     int searchBlocks = 0;
+    int cus = 256;
+    // the stage's sort and scan schedule (see TrainSchedule)
+    TrainSchedule sched;
+    bool autotune = false;   // pick the schedule per (samples, features) bucket by timing
+    bool forced = false;     // a schedule set by the caller
 };
 
 namespace {
 
-int search_blocks() {
+int device_cus() {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
         hipDeviceProp_t p;
         if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) cus = p.multiProcessorCount;
     }
     (void)hipGetLastError();
-    return 2 * cus;
+    return cus;
 }
+
+constexpr int kAheadChoices[] = {2, 4, 8};
+constexpr int kBlockChoices[] = {1, 2, 4};
+constexpr int kMaxBlocksPerCu = 4;
 
 // The wide presort over count features into out (rows outPitch apart) and
 // mask (rows outPitch / 64 words apart; mode 0 only).
@@ -1440,10 +1466,13 @@ int set_lds_limits() {
     if (done) return VJ_OK;
     VJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(presort_kernel),
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024));
-    VJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(search_kernel<uint32_t>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 2048));
-    VJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(search_kernel<int64_t>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 2048));
+    for (const void* k : {reinterpret_cast<const void*>(search_kernel<uint32_t, 2>),
+                          reinterpret_cast<const void*>(search_kernel<uint32_t, 4>),
+                          reinterpret_cast<const void*>(search_kernel<uint32_t, 8>),
+                          reinterpret_cast<const void*>(search_kernel<int64_t, 2>),
+                          reinterpret_cast<const void*>(search_kernel<int64_t, 4>),
+                          reinterpret_cast<const void*>(search_kernel<int64_t, 8>)})
+        VJ_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 2048));
     VJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(search_each_kernel),
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 2048));
     done = true;
@@ -1472,7 +1501,8 @@ extern "C" int vj_trainer_create(int winW, int winH, int maxSamples, int maxTile
     t->winH = winH;
     t->maxSamples = maxSamples;
     t->maxTile = maxTile;
-    t->searchBlocks = search_blocks();
+    t->cus = device_cus();
+    t->searchBlocks = 2 * t->cus;
     int st = t->hRound.ensure(1);
     if (st == VJ_OK) st = t->hCalib.ensure(1);
     if (st == VJ_OK) st = t->hCounter.ensure(1);
@@ -1593,6 +1623,213 @@ extern "C" int vj_trainer_select_best(vj_trainer* t, const void* dBest, int begi
     return VJ_OK;
 }
 
+namespace {
+
+// The sample order of features [0, count) of the stage, in the layout of
+// t->sched (LDS radix sort with flags in the entries, or the global sort with
+// split-mask rows).
+int presort_stage(vj_trainer* t, int count, hipStream_t s) {
+    const int n = t->n;
+    t->wide = t->sched.wide != 0;
+    if (t->wide) {
+        VJ_TRY(t->mask.ensure(static_cast<size_t>(t->pitch / 64) * t->nFeat));
+        VJ_TRY(t->wq.ensure(static_cast<size_t>(n)));
+        VJ_TRY(t->wtot.ensure(2));
+        return launch_presort_wide(t, t->feats, t->sumT.get(), t->inv.get(), nullptr, n, count, t->ord.get(),
+                                   t->pitch, t->mask.get(), 0, s);
+    }
+    hipLaunchKernelGGL(presort_kernel, dim3(count), dim3(kSortThreads), presort_lds(n), s, t->feats, 0,
+                       t->sumT.get(), t->inv.get(), nullptr, n, t->ord.get(), t->pitch, 0);
+    VJ_HIP(hipGetLastError());
+    return VJ_OK;
+}
+
+// One round's threshold scan over features [0, count) with schedule sc (the
+// layout of the presorted orders, t->wide, is the one presort_stage wrote);
+// blocks receives the grid (the entries of t->blockBest written).
+int launch_search(vj_trainer* t, const TrainSchedule& sc, int count, int& blocks, hipStream_t s) {
+    const int n = t->n;
+    blocks = std::max(1, std::min((count + 15) / 16, sc.blocks * t->cus));
+    const dim3 g(static_cast<unsigned>(blocks)), b(kSearchThreads);
+    if (t->wide) {
+        hipLaunchKernelGGL(wide_weights_kernel, dim3(1), dim3(1024), 0, s, t->sw32.get(), n, t->wq.get(),
+                           t->wtot.get());
+        VJ_HIP(hipGetLastError());
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, g, b, 0, s, t->ord.get(), t->mask.get(), t->pitch, n, count, t->wq.get(),
+                               t->wtot.get(), t->used.get(), t->blockBest.get());
+        };
+        if (sc.ahead == 2) go(search_wide_kernel<2>);
+        else if (sc.ahead == 8) go(search_wide_kernel<8>);
+        else go(search_wide_kernel<4>);
+    } else {
+        // int64 weights in LDS when they fit beside the kernel's static LDS
+        // and the schedule does not ask for packed words
+        const bool i64 = !sc.packed && static_cast<size_t>(n) * 8 <= kSearchWideLds;
+        const size_t lds = static_cast<size_t>(n) * (i64 ? 8 : 4);
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, g, b, lds, s, t->ord.get(), t->pitch, n, count, t->sw32.get(), t->used.get(),
+                               t->blockBest.get());
+        };
+        if (i64) {
+            if (sc.ahead == 2) go(search_kernel<int64_t, 2>);
+            else if (sc.ahead == 8) go(search_kernel<int64_t, 8>);
+            else go(search_kernel<int64_t, 4>);
+        } else {
+            if (sc.ahead == 2) go(search_kernel<uint32_t, 2>);
+            else if (sc.ahead == 8) go(search_kernel<uint32_t, 8>);
+            else go(search_kernel<uint32_t, 4>);
+        }
+    }
+    VJ_HIP(hipGetLastError());
+    return VJ_OK;
+}
+
+// Schedules tuned per (sample count / 1024, log2 feature count, window).
+// The first stage of a bucket picks the presort layout from a sample of its
+// features (the presort and one default scan of the sample in each layout,
+// scaled to one presort plus kTuneRounds rounds of the whole stage), presorts
+// every feature in that layout, then times every scan candidate on the whole
+// stage and keeps the fastest.  Later stages of the bucket reuse the pick.
+constexpr int kTuneSample = 16384;
+constexpr int kTuneRounds = 50;
+std::mutex gTuneMu;
+std::map<std::array<int, 5>, TrainSchedule> gTuned;
+
+std::array<int, 5> tune_key(const vj_trainer* t) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) (void)hipGetLastError();
+    int lgF = 0;
+    while ((1 << lgF) < t->nFeat) ++lgF;
+    return {dev, (t->n + 1023) / 1024, lgF, t->winW, t->winH};
+}
+
+// The best of three timed runs of launch on stream s, in ms.
+template <typename F>
+int time_best(hipStream_t s, F&& launch, float& ms) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    VJ_HIP(hipEventCreate(&e0));
+    const hipError_t ce = hipEventCreate(&e1);
+    if (ce != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(ce, "hipEventCreate", __FILE__, __LINE__);
+    }
+    int st = VJ_OK;
+    float best = 1e30f;
+    for (int rep = 0; rep < 3 && st == VJ_OK; ++rep) {
+        float m = 0.0f;
+        hipError_t he = hipEventRecord(e0, s);
+        if (he == hipSuccess) st = launch();
+        if (he == hipSuccess && st == VJ_OK) he = hipEventRecord(e1, s);
+        if (he == hipSuccess && st == VJ_OK) he = hipEventSynchronize(e1);
+        if (he == hipSuccess && st == VJ_OK) he = hipEventElapsedTime(&m, e0, e1);
+        if (he != hipSuccess && st == VJ_OK) st = hip_fail(he, "tuning timer", __FILE__, __LINE__);
+        best = std::min(best, m);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ms = best;
+    return st;
+}
+
+// The presort layout of the stage (t->sched.wide) from a sample of features.
+int tune_layout(vj_trainer* t, hipStream_t s) {
+    if (t->n > kSortMaxN) {
+        t->sched.wide = 1;
+        return VJ_OK;
+    }
+    const int sample = std::min(t->nFeat, kTuneSample);
+    const double scale = static_cast<double>(t->nFeat) / sample;
+    double bestCost = 1e300;
+    int bestWide = 0;
+    for (int wide = 0; wide <= 1; ++wide) {
+        TrainSchedule sc;
+        sc.wide = wide;
+        t->sched = sc;
+        float sortMs = 0.0f, scanMs = 0.0f;
+        VJ_TRY(time_best(s, [&] { return presort_stage(t, sample, s); }, sortMs));
+        int blocks = 0;
+        VJ_TRY(time_best(s, [&] { return launch_search(t, sc, sample, blocks, s); }, scanMs));
+        const double cost = scale * (sortMs + kTuneRounds * static_cast<double>(scanMs));
+        if (cost < bestCost) {
+            bestCost = cost;
+            bestWide = wide;
+        }
+    }
+    t->sched = TrainSchedule{};
+    t->sched.wide = bestWide;
+    return VJ_OK;
+}
+
+// The scan schedule of the stage, in the presorted layout, from timed rounds:
+// each candidate runs the stage's first two rounds (search, arg-min, weight
+// update, calibration) from the same saved state, on the host's clock, as the
+// rounds of the stage will (a scan kernel timed alone ranked the workgroup
+// counts differently from the rounds).  The candidates take turns, three
+// times round, so that a drift of the clocks over the tuning favours none;
+// each keeps its best time, and one replaces the default only when it is
+// faster by more than kTuneMargin.  The state is restored afterwards.
+constexpr double kTuneMargin = 0.03;
+constexpr int kTuneRoundsTimed = 2;  // rounds per timing (the same rounds under every candidate)
+int round_impl(vj_trainer* t, const vj_round_config* cfg, vj_weak* out, hipStream_t s);
+
+int tune_scan(vj_trainer* t, hipStream_t s) {
+    std::vector<TrainSchedule> cand;
+    for (int ahead : kAheadChoices)
+        for (int blocksPerCu : kBlockChoices)
+            for (int packed = 0; packed <= (t->sched.wide ? 0 : 1); ++packed) {
+                TrainSchedule sc = t->sched;
+                sc.ahead = ahead;
+                sc.blocks = blocksPerCu;
+                sc.packed = packed;
+                cand.push_back(sc);
+            }
+    const size_t n = static_cast<size_t>(t->n), nf = static_cast<size_t>(t->nFeat);
+    DevBuf<double> w0;
+    DevBuf<float> strong0, sw0;
+    DevBuf<uint8_t> used0;
+    VJ_TRY(w0.ensure(n));
+    VJ_TRY(strong0.ensure(n));
+    VJ_TRY(sw0.ensure(n));
+    VJ_TRY(used0.ensure(nf));
+    auto copy = [&](void* dst, const void* src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s);
+    };
+    VJ_HIP(copy(w0.get(), t->w.get(), n * 8));
+    VJ_HIP(copy(strong0.get(), t->strong.get(), n * 4));
+    VJ_HIP(copy(sw0.get(), t->sw32.get(), n * 4));
+    VJ_HIP(copy(used0.get(), t->used.get(), nf));
+    auto restore = [&]() -> int {
+        VJ_HIP(copy(t->w.get(), w0.get(), n * 8));
+        VJ_HIP(copy(t->strong.get(), strong0.get(), n * 4));
+        VJ_HIP(copy(t->sw32.get(), sw0.get(), n * 4));
+        VJ_HIP(copy(t->used.get(), used0.get(), nf));
+        VJ_HIP(hipStreamSynchronize(s));
+        return VJ_OK;
+    };
+    std::vector<double> ms(cand.size(), 1e300);
+    for (int rep = 0; rep < 3; ++rep)
+        for (size_t c = 0; c < cand.size(); ++c) {
+            VJ_TRY(restore());
+            t->sched = cand[c];
+            vj_weak out;
+            const auto t0 = std::chrono::steady_clock::now();
+            for (int r = 0; r < kTuneRoundsTimed; ++r) VJ_TRY(round_impl(t, nullptr, &out, s));
+            ms[c] = std::min(ms[c], std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
+    VJ_TRY(restore());
+    const TrainSchedule dflt{};
+    size_t best = 0, def = 0;
+    for (size_t c = 0; c < cand.size(); ++c) {
+        if (ms[c] < ms[best]) best = c;
+        if (cand[c].ahead == dflt.ahead && cand[c].blocks == dflt.blocks && cand[c].packed == dflt.packed) def = c;
+    }
+    t->sched = ms[def] <= ms[best] * (1.0 + kTuneMargin) ? cand[def] : cand[best];
+    return VJ_OK;
+}
+
+}  // namespace
+
 extern "C" int vj_trainer_stage_begin(vj_trainer* t, const uint8_t* dPatches, const int8_t* hLabels,
                                       const uint8_t* hActive, int n, int numPos, const vj_feature* dFeatures,
                                       int nFeatures, vj_stream sv) {
@@ -1607,12 +1844,6 @@ extern "C" int vj_trainer_stage_begin(vj_trainer* t, const uint8_t* dPatches, co
     const int iw = t->winW + 1, ih = t->winH + 1;
     const size_t area = static_cast<size_t>(iw) * ih;
     t->pitch = (n + 255) / 256 * 256;
-    t->wide = n > kSortMaxN;
-    if (t->wide) {
-        VJ_TRY(t->mask.ensure(static_cast<size_t>(t->pitch / 64) * nFeatures));
-        VJ_TRY(t->wq.ensure(static_cast<size_t>(n)));
-        VJ_TRY(t->wtot.ensure(2));
-    }
     VJ_TRY(t->sumT.ensure(area * n));
     VJ_TRY(t->sqT.ensure(area * n));
     VJ_TRY(t->inv.ensure(static_cast<size_t>(n)));
@@ -1623,7 +1854,7 @@ extern "C" int vj_trainer_stage_begin(vj_trainer* t, const uint8_t* dPatches, co
     VJ_TRY(t->strong.ensure(static_cast<size_t>(n)));
     VJ_TRY(t->sw32.ensure(static_cast<size_t>(n)));
     VJ_TRY(t->used.ensure(static_cast<size_t>(nFeatures)));
-    VJ_TRY(t->blockBest.ensure(static_cast<size_t>(t->searchBlocks)));
+    VJ_TRY(t->blockBest.ensure(static_cast<size_t>(kMaxBlocksPerCu) * t->cus));
     VJ_TRY(t->roundBest.ensure(1));
     VJ_TRY(t->calib.ensure(1));
     VJ_TRY(t->counter.ensure(1));
@@ -1640,15 +1871,6 @@ extern "C" int vj_trainer_stage_begin(vj_trainer* t, const uint8_t* dPatches, co
     hipLaunchKernelGGL(stage_integral_kernel, dim3((n + 255) / 256), dim3(256), 0, s, dPatches, t->winW,
                        t->winH, n, t->sumT.get(), t->sqT.get(), t->inv.get());
     VJ_HIP(hipGetLastError());
-    // sample order of every feature, once for the whole stage
-    if (t->wide) {
-        VJ_TRY(launch_presort_wide(t, dFeatures, t->sumT.get(), t->inv.get(), nullptr, n, nFeatures, t->ord.get(),
-                                   t->pitch, t->mask.get(), 0, s));
-    } else {
-        hipLaunchKernelGGL(presort_kernel, dim3(nFeatures), dim3(kSortThreads), presort_lds(n), s, dFeatures, 0,
-                           t->sumT.get(), t->inv.get(), nullptr, n, t->ord.get(), t->pitch, 0);
-        VJ_HIP(hipGetLastError());
-    }
     // weights 0.5/numPos and 0.5/activeNeg, normalised (src/main.cu:1955-1962)
     hipLaunchKernelGGL(reset_kernel, dim3((n + 255) / 256), dim3(256), 0, s, t->w.get(), t->strong.get(),
                        t->active.get(), n, numPos, 0.5 / static_cast<double>(numPos), 0.5 / static_cast<double>(activeNeg));
@@ -1656,40 +1878,92 @@ extern "C" int vj_trainer_stage_begin(vj_trainer* t, const uint8_t* dPatches, co
     hipLaunchKernelGGL(normalize_kernel, dim3(1), dim3(1024), 0, s, t->w.get(), t->label.get(), t->active.get(), n,
                        t->sw32.get());
     VJ_HIP(hipGetLastError());
-    VJ_HIP(hipStreamSynchronize(s));  // hLabels / act staging must stay valid until here
     t->n = n;
     t->numPos = numPos;
     t->nFeat = nFeatures;
     t->feats = dFeatures;
     t->stageNegBase = activeNeg;
+    // the stage's schedule: the caller's, the tuned one of this bucket, or
+    // the default; the wide layout is the only one above the LDS sort limit
+    bool tuneScan = false;
+    std::array<int, 5> key{};
+    if (!t->forced) {
+        t->sched = TrainSchedule{};
+        if (t->autotune) {
+            key = tune_key(t);
+            std::lock_guard<std::mutex> g(gTuneMu);
+            auto it = gTuned.find(key);
+            if (it != gTuned.end()) t->sched = it->second;
+            else tuneScan = true;
+        }
+        if (tuneScan) VJ_TRY(tune_layout(t, s));
+    }
+    if (n > kSortMaxN) t->sched.wide = 1;
+    // sample order of every feature, once for the whole stage
+    VJ_TRY(presort_stage(t, nFeatures, s));
+    if (tuneScan) {
+        VJ_TRY(tune_scan(t, s));
+        std::lock_guard<std::mutex> g(gTuneMu);
+        gTuned[key] = t->sched;
+    }
+    VJ_HIP(hipStreamSynchronize(s));  // hLabels / act staging must stay valid until here
+    return VJ_OK;
+}
+
+extern "C" int vj_trainer_set_autotune(vj_trainer* t, int on) {
+    VJ_CHECK(t, "set_autotune: null trainer");
+    t->autotune = on != 0;
+    t->forced = false;
+    return VJ_OK;
+}
+
+extern "C" int vj_trainer_set_schedule(vj_trainer* t, const int* sched) {
+    VJ_CHECK(t, "set_schedule: null trainer");
+    if (!sched) {
+        t->forced = false;
+        t->autotune = false;
+        return VJ_OK;
+    }
+    TrainSchedule sc;
+    sc.wide = sched[0];
+    sc.ahead = sched[1];
+    sc.blocks = sched[2];
+    sc.packed = sched[3];
+    VJ_CHECK(sc.wide == 0 || sc.wide == 1, "set_schedule: wide must be 0 or 1");
+    VJ_CHECK(std::find(std::begin(kAheadChoices), std::end(kAheadChoices), sc.ahead) != std::end(kAheadChoices),
+             "set_schedule: loads ahead must be 2, 4 or 8");
+    VJ_CHECK(std::find(std::begin(kBlockChoices), std::end(kBlockChoices), sc.blocks) != std::end(kBlockChoices),
+             "set_schedule: workgroups per CU must be 1, 2 or 4");
+    VJ_CHECK(sc.packed == 0 || sc.packed == 1, "set_schedule: packed must be 0 or 1");
+    t->sched = sc;
+    t->forced = true;
+    t->autotune = false;
+    return VJ_OK;
+}
+
+extern "C" int vj_trainer_schedule(vj_trainer* t, int* sched) {
+    VJ_CHECK(t && sched, "schedule: null argument");
+    sched[0] = t->sched.wide;
+    sched[1] = t->sched.ahead;
+    sched[2] = t->sched.blocks;
+    sched[3] = t->sched.packed;
     return VJ_OK;
 }
 
 extern "C" int vj_trainer_round(vj_trainer* t, const vj_round_config* cfg, vj_weak* out, vj_stream sv) {
     VJ_CHECK(t && out && t->n > 0, "round: no stage begun");
-    hipStream_t s = static_cast<hipStream_t>(sv);
+    return round_impl(t, cfg, out, static_cast<hipStream_t>(sv));
+}
+
+namespace {
+
+int round_impl(vj_trainer* t, const vj_round_config* cfg, vj_weak* out, hipStream_t s) {
     const float minHit = cfg ? cfg->minHitRate : 0.99f;
     const float maxFa = cfg ? cfg->maxFalseAlarm : 0.40f;
     std::memset(out, 0, sizeof *out);
     out->feature = -1;
-    const int blocks = std::min((t->nFeat + 15) / 16, t->searchBlocks);
-    if (t->wide) {
-        hipLaunchKernelGGL(wide_weights_kernel, dim3(1), dim3(1024), 0, s, t->sw32.get(), t->n, t->wq.get(),
-                           t->wtot.get());
-        VJ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(search_wide_kernel, dim3(blocks), dim3(kSearchThreads), 0, s, t->ord.get(), t->mask.get(),
-                           t->pitch, t->n, t->nFeat, t->wq.get(), t->wtot.get(), t->used.get(), t->blockBest.get());
-    } else if (static_cast<size_t>(t->n) * 8 <= kSearchWideLds) {
-        // int64 weights in LDS when they fit beside the kernel's static LDS
-        hipLaunchKernelGGL(search_kernel<int64_t>, dim3(blocks), dim3(kSearchThreads), static_cast<size_t>(t->n) * 8,
-                           s, t->ord.get(), t->pitch, t->n, t->nFeat, t->sw32.get(), t->used.get(),
-                           t->blockBest.get());
-    } else {
-        hipLaunchKernelGGL(search_kernel<uint32_t>, dim3(blocks), dim3(kSearchThreads), static_cast<size_t>(t->n) * 4,
-                           s, t->ord.get(), t->pitch, t->n, t->nFeat, t->sw32.get(), t->used.get(),
-                           t->blockBest.get());
-    }
-    VJ_HIP(hipGetLastError());
+    int blocks = 0;
+    VJ_TRY(launch_search(t, t->sched, t->nFeat, blocks, s));
     hipLaunchKernelGGL(reduce_best_kernel, dim3(1), dim3(256), 0, s, t->blockBest.get(), blocks, t->ord.get(), t->pitch,
                        t->feats, t->sumT.get(), t->inv.get(), t->n, t->wide ? 0xFFFFu : 0x7FFFu,
                        t->roundBest.get());
@@ -1738,6 +2012,8 @@ extern "C" int vj_trainer_round(vj_trainer* t, const vj_round_config* cfg, vj_we
     out->stageSatisfied = ((out->hitRate >= minHit && out->falseAlarm <= maxFa) || out->negSurvive == 0) ? 1 : 0;
     return VJ_OK;
 }
+
+}  // namespace
 
 extern "C" int vj_trainer_stage_end(vj_trainer* t, float threshold, int* remainNeg, vj_stream sv) {
     VJ_CHECK(t && remainNeg && t->n > 0, "stage_end: no stage begun");

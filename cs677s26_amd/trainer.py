@@ -84,6 +84,20 @@ class AdaBoostTrainer:
         check(lib().vj_trainer_stage_end(self.handle, float(threshold), C.byref(n), _s(stream)), "stage_end")
         return n.value
 
+    # -- sort and scan schedule (no reference counterpart) -------------------
+    def set_autotune(self, on=True):
+        check(lib().vj_trainer_set_autotune(self.handle, 1 if on else 0), "set_autotune")
+
+    def set_schedule(self, sched=None):
+        """Force (wide, ahead, blocks, packed) for the next stages; None: the default."""
+        arr = None if sched is None else np.ascontiguousarray(sched, np.int32)
+        check(lib().vj_trainer_set_schedule(self.handle, None if arr is None else arr.ctypes.data), "set_schedule")
+
+    def schedule(self):
+        out = np.zeros(4, np.int32)
+        check(lib().vj_trainer_schedule(self.handle, out.ctypes.data), "schedule")
+        return tuple(int(v) for v in out)
+
     def read_state(self):
         w = np.empty(self.n, np.float64)
         s = np.empty(self.n, np.float32)

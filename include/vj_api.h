@@ -355,13 +355,36 @@ int vj_trainer_stage_end(vj_trainer* t, float stageThreshold, int* remainNeg, vj
 int vj_trainer_read_state(vj_trainer* t, double* hWeights, float* hStrong, uint8_t* hActive,
                           float* hInvNorm);
 /* Presorted orders of features [begin, begin+count), n entries each, for
- * checking the presort.  Up to 20,480 samples an entry is the sample index
- * with bit 15 set when a valid split follows the position; above that the
- * entries are plain 16-bit indices and the flags are kept apart. */
+ * checking the presort.  In the LDS layout (up to 20,480 samples) an entry is
+ * the sample index with bit 15 set when a valid split follows the position;
+ * in the wide layout (always above 20,480 samples, or as the stage's
+ * schedule picks) the entries are plain 16-bit indices and the flags are kept
+ * apart. */
 int vj_trainer_read_orders(vj_trainer* t, int begin, int count, uint16_t* hOut);
 /* The valid-split flags of the same orders, one byte (0 or 1) per position:
  * fl(v_k + 1e-7f) < v_{k+1} (src/boosting_core.cu:395-405). */
 int vj_trainer_read_splits(vj_trainer* t, int begin, int count, uint8_t* hOut);
+
+/* Sort and scan schedule of a stage (no counterpart in the reference, whose
+ * block sizes are fixed, src/boosting_core.cu:25-29).  Four ints:
+ *   [0] wide   1: global-memory radix sort and split-mask rows (forced above
+ *              20,480 samples), 0: the LDS radix sort with flags in the entries;
+ *   [1] ahead  search: steps of 16-byte order loads in flight per lane (2, 4, 8);
+ *   [2] blocks search: workgroups per CU (1, 2, 4);
+ *   [3] packed search: packed 32-bit weights in LDS even where int64 ones fit.
+ * Every schedule gives the same rounds bit for bit.  With autotune on, the
+ * first stage_begin of each (samples / 1024, log2 features, window) bucket
+ * picks the layout from a sample of 16,384 features (presort and one scan in
+ * each layout, scaled to one presort plus 50 rounds of the stage), presorts
+ * every feature in it, then times the stage's first two rounds under every scan
+ * candidate from the same saved state (restored afterwards) and keeps the
+ * fastest (the default unless another is faster by more than 3%) for the
+ * process; later stages of the bucket reuse the pick.  set_schedule forces one (NULL: back to the default {0, 4, 2, 0})
+ * and turns autotune off; vj_trainer_schedule reports the schedule of the
+ * current stage. */
+int vj_trainer_set_autotune(vj_trainer* t, int on);
+int vj_trainer_set_schedule(vj_trainer* t, const int* sched);
+int vj_trainer_schedule(vj_trainer* t, int* sched);
 
 /* TileBestCandidate, include/vj/boosting_core.cuh (16 B): what
  * vj_trainer_select_best writes.  featureIdx is INT_MAX only when every
