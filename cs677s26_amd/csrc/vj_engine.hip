@@ -61,6 +61,130 @@ __device__ __forceinline__ uint8_t bilinear_pixel(float p00, float p01, float p1
     return static_cast<uint8_t>(v < 0 ? 0 : (v > 255 ? 255 : v));
 }
 
+// One resize depth in one launch: every level of that depth (one per image)
+// from its source level, a job per level.  Used for the large levels, where a
+// launch per depth costs little next to the work and no pixel is computed
+// twice.  A workgroup makes `rows` output rows (up to kResizeRows; fewer when
+// the depth has too few rows to fill the GPU otherwise) of a segment of up to
+// 1024 columns.  It first copies the source rectangle those pixels tap into LDS
+// with word loads (byte gathers from global memory would each hold the
+// texture addresser for a whole instruction), then each thread makes four
+// adjacent pixels of every row from LDS, keeping their column taps, and
+// stores them as one word.  A rectangle larger than kResizeLds (source levels
+// far larger than the output) is read from global memory directly.
+struct LevelJob {
+    int32_t level;    // destination level
+    int32_t itemOff;  // first work item of this job
+};
+constexpr int kResizeSeg = 1024;
+constexpr int kResizeRows = 8;
+constexpr int kResizeLds = 24 * 1024;
+
+__global__ __launch_bounds__(256) void resize_level_kernel(const LevelDesc* __restrict__ L,
+                                                           const LevelJob* __restrict__ jobs, int nJobs,
+                                                           const float* __restrict__ rs,
+                                                           const int32_t* __restrict__ src, int rows) {
+    __shared__ __align__(16) uint8_t rect[kResizeLds];
+    const int item = blockIdx.x;
+    int lo = 0, hi = nJobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].itemOff <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    const int li = jobs[lo].level;
+    const LevelDesc dst = L[li];
+    const LevelDesc sl = L[src[li]];
+    const int segs = (dst.w + kResizeSeg - 1) / kResizeSeg;
+    const int local = item - jobs[lo].itemOff;
+    const int band = local / segs;
+    const int xs = (local - band * segs) * kResizeSeg;
+    const int xe = min(xs + kResizeSeg, dst.w);
+    const int y0 = band * rows, y1 = min(y0 + rows, dst.h);
+    const float sx = rs[2 * li], sy = rs[2 * li + 1];
+    // the source rectangle: rows [sr0, sr1], columns [c0, c1] (taps are
+    // monotone), each LDS row starting at the word boundary below column c0
+    // of its source row (the pitch keeps every row's words aligned)
+    const int c0 = bilinear_taps(xs, sx, sl.w).i0, c1 = bilinear_taps(xe - 1, sx, sl.w).i1;
+    const int sr0 = bilinear_taps(y0, sy, sl.h).i0, sr1 = bilinear_taps(y1 - 1, sy, sl.h).i1;
+    const gpix_t pix = global_pix(sl.pix);
+    const int pitch = (c1 - c0 + 1 + 3 + 3) & ~3;  // room for up to 3 bytes before c0
+    const bool staged = static_cast<int64_t>(pitch) * (sr1 - sr0 + 1) <= kResizeLds;
+    if (staged) {
+        const int tid = static_cast<int>(threadIdx.x), lane = tid & (kWave - 1), wave = tid / kWave;
+        for (int r = wave; r <= sr1 - sr0; r += 256 / kWave) {
+            const gpix_t row = pix + static_cast<size_t>(sr0 + r) * sl.stride;
+            const uintptr_t lo8 = reinterpret_cast<uintptr_t>(row + c0), hi8 = reinterpret_cast<uintptr_t>(row + c1) + 1;
+            const uintptr_t alo = lo8 & ~uintptr_t{3};
+            const uintptr_t a0 = (lo8 + 3) & ~uintptr_t{3}, a1 = hi8 & ~uintptr_t{3};
+            uint8_t* dstRow = rect + r * pitch;
+            // whole words inside [c0, c1], then the up to 3 + 3 bytes around them
+            for (uintptr_t a = a0 + 4 * static_cast<uintptr_t>(lane); a < a1; a += 4 * kWave)
+                *reinterpret_cast<uint32_t*>(dstRow + (a - alo)) =
+                    *(const uint32_t __attribute__((address_space(1)))*)(a);
+            const int head = static_cast<int>(a0 - lo8), tail = static_cast<int>(hi8 - a1);
+            if (a0 > a1) {  // no whole word: every byte of the span
+                if (lane <= c1 - c0) dstRow[lo8 - alo + lane] = row[c0 + lane];
+            } else if (lane < head) {
+                dstRow[lo8 - alo + lane] = row[c0 + lane];
+            } else if (lane >= 8 && lane < 8 + tail) {
+                dstRow[a1 - alo + (lane - 8)] = row[c1 + 1 - tail + (lane - 8)];
+            }
+        }
+        __syncthreads();
+    }
+    const int x0 = xs + static_cast<int>(threadIdx.x) * 4;
+    if (x0 >= xe) return;
+    const int nx = min(4, xe - x0);
+    const int shift = static_cast<int>(reinterpret_cast<uintptr_t>(pix + c0) & 3u);  // LDS offset of column c0
+    Taps tx[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) tx[k] = bilinear_taps(min(x0 + k, xe - 1), sx, sl.w);
+    for (int y = y0; y < y1; ++y) {
+        const Taps ty = bilinear_taps(y, sy, sl.h);
+        float p[4][4];
+        if (staged) {
+            // LDS row of source row s: (s - sr0) * pitch, shifted by the
+            // row's own misalignment (the stride may not be a multiple of 4)
+            auto at = [&](int sr, int c) {
+                const uintptr_t rb = reinterpret_cast<uintptr_t>(pix + static_cast<size_t>(sr) * sl.stride + c0);
+                return static_cast<float>(rect[(sr - sr0) * pitch + static_cast<int>(rb & 3u) + (c - c0)]);
+            };
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                p[k][0] = at(ty.i0, tx[k].i0);
+                p[k][1] = at(ty.i0, tx[k].i1);
+                p[k][2] = at(ty.i1, tx[k].i0);
+                p[k][3] = at(ty.i1, tx[k].i1);
+            }
+        } else {
+            const gpix_t r0 = pix + static_cast<size_t>(ty.i0) * sl.stride;
+            const gpix_t r1 = pix + static_cast<size_t>(ty.i1) * sl.stride;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                p[k][0] = static_cast<float>(r0[tx[k].i0]);
+                p[k][1] = static_cast<float>(r0[tx[k].i1]);
+                p[k][2] = static_cast<float>(r1[tx[k].i0]);
+                p[k][3] = static_cast<float>(r1[tx[k].i1]);
+            }
+        }
+        (void)shift;
+        uint32_t packed = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            packed |= static_cast<uint32_t>(bilinear_pixel(p[k][0], p[k][1], p[k][2], p[k][3], tx[k].a, ty.a))
+                      << (8 * k);
+        const gpix_out_t out = global_pix_out(dst.pix) + static_cast<size_t>(y) * dst.stride;
+        // pyramid rows start 64-byte aligned, so four pixels from x0 (a
+        // multiple of four) form one aligned word
+        if (nx == 4) {
+            *(uint32_t __attribute__((address_space(1)))*)(out + x0) = packed;
+        } else {
+            for (int k = 0; k < nx; ++k) out[x0 + k] = static_cast<uint8_t>(packed >> (8 * k));
+        }
+    }
+}
+
 // The pyramid in a few launches: a resize group is up to kResizeDepth
 // consecutive levels of one image's resize chain (each level resized from
 // the one before, as the CPU path does), computed from the level before the
@@ -154,7 +278,7 @@ __global__ __launch_bounds__(kResizeThreads) void resize_group_kernel(const Leve
         const int y0 = ry.need0[j], ny = ry.need1[j] - y0;
         if (nx * ny > kResizeCap) return;  // the plan sizes tiles so this never happens
         const LevelDesc d = L[jb.lv[j - 1]];
-        uint8_t* out = const_cast<uint8_t*>(d.pix);
+        const gpix_out_t out = global_pix_out(d.pix);
         uint8_t* cur = buf[j & 1];
         const uint8_t* prev = buf[(j - 1) & 1];
         const int px0 = rx.need0[j - 1], pitch = rx.need1[j - 1] - px0, py0 = ry.need0[j - 1];
@@ -167,8 +291,8 @@ __global__ __launch_bounds__(kResizeThreads) void resize_group_kernel(const Leve
                 const Taps tx2 = bilinear_taps(x, sx[j], W[j - 1]);
                 float p00, p01, p10, p11;
                 if (j == 1) {
-                    const uint8_t* r0 = s0.pix + static_cast<size_t>(ty2.i0) * s0.stride;
-                    const uint8_t* r1 = s0.pix + static_cast<size_t>(ty2.i1) * s0.stride;
+                    const gpix_t r0 = global_pix(s0.pix) + static_cast<size_t>(ty2.i0) * s0.stride;
+                    const gpix_t r1 = global_pix(s0.pix) + static_cast<size_t>(ty2.i1) * s0.stride;
                     p00 = static_cast<float>(r0[tx2.i0]);
                     p01 = static_cast<float>(r0[tx2.i1]);
                     p10 = static_cast<float>(r1[tx2.i0]);
@@ -191,9 +315,9 @@ __global__ __launch_bounds__(kResizeThreads) void resize_group_kernel(const Leve
 }
 
 // Pixels x..x+3 of a row packed little-endian into one word (zero beyond w).
-__device__ __forceinline__ uint32_t load4_packed(const uint8_t* row, int x, int w) {
+__device__ __forceinline__ uint32_t load4_packed(gpix_t row, int x, int w) {
     if (x + 3 < w && ((reinterpret_cast<uintptr_t>(row + x) & 3u) == 0))
-        return *reinterpret_cast<const uint32_t*>(row + x);
+        return *(const uint32_t __attribute__((address_space(1)))*)(row + x);
     uint32_t v = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -205,7 +329,7 @@ __device__ __forceinline__ uint32_t load4_packed(const uint8_t* row, int x, int 
 __device__ __forceinline__ void load_band(const LevelDesc& lv, int x, int y0, int y1, uint32_t pv[kBandRows]) {
 #pragma unroll
     for (int r = 0; r < kBandRows; ++r)
-        pv[r] = (y0 + r < y1) ? load4_packed(lv.pix + static_cast<size_t>(y0 + r) * lv.stride, x, lv.w) : 0u;
+        pv[r] = (y0 + r < y1) ? load4_packed(global_pix(lv.pix) + static_cast<size_t>(y0 + r) * lv.stride, x, lv.w) : 0u;
 }
 
 // Integral phase 1, one wavefront per (level, band of 32 rows, chunk of 256
@@ -1268,8 +1392,17 @@ struct Plan {
     std::vector<int32_t> itemOffs, chunkOffs;
     std::vector<int64_t> lvWordOff;
     std::vector<int64_t> imgWordOff;
-    std::vector<std::vector<ResizeJob>> resizeJobs;  // per resize group
-    std::vector<int32_t> resizeItems;                // per resize group: tiles
+    // the pyramid's launches in order: one per large resize depth (a job per
+    // level), one per group of small depths (a job per image)
+    struct ResizeStep {
+        bool group;
+        int32_t first, count;  // jobs in levelJobs / groupJobs
+        int32_t items;         // workgroups
+        int32_t rows;          // a depth's launch: output rows per workgroup
+    };
+    std::vector<ResizeStep> resizeSteps;
+    std::vector<LevelJob> levelJobs;
+    std::vector<ResizeJob> groupJobs;
     std::vector<int32_t> tileLvl, tileOff;  // scanned levels and their first cascade tile
     int32_t nTiles = 0;
     int32_t maxStep = 0;
@@ -2021,7 +2154,6 @@ struct vj_engine {
     DevBuf<float> dRs;
     DevBuf<int32_t> dMeta32;
     DevBuf<int64_t> dMeta64;
-    DevBuf<ResizeJob> dJobs;
     DevBuf<uint8_t> dPlan;                     // a detect call's tables (levels, scales, offsets, jobs)
     DevBuf<int32_t> dCounts, dBlockCount, dBlockOff, dImgBase;
     // two-pass cascade: first-pass survivors and the deep pass's work list
@@ -2147,6 +2279,7 @@ struct ImageIn {
 // more than the recomputed margins).  The regions are computed with the
 // kernel's own arithmetic.
 constexpr int kResizeTiles = 256;
+constexpr int64_t kResizeBigPixels = int64_t{1} << 19;
 void resize_grid(const Plan& P, ResizeJob& jb) {
     int W[kResizeDepth + 1], H[kResizeDepth + 1];
     float sx[kResizeDepth + 1], sy[kResizeDepth + 1];
@@ -2334,24 +2467,62 @@ int build_plan(const vj_engine* e, const std::vector<ImageIn>& imgs, float sf, i
     P.totElems = totOff;
     P.rowElems = rowOff;
     P.nWords = wordOff;
-    // resize groups: depths [g * kResizeDepth + 1, (g + 1) * kResizeDepth] of
-    // every image's chain; consecutive levels of one image in one group form
-    // a job (resized levels are listed image by image, depth by depth)
-    const int nGroups = (maxDepth + kResizeDepth - 1) / kResizeDepth;
-    P.resizeJobs.assign(static_cast<size_t>(nGroups), {});
-    P.resizeItems.assign(static_cast<size_t>(nGroups), 0);
-    for (size_t k = 0; k < resized.size();) {
-        const int img = P.levels[static_cast<size_t>(resized[k].level)].image;
-        const int g = (resized[k].depth - 1) / kResizeDepth;
-        ResizeJob jb{};
-        jb.src = P.src[static_cast<size_t>(resized[k].level)];
-        while (k < resized.size() && P.levels[static_cast<size_t>(resized[k].level)].image == img &&
-               (resized[k].depth - 1) / kResizeDepth == g)
-            jb.lv[jb.n++] = resized[k++].level;
-        resize_grid(P, jb);
-        jb.itemOff = P.resizeItems[static_cast<size_t>(g)];
-        P.resizeItems[static_cast<size_t>(g)] += jb.gx * jb.gy;
-        P.resizeJobs[static_cast<size_t>(g)].push_back(jb);
+    // the pyramid's launches: a depth whose levels (over the call's images)
+    // hold at least kResizeBigPixels pixels gets a launch of its own; runs of
+    // smaller depths go into groups of up to kResizeDepth, a job per image
+    std::vector<int64_t> depthPix(static_cast<size_t>(maxDepth) + 1, 0);
+    std::vector<std::vector<int32_t>> atDepth(static_cast<size_t>(maxDepth) + 1);
+    for (const Pending& r : resized) {  // listed image by image, depth by depth
+        const LevelDesc& d = P.levels[static_cast<size_t>(r.level)];
+        depthPix[static_cast<size_t>(r.depth)] += static_cast<int64_t>(d.w) * d.h;
+        atDepth[static_cast<size_t>(r.depth)].push_back(r.level);
+    }
+    for (int dpt = 1; dpt <= maxDepth;) {
+        Plan::ResizeStep st{};
+        if (depthPix[static_cast<size_t>(dpt)] >= kResizeBigPixels) {
+            st.group = false;
+            st.first = static_cast<int32_t>(P.levelJobs.size());
+            // rows per workgroup: the most (up to kResizeRows) that still
+            // give kResizeTiles * 4 workgroups
+            int64_t segRows = 0;
+            for (int32_t li : atDepth[static_cast<size_t>(dpt)]) {
+                const LevelDesc& d = P.levels[static_cast<size_t>(li)];
+                segRows += static_cast<int64_t>((d.w + kResizeSeg - 1) / kResizeSeg) * d.h;
+            }
+            st.rows = kResizeRows;
+            while (st.rows > 1 && segRows / st.rows < 4 * kResizeTiles) st.rows >>= 1;
+            for (int32_t li : atDepth[static_cast<size_t>(dpt)]) {
+                const LevelDesc& d = P.levels[static_cast<size_t>(li)];
+                P.levelJobs.push_back({li, st.items});
+                st.items += (d.w + kResizeSeg - 1) / kResizeSeg * ((d.h + st.rows - 1) / st.rows);
+            }
+            st.count = static_cast<int32_t>(P.levelJobs.size()) - st.first;
+            ++dpt;
+        } else {
+            int end = dpt;  // depths [dpt, end)
+            while (end <= maxDepth && end - dpt < kResizeDepth && depthPix[static_cast<size_t>(end)] < kResizeBigPixels)
+                ++end;
+            st.group = true;
+            st.first = static_cast<int32_t>(P.groupJobs.size());
+            // each image's levels at these depths, in chain order
+            std::map<int32_t, ResizeJob> byImage;
+            for (int g = dpt; g < end; ++g)
+                for (int32_t li : atDepth[static_cast<size_t>(g)]) {
+                    ResizeJob& jb = byImage[P.levels[static_cast<size_t>(li)].image];
+                    if (jb.n == 0) jb.src = P.src[static_cast<size_t>(li)];
+                    jb.lv[jb.n++] = li;
+                }
+            for (auto& kv : byImage) {
+                ResizeJob& jb = kv.second;
+                resize_grid(P, jb);
+                jb.itemOff = st.items;
+                st.items += jb.gx * jb.gy;
+                P.groupJobs.push_back(jb);
+            }
+            st.count = static_cast<int32_t>(P.groupJobs.size()) - st.first;
+            dpt = end;
+        }
+        P.resizeSteps.push_back(st);
     }
     return VJ_OK;
 }
@@ -2447,15 +2618,10 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     VJ_TRY(e->dBlockCount.ensure(blockImg.size()));
     VJ_TRY(e->dBlockOff.ensure(blockImg.size()));
     VJ_TRY(e->dImgBase.ensure(static_cast<size_t>(nImages)));
-    std::vector<ResizeJob> jobs;
-    std::vector<size_t> jobStart;
-    for (const auto& jv : P.resizeJobs) {
-        jobStart.push_back(jobs.size());
-        jobs.insert(jobs.end(), jv.begin(), jv.end());
-    }
     // plan upload: every table into one staging buffer, then one copy
     const size_t stageBytes = P.levels.size() * sizeof(LevelDesc) + P.rs.size() * 4 + meta32.size() * 4 +
-                              meta64.size() * 8 + jobs.size() * sizeof(ResizeJob) + 6 * 256;
+                              meta64.size() * 8 + P.levelJobs.size() * sizeof(LevelJob) +
+                              P.groupJobs.size() * sizeof(ResizeJob) + 7 * 256;
     VJ_TRY(e->hStage.ensure(stageBytes));
     VJ_HIP(hipStreamSynchronize(s));  // staging buffer reuse across calls
     size_t stOff = 0;
@@ -2469,7 +2635,8 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     const size_t aRs = place(P.rs.data(), P.rs.size() * sizeof(float));
     const size_t a32 = place(meta32.data(), meta32.size() * sizeof(int32_t));
     const size_t a64 = place(meta64.data(), meta64.size() * sizeof(int64_t));
-    const size_t aJobs = place(jobs.data(), jobs.size() * sizeof(ResizeJob));
+    const size_t aLJobs = place(P.levelJobs.data(), P.levelJobs.size() * sizeof(LevelJob));
+    const size_t aGJobs = place(P.groupJobs.data(), P.groupJobs.size() * sizeof(ResizeJob));
     VJ_TRY(e->dPlan.ensure(std::max<size_t>(stOff, 256)));
     VJ_HIP(hipMemcpyAsync(e->dPlan.get(), e->hStage.get(), stOff, hipMemcpyHostToDevice, s));
 
@@ -2477,7 +2644,8 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     const float* dRs = reinterpret_cast<const float*>(e->dPlan.get() + aRs);
     const int32_t* m32 = reinterpret_cast<const int32_t*>(e->dPlan.get() + a32);
     const int64_t* m64 = reinterpret_cast<const int64_t*>(e->dPlan.get() + a64);
-    const ResizeJob* dJobs = reinterpret_cast<const ResizeJob*>(e->dPlan.get() + aJobs);
+    const LevelJob* dLJobs = reinterpret_cast<const LevelJob*>(e->dPlan.get() + aLJobs);
+    const ResizeJob* dGJobs = reinterpret_cast<const ResizeJob*>(e->dPlan.get() + aGJobs);
     const int nLevels = static_cast<int>(P.levels.size());
 
     size_t ldsBytes = 0;
@@ -2503,12 +2671,16 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
             VJ_HIP(hipGetLastError());
         }
         if (e->timing) VJ_HIP(hipEventRecord(e->ev[0], ls));
-        // 1. pyramid: one launch per resize group (each group needs the one before)
-        for (size_t grp = 0; grp < P.resizeJobs.size(); ++grp) {
-            const int n = static_cast<int>(P.resizeJobs[grp].size());
-            if (!n || !P.resizeItems[grp]) continue;
-            hipLaunchKernelGGL(resize_group_kernel, dim3(P.resizeItems[grp]), dim3(kResizeThreads), 0, ls, dL,
-                               dJobs + jobStart[grp], n, dRs);
+        // 1. pyramid: one launch per large depth or group of small depths
+        //    (each needs the one before)
+        for (const Plan::ResizeStep& st : P.resizeSteps) {
+            if (!st.count || !st.items) continue;
+            if (st.group)
+                hipLaunchKernelGGL(resize_group_kernel, dim3(st.items), dim3(kResizeThreads), 0, ls, dL,
+                                   dGJobs + st.first, st.count, dRs);
+            else
+                hipLaunchKernelGGL(resize_level_kernel, dim3(st.items), dim3(256), 0, ls, dL, dLJobs + st.first,
+                                   st.count, dRs, m32 + oSrc, st.rows);
             VJ_HIP(hipGetLastError());
         }
         if (e->timing) VJ_HIP(hipEventRecord(e->ev[1], ls));
@@ -2594,10 +2766,12 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     // valid while every grid size, scalar argument and buffer address matches.
     std::vector<int64_t> key;
     auto k = [&key](int64_t v) { key.push_back(v); };
-    for (size_t grp = 0; grp < P.resizeJobs.size(); ++grp) {
-        k(static_cast<int64_t>(P.resizeJobs[grp].size()));
-        k(P.resizeItems[grp]);
-        k(static_cast<int64_t>(jobStart[grp]));
+    for (const Plan::ResizeStep& st : P.resizeSteps) {
+        k(st.group ? 1 : 0);
+        k(st.first);
+        k(st.count);
+        k(st.items);
+        k(st.rows);
     }
     for (int64_t v : {int64_t(e->stageSplit), int64_t(stageTail), int64_t(deepGrid), int64_t(nLevels), int64_t(P.nItems), int64_t(P.nChunkItems), int64_t(P.nTiles),
                       int64_t(P.tileLvl.size()), int64_t(ldsBytes), int64_t(nBlocks), int64_t(nImages), int64_t(cap),
@@ -2606,7 +2780,8 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
                       int64_t(oBlockWord), int64_t(oBlockImg), int64_t(oImgBlock)})
         k(v);
     for (const void* ptr : {static_cast<const void*>(dL), static_cast<const void*>(m32), static_cast<const void*>(m64),
-                            static_cast<const void*>(dJobs), static_cast<const void*>(dRs),
+                            static_cast<const void*>(dLJobs), static_cast<const void*>(dGJobs),
+                            static_cast<const void*>(dRs),
                             static_cast<const void*>(e->colS.get()), static_cast<const void*>(e->colQ.get()),
                             static_cast<const void*>(e->rowS.get()), static_cast<const void*>(e->rowQ.get()),
                             static_cast<const void*>(e->totS.get()), static_cast<const void*>(e->totQ.get()),

@@ -71,6 +71,14 @@ struct KStage {
     int32_t pad[3];
 };
 
+// Pixel pointers read from a LevelDesc in memory have no known address space,
+// so the compiler would access them with flat instructions (which also count
+// against the LDS wait counter); they always point to global memory.
+typedef const uint8_t __attribute__((address_space(1)))* gpix_t;
+typedef uint8_t __attribute__((address_space(1)))* gpix_out_t;
+__device__ __forceinline__ gpix_t global_pix(const uint8_t* p) { return (gpix_t)p; }
+__device__ __forceinline__ gpix_out_t global_pix_out(const uint8_t* p) { return (gpix_out_t)(const_cast<uint8_t*>(p)); }
+
 __device__ __forceinline__ int lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
