@@ -457,7 +457,7 @@ __global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restric
     const int x = c * kChunkCols + lane * 4;
     const int y0 = b * kBandRows;
     const int y1 = min(lv.h, y0 + kBandRows);
-    const int iw = lv.w + 1;
+    const int64_t pitch = lv.iiPitch;
     uint32_t* outS = iiS + lv.iiOff;
     uint64_t* outQ = iiQ + lv.iiOff;
 
@@ -513,21 +513,46 @@ __global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restric
             aQ[k] = rq;
         }
     }
-    if (b == 0) {  // row 0 of the integral is zero
+    // A lane writes integral elements x .. x+3 of each row: ii at pixel
+    // columns x-1 .. x+2, its left neighbour's last value and its first three
+    // (so that with a row pitch of whole 128-byte lines every store is one
+    // aligned 16-byte piece); lane 0 takes the value left of the chunk, and
+    // the lane whose last pixel is the row's last (x + 4 == w) at the end of
+    // the last chunk also writes element w.
+    const int w = lv.w;
+    const bool tailEl = x + 4 == w && lane == kWave - 1;
+    auto put_row = [&](int64_t row, uint32_t s0, const uint32_t (&sv)[4], uint64_t q0, const uint64_t (&qv)[4]) {
+        uint32_t* rS = outS + row * pitch;
+        uint64_t* rQ = outQ + row * pitch;
+        if (x + 3 <= w) {
+            *reinterpret_cast<u32x4_a4*>(rS + x) = u32x4_a4{s0, sv[0], sv[1], sv[2]};
+            *reinterpret_cast<u64x2_a8*>(rQ + x) = u64x2_a8{q0, qv[0]};
+            *reinterpret_cast<u64x2_a8*>(rQ + x + 2) = u64x2_a8{qv[1], qv[2]};
+        } else if (x <= w) {
+            rS[x] = s0;
+            rQ[x] = q0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (x + k < lv.w) {
-                outS[x + k + 1] = 0u;
-                outQ[x + k + 1] = 0u;
-            }
-        if (c == 0 && lane == 0) {
-            outS[0] = 0u;
-            outQ[0] = 0u;
+            for (int k = 1; k < 4; ++k)
+                if (x + k <= w) {
+                    rS[x + k] = sv[k - 1];
+                    rQ[x + k] = qv[k - 1];
+                }
         }
+        if (tailEl) {
+            rS[w] = sv[3];
+            rQ[w] = qv[3];
+        }
+    };
+    if (b == 0) {  // row 0 of the integral is zero
+        const uint32_t zs[4] = {0u, 0u, 0u, 0u};
+        const uint64_t zq[4] = {0u, 0u, 0u, 0u};
+        put_row(0, 0u, zs, 0u, zq);
     }
+    // the value left of the chunk (element c * 256) on the current row
+    uint32_t leftS = ccS;
+    uint64_t leftQ = ccQ;
     uint32_t pv[kBandRows];
     load_band(lv, x, y0, y1, pv);
-    const bool full = x + 3 < lv.w;
 #pragma unroll
     for (int r = 0; r < kBandRows; ++r) {
         const int y = y0 + r;
@@ -539,8 +564,6 @@ __global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restric
         const uint32_t rowCarryQ = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rcQ), r));
         uint32_t rs = wave_incl_scan_dpp(ls) - ls + rowCarryS;
         uint32_t rq = wave_incl_scan_dpp(lq) - lq + rowCarryQ;
-        uint32_t* rowOutS = outS + static_cast<int64_t>(y + 1) * iw;
-        uint64_t* rowOutQ = outQ + static_cast<int64_t>(y + 1) * iw;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             rs += p[k];
@@ -548,24 +571,15 @@ __global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restric
             aS[k] += rs;
             aQ[k] += rq;
         }
-        if (full) {
-            // columns x+1 .. x+4: one 16-byte store of sums, two of squares
-            // (global accesses need only element alignment)
-            *reinterpret_cast<u32x4_a4*>(rowOutS + x + 1) = u32x4_a4{aS[0], aS[1], aS[2], aS[3]};
-            *reinterpret_cast<u64x2_a8*>(rowOutQ + x + 1) = u64x2_a8{aQ[0], aQ[1]};
-            *reinterpret_cast<u64x2_a8*>(rowOutQ + x + 3) = u64x2_a8{aQ[2], aQ[3]};
-        } else {
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (x + k < lv.w) {
-                    rowOutS[x + k + 1] = aS[k];
-                    rowOutQ[x + k + 1] = aQ[k];
-                }
+        leftS += rowCarryS;
+        leftQ += rowCarryQ;
+        uint32_t prevS = static_cast<uint32_t>(__shfl_up(static_cast<int>(aS[3]), 1, kWave));
+        uint64_t prevQ = static_cast<uint64_t>(__shfl_up(static_cast<unsigned long long>(aQ[3]), 1, kWave));
+        if (lane == 0) {
+            prevS = leftS;
+            prevQ = leftQ;
         }
-        if (c == 0 && lane == 0) {
-            rowOutS[0] = 0u;
-            rowOutQ[0] = 0u;
-        }
+        put_row(y + 1, prevS, aS, prevQ, aQ);
     }
 }
 
@@ -664,7 +678,7 @@ __device__ __forceinline__ void stage_footprint(const TileGeo& g, const uint32_t
     constexpr int kWaves = kCascadeBlock / kWave;
     const int FW = (g.TW - 1) * g.step + winW + 1;
     const int ix0 = g.wx0 * g.step, iy0 = g.wy0 * g.step;
-    const int iw = g.lv.w + 1;
+    const int64_t iw = g.lv.iiPitch;
     const uint32_t* sumL = iiS + g.lv.iiOff;
     const int lane = lane_id(), wave = static_cast<int>(threadIdx.x) / kWave;
     const int cx0 = ca + lane, cx1 = cx0 + kWave;
@@ -706,7 +720,7 @@ __device__ __forceinline__ void stage_window_box(const TileGeo& g, const uint32_
 __device__ __forceinline__ void gate_windows(const TileGeo& g, const uint32_t* __restrict__ iiS,
                                              const uint64_t* __restrict__ iiQ, CascadeSmem& sm, int winW,
                                              int winH) {
-    const int iw = g.lv.w + 1;
+    const int64_t iw = g.lv.iiPitch;
     const uint32_t* sumL = iiS + g.lv.iiOff;
     const uint64_t* sqL = iiQ + g.lv.iiOff;
     const int nw = winW - 2, nh = winH - 2, narea = nw * nh;
@@ -2422,6 +2436,7 @@ int build_plan(const vj_engine* e, const std::vector<ImageIn>& imgs, float sf, i
         }
         d.nBands = (d.h + kBandRows - 1) / kBandRows;
         d.nChunks = (d.w + kChunkCols - 1) / kChunkCols;
+        d.iiPitch = static_cast<int32_t>(align_up(d.w + 1, kIIPitchAlign));
         d.scanW = (d.w - e->winW + 1 + d.step - 1) / d.step;
         d.scanH = (d.h - e->winH + 1 + d.step - 1) / d.step;
         d.rowWords = (d.scanW + 63) / 64;
@@ -2434,7 +2449,7 @@ int build_plan(const vj_engine* e, const std::vector<ImageIn>& imgs, float sf, i
         P.lvWordOff.push_back(wordOff);
         items += d.nBands * d.nChunks;
         chunkItems += d.nChunks;
-        iiOff += align_up(static_cast<int64_t>(d.w + 1) * (d.h + 1), 64);
+        iiOff += align_up(static_cast<int64_t>(d.iiPitch) * (d.h + 1), 64);
         colOff += static_cast<int64_t>(d.nBands) * d.w;
         totOff += static_cast<int64_t>(d.nBands) * d.nChunks;
         rowOff += static_cast<int64_t>(d.h) * d.nChunks;
@@ -2925,6 +2940,7 @@ extern "C" int vj_engine_integral_batch(vj_engine* e, const uint8_t* dGray, int 
         d.step = 1;
         d.nBands = (h + kBandRows - 1) / kBandRows;
         d.nChunks = (w + kChunkCols - 1) / kChunkCols;
+        d.iiPitch = w + 1;  // the view's dense layout
         VJ_TRY(e->colS.ensure(static_cast<size_t>(d.nBands) * w));
         VJ_TRY(e->colQ.ensure(static_cast<size_t>(d.nBands) * w));
         VJ_TRY(e->totS.ensure(static_cast<size_t>(d.nBands) * d.nChunks));
@@ -3053,6 +3069,7 @@ extern "C" int vj_integral_image_noalloc(const uint8_t* dGray, int w, int h, int
     d.step = 1;
     d.nBands = (h + kBandRows - 1) / kBandRows;
     d.nChunks = (w + kChunkCols - 1) / kChunkCols;
+    d.iiPitch = w + 1;  // the caller's dense (h+1) x (w+1) layout
     VJ_CHECK(static_cast<int64_t>(d.nBands) * w <= kNaCols && static_cast<int64_t>(h) * d.nChunks <= kNaRows &&
                  static_cast<int64_t>(d.nBands) * d.nChunks <= kNaTots,
              "integral_image_noalloc: image exceeds the scratch");

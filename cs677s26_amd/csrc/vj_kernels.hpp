@@ -50,7 +50,13 @@ struct LevelDesc {
     int32_t chunkItemOff; // first column-chunk item
     int32_t tileOff;      // first cascade tile
     int32_t srcLevel;     // level this one is resized from (-1: level 0 of its image)
+    int32_t iiPitch;      // elements per integral row: w + 1, or padded to whole 128-byte lines
 };
+
+// Integral row pitch of the detect pyramid, in elements: whole 128-byte lines
+// of u32 sums (and 256 bytes of u64 square sums), so the integral kernels'
+// 16-byte stores are aligned.
+constexpr int kIIPitchAlign = 32;
 
 // Cascade stump with its corner offsets decoded for one LDS tile pitch.
 // 64 bytes so a wavefront reads it with one scalar load.
