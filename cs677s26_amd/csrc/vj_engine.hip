@@ -1399,6 +1399,15 @@ __global__ __launch_bounds__(256) void emit_kernel(const LevelDesc* __restrict__
 // Host side
 // ===========================================================================
 
+// Offsets of a plan's tables: elements in the 32- and 64-bit metadata arrays
+// and bytes in the engine's plan buffer.
+struct PlanTables {
+    size_t oItem = 0, oChunk = 0, oSrc = 0, oLvl = 0, oTile = 0, oLvWord = 0, oBlockWord = 0, oBlockImg = 0,
+           oImgBlock = 0;
+    size_t aLv = 0, aRs = 0, a32 = 0, a64 = 0, aLJobs = 0, aGJobs = 0;
+    int nBlocks = 0;
+};
+
 struct Plan {
     std::vector<LevelDesc> levels;
     std::vector<float> rs;          // per level: resize scale x, y
@@ -1417,6 +1426,7 @@ struct Plan {
     std::vector<ResizeStep> resizeSteps;
     std::vector<LevelJob> levelJobs;
     std::vector<ResizeJob> groupJobs;
+    PlanTables tab;  // where run_pipeline placed the tables in the plan buffer
     std::vector<int32_t> tileLvl, tileOff;  // scanned levels and their first cascade tile
     int32_t nTiles = 0;
     int32_t maxStep = 0;
@@ -2218,6 +2228,12 @@ struct vj_engine {
     // shape is captured on its second consecutive call (lastKey): a caller
     // whose image sizes change every call launches directly and pays no
     // capture.
+    // the last detect call's plan, reused by a call with the same images
+    Plan plan;
+    std::vector<int64_t> planKey;
+    const uint8_t* planPyr = nullptr;
+    bool planValid = false;
+
     bool graphs = true;
     std::vector<int64_t> graphKey, lastKey;
     hipGraphExec_t graphExec = nullptr;
@@ -2563,7 +2579,7 @@ int upload(DevBuf<T>& dst, const std::vector<T>& v, PinnedBuf<uint8_t>& staging,
 
 // Runs the whole detect pipeline for the images of a plan; leaves per-image
 // counts in e->dCounts and detections (capped per image) in e->dOut.
-int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
+int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s, bool tablesCurrent) {
     // buffers
     VJ_TRY(e->pyr.ensure(static_cast<size_t>(std::max<int64_t>(P.pyrBytes, 1))));
     {
@@ -2596,64 +2612,74 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     VJ_TRY(e->dCounts.ensure(static_cast<size_t>(nImages)));
     VJ_TRY(e->dOut.ensure(static_cast<size_t>(cap) * nImages));
 
-    // plan upload (one staging buffer, several async copies)
-    std::vector<int32_t> meta32;
-    const size_t oItem = meta32.size();
-    meta32.insert(meta32.end(), P.itemOffs.begin(), P.itemOffs.end());
-    const size_t oChunk = meta32.size();
-    meta32.insert(meta32.end(), P.chunkOffs.begin(), P.chunkOffs.end());
-    const size_t oSrc = meta32.size();
-    meta32.insert(meta32.end(), P.src.begin(), P.src.end());
-    const size_t oLvl = meta32.size();
-    meta32.insert(meta32.end(), P.tileLvl.begin(), P.tileLvl.end());
-    const size_t oTile = meta32.size();
-    meta32.insert(meta32.end(), P.tileOff.begin(), P.tileOff.end());
-    std::vector<int64_t> meta64;
-    const size_t oLvWord = meta64.size();
-    meta64.insert(meta64.end(), P.lvWordOff.begin(), P.lvWordOff.end());
-    // compaction blocks: each image's words cut into runs of kWordBlock
-    std::vector<int64_t> blockWord;
-    std::vector<int32_t> blockImg, imgBlock;
-    for (int im = 0; im < nImages; ++im) {
-        imgBlock.push_back(static_cast<int32_t>(blockImg.size()));
-        for (int64_t a = P.imgWordOff[static_cast<size_t>(im)]; a < P.imgWordOff[static_cast<size_t>(im) + 1];
-             a += kWordBlock) {
-            blockWord.push_back(a);
-            blockImg.push_back(im);
+    // The call's tables (levels, scales, offsets, jobs, compaction blocks) in
+    // one staging buffer and one copy, unless the engine's plan buffer already
+    // holds this very plan (the previous call's, with the same images).
+    if (!tablesCurrent) {
+        PlanTables& T = P.tab;
+        std::vector<int32_t> meta32;
+        T.oItem = meta32.size();
+        meta32.insert(meta32.end(), P.itemOffs.begin(), P.itemOffs.end());
+        T.oChunk = meta32.size();
+        meta32.insert(meta32.end(), P.chunkOffs.begin(), P.chunkOffs.end());
+        T.oSrc = meta32.size();
+        meta32.insert(meta32.end(), P.src.begin(), P.src.end());
+        T.oLvl = meta32.size();
+        meta32.insert(meta32.end(), P.tileLvl.begin(), P.tileLvl.end());
+        T.oTile = meta32.size();
+        meta32.insert(meta32.end(), P.tileOff.begin(), P.tileOff.end());
+        std::vector<int64_t> meta64;
+        T.oLvWord = meta64.size();
+        meta64.insert(meta64.end(), P.lvWordOff.begin(), P.lvWordOff.end());
+        // compaction blocks: each image's words cut into runs of kWordBlock
+        std::vector<int64_t> blockWord;
+        std::vector<int32_t> blockImg, imgBlock;
+        for (int im = 0; im < nImages; ++im) {
+            imgBlock.push_back(static_cast<int32_t>(blockImg.size()));
+            for (int64_t a = P.imgWordOff[static_cast<size_t>(im)]; a < P.imgWordOff[static_cast<size_t>(im) + 1];
+                 a += kWordBlock) {
+                blockWord.push_back(a);
+                blockImg.push_back(im);
+            }
         }
+        imgBlock.push_back(static_cast<int32_t>(blockImg.size()));
+        blockWord.push_back(P.imgWordOff.back());
+        T.oBlockWord = meta64.size();
+        meta64.insert(meta64.end(), blockWord.begin(), blockWord.end());
+        T.oBlockImg = meta32.size();
+        meta32.insert(meta32.end(), blockImg.begin(), blockImg.end());
+        T.oImgBlock = meta32.size();
+        meta32.insert(meta32.end(), imgBlock.begin(), imgBlock.end());
+        T.nBlocks = static_cast<int>(blockImg.size());
+        const size_t stageBytes = P.levels.size() * sizeof(LevelDesc) + P.rs.size() * 4 + meta32.size() * 4 +
+                                  meta64.size() * 8 + P.levelJobs.size() * sizeof(LevelJob) +
+                                  P.groupJobs.size() * sizeof(ResizeJob) + 7 * 256;
+        VJ_TRY(e->hStage.ensure(stageBytes));
+        VJ_HIP(hipStreamSynchronize(s));  // staging buffer reuse across calls
+        size_t stOff = 0;
+        auto place = [&](const void* src, size_t bytes) {
+            const size_t at = stOff;
+            if (bytes) std::memcpy(e->hStage.get() + at, src, bytes);
+            stOff += static_cast<size_t>(align_up(static_cast<int64_t>(bytes), 256));
+            return at;
+        };
+        T.aLv = place(P.levels.data(), P.levels.size() * sizeof(LevelDesc));
+        T.aRs = place(P.rs.data(), P.rs.size() * sizeof(float));
+        T.a32 = place(meta32.data(), meta32.size() * sizeof(int32_t));
+        T.a64 = place(meta64.data(), meta64.size() * sizeof(int64_t));
+        T.aLJobs = place(P.levelJobs.data(), P.levelJobs.size() * sizeof(LevelJob));
+        T.aGJobs = place(P.groupJobs.data(), P.groupJobs.size() * sizeof(ResizeJob));
+        VJ_TRY(e->dPlan.ensure(std::max<size_t>(stOff, 256)));
+        VJ_HIP(hipMemcpyAsync(e->dPlan.get(), e->hStage.get(), stOff, hipMemcpyHostToDevice, s));
     }
-    imgBlock.push_back(static_cast<int32_t>(blockImg.size()));
-    blockWord.push_back(P.imgWordOff.back());
-    const size_t oBlockWord = meta64.size();
-    meta64.insert(meta64.end(), blockWord.begin(), blockWord.end());
-    const size_t oBlockImg = meta32.size();
-    meta32.insert(meta32.end(), blockImg.begin(), blockImg.end());
-    const size_t oImgBlock = meta32.size();
-    meta32.insert(meta32.end(), imgBlock.begin(), imgBlock.end());
-    VJ_TRY(e->dBlockCount.ensure(blockImg.size()));
-    VJ_TRY(e->dBlockOff.ensure(blockImg.size()));
+    VJ_TRY(e->dBlockCount.ensure(static_cast<size_t>(P.tab.nBlocks)));
+    VJ_TRY(e->dBlockOff.ensure(static_cast<size_t>(P.tab.nBlocks)));
     VJ_TRY(e->dImgBase.ensure(static_cast<size_t>(nImages)));
-    // plan upload: every table into one staging buffer, then one copy
-    const size_t stageBytes = P.levels.size() * sizeof(LevelDesc) + P.rs.size() * 4 + meta32.size() * 4 +
-                              meta64.size() * 8 + P.levelJobs.size() * sizeof(LevelJob) +
-                              P.groupJobs.size() * sizeof(ResizeJob) + 7 * 256;
-    VJ_TRY(e->hStage.ensure(stageBytes));
-    VJ_HIP(hipStreamSynchronize(s));  // staging buffer reuse across calls
-    size_t stOff = 0;
-    auto place = [&](const void* src, size_t bytes) {
-        const size_t at = stOff;
-        if (bytes) std::memcpy(e->hStage.get() + at, src, bytes);
-        stOff += static_cast<size_t>(align_up(static_cast<int64_t>(bytes), 256));
-        return at;
-    };
-    const size_t aLv = place(P.levels.data(), P.levels.size() * sizeof(LevelDesc));
-    const size_t aRs = place(P.rs.data(), P.rs.size() * sizeof(float));
-    const size_t a32 = place(meta32.data(), meta32.size() * sizeof(int32_t));
-    const size_t a64 = place(meta64.data(), meta64.size() * sizeof(int64_t));
-    const size_t aLJobs = place(P.levelJobs.data(), P.levelJobs.size() * sizeof(LevelJob));
-    const size_t aGJobs = place(P.groupJobs.data(), P.groupJobs.size() * sizeof(ResizeJob));
-    VJ_TRY(e->dPlan.ensure(std::max<size_t>(stOff, 256)));
-    VJ_HIP(hipMemcpyAsync(e->dPlan.get(), e->hStage.get(), stOff, hipMemcpyHostToDevice, s));
+    const size_t oItem = P.tab.oItem, oChunk = P.tab.oChunk, oSrc = P.tab.oSrc, oLvl = P.tab.oLvl,
+                 oTile = P.tab.oTile, oLvWord = P.tab.oLvWord, oBlockWord = P.tab.oBlockWord,
+                 oBlockImg = P.tab.oBlockImg, oImgBlock = P.tab.oImgBlock;
+    const size_t aLv = P.tab.aLv, aRs = P.tab.aRs, a32 = P.tab.a32, a64 = P.tab.a64, aLJobs = P.tab.aLJobs,
+                 aGJobs = P.tab.aGJobs;
 
     const LevelDesc* dL = reinterpret_cast<const LevelDesc*>(e->dPlan.get() + aLv);
     const float* dRs = reinterpret_cast<const float*>(e->dPlan.get() + aRs);
@@ -2669,7 +2695,7 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s) {
     // deep and tail passes: the leaf values after the footprint, 16-byte aligned
     const int leafOff = static_cast<int>((ldsBytes / sizeof(float) + 3) & ~size_t(3));
     const size_t deepLds = (static_cast<size_t>(leafOff) + kLeafCap) * sizeof(float);
-    const int nBlocks = static_cast<int>(blockImg.size());
+    const int nBlocks = P.tab.nBlocks;
     const int deepGrid = deep ? std::min(P.nTiles * kDeepSlices, kDeepGrid) : 0;
     const bool tail = deep && e->stageTail < e->nStages;
     const int stageTail = deep ? e->stageTail : e->nStages;
@@ -3182,6 +3208,7 @@ extern "C" int vj_engine_set_cascade(vj_engine* e, const vj_feature* dFeatures, 
     e->nModelFeat = nF;
     e->nStumps = nS;
     e->nStages = nSt;
+    e->planValid = false;
     // The first pass runs the leading stages whose stumps add up to at most
     // kSplitStumps (at least one stage); the rest go to the deep pass.
     {
@@ -3243,7 +3270,8 @@ int autotune(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int minOb
             for (int rep = 0; rep < 2 && status == VJ_OK; ++rep) {
                 Plan P;
                 status = build_plan(e, imgs, sf, minObj, P);
-                if (status == VJ_OK) status = run_pipeline(e, P, capPer, s);
+                e->planValid = false;  // the tuning plans overwrite the plan buffer
+                if (status == VJ_OK) status = run_pipeline(e, P, capPer, s, false);
                 float ms = 0.0f;
                 if (status == VJ_OK && hipStreamSynchronize(s) == hipSuccess &&
                     hipEventElapsedTime(&ms, e->ev[2], e->ev[3]) == hipSuccess) {
@@ -3280,10 +3308,35 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
     }
     const int capPer = std::max(1, std::min(maxDet, e->maxDet));
     if (e->autotune) VJ_TRY(autotune(e, imgs, sf, minObj, capPer, s));
-    Plan P;
-    VJ_TRY(build_plan(e, imgs, sf, minObj, P));
+    // The plan depends only on the images (addresses, sizes, strides), the
+    // scale factor, the minimum object size and the tile rows; a call that
+    // repeats the previous call's reuses its plan and the tables it left in
+    // the plan buffer (while the pyramid buffer has not moved).
+    std::vector<int64_t> key;
+    {
+        uint32_t sfBits;
+        std::memcpy(&sfBits, &sf, 4);
+        for (int64_t v : {int64_t(sfBits), int64_t(minObj), int64_t(capPer), int64_t(e->tileRows[1]),
+                          int64_t(e->tileRows[2]), int64_t(e->tileRows[3]), int64_t(e->nStages)})
+            key.push_back(v);
+        for (const ImageIn& in : imgs)
+            for (int64_t v : {int64_t(reinterpret_cast<uintptr_t>(in.pix)), int64_t(in.w), int64_t(in.h),
+                              int64_t(in.stride)})
+                key.push_back(v);
+    }
+    const bool reuse = e->planValid && key == e->planKey && e->pyr.get() == e->planPyr;
+    Plan& P = e->plan;
+    if (!reuse) {
+        e->planValid = false;
+        VJ_TRY(build_plan(e, imgs, sf, minObj, P));
+    }
     const int nImages = static_cast<int>(imgs.size());
-    VJ_TRY(run_pipeline(e, P, capPer, s));
+    VJ_TRY(run_pipeline(e, P, capPer, s, reuse));
+    if (!reuse) {
+        e->planKey = std::move(key);
+        e->planPyr = e->pyr.get();
+        e->planValid = true;
+    }
     // grouping on the device, every image of the call in the same launches:
     // sweep order, similar pairs (and their unions), component statistics,
     // the reference's DSU roots of the kept components, the clusters
