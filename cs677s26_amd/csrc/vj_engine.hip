@@ -338,7 +338,8 @@ __device__ __forceinline__ void load_band(const LevelDesc& lv, int x, int y0, in
 __global__ __launch_bounds__(64) void ii_colsum_kernel(const LevelDesc* __restrict__ L, int nItems,
                                                        const int32_t* __restrict__ itemOffs, int nLevels,
                                                        uint32_t* __restrict__ colS, uint32_t* __restrict__ colQ,
-                                                       uint32_t* __restrict__ rowS, uint32_t* __restrict__ rowQ) {
+                                                       uint32_t* __restrict__ rowS, uint32_t* __restrict__ rowQ,
+                                                       uint32_t* __restrict__ totS, uint64_t* __restrict__ totQ) {
     const int item = blockIdx.x;
     if (item >= nItems) return;
     const int li = find_segment(itemOffs, nLevels, item);
@@ -351,6 +352,8 @@ __global__ __launch_bounds__(64) void ii_colsum_kernel(const LevelDesc* __restri
     const int y0 = b * kBandRows;
     const int y1 = min(lv.h, y0 + kBandRows);
     uint32_t cs[4] = {0, 0, 0, 0}, cq[4] = {0, 0, 0, 0};
+    uint32_t bandS = 0;
+    uint64_t bandQ = 0;
     uint32_t pv[kBandRows];
     load_band(lv, x, y0, y1, pv);
 #pragma unroll
@@ -368,10 +371,18 @@ __global__ __launch_bounds__(64) void ii_colsum_kernel(const LevelDesc* __restri
         }
         rs = wave_sum_dpp(rs);
         rq = wave_sum_dpp(rq);
+        bandS += rs;
+        bandQ += rq;
         if (lane == 0) {
             rowS[lv.rowOff + static_cast<int64_t>(y) * lv.nChunks + c] = rs;
             rowQ[lv.rowOff + static_cast<int64_t>(y) * lv.nChunks + c] = rq;
         }
+    }
+    // the band's total over the chunk (turned into a prefix over bands by
+    // the vertical scan)
+    if (lane == 0) {
+        totS[lv.totOff + static_cast<int64_t>(b) * lv.nChunks + c] = bandS;
+        totQ[lv.totOff + static_cast<int64_t>(b) * lv.nChunks + c] = bandQ;
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -384,52 +395,64 @@ __global__ __launch_bounds__(64) void ii_colsum_kernel(const LevelDesc* __restri
 
 // Integral phase 2, one workgroup per (level, chunk of 256 columns): turns
 // the band column sums into exclusive vertical prefixes (column sums above
-// each band, in place) and records each band's total of them over the chunk
-// (per-wave partial totals in LDS, 256 bands at a time).
+// each band, in place), and the band totals of the chunk (from phase 1) into
+// their exclusive prefix over the bands: the sum of all pixels above the band
+// in the chunk's columns.
 __global__ __launch_bounds__(256) void ii_vscan_kernel(const LevelDesc* __restrict__ L,
                                                        const int32_t* __restrict__ chunkOffs, int nLevels,
                                                        uint32_t* __restrict__ colS, uint32_t* __restrict__ colQ,
-                                                       uint32_t* __restrict__ totS, uint64_t* __restrict__ totQ) {
-    constexpr int kBandBlock = 256;
-    __shared__ uint32_t partS[kBandBlock][4];
-    __shared__ uint64_t partQ[kBandBlock][4];
+                                                       uint32_t* __restrict__ totS, uint64_t* __restrict__ totQ,
+                                                       int32_t* __restrict__ zeroCtl, int nZero) {
+    // the cascade's work-list counters (detect only), zeroed here to save a launch
+    if (zeroCtl && blockIdx.x == 0)
+        for (int i = static_cast<int>(threadIdx.x); i < nZero; i += blockDim.x) zeroCtl[i] = 0;
     const int item = blockIdx.x;
     const int li = find_segment(chunkOffs, nLevels, item);
     const LevelDesc lv = L[li];
     const int c = item - lv.chunkItemOff;
     const int x = c * kChunkCols + threadIdx.x;
-    const int wave = threadIdx.x >> 6;
-    uint32_t s = 0, q = 0;
-    for (int b0 = 0; b0 < lv.nBands; b0 += kBandBlock) {
-        const int b1 = min(lv.nBands, b0 + kBandBlock);
-        for (int b = b0; b < b1; ++b) {
-            uint32_t vs = s, vq = q;
-            if (x < lv.w) {
-                const int64_t i = lv.colOff + static_cast<int64_t>(b) * lv.w + x;
-                const uint32_t a = colS[i], aq = colQ[i];
+    if (x < lv.w) {
+        uint32_t s = 0, q = 0;
+        constexpr int kU = 4;  // bands loaded ahead of their use
+        int b0 = 0;
+        for (; b0 + kU <= lv.nBands; b0 += kU) {
+            uint32_t a[kU], aq[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int64_t i = lv.colOff + static_cast<int64_t>(b0 + u) * lv.w + x;
+                a[u] = colS[i];
+                aq[u] = colQ[i];
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int64_t i = lv.colOff + static_cast<int64_t>(b0 + u) * lv.w + x;
                 colS[i] = s;
                 colQ[i] = q;
-                s += a;
-                q += aq;
-            } else {
-                vs = 0;
-                vq = 0;
-            }
-            const uint32_t ws = wave_sum(vs);
-            const uint64_t wq = wave_sum(static_cast<uint64_t>(vq));
-            if ((threadIdx.x & 63) == 0) {
-                partS[b - b0][wave] = ws;
-                partQ[b - b0][wave] = wq;
+                s += a[u];
+                q += aq[u];
             }
         }
-        __syncthreads();
-        for (int b = b0 + static_cast<int>(threadIdx.x); b < b1; b += blockDim.x) {
-            totS[lv.totOff + static_cast<int64_t>(b) * lv.nChunks + c] =
-                partS[b - b0][0] + partS[b - b0][1] + partS[b - b0][2] + partS[b - b0][3];
-            totQ[lv.totOff + static_cast<int64_t>(b) * lv.nChunks + c] =
-                partQ[b - b0][0] + partQ[b - b0][1] + partQ[b - b0][2] + partQ[b - b0][3];
+        for (int b = b0; b < lv.nBands; ++b) {
+            const int64_t i = lv.colOff + static_cast<int64_t>(b) * lv.w + x;
+            const uint32_t a = colS[i], aq = colQ[i];
+            colS[i] = s;
+            colQ[i] = q;
+            s += a;
+            q += aq;
         }
-        __syncthreads();  // the partial totals are rewritten by the next block
+    }
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        uint64_t q = 0;
+        for (int b = 0; b < lv.nBands; ++b) {
+            const int64_t i = lv.totOff + static_cast<int64_t>(b) * lv.nChunks + c;
+            const uint32_t a = totS[i];
+            const uint64_t aq = totQ[i];
+            totS[i] = s;
+            totQ[i] = q;
+            s += a;
+            q += aq;
+        }
     }
 }
 
@@ -615,6 +638,9 @@ constexpr int kCascadeBlock = 256;
 // per tile; a work item is tile * kDeepSlices + slice).
 constexpr int kDeepSlice = 64;
 constexpr int kDeepSlices = kTileMaxWin / kDeepSlice;
+// Layout of the deep and tail passes' counters (see deep_list_kernel).
+constexpr int kDeepHist = 4, kDeepCursor = kDeepHist + kTileMaxWin + 1, kTailCtl = kDeepCursor + kTileMaxWin + 1;
+constexpr int kDeepCtlInts = kTailCtl + 2;
 // Stumps of the leading stages the first pass runs, and the deep pass's
 // persistent grid (eight workgroups on each of 256 CUs).
 constexpr int kSplitStumps = 128;
@@ -1047,6 +1073,7 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(
             survBase = n ? atomicAdd(survTotal, n) : 0;
             tileSurvOff[t] = survBase;
             tileCount[t] = n;
+            if (n > 0) atomicAdd(&survTotal[kDeepHist + n], 1);  // the work list's histogram
         }
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += kCascadeBlock) {
@@ -1060,75 +1087,61 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(
 // Deep-pass work list: every tile with survivors after the first pass
 // contributes ceil(count / kDeepSlice) items (tile * kDeepSlices + slice),
 // ordered by survivor count, largest first, so the longest items start
-// first.  Built in three launches: a histogram of the tiles' survivor
-// counts, each count's first item (counts in descending order) with the item
-// total and a zeroed work counter, then every tile's items at its count's
-// offset (tiles of one count in any order).  ctl layout: [0] survivor total,
-// [1] items, [2] work counter, then per count its tile histogram (replaced
-// by its first item) and its cursor, then the tail pass's item total and
-// work counter.
-constexpr int kDeepHist = 4, kDeepCursor = kDeepHist + kTileMaxWin + 1, kTailCtl = kDeepCursor + kTileMaxWin + 1;
-constexpr int kDeepCtlInts = kTailCtl + 2;
+// first.  The counters are zeroed by the integral pass (ii_vscan_kernel), the
+// first pass adds each tile to the histogram of survivor counts, and one
+// launch then turns the histogram into each count's first item (counts in
+// descending order, recomputed by every workgroup) and places every tile's
+// items at its count's offset (tiles of one count in any order).  ctl layout:
+// [0] survivor total, [1] items, [2] work counter, then per count its tile
+// histogram and its cursor, then the tail pass's item total and work counter.
 
 __global__ __launch_bounds__(256) void zero_u64_kernel(unsigned long long* __restrict__ a, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = 0ull;
 }
 
-__global__ __launch_bounds__(256) void deep_reset_kernel(int32_t* __restrict__ ctl) {
-    for (int i = threadIdx.x; i < kDeepCtlInts; i += blockDim.x) ctl[i] = 0;
-}
+constexpr int kListThreads = 256;
+static_assert(kTileMaxWin == 2 * kListThreads, "the count scan takes two counts per thread");
 
-__global__ __launch_bounds__(256) void deep_hist_kernel(const int32_t* __restrict__ tileCount, int nTiles,
-                                                        int32_t* __restrict__ ctl) {
-    __shared__ int32_t h[kTileMaxWin + 1];
-    for (int c = threadIdx.x; c <= kTileMaxWin; c += blockDim.x) h[c] = 0;
-    __syncthreads();
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nTiles; t += gridDim.x * blockDim.x) {
-        const int c = tileCount[t];
-        if (c > 0) atomicAdd(&h[c], 1);
-    }
-    __syncthreads();
-    for (int c = threadIdx.x; c <= kTileMaxWin; c += blockDim.x)
-        if (h[c]) atomicAdd(&ctl[kDeepHist + c], h[c]);
-}
-
-// One workgroup of kTileMaxWin threads; thread i takes count kTileMaxWin - i.
-__global__ __launch_bounds__(kTileMaxWin) void deep_base_kernel(int32_t* __restrict__ ctl) {
-    __shared__ int32_t waveTot[kTileMaxWin / kWave];
-    const int c = kTileMaxWin - static_cast<int>(threadIdx.x);
-    const int32_t v = ctl[kDeepHist + c] * ((c + kDeepSlice - 1) / kDeepSlice);
-    const int32_t inc = wave_incl_scan(v);
-    if (lane_id() == kWave - 1) waveTot[threadIdx.x / kWave] = inc;
-    __syncthreads();
-    int32_t base = inc - v, total = 0;
-    for (int w = 0; w < kTileMaxWin / kWave; ++w) {
-        if (w < static_cast<int>(threadIdx.x / kWave)) base += waveTot[w];
-        total += waveTot[w];
-    }
-    ctl[kDeepHist + c] = base;
-    if (threadIdx.x == 0) {
-        ctl[1] = total;
-        ctl[2] = 0;
-    }
-}
-
-__global__ __launch_bounds__(256) void deep_scatter_kernel(const int32_t* __restrict__ tileCount, int nTiles,
-                                                           int32_t* __restrict__ ctl, int32_t* __restrict__ items) {
+__global__ __launch_bounds__(kListThreads) void deep_list_kernel(const int32_t* __restrict__ tileCount, int nTiles,
+                                                                 int32_t* __restrict__ ctl,
+                                                                 int32_t* __restrict__ items) {
+    __shared__ int32_t base[kTileMaxWin + 1];
     __shared__ int32_t cnt[kTileMaxWin + 1];
     __shared__ int32_t off[kTileMaxWin + 1];
-    for (int t0 = blockIdx.x * blockDim.x; t0 < nTiles; t0 += gridDim.x * blockDim.x) {
-        for (int c = threadIdx.x; c <= kTileMaxWin; c += blockDim.x) cnt[c] = 0;
+    __shared__ int32_t waveTot[kListThreads / kWave];
+    const int tid = static_cast<int>(threadIdx.x);
+    // each count's first item: thread i takes counts kTileMaxWin - 2i and
+    // kTileMaxWin - 2i - 1 (descending counts)
+    {
+        const int c0 = kTileMaxWin - 2 * tid, c1 = c0 - 1;
+        const int32_t v0 = ctl[kDeepHist + c0] * ((c0 + kDeepSlice - 1) / kDeepSlice);
+        const int32_t v1 = ctl[kDeepHist + c1] * ((c1 + kDeepSlice - 1) / kDeepSlice);
+        const int32_t t = v0 + v1;
+        const int32_t inc = wave_incl_scan(t);
+        if (lane_id() == kWave - 1) waveTot[tid / kWave] = inc;
         __syncthreads();
-        const int t = t0 + static_cast<int>(threadIdx.x);
+        int32_t b = inc - t, total = 0;
+        for (int w = 0; w < kListThreads / kWave; ++w) {
+            if (w < tid / kWave) b += waveTot[w];
+            total += waveTot[w];
+        }
+        base[c0] = b;
+        base[c1] = b + v0;
+        if (blockIdx.x == 0 && tid == 0) ctl[1] = total;
+    }
+    for (int t0 = blockIdx.x * kListThreads; t0 < nTiles; t0 += gridDim.x * kListThreads) {
+        for (int c = tid; c <= kTileMaxWin; c += kListThreads) cnt[c] = 0;
+        __syncthreads();
+        const int t = t0 + tid;
         const int c = t < nTiles ? tileCount[t] : 0;
         const int local = c > 0 ? atomicAdd(&cnt[c], 1) : 0;
         __syncthreads();
-        for (int b = threadIdx.x; b <= kTileMaxWin; b += blockDim.x)
+        for (int b = tid; b <= kTileMaxWin; b += kListThreads)
             if (cnt[b]) off[b] = atomicAdd(&ctl[kDeepCursor + b], cnt[b]);
         __syncthreads();
         if (c > 0) {
             const int ns = (c + kDeepSlice - 1) / kDeepSlice;
-            const int pos = ctl[kDeepHist + c] + (off[c] + local) * ns;
+            const int pos = base[c] + (off[c] + local) * ns;
             for (int k = 0; k < ns; ++k) items[pos + k] = t * kDeepSlices + k;
         }
         __syncthreads();  // cnt and off are rewritten by the next chunk
@@ -2728,10 +2741,12 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s, bool tablesCurre
         // 2. integrals of every level
         if (P.nItems > 0) {
             hipLaunchKernelGGL(ii_colsum_kernel, dim3(P.nItems), dim3(64), 0, ls, dL, P.nItems, m32 + oItem, nLevels,
-                               e->colS.get(), e->colQ.get(), e->rowS.get(), e->rowQ.get());
+                               e->colS.get(), e->colQ.get(), e->rowS.get(), e->rowQ.get(), e->totS.get(),
+                               e->totQ.get());
             VJ_HIP(hipGetLastError());
             hipLaunchKernelGGL(ii_vscan_kernel, dim3(P.nChunkItems), dim3(256), 0, ls, dL, m32 + oChunk, nLevels,
-                               e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get());
+                               e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get(),
+                               deep ? e->deepCtl.get() : nullptr, kDeepCtlInts);
             VJ_HIP(hipGetLastError());
             hipLaunchKernelGGL(ii_write_kernel, dim3(P.nItems), dim3(64), 0, ls, dL, P.nItems, m32 + oItem, nLevels,
                                e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get(), e->rowS.get(),
@@ -2742,10 +2757,6 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s, bool tablesCurre
         //    largest window step present (its tile footprint is the widest)
         if (e->timing) VJ_HIP(hipEventRecord(e->ev[2], ls));
         if (P.nTiles > 0) {
-            if (deep) {
-                hipLaunchKernelGGL(deep_reset_kernel, dim3(1), dim3(256), 0, ls, e->deepCtl.get());
-                VJ_HIP(hipGetLastError());
-            }
             hipLaunchKernelGGL(enter ? cascade_kernel<true> : cascade_kernel<false>, dim3(P.nTiles), dim3(kCascadeBlock), ldsBytes, ls, dL, m32 + oLvl,
                                m32 + oTile, static_cast<int>(P.tileLvl.size()), P.nTiles, e->iiS.get(), e->iiQ.get(),
                                e->kst.get(), e->nStumps, e->kstage.get(), e->nStages, e->stageSplit, e->winW, e->winH,
@@ -2753,13 +2764,8 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s, bool tablesCurre
                                e->tileSurvOff.get(), e->tileCount.get(), enter);
             VJ_HIP(hipGetLastError());
             if (deep) {
-                const int listGrid = std::min((P.nTiles + 255) / 256, 1024);
-                hipLaunchKernelGGL(deep_hist_kernel, dim3(listGrid), dim3(256), 0, ls, e->tileCount.get(), P.nTiles,
-                                   e->deepCtl.get());
-                VJ_HIP(hipGetLastError());
-                hipLaunchKernelGGL(deep_base_kernel, dim3(1), dim3(kTileMaxWin), 0, ls, e->deepCtl.get());
-                VJ_HIP(hipGetLastError());
-                hipLaunchKernelGGL(deep_scatter_kernel, dim3(listGrid), dim3(256), 0, ls, e->tileCount.get(),
+                const int listGrid = std::min((P.nTiles + kListThreads - 1) / kListThreads, 1024);
+                hipLaunchKernelGGL(deep_list_kernel, dim3(listGrid), dim3(kListThreads), 0, ls, e->tileCount.get(),
                                    P.nTiles, e->deepCtl.get(), e->deepItems.get());
                 VJ_HIP(hipGetLastError());
                 hipLaunchKernelGGL(enter ? cascade_deep_kernel<true> : cascade_deep_kernel<false>, dim3(deepGrid), dim3(kCascadeBlock), deepLds, ls, dL,
@@ -2982,10 +2988,11 @@ extern "C" int vj_engine_integral_batch(vj_engine* e, const uint8_t* dGray, int 
         VJ_TRY(upload(e->dMeta32, offs, e->hStage, stOff, s));
         const int items = d.nBands * d.nChunks;
         hipLaunchKernelGGL(ii_colsum_kernel, dim3(items), dim3(64), 0, s, e->dLevels.get(), items, e->dMeta32.get(), 1,
-                           e->colS.get(), e->colQ.get(), e->rowS.get(), e->rowQ.get());
+                           e->colS.get(), e->colQ.get(), e->rowS.get(), e->rowQ.get(), e->totS.get(), e->totQ.get());
         VJ_HIP(hipGetLastError());
         hipLaunchKernelGGL(ii_vscan_kernel, dim3(d.nChunks), dim3(256), 0, s, e->dLevels.get(),
-                           e->dMeta32.get() + 1, 1, e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get());
+                           e->dMeta32.get() + 1, 1, e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get(),
+                           nullptr, 0);
         VJ_HIP(hipGetLastError());
         hipLaunchKernelGGL(ii_write_kernel, dim3(items), dim3(64), 0, s, e->dLevels.get(), items, e->dMeta32.get(), 1,
                            e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get(), e->rowS.get(),
@@ -3113,9 +3120,11 @@ extern "C" int vj_integral_image_noalloc(const uint8_t* dGray, int w, int h, int
     const int items = d.nBands * d.nChunks;
     hipLaunchKernelGGL(na_setup_kernel, dim3(1), dim3(64), 0, s, d, base);
     VJ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(ii_colsum_kernel, dim3(items), dim3(64), 0, s, L, items, offs, 1, colS, colQ, rowS, rowQ);
+    hipLaunchKernelGGL(ii_colsum_kernel, dim3(items), dim3(64), 0, s, L, items, offs, 1, colS, colQ, rowS, rowQ,
+                       totS, totQ);
     VJ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(ii_vscan_kernel, dim3(d.nChunks), dim3(256), 0, s, L, offs + 1, 1, colS, colQ, totS, totQ);
+    hipLaunchKernelGGL(ii_vscan_kernel, dim3(d.nChunks), dim3(256), 0, s, L, offs + 1, 1, colS, colQ, totS, totQ,
+                       nullptr, 0);
     VJ_HIP(hipGetLastError());
     hipLaunchKernelGGL(ii_write_kernel, dim3(items), dim3(64), 0, s, L, items, offs, 1, colS, colQ, totS, totQ, rowS,
                        rowQ, dSum, dSq);
