@@ -487,7 +487,7 @@ def detect_fields(args):
     casc_ms = d["phase_ms"][2]
     casc_bytes = d["stats"]["cascadeBytes"]
     achieved = casc_bytes / (casc_ms * 1e-3) / 1e9 if casc_ms > 0 else 0.0
-    out["roofline"] = {"bound": "hbm", "kernel": "cascade_kernel+deep_hist/base/scatter_kernel+cascade_deep_kernel+cascade_tail_kernel",
+    out["roofline"] = {"bound": "hbm", "kernel": "cascade_kernel+deep_list_kernel+cascade_deep_kernel+cascade_tail_kernel",
                        "achieved": achieved, "peak": HBM_PEAK_GBS,
                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                        "traffic": measured_traffic("cascade_kernel_detect1080"),
