@@ -164,6 +164,27 @@ def test_detect_edge_cases(cascade3):
     assert len(eng3.detectMultiScale(d, 333, 250, 341, 1.25, 4, 24, 200000, False)) == 0
 
 
+def test_detect_large_levels_with_unaligned_rows(cascade3):
+    """Levels large enough for the pyramid's per-depth launches (at least 512 K
+    pixels at a depth), read from rows that start at odd addresses: an odd
+    width, then the same image in a buffer with an odd wider stride, then a
+    minimum object size that skips level 0 (the first level resized straight
+    from the input at a larger ratio).  Raw boxes equal the oracle's."""
+    say("detect 1283x777, odd strides, per-depth resize launches")
+    feats, stumps, stages = cascade3
+    w, h = 1283, 777
+    img = synth.detect_image(w, h, 60, 24, 90, seed=17)
+    eng = _engine(w, h, cascade3)
+    padded = np.zeros((h, w + 8), np.uint8)
+    padded[:, :w] = img
+    for buf, stride in ((img, w), (padded, w + 8)):
+        d = DeviceArray.from_host(buf)
+        for min_obj, sf in ((24, 1.25), (31, 1.5)):
+            raw = eng.detectMultiScale(d, w, h, stride, sf, 4, min_obj, 200000, False)
+            want, _ = O.detect_raw(feats, stumps, stages, img, sf=sf, min_obj=min_obj)
+            assert det_tuples(raw) == det_tuples(want), (stride, min_obj, sf)
+
+
 def test_detect_batch_equals_single(cascade3):
     say("detect batch of 6 uneven images")
     sizes = [(640, 480), (333, 250), (1000, 431), (24, 24), (800, 600), (97, 201)]
