@@ -112,23 +112,28 @@ __device__ __forceinline__ float response(const vj_feature& f, const uint32_t* _
 // before +0, as the reference's radix sort orders them), rebased to the
 // feature's smallest key so that only the bits in which the feature's keys
 // differ are sorted, 8 per pass.  The sorted positions are padded to whole
-// waves (nPad = n rounded up to 1024) with the largest key, which sorts them
-// after every real sample; stability keeps equal keys in sample order.
-// Each wave owns nPad/16 consecutive positions and ranks them 64 at a time:
-// eight ballots give every lane the lanes holding the same digit, so a
-// lane's rank is its count of same-digit lanes below it plus the wave's
-// running count of that digit; one exclusive scan over (digit, wave) then
-// places every element.  Output per position k: the sample index, with bit
-// 15 set when fl(v_k + 1e-7f) < v_{k+1} (src/boosting_core.cu:395-405).
-// mode 0 computes the responses from the integrals; mode 1 reads them from
-// resp (row f of [count][n]) and writes plain indices (sortSamplesPerFeature
-// semantics).
+// waves (nPad = NB * 1024) with the largest key, which sorts them after every
+// real sample; stability keeps equal keys in sample order.  Each wave owns
+// NB * 64 consecutive positions and ranks them 64 at a time: eight ballots
+// give every lane the lanes holding the same digit, so a lane's rank is its
+// count of same-digit lanes below it plus the wave's running count of that
+// digit; one exclusive scan over (digit, wave) then places every element.
+// Each lane computes the responses of the positions it ranks in the first
+// pass (coalesced gathers of 64 consecutive samples).  Output per position
+// k: the sample index, with bit 15 set when fl(v_k + 1e-7f) < v_{k+1}
+// (src/boosting_core.cu:395-405).  mode 0 computes the responses from the
+// integrals; mode 1 reads them from resp (row f of [count][n]) and writes
+// plain indices (sortSamplesPerFeature semantics).  Workgroup i takes
+// feature xcd_spread(i, count): the workgroups that one XCD runs together
+// take consecutive features, whose corner rows mostly coincide, so the
+// gathers hit that XCD's L2.
 // ---------------------------------------------------------------------------
 constexpr int kSortWaves = kSortThreads / kWave;
 
 // The lanes of a full wavefront whose 8-bit digit equals this lane's: a lane
 // differs from this one where some bit differs, so OR over the bits of
-// (lanes with the bit set) XOR (all ones if this lane has it), then invert.
+// (lanes with the bit set) XOR (all ones if this lane has it); the last bit's
+// step takes the complement.
 __device__ __forceinline__ uint64_t same_digit_lanes(uint32_t dg) {
     uint32_t xlo = 0u, xhi = 0u;
 #pragma unroll
@@ -137,58 +142,114 @@ __device__ __forceinline__ uint64_t same_digit_lanes(uint32_t dg) {
         // the ballot of ext != 0, compared on ext itself
         uint64_t bb;
         asm("v_cmp_ne_u32_e64 %0, 0, %1" : "=s"(bb) : "v"(ext));
-        xlo = __builtin_amdgcn_bitop3_b32(ext, xlo, static_cast<uint32_t>(bb), 0xDE);   // xlo | (bb ^ ext)
-        xhi = __builtin_amdgcn_bitop3_b32(ext, xhi, static_cast<uint32_t>(bb >> 32), 0xDE);
+        if (b < 7) {
+            xlo = __builtin_amdgcn_bitop3_b32(ext, xlo, static_cast<uint32_t>(bb), 0xDE);   // xlo | (bb ^ ext)
+            xhi = __builtin_amdgcn_bitop3_b32(ext, xhi, static_cast<uint32_t>(bb >> 32), 0xDE);
+        } else {
+            xlo = __builtin_amdgcn_bitop3_b32(ext, xlo, static_cast<uint32_t>(bb), 0x21);   // ~(xlo | (bb ^ ext))
+            xhi = __builtin_amdgcn_bitop3_b32(ext, xhi, static_cast<uint32_t>(bb >> 32), 0x21);
+        }
     }
-    return ~((static_cast<uint64_t>(xhi) << 32) | xlo);
+    return (static_cast<uint64_t>(xhi) << 32) | xlo;
 }
 // Digit counters: one row of 256 per wave, rows kCntPitch words apart, so a
 // wave's lanes reading the counters of distinct digits hit distinct banks.
 constexpr int kCntPitch = 257;
 static_assert(kSortWaves == 16 && 256 * kSortWaves == 4 * kSortThreads, "the counter scan takes 4 entries per thread");
 
+// Block i of a count-block grid -> item: the hardware deals blocks to the 8
+// XCDs in turn, so XCD x runs blocks x, x + 8, ...; they take the x-th
+// contiguous eighth of [0, count).
+__device__ __forceinline__ int xcd_spread(int i, int count) {
+    const int q = count >> 3, r = count & 7, xcd = i & 7, pos = i >> 3;
+    return xcd < r ? xcd * (q + 1) + pos : r * (q + 1) + (xcd - r) * q + pos;
+}
+
+// Keys of positions p0 + 64 i (i < NB) of feature ft with R rectangles
+// (samples past n read sample n - 1; the caller replaces their keys).  The
+// corner rows are uniform base pointers and a lane's sample a 32-bit byte
+// offset from them; four positions' corners are loaded before any is used.
+template <int NB, int R>
+__device__ __forceinline__ void feature_keys(const vj_feature& ft, const uint32_t* __restrict__ sumT,
+                                             const float* __restrict__ inv, int n, int p0, uint32_t (&kr)[NB]) {
+    const uint8_t* c[R][4];
+    float w[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        c[r][0] = reinterpret_cast<const uint8_t*>(sumT + static_cast<size_t>(ft.r[r].p0) * n);
+        c[r][1] = reinterpret_cast<const uint8_t*>(sumT + static_cast<size_t>(ft.r[r].p1) * n);
+        c[r][2] = reinterpret_cast<const uint8_t*>(sumT + static_cast<size_t>(ft.r[r].p2) * n);
+        c[r][3] = reinterpret_cast<const uint8_t*>(sumT + static_cast<size_t>(ft.r[r].p3) * n);
+        w[r] = ft.r[r].w;
+    }
+    const uint8_t* iv = reinterpret_cast<const uint8_t*>(inv);
+    constexpr int G = 4;
+#pragma unroll
+    for (int i0 = 0; i0 < NB; i0 += G) {
+        uint32_t v[G][R][4];
+        float sc[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (i0 + g < NB) {
+                const uint32_t off = static_cast<uint32_t>(min(p0 + (i0 + g) * kWave, n - 1)) * 4u;
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) v[g][r][k] = *reinterpret_cast<const uint32_t*>(c[r][k] + off);
+                sc[g] = *reinterpret_cast<const float*>(iv + off);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (i0 + g < NB) {
+                float s[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    s[r] = static_cast<float>(static_cast<int32_t>(v[g][r][0] - v[g][r][1] - v[g][r][2] + v[g][r][3]));
+                // src/main.cu:476-496: w0*S0 + w1*S1 (+ w2*S2), then * invNorm
+                float resp = w[0] * s[0] + w[1] * s[1];
+                if (R == 3) resp += w[2] * s[2];
+                kr[i0 + g] = float_key(resp * sc[g]);
+            }
+        }
+    }
+}
+
+template <int NB>
 __global__ __launch_bounds__(kSortThreads) void presort_kernel(const vj_feature* __restrict__ feats, int fBegin,
-                                                               const uint32_t* __restrict__ sumT,
+                                                               int count, const uint32_t* __restrict__ sumT,
                                                                const float* __restrict__ inv,
                                                                const float* __restrict__ resp, int n,
                                                                uint16_t* __restrict__ out, int64_t outPitch,
                                                                int mode) {
     extern __shared__ uint8_t smem[];
-    const int nPad = (n + kSortThreads - 1) / kSortThreads * kSortThreads;
-    const int nb = nPad / kSortThreads;                                  // 64-position batches per wave
+    constexpr int nPad = NB * kSortThreads;
     uint32_t* cnt = reinterpret_cast<uint32_t*>(smem);                   // [kSortWaves][kCntPitch]
-    uint32_t* keys = cnt + kCntPitch * kSortWaves;                       // [nPad]
+    uint32_t* keys = cnt + kCntPitch * kSortWaves;                       // [nPad] rebased keys
     uint16_t* idx = reinterpret_cast<uint16_t*>(keys + nPad);            // [nPad]
     __shared__ uint32_t redMin[kSortWaves], redMax[kSortWaves], waveTot[kSortWaves];
 
-    const int f = blockIdx.x;
+    const int f = xcd_spread(blockIdx.x, count);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int base = wave * NB * kWave;           // this wave's positions
+    // per element: the rebased key, and (sample index << 16) | rank
+    uint32_t kr[NB], pk[NB];
     uint32_t kmin = 0xFFFFFFFFu, kmax = 0u;
     if (mode == 0) {
         const vj_feature ft = feats[fBegin + f];
-#pragma unroll
-        for (int j = 0; j < kSortMaxPerThread; ++j) {
-            const int s = tid + j * kSortThreads;
-            if (s < n) {
-                const uint32_t k = float_key(response(ft, sumT, inv, n, s));
-                keys[s] = k;
-                idx[s] = static_cast<uint16_t>(s);
-                kmin = min(kmin, k);
-                kmax = max(kmax, k);
-            }
-        }
+        if (ft.rectCount == 3) feature_keys<NB, 3>(ft, sumT, inv, n, base + lane, kr);
+        else feature_keys<NB, 2>(ft, sumT, inv, n, base + lane, kr);
     } else {
         const float* row = resp + static_cast<size_t>(f) * n;
 #pragma unroll
-        for (int j = 0; j < kSortMaxPerThread; ++j) {
-            const int s = tid + j * kSortThreads;
-            if (s < n) {
-                const uint32_t k = float_key(row[s]);
-                keys[s] = k;
-                idx[s] = static_cast<uint16_t>(s);
-                kmin = min(kmin, k);
-                kmax = max(kmax, k);
-            }
+        for (int i = 0; i < NB; ++i) kr[i] = float_key(row[min(base + i * kWave + lane, n - 1)]);
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        pk[i] = static_cast<uint32_t>(base + i * kWave + lane) << 16;
+        if ((pk[i] >> 16) < static_cast<uint32_t>(n)) {
+            kmin = min(kmin, kr[i]);
+            kmax = max(kmax, kr[i]);
         }
     }
 #pragma unroll
@@ -203,41 +264,42 @@ __global__ __launch_bounds__(kSortThreads) void presort_kernel(const vj_feature*
     __syncthreads();
     kmin = redMin[0];
     kmax = redMax[0];
+#pragma unroll
     for (int w = 1; w < kSortWaves; ++w) {
         kmin = min(kmin, redMin[w]);
         kmax = max(kmax, redMax[w]);
     }
-    for (int s = n + tid; s < nPad; s += kSortThreads) {
-        keys[s] = kmax;
-        idx[s] = 0xFFFFu;
+    const uint32_t span = kmax - kmin;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        keys[base + i * kWave + lane] = (pk[i] >> 16) < static_cast<uint32_t>(n) ? kr[i] - kmin : span;
+        idx[base + i * kWave + lane] = static_cast<uint16_t>(pk[i] >> 16);
     }
     __syncthreads();
 
-    const uint32_t span = kmax - kmin;
     const int bits = span ? 32 - __clz(static_cast<int>(span)) : 0;
-    const int base = wave * nb * kWave;
+    uint32_t* wcnt = cnt + wave * kCntPitch;
     for (int shift = 0; shift < bits; shift += 8) {
-        uint32_t kr[kSortMaxPerThread], rk[kSortMaxPerThread];
-        uint16_t ir[kSortMaxPerThread];
 #pragma unroll
-        for (int i = 0; i < kSortMaxPerThread; ++i) {
-            if (i < nb) {
-                kr[i] = keys[base + i * kWave + lane];
-                ir[i] = idx[base + i * kWave + lane];
-            }
+        for (int i = 0; i < NB; ++i) {
+            kr[i] = keys[base + i * kWave + lane];
+            pk[i] = static_cast<uint32_t>(idx[base + i * kWave + lane]) << 16;
         }
-        uint32_t* wcnt = cnt + wave * kCntPitch;
-        for (int d = lane; d < 256; d += kWave) wcnt[d] = 0u;
 #pragma unroll
-        for (int i = 0; i < kSortMaxPerThread; ++i) {
-            if (i < nb) {
-                const uint32_t dg = ((kr[i] - kmin) >> shift) & 0xFFu;
-                const uint64_t m = same_digit_lanes(dg);
-                const uint32_t below = static_cast<uint32_t>(mask_rank(m));
-                const uint32_t c = wcnt[dg];
-                rk[i] = c + below;
-                if (below == 0) wcnt[dg] = c + static_cast<uint32_t>(__popcll(m));
-            }
+        for (int d = lane; d < 256; d += kWave) wcnt[d] = 0u;
+        // rank: same-digit lanes below this one plus the wave's earlier
+        // elements of the digit; every lane of a digit writes the same new
+        // count
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const uint32_t dg = __builtin_amdgcn_ubfe(kr[i], static_cast<uint32_t>(shift), 8u);
+            const uint64_t m = same_digit_lanes(dg);
+            const uint32_t c = wcnt[dg];
+            pk[i] |= c + static_cast<uint32_t>(mask_rank(m));
+            wcnt[dg] = c + static_cast<uint32_t>(__popcll(m));
+            // the rank is formed here: sunk to the scatter, every element's
+            // mask and count would stay live across the scan
+            asm volatile("" : "+v"(pk[i]));
         }
         __syncthreads();
         // exclusive scan over (digit, wave) in that order: 4 consecutive
@@ -265,24 +327,40 @@ __global__ __launch_bounds__(kSortThreads) void presort_kernel(const vj_feature*
         }
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < kSortMaxPerThread; ++i) {
-            if (i < nb) {
-                const uint32_t dg = ((kr[i] - kmin) >> shift) & 0xFFu;
-                const uint32_t pos = wcnt[dg] + rk[i];
-                keys[pos] = kr[i];
-                idx[pos] = ir[i];
-            }
+        for (int i = 0; i < NB; ++i) {
+            const uint32_t dg = __builtin_amdgcn_ubfe(kr[i], static_cast<uint32_t>(shift), 8u);
+            const uint32_t pos = wcnt[dg] + (pk[i] & 0xFFFFu);
+            keys[pos] = kr[i];
+            idx[pos] = static_cast<uint16_t>(pk[i] >> 16);
         }
         __syncthreads();
     }
     uint16_t* o = out + static_cast<int64_t>(f) * outPitch;
-    for (int k = tid; k < n; k += kSortThreads) {
-        uint32_t e = idx[k];
-        if (mode == 0 && k + 1 < n) {
-            const float v0 = key_float(keys[k]), v1 = key_float(keys[k + 1]);
-            if (v0 + 1e-7f < v1) e |= 0x8000u;
+    if (mode != 0) {
+        for (int k = tid; k < n; k += kSortThreads) o[k] = idx[k];
+        return;
+    }
+    // four positions per lane: one 8-byte store of the entries (mode 0 rows
+    // are padded to 256 entries, so o + k is 8-byte aligned), the split flags
+    // from the keys of positions k .. k + 4
+    for (int k = tid * 4; k < n; k += kSortThreads * 4) {
+        const uint2 iv = *reinterpret_cast<const uint2*>(idx + k);
+        const uint4 kv = *reinterpret_cast<const uint4*>(keys + k);
+        const uint32_t k4 = k + 4 < n ? keys[k + 4] : 0u;
+        const float v0 = key_float(kv.x + kmin), v1 = key_float(kv.y + kmin), v2 = key_float(kv.z + kmin),
+                    v3 = key_float(kv.w + kmin), v4 = key_float(k4 + kmin);
+        uint32_t lo = iv.x, hi = iv.y;
+        if (k + 1 < n && v0 + 1e-7f < v1) lo |= 0x8000u;
+        if (k + 2 < n && v1 + 1e-7f < v2) lo |= 0x80000000u;
+        if (k + 3 < n && v2 + 1e-7f < v3) hi |= 0x8000u;
+        if (k + 4 < n && v3 + 1e-7f < v4) hi |= 0x80000000u;
+        if (k + 4 <= n) {
+            *reinterpret_cast<uint2*>(o + k) = make_uint2(lo, hi);
+        } else {
+            o[k] = static_cast<uint16_t>(lo);      // k + 3 >= n here
+            if (k + 1 < n) o[k + 1] = static_cast<uint16_t>(lo >> 16);
+            if (k + 2 < n) o[k + 2] = static_cast<uint16_t>(hi);
         }
-        o[k] = static_cast<uint16_t>(e);
     }
 }
 
@@ -1444,6 +1522,28 @@ size_t presort_lds(int n) {
     return static_cast<size_t>(kCntPitch) * kSortWaves * 4 + nPad * 4 + nPad * 2;
 }
 
+// presort_kernel<NB> for NB = ceil(n / 1024) positions per thread (1..20).
+template <int NB = 1>
+void launch_presort(int count, hipStream_t s, const vj_feature* feats, const uint32_t* sumT, const float* inv,
+                    const float* resp, int n, uint16_t* out, int64_t outPitch, int mode) {
+    if constexpr (NB <= kSortMaxPerThread) {
+        if ((n + kSortThreads - 1) / kSortThreads != NB)
+            return launch_presort<NB + 1>(count, s, feats, sumT, inv, resp, n, out, outPitch, mode);
+        hipLaunchKernelGGL(presort_kernel<NB>, dim3(count), dim3(kSortThreads), presort_lds(n), s, feats, 0, count,
+                           sumT, inv, resp, n, out, outPitch, mode);
+    }
+}
+
+template <int NB = 1>
+int presort_lds_limits() {
+    if constexpr (NB <= kSortMaxPerThread) {
+        VJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(presort_kernel<NB>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024));
+        return presort_lds_limits<NB + 1>();
+    }
+    return VJ_OK;
+}
+
 // Every corner of every feature must index the (winW+1) x (winH+1) window
 // integral; checked on the host before any kernel reads through them.
 int validate_features(const vj_feature* dFeatures, int begin, int count, int winW, int winH, hipStream_t s) {
@@ -1464,8 +1564,7 @@ int validate_features(const vj_feature* dFeatures, int begin, int count, int win
 int set_lds_limits() {
     static bool done = false;
     if (done) return VJ_OK;
-    VJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(presort_kernel),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024));
+    VJ_TRY(presort_lds_limits());
     for (const void* k : {reinterpret_cast<const void*>(search_kernel<uint32_t, 2>),
                           reinterpret_cast<const void*>(search_kernel<uint32_t, 4>),
                           reinterpret_cast<const void*>(search_kernel<uint32_t, 8>),
@@ -1552,8 +1651,8 @@ extern "C" int vj_trainer_sort_per_feature(vj_trainer* t, const float* dResp, in
     if (n > kSortMaxN)
         return launch_presort_wide(t, nullptr, nullptr, nullptr, dResp, n, count, dOut, static_cast<int64_t>(n),
                                    nullptr, 1, static_cast<hipStream_t>(sv));
-    hipLaunchKernelGGL(presort_kernel, dim3(count), dim3(kSortThreads), presort_lds(n), static_cast<hipStream_t>(sv),
-                       nullptr, 0, nullptr, nullptr, dResp, n, dOut, static_cast<int64_t>(n), 1);
+    launch_presort(count, static_cast<hipStream_t>(sv), nullptr, nullptr, nullptr, dResp, n, dOut,
+                   static_cast<int64_t>(n), 1);
     VJ_HIP(hipGetLastError());
     return VJ_OK;
 }
@@ -1638,8 +1737,7 @@ int presort_stage(vj_trainer* t, int count, hipStream_t s) {
         return launch_presort_wide(t, t->feats, t->sumT.get(), t->inv.get(), nullptr, n, count, t->ord.get(),
                                    t->pitch, t->mask.get(), 0, s);
     }
-    hipLaunchKernelGGL(presort_kernel, dim3(count), dim3(kSortThreads), presort_lds(n), s, t->feats, 0,
-                       t->sumT.get(), t->inv.get(), nullptr, n, t->ord.get(), t->pitch, 0);
+    launch_presort(count, s, t->feats, t->sumT.get(), t->inv.get(), nullptr, n, t->ord.get(), t->pitch, 0);
     VJ_HIP(hipGetLastError());
     return VJ_OK;
 }
