@@ -459,7 +459,10 @@ __global__ __launch_bounds__(256) void ii_vscan_kernel(const LevelDesc* __restri
 // Integral phase 3, one wavefront per (level, band, chunk): the integral at
 // the band's top edge comes from the vertical prefixes (scanned across the
 // chunk and the chunks to its left), each row adds its row prefix; writes
-// ii[y+1][x+1] as u32 sum and u64 square sum.
+// ii[y+1][x+1] as u32 sum and QT square sum: u64 for the integral views, u32
+// (the square sum modulo 2^32) for detect, whose window square sums are below
+// 2^32 and so exact as u32 differences.
+template <typename QT>
 __global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restrict__ L, int nItems,
                                                       const int32_t* __restrict__ itemOffs, int nLevels,
                                                       const uint32_t* __restrict__ colS,
@@ -468,7 +471,7 @@ __global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restric
                                                       const uint64_t* __restrict__ totQ,
                                                       const uint32_t* __restrict__ rowS,
                                                       const uint32_t* __restrict__ rowQ,
-                                                      uint32_t* __restrict__ iiS, uint64_t* __restrict__ iiQ) {
+                                                      uint32_t* __restrict__ iiS, QT* __restrict__ iiQ) {
     const int item = blockIdx.x;
     if (item >= nItems) return;
     const int li = find_segment(itemOffs, nLevels, item);
@@ -482,16 +485,16 @@ __global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restric
     const int y1 = min(lv.h, y0 + kBandRows);
     const int64_t pitch = lv.iiPitch;
     uint32_t* outS = iiS + lv.iiOff;
-    uint64_t* outQ = iiQ + lv.iiOff;
+    QT* outQ = iiQ + lv.iiOff;
 
     // carry from the chunks to the left: band totals, one chunk per lane (64
     // chunks at a time), and per-row totals, loaded four chunks at a time
     const int64_t tb = lv.totOff + static_cast<int64_t>(b) * lv.nChunks;
     uint32_t ccS = 0;
-    uint64_t ccQ = 0;
+    QT ccQ = 0;
     for (int k0 = 0; k0 < c; k0 += kWave) {
         ccS += wave_sum(k0 + lane < c ? totS[tb + k0 + lane] : 0u);
-        ccQ += wave_sum(k0 + lane < c ? totQ[tb + k0 + lane] : uint64_t{0});
+        ccQ += wave_sum(k0 + lane < c ? static_cast<QT>(totQ[tb + k0 + lane]) : QT{0});
     }
     uint32_t rcS = 0, rcQ = 0;
     if (lane < kBandRows && y0 + lane < lv.h) {
@@ -520,14 +523,14 @@ __global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restric
         vQ[k] = in ? colQ[lv.colOff + static_cast<int64_t>(b) * lv.w + x + k] : 0u;
     }
     const uint32_t tS = vS[0] + vS[1] + vS[2] + vS[3];
-    const uint64_t tQ = static_cast<uint64_t>(vQ[0]) + vQ[1] + vQ[2] + vQ[3];
+    const QT tQ = static_cast<QT>(vQ[0]) + vQ[1] + vQ[2] + vQ[3];
     const uint32_t eS = wave_incl_scan(tS) - tS;
-    const uint64_t eQ = wave_incl_scan(tQ) - tQ;
+    const QT eQ = wave_incl_scan(tQ) - tQ;
     uint32_t aS[4];
-    uint64_t aQ[4];
+    QT aQ[4];
     {
         uint32_t rs = ccS + eS;
-        uint64_t rq = ccQ + eQ;
+        QT rq = ccQ + eQ;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             rs += vS[k];
@@ -544,13 +547,17 @@ __global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restric
     // the last chunk also writes element w.
     const int w = lv.w;
     const bool tailEl = x + 4 == w && lane == kWave - 1;
-    auto put_row = [&](int64_t row, uint32_t s0, const uint32_t (&sv)[4], uint64_t q0, const uint64_t (&qv)[4]) {
+    auto put_row = [&](int64_t row, uint32_t s0, const uint32_t (&sv)[4], QT q0, const QT (&qv)[4]) {
         uint32_t* rS = outS + row * pitch;
-        uint64_t* rQ = outQ + row * pitch;
+        QT* rQ = outQ + row * pitch;
         if (x + 3 <= w) {
             *reinterpret_cast<u32x4_a4*>(rS + x) = u32x4_a4{s0, sv[0], sv[1], sv[2]};
-            *reinterpret_cast<u64x2_a8*>(rQ + x) = u64x2_a8{q0, qv[0]};
-            *reinterpret_cast<u64x2_a8*>(rQ + x + 2) = u64x2_a8{qv[1], qv[2]};
+            if constexpr (sizeof(QT) == 4) {
+                *reinterpret_cast<u32x4_a4*>(rQ + x) = u32x4_a4{q0, qv[0], qv[1], qv[2]};
+            } else {
+                *reinterpret_cast<u64x2_a8*>(rQ + x) = u64x2_a8{q0, qv[0]};
+                *reinterpret_cast<u64x2_a8*>(rQ + x + 2) = u64x2_a8{qv[1], qv[2]};
+            }
         } else if (x <= w) {
             rS[x] = s0;
             rQ[x] = q0;
@@ -568,12 +575,12 @@ __global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restric
     };
     if (b == 0) {  // row 0 of the integral is zero
         const uint32_t zs[4] = {0u, 0u, 0u, 0u};
-        const uint64_t zq[4] = {0u, 0u, 0u, 0u};
-        put_row(0, 0u, zs, 0u, zq);
+        const QT zq[4] = {0u, 0u, 0u, 0u};
+        put_row(0, 0u, zs, QT{0}, zq);
     }
     // the value left of the chunk (element c * 256) on the current row
     uint32_t leftS = ccS;
-    uint64_t leftQ = ccQ;
+    QT leftQ = ccQ;
     uint32_t pv[kBandRows];
     load_band(lv, x, y0, y1, pv);
 #pragma unroll
@@ -597,7 +604,9 @@ __global__ __launch_bounds__(64) void ii_write_kernel(const LevelDesc* __restric
         leftS += rowCarryS;
         leftQ += rowCarryQ;
         uint32_t prevS = static_cast<uint32_t>(__shfl_up(static_cast<int>(aS[3]), 1, kWave));
-        uint64_t prevQ = static_cast<uint64_t>(__shfl_up(static_cast<unsigned long long>(aQ[3]), 1, kWave));
+        QT prevQ;
+        if constexpr (sizeof(QT) == 4) prevQ = static_cast<QT>(__shfl_up(static_cast<int>(aQ[3]), 1, kWave));
+        else prevQ = static_cast<QT>(__shfl_up(static_cast<unsigned long long>(aQ[3]), 1, kWave));
         if (lane == 0) {
             prevS = leftS;
             prevQ = leftQ;
@@ -744,11 +753,11 @@ __device__ __forceinline__ void stage_window_box(const TileGeo& g, const uint32_
 // the live list; a tile has at most two windows per thread, both gated
 // before either is appended so their global loads overlap.
 __device__ __forceinline__ void gate_windows(const TileGeo& g, const uint32_t* __restrict__ iiS,
-                                             const uint64_t* __restrict__ iiQ, CascadeSmem& sm, int winW,
+                                             const uint32_t* __restrict__ iiQ, CascadeSmem& sm, int winW,
                                              int winH) {
     const int64_t iw = g.lv.iiPitch;
     const uint32_t* sumL = iiS + g.lv.iiOff;
-    const uint64_t* sqL = iiQ + g.lv.iiOff;
+    const uint32_t* sqL = iiQ + g.lv.iiOff;
     const int nw = winW - 2, nh = winH - 2, narea = nw * nh;
     const int64_t np0 = 1 + iw, np1 = (1 + nw) + iw;
     const int64_t np2 = 1 + static_cast<int64_t>(iw) * (1 + nh), np3 = (1 + nw) + static_cast<int64_t>(iw) * (1 + nh);
@@ -1012,7 +1021,7 @@ __device__ int run_stages(const TileGeo& g, CascadeSmem& sm, const float* lds, c
 template <bool kCount>
 __global__ __launch_bounds__(kCascadeBlock) void cascade_kernel(
     const LevelDesc* __restrict__ L, const int32_t* __restrict__ lvlIdx, const int32_t* __restrict__ tileOffs,
-    int nLv, int nTiles, const uint32_t* __restrict__ iiS, const uint64_t* __restrict__ iiQ,
+    int nLv, int nTiles, const uint32_t* __restrict__ iiS, const uint32_t* __restrict__ iiQ,
     const KStump* __restrict__ stumpsAll, int nStumps, const KStage* __restrict__ stages, int nStages,
     int stageSplit, int winW, int winH, uint64_t* __restrict__ words, float* __restrict__ scores,
     int32_t* __restrict__ survTotal, uint16_t* __restrict__ survIdx, float* __restrict__ survInv,
@@ -2184,7 +2193,8 @@ struct vj_engine {
     // detect work buffers
     DevBuf<uint8_t> pyr;
     DevBuf<uint32_t> iiS, colS, colQ, rowS, rowQ, totS;
-    DevBuf<uint64_t> iiQ, totQ;
+    DevBuf<uint32_t> iiQ;  // detect's square-sum integrals modulo 2^32
+    DevBuf<uint64_t> totQ;
     DevBuf<uint64_t> words;
     DevBuf<float> scores;
     DevBuf<LevelDesc> dLevels;
@@ -2490,9 +2500,9 @@ int build_plan(const vj_engine* e, const std::vector<ImageIn>& imgs, float sf, i
         P.maxStep = std::max(P.maxStep, d.step);
         P.windows += static_cast<int64_t>(d.scanW) * d.scanH;
         P.pyramidPixels += static_cast<int64_t>(d.w) * d.h;
-        P.integralBytes += static_cast<int64_t>(d.w) * d.h + 12ll * (d.w + 1) * (d.h + 1);
+        P.integralBytes += static_cast<int64_t>(d.w) * d.h + 8ll * (d.w + 1) * (d.h + 1);  // u32 sum + u32 square sum
         if (d.scaleIdx != 0) P.integralBytes += static_cast<int64_t>(d.w) * d.h;  // resize write
-        P.cascadeBytes += 12ll * (d.w + 1) * (d.h + 1);
+        P.cascadeBytes += 8ll * (d.w + 1) * (d.h + 1);
     }
     while (curImg < static_cast<int>(imgs.size())) {
         P.imgWordOff.push_back(wordOff);
@@ -2748,7 +2758,7 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s, bool tablesCurre
                                e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get(),
                                deep ? e->deepCtl.get() : nullptr, kDeepCtlInts);
             VJ_HIP(hipGetLastError());
-            hipLaunchKernelGGL(ii_write_kernel, dim3(P.nItems), dim3(64), 0, ls, dL, P.nItems, m32 + oItem, nLevels,
+            hipLaunchKernelGGL(ii_write_kernel<uint32_t>, dim3(P.nItems), dim3(64), 0, ls, dL, P.nItems, m32 + oItem, nLevels,
                                e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get(), e->rowS.get(),
                                e->rowQ.get(), e->iiS.get(), e->iiQ.get());
             VJ_HIP(hipGetLastError());
@@ -2994,7 +3004,7 @@ extern "C" int vj_engine_integral_batch(vj_engine* e, const uint8_t* dGray, int 
                            e->dMeta32.get() + 1, 1, e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get(),
                            nullptr, 0);
         VJ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(ii_write_kernel, dim3(items), dim3(64), 0, s, e->dLevels.get(), items, e->dMeta32.get(), 1,
+        hipLaunchKernelGGL(ii_write_kernel<uint64_t>, dim3(items), dim3(64), 0, s, e->dLevels.get(), items, e->dMeta32.get(), 1,
                            e->colS.get(), e->colQ.get(), e->totS.get(), e->totQ.get(), e->rowS.get(),
                            e->rowQ.get(), e->bSum.get(), e->bSq.get());
         VJ_HIP(hipGetLastError());
@@ -3126,7 +3136,7 @@ extern "C" int vj_integral_image_noalloc(const uint8_t* dGray, int w, int h, int
     hipLaunchKernelGGL(ii_vscan_kernel, dim3(d.nChunks), dim3(256), 0, s, L, offs + 1, 1, colS, colQ, totS, totQ,
                        nullptr, 0);
     VJ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(ii_write_kernel, dim3(items), dim3(64), 0, s, L, items, offs, 1, colS, colQ, totS, totQ, rowS,
+    hipLaunchKernelGGL(ii_write_kernel<uint64_t>, dim3(items), dim3(64), 0, s, L, items, offs, 1, colS, colQ, totS, totQ, rowS,
                        rowQ, dSum, dSq);
     VJ_HIP(hipGetLastError());
     return VJ_OK;
@@ -3594,7 +3604,7 @@ extern "C" int vj_engine_detect_footprint(vj_engine* e, int w, int h, float sf, 
         std::vector<ImageIn> imgs{{nullptr, w, h, w}};
         VJ_TRY(build_plan(e, imgs, sf, minObj, P));
         const int64_t cap = std::max(1, std::min(maxDet, e->maxDet));
-        *bytes = static_cast<int64_t>(w) * h + P.pyrBytes + 12 * P.iiElems + 8 * P.colElems + 12 * P.totElems +
+        *bytes = static_cast<int64_t>(w) * h + P.pyrBytes + 8 * P.iiElems + 8 * P.colElems + 12 * P.totElems +
                  8 * P.rowElems + 8 * P.nWords + 4 * 64 * P.nWords + 24 * cap + 6 * P.windows +
                  44 * static_cast<int64_t>(P.nTiles) +
                  static_cast<int64_t>(P.levels.size()) * static_cast<int64_t>(sizeof(LevelDesc) + 64);
