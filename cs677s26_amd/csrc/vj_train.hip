@@ -1095,14 +1095,33 @@ __global__ __launch_bounds__(1024) void normalize_kernel(double* __restrict__ w,
                                                          float* __restrict__ sw32) {
     __shared__ uint64_t sLo[16], sHi[16];
     __shared__ double sS;
+    // kNormU samples per thread per step, all loaded before any is used (one
+    // workgroup: without loads in flight the loop waits on each one)
+    constexpr int kNormU = 8;
+    const int bd = static_cast<int>(blockDim.x);
     uint64_t lo = 0, hi = 0;
-    for (int s = threadIdx.x; s < n; s += blockDim.x) {
-        if (!active[s]) w[s] = 0.0;
-        uint64_t a, b;
-        fixed90(w[s], a, b);
-        const uint64_t nl = lo + a;
-        hi += b + (nl < lo ? 1ull : 0ull);
-        lo = nl;
+    for (int s0 = threadIdx.x; s0 < n; s0 += bd * kNormU) {
+        double wv[kNormU];
+        uint8_t av[kNormU];
+#pragma unroll
+        for (int u = 0; u < kNormU; ++u) {
+            const int s = s0 + u * bd;
+            wv[u] = s < n ? w[s] : 0.0;
+            av[u] = s < n ? active[s] : 1;
+        }
+#pragma unroll
+        for (int u = 0; u < kNormU; ++u) {
+            const int s = s0 + u * bd;
+            if (!av[u]) {
+                w[s] = 0.0;
+                wv[u] = 0.0;
+            }
+            uint64_t a, b;
+            fixed90(wv[u], a, b);
+            const uint64_t nl = lo + a;
+            hi += b + (nl < lo ? 1ull : 0ull);
+            lo = nl;
+        }
     }
     // wave reduction of the 128-bit partial sums
 #pragma unroll
@@ -1128,12 +1147,28 @@ __global__ __launch_bounds__(1024) void normalize_kernel(double* __restrict__ w,
     }
     __syncthreads();
     const double S = sS;
-    for (int s = threadIdx.x; s < n; s += blockDim.x) {
-        double x = w[s];
-        if (S > 1e-30) x = x / S;
-        w[s] = x;
-        const float f = static_cast<float>(x);
-        sw32[s] = active[s] ? (label[s] > 0 ? f : -f) : 0.0f;
+    for (int s0 = threadIdx.x; s0 < n; s0 += bd * kNormU) {
+        double wv[kNormU];
+        uint8_t av[kNormU];
+        int8_t lv[kNormU];
+#pragma unroll
+        for (int u = 0; u < kNormU; ++u) {
+            const int s = s0 + u * bd;
+            wv[u] = s < n ? w[s] : 0.0;
+            av[u] = s < n ? active[s] : 0;
+            lv[u] = s < n ? label[s] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kNormU; ++u) {
+            const int s = s0 + u * bd;
+            if (s < n) {
+                double x = wv[u];
+                if (S > 1e-30) x = x / S;
+                w[s] = x;
+                const float f = static_cast<float>(x);
+                sw32[s] = av[u] ? (lv[u] > 0 ? f : -f) : 0.0f;
+            }
+        }
     }
 }
 
@@ -1158,44 +1193,90 @@ __global__ __launch_bounds__(1024) void calibrate_kernel(const float* __restrict
     __shared__ uint32_t hist[256];
     __shared__ uint32_t sPrefix, sMask, sK;
     __shared__ uint32_t red[4][16];
+    __shared__ uint32_t binTot[4];
     if (threadIdx.x == 0) {
         sPrefix = 0;
         sMask = 0;
         sK = static_cast<uint32_t>(cutIdx);
     }
+    constexpr int kCalU = 8;  // scores per thread per step, all loaded before any is used
+    const int bd = static_cast<int>(blockDim.x);
+    const int tid = static_cast<int>(threadIdx.x), lane = tid & 63;
     for (int pass = 3; pass >= 0; --pass) {
         const int shift = pass * 8;
-        for (int b = threadIdx.x; b < 256; b += blockDim.x) hist[b] = 0;
+        for (int b = tid; b < 256; b += bd) hist[b] = 0;
         __syncthreads();
         const uint32_t prefix = sPrefix, mask = sMask;
-        for (int i = threadIdx.x; i < numPos; i += blockDim.x) {
-            const uint32_t k = float_key(strong[i]);
-            if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+        for (int i0 = tid; i0 < numPos; i0 += bd * kCalU) {
+            uint32_t kv[kCalU];
+#pragma unroll
+            for (int u = 0; u < kCalU; ++u) {
+                const int i = i0 + u * bd;
+                kv[u] = i < numPos ? float_key(strong[i]) : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < kCalU; ++u) {
+                const int i = i0 + u * bd;
+                const bool in = i < numPos && (kv[u] & mask) == prefix;
+                const uint32_t dg = (kv[u] >> shift) & 255u;
+                // the lanes sharing the first participating lane's digit add
+                // with one atomic (the leading digits of the scores mostly
+                // agree); the others add one by one
+                const uint64_t part = __ballot(in);
+                if (!part) continue;
+                const int lead = __ffsll(static_cast<long long>(part)) - 1;
+                const uint32_t d0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(dg), lead));
+                const uint64_t same = __ballot(in && dg == d0);
+                if (lane == lead) atomicAdd(&hist[d0], static_cast<uint32_t>(__popcll(same)));
+                if (in && dg != d0) atomicAdd(&hist[dg], 1u);
+            }
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t kk = sK, d = 0;
-            for (; d < 255; ++d) {
-                if (kk < hist[d]) break;
-                kk -= hist[d];
+        // the digit holding the k-th remaining score: an exclusive scan over
+        // the 256 bins (one per thread of the first four waves)
+        uint32_t h = 0, inc = 0;
+        if (tid < 256) {  // whole waves
+            h = hist[tid];
+            inc = wave_incl_scan_dpp(h);
+            if (lane == 63) binTot[tid >> 6] = inc;
+        }
+        const uint32_t kk = sK;
+        __syncthreads();
+        if (tid < 256) {
+            uint32_t before = inc - h;
+            for (int wv = 0; wv < (tid >> 6); ++wv) before += binTot[wv];
+            // the bin with before <= kk < before + h; bin 255 takes the rest
+            // (the intervals of the bins are disjoint, so one thread writes)
+            if (kk >= before && (kk - before < h || tid == 255)) {
+                sK = kk - before;
+                sPrefix = prefix | (static_cast<uint32_t>(tid) << shift);
+                sMask = mask | (255u << shift);
             }
-            sK = kk;
-            sPrefix = prefix | (d << shift);
-            sMask = mask | (255u << shift);
         }
         __syncthreads();
     }
     const float cut = key_float(sPrefix);
     uint32_t c[4] = {0, 0, 0, 0};
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const float s = strong[i];
-        const uint32_t ge = s >= cut, gt = s > cut;
-        if (i < numPos) {
-            c[0] += ge;
-            c[1] += gt;
-        } else if (active[i]) {
-            c[2] += ge;
-            c[3] += gt;
+    for (int i0 = tid; i0 < n; i0 += bd * kCalU) {
+        float sv[kCalU];
+        uint8_t av[kCalU];
+#pragma unroll
+        for (int u = 0; u < kCalU; ++u) {
+            const int i = i0 + u * bd;
+            sv[u] = i < n ? strong[i] : 0.0f;
+            av[u] = (i < n && i >= numPos) ? active[i] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kCalU; ++u) {
+            const int i = i0 + u * bd;
+            const uint32_t ge = sv[u] >= cut, gt = sv[u] > cut;
+            if (i < numPos) {
+                c[0] += ge;
+                c[1] += gt;
+            } else if (i < n && av[u]) {
+                c[2] += ge;
+                c[3] += gt;
+            }
         }
     }
 #pragma unroll
