@@ -112,6 +112,7 @@ def batched_leg(args):
     eng.detect_batch(dev, spec, 1.25, 4, 24, 200000, True)
     eng.detect_batch(dev, spec, 1.25, 4, 24, 200000, True)
     tile_rows = eng.tile_rows()
+    split = eng.cascade_split()
     synchronize()
     reps = max(1, min(args.steps, 3))
     t0 = time.perf_counter()
@@ -141,6 +142,7 @@ def batched_leg(args):
     mpix = sum(w * h for w, h in sizes) / 1e6
     return {"mpix_per_s": mpix * reps / wall, "mpix_per_batch": mpix, "ms_per_batch": 1e3 * wall / reps,
             "plain_per_image_mpix_per_s": mpix / wall_plain, "tile_rows": tile_rows,
+            "tail_pass": split[1] < split[2],
             "speedup_vs_plain": (mpix * reps / wall) / (mpix / wall_plain), "integral_roofline": int_roof}
 
 

@@ -121,6 +121,7 @@ _PROTOS = {
     "vj_engine_set_graphs": (I, [P, I]),
     "vj_engine_set_autotune": (I, [P, I]),
     "vj_engine_tile_rows": (I, [P, C.POINTER(I)]),
+    "vj_engine_cascade_split": (I, [P, C.POINTER(I)]),
     "vj_server_create": (I, [C.POINTER(ServerConfig), P, I, P, I, P, I, PP]),
     "vj_server_submit": (I, [P, P, I, I, I, C.POINTER(I64)]),
     "vj_server_wait": (I, [P, I64, P, I, C.POINTER(I), C.POINTER(D)]),

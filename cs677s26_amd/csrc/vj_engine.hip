@@ -2206,6 +2206,7 @@ struct vj_engine {
     // two-pass cascade: first-pass survivors and the deep pass's work list
     int stageSplit = 0;                        // stages run by the first pass
     int stageTail = 0;                         // first stage of the tail pass (nStages: none)
+    int stageTailDefault = 0;                  // stageTail from the split rule (autotune off)
     DevBuf<uint16_t> survIdx;
     DevBuf<float> survInv;
     DevBuf<int32_t> tileSurvOff, tileCount, deepItems, deepCtl;  // deepCtl: total, items, counter, list scratch
@@ -2244,7 +2245,7 @@ struct vj_engine {
     // by the autotuner when it is on
     int tileRows[4] = {0, tile_rows_for_step(1), tile_rows_for_step(2), tile_rows_for_step(3)};
     bool autotune = false;
-    std::map<std::vector<int64_t>, std::array<int, 4>> tuned;
+    std::map<std::vector<int64_t>, std::array<int, 5>> tuned;  // tile rows 1..3, tail start
 
     // the detect launch sequence captured as a HIP graph, replayed while the
     // plan shape, the tile rows and every buffer address stay the same.  A
@@ -3239,7 +3240,7 @@ extern "C" int vj_engine_set_cascade(vj_engine* e, const vj_feature* dFeatures, 
         // (at least one stage); the tail pass runs the rest
         cum = 0;
         while (k < nSt && cum < kDeepStumps) cum += hStages[k++].ntrees;
-        e->stageTail = k;
+        e->stageTail = e->stageTailDefault = k;
     }
     return VJ_OK;
 }
@@ -3261,10 +3262,13 @@ constexpr std::array<int, 4> kTileCandidates[] = {
     {0, 8, 8, 8}, {0, 8, 4, 4}, {0, 4, 4, 4}, {0, 8, 8, 4}, {0, 4, 8, 8}};
 
 // Autotuner: the first call of a batch shape (image sizes, scale factor,
-// minimum object size, cascade size) times the cascade kernel for each
-// candidate tile shape (best of two runs, HIP events on the call's stream)
-// and keeps the fastest for that shape; later calls reuse it.  Every shape
-// gives the same detections; only the speed differs.
+// minimum object size, cascade size) times the cascade kernels for each
+// candidate tile shape, with and without the tail pass (the deep pass then
+// runs every remaining stage: slower on one frame, whose few late windows
+// the tail spreads over more lanes, faster on large batches), best of two
+// runs, HIP events on the call's stream, and keeps the fastest for that
+// shape; later calls reuse it.  Every choice gives the same detections;
+// only the speed differs.
 int autotune(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int minObj, int capPer, hipStream_t s) {
     std::vector<int64_t> key;
     uint32_t sfBits;
@@ -3281,10 +3285,16 @@ int autotune(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int minOb
         const bool timing = e->timing;
         e->timing = true;
         float best = 0.0f;
-        std::array<int, 4> bestRows = kTileCandidates[0];
+        std::array<int, 5> bestRows{};
         int status = VJ_OK;
+        std::vector<std::array<int, 5>> cands;
         for (const std::array<int, 4>& c : kTileCandidates) {
-            std::copy(c.begin(), c.end(), e->tileRows);
+            cands.push_back({c[0], c[1], c[2], c[3], e->stageTailDefault});
+            if (e->stageTailDefault < e->nStages) cands.push_back({c[0], c[1], c[2], c[3], e->nStages});
+        }
+        for (const std::array<int, 5>& c : cands) {
+            std::copy(c.begin(), c.begin() + 4, e->tileRows);
+            e->stageTail = c[4];
             float t = -1.0f;
             for (int rep = 0; rep < 2 && status == VJ_OK; ++rep) {
                 Plan P;
@@ -3309,7 +3319,8 @@ int autotune(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int minOb
         if (status != VJ_OK) return status;
         it = e->tuned.emplace(key, bestRows).first;
     }
-    std::copy(it->second.begin(), it->second.end(), e->tileRows);
+    std::copy(it->second.begin(), it->second.begin() + 4, e->tileRows);
+    e->stageTail = it->second[4];
     return VJ_OK;
 }
 
@@ -3624,8 +3635,18 @@ extern "C" int vj_engine_set_graphs(vj_engine* e, int on) {
 extern "C" int vj_engine_set_autotune(vj_engine* e, int on) {
     VJ_CHECK(e, "vj_engine_set_autotune: null");
     e->autotune = on != 0;
-    if (!e->autotune)
+    if (!e->autotune) {
         for (int st = 1; st <= 3; ++st) e->tileRows[st] = tile_rows_for_step(st);
+        e->stageTail = e->stageTailDefault;
+    }
+    return VJ_OK;
+}
+
+extern "C" int vj_engine_cascade_split(vj_engine* e, int* split3) {
+    VJ_CHECK(e && split3, "vj_engine_cascade_split: null");
+    split3[0] = e->stageSplit;
+    split3[1] = e->stageTail;
+    split3[2] = e->nStages;
     return VJ_OK;
 }
 

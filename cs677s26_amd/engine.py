@@ -160,7 +160,8 @@ class FaceVisionEngine:
         check(lib().vj_engine_set_graphs(self.handle, int(bool(on))), "set_graphs")
 
     def set_autotune(self, on=True):
-        """Pick the cascade tile shape per batch shape by timing candidates."""
+        """Pick the cascade tile shape and the tail-pass split per batch shape by
+        timing candidates; off restores the defaults."""
         check(lib().vj_engine_set_autotune(self.handle, int(bool(on))), "set_autotune")
 
     def tile_rows(self):
@@ -168,6 +169,13 @@ class FaceVisionEngine:
         a = (C.c_int * 3)()
         check(lib().vj_engine_tile_rows(self.handle, a), "tile_rows")
         return list(a)
+
+    def cascade_split(self):
+        """(first deep-pass stage, first tail-pass stage, stages) in use; the tail
+        pass is off when the second equals the third."""
+        a = (C.c_int * 3)()
+        check(lib().vj_engine_cascade_split(self.handle, a), "cascade_split")
+        return tuple(a)
 
     def phase_ms(self):
         """Device ms of the last detect: pyramid, integrals, cascade, compaction."""

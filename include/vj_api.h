@@ -246,12 +246,17 @@ int vj_engine_detect_batch(vj_engine* e, int nImages, const uint8_t* const* dIma
  * shape and the engine's buffers are unchanged; a shape seen once is launched
  * directly.  Autotune (default off): the first call of each batch shape
  * (image sizes, scale factor, minimum object size, cascade) times the
- * cascade kernel over a few candidate tile shapes and keeps the fastest for
- * that shape.  Neither changes any detection.  vj_engine_tile_rows reports
- * the cascade tile rows in use for window steps 1, 2 and 3. */
+ * cascade kernels over a few candidate tile shapes, each with and without
+ * the tail pass (without it the deep pass runs every remaining stage), and
+ * keeps the fastest for that shape; turning it off restores the defaults.
+ * Neither changes any detection.  vj_engine_tile_rows reports the cascade
+ * tile rows in use for window steps 1, 2 and 3; vj_engine_cascade_split
+ * reports {first deep-pass stage, first tail-pass stage, stages} in use (the
+ * tail pass is off when the second equals the third). */
 int vj_engine_set_graphs(vj_engine* e, int on);
 int vj_engine_set_autotune(vj_engine* e, int on);
 int vj_engine_tile_rows(vj_engine* e, int* rows3);
+int vj_engine_cascade_split(vj_engine* e, int* split3);
 
 /* Device bytes one detect of a w x h image occupies in an engine's work
  * buffers (input, pyramid, integrals and their scratch, window mask and

@@ -408,4 +408,6 @@ def test_batch64_workload_equals_plain_per_image():
         plain.set_cascade_host(feats, stumps, stages)
         for i, (d, (w, h)) in enumerate(zip(dev, sizes)):
             assert det_tuples(plain.detectMultiScale(d, w, h, w, 1.25, 4, 24, 200000, grouping)) == first[i], (grouping, i)
-    say(f"  raw boxes {sum(len(x) for x in first)}")
+    say(f"  raw boxes {sum(len(x) for x in first)}, tuned tile rows {eng.tile_rows()}, split {eng.cascade_split()}")
+    s0, s1, ns = eng.cascade_split()
+    assert 0 < s0 <= s1 <= ns == len(stages)
