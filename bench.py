@@ -513,7 +513,7 @@ def detect_fields(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rounds", type=int, default=50)
     ap.add_argument("--requests", type=int, default=240, help="served leg: requests in the Poisson stream")

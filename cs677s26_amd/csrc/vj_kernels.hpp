@@ -54,8 +54,8 @@ struct LevelDesc {
 };
 
 // Integral row pitch of the detect pyramid, in elements: whole 128-byte lines
-// of u32 sums (and 256 bytes of u64 square sums), so the integral kernels'
-// 16-byte stores are aligned.
+// of u32 sums and of u32 square sums, so the integral kernels' 16-byte stores
+// are aligned.
 constexpr int kIIPitchAlign = 32;
 
 // Cascade stump with its corner offsets decoded for one LDS tile pitch.
