@@ -2231,6 +2231,7 @@ struct vj_engine {
     PinnedBuf<vj_detection> hOutG;
     // sweep order: per image its width (its x bins), the boxes listed by x
     DevBuf<int32_t> gImgW, gBucket;
+    std::vector<int32_t> imgWUp;  // the image widths gImgW holds
     PinnedBuf<int32_t> hImgW;
     DevBuf<int4> gSorted;
     DevBuf<int64_t> gCtl;
@@ -3037,6 +3038,37 @@ extern "C" int vj_engine_integral_view(vj_engine* e, vj_integral_view* out) {
     return VJ_OK;
 }
 
+// The call's small results in one launch instead of four copies: the
+// grouping's control words, each image's cluster counts and flags, the raw
+// counts and the first grouped boxes (as many as the images' kept clusters,
+// at most nOut), stored straight into the caller-visible pinned buffers
+// (device addresses of host memory; visible to the host once the stream is
+// synchronised).
+__global__ __launch_bounds__(256) void publish_kernel(const int64_t* __restrict__ ctl,
+                                                      const int32_t* __restrict__ grpImg,
+                                                      const int32_t* __restrict__ counts,
+                                                      const vj_detection* __restrict__ outG, int nImages, int nOut,
+                                                      int64_t* __restrict__ hCtl, int32_t* __restrict__ hGrpImg,
+                                                      int32_t* __restrict__ hCounts, vj_detection* __restrict__ hOutG) {
+    __shared__ int32_t part[4];
+    const int tid = static_cast<int>(threadIdx.x);
+    if (tid < 2) hCtl[tid] = ctl[tid];
+    int32_t kept = 0;
+    for (int i = tid; i < nImages; i += 256) {
+        hGrpImg[3 * i] = grpImg[3 * i];
+        hGrpImg[3 * i + 1] = grpImg[3 * i + 1];
+        hGrpImg[3 * i + 2] = grpImg[3 * i + 2];
+        hCounts[i] = counts[i];
+        kept += max(grpImg[3 * i], 0);
+    }
+    kept = wave_sum(kept);
+    if ((tid & (kWave - 1)) == 0) part[tid / kWave] = kept;
+    __syncthreads();
+    const int64_t total = static_cast<int64_t>(part[0]) + part[1] + part[2] + part[3];
+    const int m = static_cast<int>(total < nOut ? total : static_cast<int64_t>(nOut));
+    for (int k = tid; k < m; k += 256) hOutG[k] = outG[k];
+}
+
 // ---- one image's integral without allocating ------------------------------
 // Scratch of vj_integral_image_noalloc: the level descriptor and item table
 // of the integral kernels, then the column band sums, row-chunk totals and
@@ -3382,8 +3414,15 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
             e->hImgW.get()[i] = imgs[static_cast<size_t>(i)].w;
             maxW = std::max(maxW, imgs[static_cast<size_t>(i)].w);
         }
+        // the widths go up only when they differ from those already there (a
+        // buffer that grows is a new allocation and gets them again)
+        const bool fresh = !e->gImgW.get() || static_cast<size_t>(nImages) > e->gImgW.capacity();
         VJ_TRY(e->gImgW.ensure(static_cast<size_t>(nImages)));
-        VJ_HIP(hipMemcpyAsync(e->gImgW.get(), e->hImgW.get(), sizeof(int32_t) * nImages, hipMemcpyHostToDevice, s));
+        const std::vector<int32_t> widths(e->hImgW.get(), e->hImgW.get() + nImages);
+        if (fresh || widths != e->imgWUp) {
+            VJ_HIP(hipMemcpyAsync(e->gImgW.get(), e->hImgW.get(), sizeof(int32_t) * nImages, hipMemcpyHostToDevice, s));
+            e->imgWUp = widths;
+        }
         VJ_TRY(e->gBucket.ensure(capTotal));
         VJ_TRY(e->gPairCnt.ensure(capTotal));
         VJ_TRY(e->gPairOff.ensure(capTotal));
@@ -3436,18 +3475,29 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
         VJ_HIP(hipGetLastError());
         VJ_TRY(e->hCtl.ensure(2));
         VJ_TRY(e->hGrpImg.ensure(3 * static_cast<size_t>(nImages)));
-        VJ_HIP(hipMemcpyAsync(e->hCtl.get(), e->gCtl.get(), 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        VJ_HIP(hipMemcpyAsync(e->hGrpImg.get(), e->gGrpImg.get(), 3 * sizeof(int32_t) * nImages,
-                              hipMemcpyDeviceToHost, s));
         // the first grouped boxes come back with the counts, so a call with
         // few of them needs no second round trip
         outPrefetch = std::min<size_t>(capTotal, kOutPrefetch);
         VJ_TRY(e->hOutG.ensure(outPrefetch));
-        VJ_HIP(hipMemcpyAsync(e->hOutG.get(), e->gOutG.get(), sizeof(vj_detection) * outPrefetch,
-                              hipMemcpyDeviceToHost, s));
     }
     VJ_TRY(e->hCounts.ensure(static_cast<size_t>(nImages)));
-    VJ_HIP(hipMemcpyAsync(e->hCounts.get(), e->dCounts.get(), sizeof(int32_t) * nImages, hipMemcpyDeviceToHost, s));
+    if (devGroup) {
+        // one launch stores the control words, per-image counts and the first
+        // grouped boxes straight into the pinned buffers
+        void *dCtl, *dGrpImg, *dCounts, *dOutG;
+        VJ_HIP(hipHostGetDevicePointer(&dCtl, e->hCtl.get(), 0));
+        VJ_HIP(hipHostGetDevicePointer(&dGrpImg, e->hGrpImg.get(), 0));
+        VJ_HIP(hipHostGetDevicePointer(&dCounts, e->hCounts.get(), 0));
+        VJ_HIP(hipHostGetDevicePointer(&dOutG, e->hOutG.get(), 0));
+        hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(256), 0, s, e->gCtl.get(), e->gGrpImg.get(),
+                           e->dCounts.get(), e->gOutG.get(), nImages, static_cast<int>(outPrefetch),
+                           static_cast<int64_t*>(dCtl), static_cast<int32_t*>(dGrpImg),
+                           static_cast<int32_t*>(dCounts), static_cast<vj_detection*>(dOutG));
+        VJ_HIP(hipGetLastError());
+    } else {
+        VJ_HIP(hipMemcpyAsync(e->hCounts.get(), e->dCounts.get(), sizeof(int32_t) * nImages, hipMemcpyDeviceToHost,
+                              s));
+    }
     VJ_HIP(hipStreamSynchronize(s));
     if (e->timing)
         for (int k = 0; k < 4; ++k) VJ_HIP(hipEventElapsedTime(&e->phaseMs[k], e->ev[k], e->ev[k + 1]));
