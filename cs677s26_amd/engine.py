@@ -38,6 +38,14 @@ class FaceVisionEngine:
         self.handle = h.value
         self.win = (int(win[0]), int(win[1]))
         self.maxDetections = int(maxDetections)
+        self._out = np.empty(0, DETECTION)  # detection buffer reused across calls
+
+    def _out_buffer(self, cap):
+        # the library writes the first n entries and only those are read, so
+        # the buffer is neither zeroed nor reallocated while it is big enough
+        if len(self._out) < cap:
+            self._out = np.empty(cap, DETECTION)
+        return self._out
 
     def close(self):
         if getattr(self, "handle", None):
@@ -122,7 +130,7 @@ class FaceVisionEngine:
     def detectMultiScale(self, dImage, imageW, imageH, imageStride, scaleFactor=1.25, minNeighbors=4,
                          minObjectSize=24, maxDetections=200000, applyGrouping=True, stream=None):
         cap = max(1, min(int(maxDetections), self.maxDetections))
-        out = np.zeros(cap, DETECTION)
+        out = self._out_buffer(cap)
         n = C.c_int()
         check(lib().vj_engine_detect(self.handle, _ptr(dImage), int(imageW), int(imageH), int(imageStride),
                                      float(scaleFactor), int(minNeighbors), int(minObjectSize), int(maxDetections),
@@ -141,7 +149,7 @@ class FaceVisionEngine:
         counts = (C.c_int * k)()
         if cap is None:
             cap = k * max(1, min(int(maxDetections), self.maxDetections))
-        out = np.zeros(max(1, cap), DETECTION)
+        out = self._out_buffer(max(1, cap))
         check(lib().vj_engine_detect_batch(self.handle, k, ptrs, ws, hs, strides, float(scaleFactor),
                                            int(minNeighbors), int(minObjectSize), int(maxDetections),
                                            int(bool(applyGrouping)), out.ctypes.data, cap, counts, _s(stream)),
