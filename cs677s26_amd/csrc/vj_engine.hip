@@ -1577,21 +1577,25 @@ __host__ __device__ inline int sweep_shift(int w) {
     return sh;
 }
 
-inline size_t sweep_lds_bytes(int maxW) {
+// An image with at most kSweepLdsBoxes boxes keeps its buckets in LDS
+// (after the bins): the tie counts then read LDS, not global memory.
+constexpr int kSweepLdsBoxes = 16384;
+
+inline size_t sweep_lds_bytes(int maxW, int ldsBoxes) {
     const size_t nb = static_cast<size_t>(maxW >> sweep_shift(maxW)) + 1;
-    return sizeof(int32_t) * (2 * nb + 1);
+    return sizeof(int32_t) * (2 * nb + 1 + static_cast<size_t>(ldsBoxes));
 }
 
 __global__ __launch_bounds__(kSweepThreads) void pair_sweep_kernel(const vj_detection* __restrict__ dets,
                                                                   const int32_t* __restrict__ counts,
                                                                   const int32_t* __restrict__ imgBase,
                                                                   const int32_t* __restrict__ imgW, int cap,
-                                                                  int32_t* __restrict__ bucket,
+                                                                  int ldsBoxes, int32_t* __restrict__ bucket,
                                                                   int4* __restrict__ sorted,
                                                                   int32_t* __restrict__ order,
                                                                   int32_t* __restrict__ grpImg,
                                                                   int64_t* __restrict__ ctl) {
-    extern __shared__ int32_t xs[];  // cnt[nb] (then the fill cursor), first[nb + 1]
+    extern __shared__ int32_t xs[];  // cnt[nb] (then the fill cursor), first[nb + 1], LDS buckets
     __shared__ int32_t waveTot[kSweepThreads / kWave];
     const int i = blockIdx.x;
     if (threadIdx.x == 0) {
@@ -1631,12 +1635,12 @@ __global__ __launch_bounds__(kSweepThreads) void pair_sweep_kernel(const vj_dete
     __syncthreads();
     for (int k = threadIdx.x; k < nb; k += kSweepThreads) cnt[k] = 0;
     __syncthreads();
-    int32_t* B = bucket + imgBase[i];
+    int32_t* B = n <= ldsBoxes ? xs + 2 * nb + 1 : bucket + imgBase[i];
     for (int g = threadIdx.x; g < n; g += kSweepThreads) {
         const int x = bin(D[g].x);
         B[first[x] + atomicAdd(&cnt[x], 1)] = g;
     }
-    __syncthreads();  // the bucket (global, this workgroup's own) is complete
+    __syncthreads();  // the bucket (LDS, or global and this workgroup's own) is complete
     for (int g = threadIdx.x; g < n; g += kSweepThreads) {
         const vj_detection d = D[g];
         const int x = bin(d.x);
@@ -1870,15 +1874,16 @@ __global__ __launch_bounds__(256) void grp_list_kernel(const int32_t* __restrict
 // in order is exactly this order (entries of other components are masked
 // out).  It is read 64 entries at a time through an LDS window of
 // kReplayEdges entries (each entry with its owner position), reloaded by all
-// lanes together.  Per chunk the lanes find the roots and ranks of both ends
-// of their entries at once; the unions the chunk needs are then applied one
-// by one in order from registers (every lane whose root lost takes the
-// winner and its new rank), and each union writes only the loser's parent
-// and the winner's rank to LDS for the chunks after.  The replay stops once
+// lanes together.  Per step of kReplayU chunks the lanes find the roots and
+// ranks of both ends of their entries at once; the unions the chunks need
+// are then applied one by one in order from registers (every slot whose
+// root lost takes the winner and its new rank), and each union writes only
+// the loser's parent and the winner's rank to LDS for the steps after.  The replay stops once
 // the members form one set.  The root's raw index orders the clusters.  A
 // span of more than kReplaySpan positions leaves its image to the host.
 constexpr int kReplaySpan = 4096;
 constexpr int kReplayEdges = 8192;
+constexpr int kReplayU = 4;  // chunks of 64 pair entries per replay step
 struct ReplayLds {
     int32_t off[kReplaySpan + 1];                 // pair-list offsets, relative to the span's first
     unsigned long long member[kReplaySpan / 64];  // member bits
@@ -1990,15 +1995,17 @@ __global__ __launch_bounds__(64) void grp_replay_kernel(
             __syncthreads();  // the staged span is complete; the old window is no longer read
             winLo = from;
             winHi = min(E, from + kReplayEdges);
-            for (int64_t t0 = from; t0 < winHi; t0 += 8 * kWave) {
-                int32_t pv[8];
+            // kWinLoads loads in flight per lane before the first is used
+            constexpr int kWinLoads = 32;
+            for (int64_t t0 = from; t0 < winHi; t0 += kWinLoads * kWave) {
+                int32_t pv[kWinLoads];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < kWinLoads; ++u) {
                     const int64_t t = t0 + u * kWave + lane;
                     pv[u] = t < winHi ? pairs[e0 + t] : 0;
                 }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < kWinLoads; ++u) {
                     const int64_t t = t0 + u * kWave + lane;
                     const int32_t q = base + pv[u] - r;
                     if (t < winHi) S.edge[t - from] = static_cast<int16_t>(q >= 0 && q < span ? q : -1);
@@ -2018,42 +2025,93 @@ __global__ __launch_bounds__(64) void grp_replay_kernel(
             __syncthreads();
         };
         load_window(0);
-        // once the members form one set no later pair changes a root
+        // once the members form one set no later pair changes a root.  Each
+        // step takes kReplayU chunks of 64 entries (entry t0 + 64 u + lane in
+        // slot u): the roots of all their ends are found together, walking
+        // the parent links in lockstep (every link of a round is read before
+        // any is used), then the chunks' unions are applied in entry order,
+        // each one relabelling the roots held by the slots not yet done.
         int sets = n;
-        for (int64_t t0 = 0; t0 < E && sets > 1; t0 += kWave) {
-            if (t0 + kWave > winHi && winHi < E) load_window(t0);
-            const int64_t t = t0 + lane;
-            int src = 0, dst = -1;
-            if (t < winHi) {
-                src = S.owner[t - winLo];
-                dst = S.edge[t - winLo];
-            }
-            // (a member's partners are members, so inside the span)
-            const bool valid = dst >= 0 && ((S.member[src >> 6] >> (src & 63)) & 1ull);
-            int rs = 0, rq = 0, ks = 0, kq = 0;
-            if (valid) {
-                rs = find(src);
-                rq = find(dst);
-                ks = S.rnk[rs];
-                kq = S.rnk[rq];
-            }
-            uint64_t nm = __ballot(valid && rs != rq);
-            while (nm) {
-                const int f = __ffsll(static_cast<long long>(nm)) - 1;  // uniform: read the lane directly
-                const int a = __builtin_amdgcn_readlane(rs, f), b = __builtin_amdgcn_readlane(rq, f);
-                const int ra = __builtin_amdgcn_readlane(ks, f), rb = __builtin_amdgcn_readlane(kq, f);
-                const int win = ra < rb ? b : a, lose = ra < rb ? a : b;
-                const int rw = ra == rb ? ra + 1 : max(ra, rb);
-                if (lane == 0) {
-                    S.par[lose] = static_cast<int16_t>(win);
-                    S.rnk[win] = static_cast<uint8_t>(rw);
+        for (int64_t t0 = 0; t0 < E && sets > 1; t0 += kReplayU * kWave) {
+            if (t0 + kReplayU * kWave > winHi && winHi < E) load_window(t0);
+            bool valid[kReplayU];
+            int x[2 * kReplayU];
+#pragma unroll
+            for (int u = 0; u < kReplayU; ++u) {
+                const int64_t t = t0 + u * kWave + lane;
+                int src = 0, dst = -1;
+                if (t < winHi) {
+                    src = S.owner[t - winLo];
+                    dst = S.edge[t - winLo];
                 }
-                rs = rs == lose ? win : rs;
-                rq = rq == lose ? win : rq;
-                ks = rs == win ? rw : ks;
-                kq = rq == win ? rw : kq;
-                nm = __ballot(valid && rs != rq);
-                --sets;
+                // (a member's partners are members, so inside the span)
+                valid[u] = dst >= 0 && ((S.member[src >> 6] >> (src & 63)) & 1ull);
+                x[2 * u] = valid[u] ? src : 0;
+                x[2 * u + 1] = valid[u] ? dst : 0;
+            }
+            int start[2 * kReplayU];
+#pragma unroll
+            for (int k = 0; k < 2 * kReplayU; ++k) start[k] = x[k];
+            for (;;) {
+                int p[2 * kReplayU];
+#pragma unroll
+                for (int k = 0; k < 2 * kReplayU; ++k) p[k] = S.par[x[k]];
+                bool moved = false;
+#pragma unroll
+                for (int k = 0; k < 2 * kReplayU; ++k) {
+                    moved |= p[k] != x[k];
+                    x[k] = p[k];
+                }
+                if (!__ballot(moved)) break;
+            }
+            int rs[kReplayU], rq[kReplayU], ks[kReplayU], kq[kReplayU];
+#pragma unroll
+            for (int u = 0; u < kReplayU; ++u) {
+                rs[u] = x[2 * u];
+                rq[u] = x[2 * u + 1];
+                ks[u] = S.rnk[rs[u]];
+                kq[u] = S.rnk[rq[u]];
+            }
+            // path compression of the ends (every lane writing a node writes
+            // the root all lanes found for it; no root changes)
+#pragma unroll
+            for (int u = 0; u < kReplayU; ++u) {
+                if (valid[u]) {
+                    S.par[start[2 * u]] = static_cast<int16_t>(rs[u]);
+                    S.par[start[2 * u + 1]] = static_cast<int16_t>(rq[u]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kReplayU; ++u) {
+                // the roots of this slot were relabelled by every earlier
+                // union; their ranks may have grown since, so they are read
+                // again (the unions wrote them to LDS)
+                if (u > 0) {
+                    ks[u] = S.rnk[rs[u]];
+                    kq[u] = S.rnk[rq[u]];
+                }
+                const uint64_t vm = __ballot(valid[u]);
+                uint64_t nm = vm & __builtin_amdgcn_sicmp(rs[u], rq[u], 33 /* ne */);
+                while (nm) {
+                    const int f = __ffsll(static_cast<long long>(nm)) - 1;  // uniform: read the lane directly
+                    const int a = __builtin_amdgcn_readlane(rs[u], f), b = __builtin_amdgcn_readlane(rq[u], f);
+                    const int ra = __builtin_amdgcn_readlane(ks[u], f), rb = __builtin_amdgcn_readlane(kq[u], f);
+                    const int win = ra < rb ? b : a, lose = ra < rb ? a : b;
+                    const int rw = ra == rb ? ra + 1 : max(ra, rb);
+                    if (lane == 0) {
+                        S.par[lose] = static_cast<int16_t>(win);
+                        S.rnk[win] = static_cast<uint8_t>(rw);
+                    }
+#pragma unroll
+                    for (int v = u; v < kReplayU; ++v) {
+                        rs[v] = rs[v] == lose ? win : rs[v];
+                        rq[v] = rq[v] == lose ? win : rq[v];
+                    }
+                    ks[u] = rs[u] == win ? rw : ks[u];
+                    kq[u] = rq[u] == win ? rw : kq[u];
+                    nm = vm & __builtin_amdgcn_sicmp(rs[u], rq[u], 33 /* ne */);
+                    --sets;
+                }
             }
         }
         const int root = find(0);
@@ -2912,9 +2970,11 @@ extern "C" int vj_engine_create(int winW, int winH, int maxSamples, int maxImage
         return VJ_DEVICE_ERROR;
     }
     {
-        // the sweep kernel's x bins (at most kSweepMaxBins) take just over 64 KB of LDS
-        static const hipError_t ldsAttr = hipFuncSetAttribute(reinterpret_cast<const void*>(pair_sweep_kernel),
-                                                              hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+        // the sweep kernel's x bins (at most kSweepMaxBins) and LDS buckets
+        // (kSweepLdsBoxes) take about 128 KB of LDS
+        static const hipError_t ldsAttr = hipFuncSetAttribute(
+            reinterpret_cast<const void*>(pair_sweep_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+            static_cast<int>(sweep_lds_bytes(kSweepMaxBins - 1, kSweepLdsBoxes)));
         if (ldsAttr != hipSuccess) {
             (void)hipGetLastError();
             set_error("vj_engine_create: cannot raise the grouping kernel's LDS limit");
@@ -3438,9 +3498,10 @@ int detect_images(vj_engine* e, const std::vector<ImageIn>& imgs, float sf, int 
         VJ_TRY(e->gGrpImg.ensure(3 * static_cast<size_t>(nImages)));
         VJ_TRY(e->gCtl.ensure(2));
         VJ_TRY(e->gScanTot.ensure((capTotal + kScanItems - 1) / kScanItems));
-        hipLaunchKernelGGL(pair_sweep_kernel, dim3(nImages), dim3(kSweepThreads), sweep_lds_bytes(maxW), s,
-                           e->dOut.get(), e->dCounts.get(), e->dImgBase.get(), e->gImgW.get(), capPer,
-                           e->gBucket.get(), e->gSorted.get(), e->gOrder.get(), e->gGrpImg.get(), e->gCtl.get());
+        const int sweepLdsBoxes = std::min(capPer, kSweepLdsBoxes);
+        hipLaunchKernelGGL(pair_sweep_kernel, dim3(nImages), dim3(kSweepThreads), sweep_lds_bytes(maxW, sweepLdsBoxes),
+                           s, e->dOut.get(), e->dCounts.get(), e->dImgBase.get(), e->gImgW.get(), capPer,
+                           sweepLdsBoxes, e->gBucket.get(), e->gSorted.get(), e->gOrder.get(), e->gGrpImg.get(), e->gCtl.get());
         VJ_HIP(hipGetLastError());
         hipLaunchKernelGGL(pair_find_kernel<false>, dim3(2048), dim3(256), 0, s, e->dCounts.get(), e->dImgBase.get(),
                            nImages, capPer, 0.2f, e->gSorted.get(), e->gPairCnt.get(), nullptr, nullptr, int64_t{0},
