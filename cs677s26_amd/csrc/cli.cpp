@@ -307,6 +307,58 @@ bool stage_negatives(const std::string& cacheDir, const std::vector<std::string>
 
 }  // namespace
 
+// VJ_DEBUG_EVERY diagnostics of the train verb (src/main.cu:1708-1720,
+// 1967-1983, 2105-2162, 2227-2283): host copies of the trainer's state, and
+// the chosen feature's responses recomputed on the host from the stage's
+// patches (src/main.cu:476-496: w0*S0 + w1*S1 (+ w2*S2), then * invNorm, each
+// S the float of an integer rectangle sum of the patch integral).
+struct TrainDebug {
+    int every = 0;
+    int n = 0, numPos = 0, win = 24;
+    std::vector<uint32_t> ii;  // [n][(win+1)^2] patch integrals
+    std::vector<double> w;
+    std::vector<float> strong, inv;
+    std::vector<uint8_t> active;
+
+    void integrals(const uint8_t* patches) {
+        const int iw = win + 1, area = iw * iw;
+        ii.assign(static_cast<size_t>(n) * area, 0u);
+        for (int s = 0; s < n; ++s) {
+            const uint8_t* p = patches + static_cast<size_t>(s) * win * win;
+            uint32_t* t = ii.data() + static_cast<size_t>(s) * area;
+            for (int y = 0; y < win; ++y) {
+                uint32_t row = 0;
+                for (int x = 0; x < win; ++x) {
+                    row += p[y * win + x];
+                    t[(y + 1) * iw + x + 1] = t[y * iw + x + 1] + row;
+                }
+            }
+        }
+    }
+    float response(const vj_feature& f, int s) const {
+        const uint32_t* t = ii.data() + static_cast<size_t>(s) * (win + 1) * (win + 1);
+        auto sum = [t](const vj_rect& r) {
+            return static_cast<float>(static_cast<int32_t>(t[r.p0] - t[r.p1] - t[r.p2] + t[r.p3]));
+        };
+        float resp = f.r[0].w * sum(f.r[0]) + f.r[1].w * sum(f.r[1]);
+        if (f.rectCount == 3) resp += f.r[2].w * sum(f.r[2]);
+        return resp * inv[static_cast<size_t>(s)];
+    }
+    bool read(vj_trainer* t) {
+        w.resize(static_cast<size_t>(n));
+        strong.resize(static_cast<size_t>(n));
+        inv.resize(static_cast<size_t>(n));
+        active.resize(static_cast<size_t>(n));
+        return vj_trainer_read_state(t, w.data(), strong.data(), active.data(), inv.data()) == VJ_OK;
+    }
+    void weight_sums(float& pos, float& neg) const {
+        pos = 0.0f;
+        neg = 0.0f;
+        for (int i = 0; i < numPos; ++i) pos += static_cast<float>(w[static_cast<size_t>(i)]);
+        for (int i = numPos; i < n; ++i) neg += static_cast<float>(w[static_cast<size_t>(i)]);
+    }
+};
+
 int run_train(int argc, char** argv) {
     constexpr int kStages = 38, kTotal = 6061, kPos = 10000, kNeg = 10000, kWin = 24, kPerImageKeep = 256;
     const int modelArg = argc - 1;
@@ -418,6 +470,20 @@ int run_train(int argc, char** argv) {
     std::vector<vj_stage> stages;
     const int mineCap = std::max(20000, numNeg * 2);
     std::vector<vj_detection> mined(static_cast<size_t>(mineCap));
+    TrainDebug dbg;
+    if (const char* env = std::getenv("VJ_DEBUG_EVERY")) {
+        try {
+            dbg.every = std::max(0, std::stoi(env));
+        } catch (...) {
+            dbg.every = 0;
+        }
+    }
+    if (dbg.every > 0) {
+        dbg.n = n;
+        dbg.numPos = numPos;
+        dbg.win = kWin;
+        std::cout << "[INFO] Debug logging enabled: VJ_DEBUG_EVERY=" << dbg.every << "\n";
+    }
     const auto t0 = std::chrono::high_resolution_clock::now();
     std::cout << "[TRAIN] Start cascade training: maxStages=" << maxStages << " maxWeakPerStage=" << maxWeak
               << " totalFeatureBudget=" << kTotal << "\n";
@@ -508,12 +574,24 @@ int run_train(int argc, char** argv) {
         if (vj_trainer_stage_begin(tr, static_cast<const uint8_t*>(dTrain.p), labels.data(), nullptr, n, numPos,
                                    static_cast<const vj_feature*>(dPool.p), poolN, nullptr) != VJ_OK)
             return fail("stage begin");
+        if (dbg.every > 0) {
+            if (!dbg.read(tr)) return fail("read trainer state");
+            dbg.integrals(trainRaw.data());
+            float pw, nw;
+            dbg.weight_sums(pw, nw);
+            std::cout << "[DBG] stage=" << stage << " initWeightSum pos=" << pw << " neg=" << nw
+                      << " activeNeg=" << numNeg << "\n";
+        }
         const int first = static_cast<int>(stumps.size());
         const int limit = std::min(maxWeak, kTotal - first);
         vj_round_config cfg{minHit, maxFa};
         float stageThreshold = 0.0f;
         bool satisfied = false;
         for (int r = 0; r < limit; ++r) {
+            const bool emitDebug = dbg.every > 0 && (r % dbg.every == 0 || r == limit - 1);
+            // the weights before the round: the update multiplies the
+            // misclassified ones by exp(C) before the normalisation
+            if (emitDebug && !dbg.read(tr)) return fail("read trainer state");
             vj_weak wk;
             if (vj_trainer_round(tr, &cfg, &wk, nullptr) != VJ_OK) return fail("round");
             if (wk.feature < 0) {
@@ -522,6 +600,57 @@ int run_train(int argc, char** argv) {
             }
             stumps.push_back({wk.feature, wk.theta, wk.left, wk.right});
             stageThreshold = wk.stageThreshold;
+            if (emitDebug) {
+                const vj_feature& f = pool[static_cast<size_t>(wk.feature)];
+                const double grow = std::exp(wk.alpha);
+                double errP = 0.0, errN = 0.0;
+                for (int i = 0; i < n; ++i) {
+                    if (!dbg.active[static_cast<size_t>(i)]) continue;
+                    const float resp = dbg.response(f, i);
+                    const int y = labels[static_cast<size_t>(i)];
+                    const bool lt = resp < wk.theta;
+                    const int predP = lt ? -1 : 1, predN = lt ? 1 : -1;
+                    const int predUsed = wk.parity > 0 ? predP : predN;
+                    const double wi = dbg.w[static_cast<size_t>(i)] * (predUsed != y ? grow : 1.0);
+                    const float wf = static_cast<float>(wi);
+                    if (predP != y) errP += wf;
+                    if (predN != y) errN += wf;
+                }
+                std::cout << "[DBG] stage=" << stage << " weak=" << r << " bestParityRaw=" << wk.parity
+                          << " parityUsed=" << wk.parity << " errP=" << errP << " errN=" << errN
+                          << " theta=" << wk.theta << "\n";
+                if (!dbg.read(tr)) return fail("read trainer state");
+                float posMin = std::numeric_limits<float>::infinity(), posMax = -posMin;
+                double posSum = 0.0;
+                std::vector<float> posScores(dbg.strong.begin(), dbg.strong.begin() + numPos);
+                for (float sc : posScores) {
+                    posMin = std::min(posMin, sc);
+                    posMax = std::max(posMax, sc);
+                    posSum += sc;
+                }
+                float posMedian = 0.0f;
+                if (!posScores.empty()) {
+                    auto mid = posScores.begin() + static_cast<std::ptrdiff_t>(posScores.size() / 2);
+                    std::nth_element(posScores.begin(), mid, posScores.end());
+                    posMedian = *mid;
+                }
+                float negMin = std::numeric_limits<float>::infinity(), negMax = -negMin;
+                double negSum = 0.0;
+                for (int i = numPos; i < n; ++i) {
+                    const float sc = dbg.strong[static_cast<size_t>(i)];
+                    negMin = std::min(negMin, sc);
+                    negMax = std::max(negMax, sc);
+                    negSum += sc;
+                }
+                const float negMean = n == numPos ? 0.0f : static_cast<float>(negSum / static_cast<double>(n - numPos));
+                float pw, nw;
+                dbg.weight_sums(pw, nw);
+                std::cout << "[DBG] stage=" << stage << " weak=" << r << " score_pos[min,max,mean,p50]=[" << posMin
+                          << "," << posMax << "," << static_cast<float>(posSum / static_cast<double>(numPos)) << ","
+                          << posMedian << "]" << " score_neg[min,max,mean]=[" << negMin << "," << negMax << ","
+                          << negMean << "]" << " threshold=" << wk.stageThreshold << " weightSum[pos,neg]=[" << pw
+                          << "," << nw << "]\n";
+            }
             std::cout << "[TRAIN] stage=" << stage << " weak=" << r << " feature=" << wk.feature << " err="
                       << static_cast<float>(std::min(std::max(wk.err, 1e-6), 1.0 - 1e-6)) << " alpha(C)=" << wk.alpha
                       << " parity=" << wk.parity << " hit=" << wk.hitRate << " fa=" << wk.falseAlarm
