@@ -271,23 +271,41 @@ def stump_leg(args):
     tune_s = time.perf_counter() - tt
     stage()
     t0, t1, t2, rounds = stage()
+    scanned = tr.pruning_stats()  # features scanned in the timed stage's rounds (seed passes included)
     sched = tr.schedule()
+    # the same stage with every feature scanned every round (no pruning)
+    tr.set_pruning(False)
+    f0, f1, f2, full_rounds = stage()
+    tr.set_pruning(True)
     tr.set_schedule(None)
     d0, d1, d2, _ = stage()
+    same = [(r["feature"], r["split"], r["parity"], r["err"]) for r in rounds] == \
+        [(r["feature"], r["split"], r["parity"], r["err"]) for r in full_rounds]
     round_ms = 1e3 * (t2 - t1) / max(1, args.rounds)
-    # the threshold scan streams each feature's u16 order with its split bit:
-    # F*N*2 + F*N/8 bytes per round, over the whole round's wall time
-    scan_bytes = len(pool) * n * 2 + len(pool) * n // 8
+    full_ms = 1e3 * (f2 - f1) / max(1, args.rounds)
+    # the threshold scan streams each scanned feature's u16 order with its
+    # split bit: N*2 + N/8 bytes per feature, over the whole round's wall time
+    per_feature = n * 2 + n // 8
+    scan_bytes = scanned * per_feature // max(1, args.rounds)
+    full_bytes = len(pool) * per_feature
     scan_gbs = scan_bytes / (round_ms * 1e-3) / 1e9 if round_ms > 0 else 0.0
+    per_feature_traffic = measured_traffic_search()
+    full_gbs = full_bytes / (full_ms * 1e-3) / 1e9 if full_ms > 0 else 0.0
     return {"s_per_stage": t2 - t0, "presort_s": t1 - t0, "round_ms": round_ms,
+            "pruning": {"features_scanned_per_round": scanned / max(1, args.rounds),
+                        "full_scan_s_per_stage": f2 - f0, "full_scan_round_ms": full_ms,
+                        "rounds_equal_full_scan": same},
             "schedule": {"wide": sched[0], "loads_ahead": sched[1], "blocks_per_cu": sched[2], "packed": sched[3],
                          "tune_s": tune_s, "default_s_per_stage": d2 - d0, "default_presort_s": d1 - d0},
             "rounds": args.rounds, "features": len(pool), "samples": n,
             "first_feature": rounds[0]["feature"] if rounds else -1,
-            "search_roofline": {"bound": "hbm", "kernel": "search_kernel (whole round)", "achieved": scan_gbs,
+            "search_roofline": {"bound": "hbm", "kernel": "search_kernel (whole round, pruned)", "achieved": scan_gbs,
                                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": scan_gbs / HBM_PEAK_GBS,
                                 "algorithmic_bytes": scan_bytes, "avg_ms": round_ms,
-                                "traffic": measured_traffic_search()}}
+                                "traffic": None if per_feature_traffic is None
+                                else per_feature_traffic * scanned / max(1, args.rounds),
+                                "full_scan": {"achieved": full_gbs, "frac": full_gbs / HBM_PEAK_GBS,
+                                              "algorithmic_bytes": full_bytes, "avg_ms": full_ms}}}
 
 
 def cpu_info():
@@ -405,13 +423,13 @@ def measured_traffic(name):
 
 
 def measured_traffic_search():
-    """HBM bytes of one search_kernel dispatch from the committed rocprofv3
-    --pmc summary of the stump search (None if not measured)."""
+    """HBM bytes per scanned feature of search_kernel from the committed
+    rocprofv3 --pmc summary of the stump search (None if not measured)."""
     path = os.path.join(ROOT, "profiles", "stump_search_162336x20000_kernels.md")
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_dispatch (search_kernel):"):
+                if line.startswith("traffic_bytes_per_feature (search_kernel):"):
                     return float(line.split(":", 1)[1])
     except OSError:
         return None

@@ -143,6 +143,8 @@ _PROTOS = {
     "vj_trainer_set_autotune": (I, [P, I]),
     "vj_trainer_set_schedule": (I, [P, P]),
     "vj_trainer_schedule": (I, [P, P]),
+    "vj_trainer_set_pruning": (I, [P, I]),
+    "vj_trainer_pruning_stats": (I, [P, P]),
     "vj_cpu_detect_raw": (I, [P, I, P, I, P, I, I, I, P, I, I, I, F, I, I, I, P, I, C.POINTER(I)]),
     "vj_cpu_best_weak": (I, [P, I, I, I, P, P, P, P, I, P]),
 }

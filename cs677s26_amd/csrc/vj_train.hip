@@ -828,12 +828,28 @@ __device__ __forceinline__ void load_weights(const float* __restrict__ sw32, int
     }
 }
 
+// Exact pruning of the search (no counterpart in the reference, which scans
+// every feature every round).  A feature's error is the sum of the search
+// weights' magnitudes q over the samples its best stump misclassifies, and
+// every round's weights are at least rho times the previous round's, sample
+// by sample (rho: the smallest ratio, rounded down; normalize_kernel).  So a
+// feature's error is at least rho times its error of the round before, and
+// lb[f] (its last exact error, scaled down by every rho since and rounded
+// down) bounds it from below.  A feature whose bound exceeds the smallest
+// error found so far this round (gBest, which only falls to the round's
+// minimum) can neither win nor tie, and is skipped; the chosen stump is the
+// one the full scan finds.  A seed pass first scans the previous round's
+// per-workgroup winners, so gBest starts near the round's minimum.
 template <typename W, int kAhead>
 __global__ __launch_bounds__(kSearchThreads) __attribute__((amdgpu_waves_per_eu(1, 4))) void search_kernel(const uint16_t* __restrict__ ord, int64_t pitch,
                                                                int n, int nFeatures,
                                                                const float* __restrict__ sw32,
                                                                const uint8_t* __restrict__ used,
-                                                               WaveBest* __restrict__ blockBest) {
+                                                               WaveBest* __restrict__ blockBest,
+                                                               double* __restrict__ lb, const double* __restrict__ rho,
+                                                               unsigned long long* __restrict__ gBest,
+                                                               const WaveBest* __restrict__ seeds, int nSeeds,
+                                                               unsigned long long* __restrict__ scanned) {
     extern __shared__ int64_t lsmem[];
     W* lsw = reinterpret_cast<W*>(lsmem);  // [n] search weights
     __shared__ int64_t totP[16], totN[16];
@@ -845,17 +861,58 @@ __global__ __launch_bounds__(kSearchThreads) __attribute__((amdgpu_waves_per_eu(
     int64_t bE = INT64_MAX;
     int32_t bK = -1, bP = +1, bF = INT_MAX;
     const int nWavesTotal = gridDim.x * (blockDim.x >> 6);
-    for (int f = blockIdx.x * (blockDim.x >> 6) + wave; f < nFeatures; f += nWavesTotal) {
-        if (used && used[f]) continue;
-        int64_t fE;
-        int32_t fK, fP;
-        scan_feature<false, false, kAhead>(ord + static_cast<int64_t>(f) * pitch, nullptr, n, lsw, totPos, totNeg, lane,
-                                           fE, fK, fP);
-        if (fK >= 0 && (fE < bE || (fE == bE && f < bF))) {
-            bE = fE;
-            bF = f;
+    const bool prune = lb != nullptr;
+    const double grow = prune && !seeds ? rho[0] : 0.0;
+    const int count = seeds ? nSeeds : nFeatures;
+    unsigned long long done = 0;
+    // this wave's features k0, k0 + T, k0 + 2T, ... (T waves in the grid),
+    // 64 at a time: lane j loads the j-th one's index, used flag and bound
+    for (int k0 = blockIdx.x * (blockDim.x >> 6) + wave; k0 < count; k0 += kWave * nWavesTotal) {
+        const int kk = k0 + lane * nWavesTotal;
+        int fl = -1;
+        if (kk < count) fl = seeds ? seeds[kk].feature : kk;
+        if (fl >= nFeatures) fl = -1;
+        const bool take = fl >= 0 && !(used && used[fl]);
+        double bl = 0.0;
+        if (take && prune && !seeds) bl = lb[fl] * grow * (1.0 - 0x1p-50);
+        unsigned long long g =
+            prune ? __hip_atomic_load(gBest, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ~0ull;
+        uint64_t todo = __ballot(take);
+        while (todo) {
+            const int j = __ffsll(static_cast<long long>(todo)) - 1;
+            todo &= todo - 1;
+            const int f = __builtin_amdgcn_readlane(fl, j);
+            if (prune && !seeds) {
+                const uint64_t bb = static_cast<uint64_t>(__double_as_longlong(bl));
+                const double b = __longlong_as_double(static_cast<long long>(
+                    (static_cast<uint64_t>(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(bb >> 32), j)))
+                     << 32) |
+                    static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(bb), j))));
+                if (b > static_cast<double>(g) * (1.0 + 0x1p-40)) {
+                    if (lane == 0) lb[f] = b;  // skipped: the bound, one round on
+                    continue;
+                }
+            }
+            int64_t fE;
+            int32_t fK, fP;
+            scan_feature<false, false, kAhead>(ord + static_cast<int64_t>(f) * pitch, nullptr, n, lsw, totPos, totNeg,
+                                               lane, fE, fK, fP);
+            ++done;
+            if (prune) {
+                if (!seeds && lane == 0) lb[f] = static_cast<double>(fE) * (1.0 - 0x1p-50);
+                if (fK >= 0 && static_cast<unsigned long long>(fE) < g) {
+                    g = static_cast<unsigned long long>(fE);
+                    if (lane == 0) atomicMin(gBest, g);
+                }
+            }
+            if (!seeds && fK >= 0 && (fE < bE || (fE == bE && f < bF))) {
+                bE = fE;
+                bF = f;
+            }
         }
     }
+    if (scanned && lane == 0 && done) atomicAdd(scanned, done);
+    if (seeds) return;  // the seed pass only lowers gBest (uniform over the workgroup)
     // the wave's winner again, now with its split and parity (same error)
     if (bF != INT_MAX) {
         int64_t fE;
@@ -1092,9 +1149,11 @@ __device__ __forceinline__ double u128_to_double(uint64_t lo, uint64_t hi) {
 // inactive).  One workgroup.
 __global__ __launch_bounds__(1024) void normalize_kernel(double* __restrict__ w, const int8_t* __restrict__ label,
                                                          const uint8_t* __restrict__ active, int n,
-                                                         float* __restrict__ sw32) {
+                                                         float* __restrict__ sw32, double* __restrict__ rho,
+                                                         unsigned long long* __restrict__ gBest) {
     __shared__ uint64_t sLo[16], sHi[16];
     __shared__ double sS;
+    __shared__ double sRho[16];
     // kNormU samples per thread per step, all loaded before any is used (one
     // workgroup: without loads in flight the loop waits on each one)
     constexpr int kNormU = 8;
@@ -1147,16 +1206,21 @@ __global__ __launch_bounds__(1024) void normalize_kernel(double* __restrict__ w,
     }
     __syncthreads();
     const double S = sS;
+    // the smallest ratio of a sample's new search-weight magnitude (as the
+    // scan adds it) to its previous one: the search's pruning factor
+    double rmin = INFINITY;
     for (int s0 = threadIdx.x; s0 < n; s0 += bd * kNormU) {
         double wv[kNormU];
         uint8_t av[kNormU];
         int8_t lv[kNormU];
+        float ov[kNormU];
 #pragma unroll
         for (int u = 0; u < kNormU; ++u) {
             const int s = s0 + u * bd;
             wv[u] = s < n ? w[s] : 0.0;
             av[u] = s < n ? active[s] : 0;
             lv[u] = s < n ? label[s] : 0;
+            ov[u] = s < n && rho ? sw32[s] : 0.0f;
         }
 #pragma unroll
         for (int u = 0; u < kNormU; ++u) {
@@ -1166,10 +1230,36 @@ __global__ __launch_bounds__(1024) void normalize_kernel(double* __restrict__ w,
                 if (S > 1e-30) x = x / S;
                 w[s] = x;
                 const float f = static_cast<float>(x);
-                sw32[s] = av[u] ? (lv[u] > 0 ? f : -f) : 0.0f;
+                const float nw = av[u] ? (lv[u] > 0 ? f : -f) : 0.0f;
+                sw32[s] = nw;
+                if (rho) {
+                    const int64_t qo = unpack_weight(pack_weight(ov[u])), qn = unpack_weight(pack_weight(nw));
+                    const double ao = static_cast<double>(qo < 0 ? -qo : qo), an = static_cast<double>(qn < 0 ? -qn : qn);
+                    if (ao > 0.0) rmin = fmin(rmin, an / ao);
+                }
             }
         }
     }
+    if (rho) {
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) rmin = fmin(rmin, __shfl_xor(rmin, d, kWave));
+        if ((threadIdx.x & 63) == 0) sRho[threadIdx.x >> 6] = rmin;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r = sRho[0];
+            for (int k = 1; k < static_cast<int>(blockDim.x >> 6); ++k) r = fmin(r, sRho[k]);
+            // each ratio is within 3 units of 2^-53 of the exact one: the
+            // margin keeps rho at or below every exact ratio
+            rho[0] = r < INFINITY ? r * (1.0 - 0x1p-40) : 0.0;
+            gBest[0] = ~0ull;
+        }
+    }
+}
+
+// Fills a[0..n) with v (the search's error bounds at a stage start).
+__global__ __launch_bounds__(256) void fill_f64_kernel(double* __restrict__ a, int n, double v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
 }
 
 // Stage weight reset (src/main.cu:908-924): positives 0.5/numPos, active
@@ -1561,6 +1651,13 @@ struct vj_trainer {
     PinnedBuf<uint32_t> hCounter;
     int searchBlocks = 0;
     int cus = 256;
+    // exact pruning of the search (see search_kernel): per-feature error
+    // bounds, the weights' growth factor, the round's smallest error so far,
+    // features scanned since the stage began, and the last round's grid
+    bool prune = true;
+    DevBuf<double> lb, rho;
+    DevBuf<unsigned long long> gBest, scanCnt;
+    int lastBlocks = 0;
     // the stage's sort and scan schedule (see TrainSchedule)
     TrainSchedule sched;
     bool autotune = false;   // pick the schedule per (samples, features) bucket by timing
@@ -1826,7 +1923,7 @@ int presort_stage(vj_trainer* t, int count, hipStream_t s) {
 // One round's threshold scan over features [0, count) with schedule sc (the
 // layout of the presorted orders, t->wide, is the one presort_stage wrote);
 // blocks receives the grid (the entries of t->blockBest written).
-int launch_search(vj_trainer* t, const TrainSchedule& sc, int count, int& blocks, hipStream_t s) {
+int launch_search(vj_trainer* t, const TrainSchedule& sc, int count, int& blocks, hipStream_t s, bool prune = false) {
     const int n = t->n;
     blocks = std::max(1, std::min((count + 15) / 16, sc.blocks * t->cus));
     const dim3 g(static_cast<unsigned>(blocks)), b(kSearchThreads);
@@ -1846,10 +1943,19 @@ int launch_search(vj_trainer* t, const TrainSchedule& sc, int count, int& blocks
         // and the schedule does not ask for packed words
         const bool i64 = !sc.packed && static_cast<size_t>(n) * 8 <= kSearchWideLds;
         const size_t lds = static_cast<size_t>(n) * (i64 ? 8 : 4);
+        double* lb = prune ? t->lb.get() : nullptr;
         auto go = [&](auto kern) {
+            // the seed pass: the previous round's per-workgroup winners
+            if (prune && t->lastBlocks > 0)
+                hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((t->lastBlocks + 15) / 16)), b, lds, s,
+                                   t->ord.get(), t->pitch, n, count, t->sw32.get(), t->used.get(), t->blockBest.get(),
+                                   lb, t->rho.get(), t->gBest.get(), t->blockBest.get(), t->lastBlocks,
+                                   t->scanCnt.get());
             hipLaunchKernelGGL(kern, g, b, lds, s, t->ord.get(), t->pitch, n, count, t->sw32.get(), t->used.get(),
-                               t->blockBest.get());
+                               t->blockBest.get(), lb, t->rho.get(), t->gBest.get(), nullptr, 0,
+                               prune ? t->scanCnt.get() : nullptr);
         };
+        if (prune) t->lastBlocks = blocks;
         if (i64) {
             if (sc.ahead == 2) go(search_kernel<int64_t, 2>);
             else if (sc.ahead == 8) go(search_kernel<int64_t, 8>);
@@ -1861,6 +1967,15 @@ int launch_search(vj_trainer* t, const TrainSchedule& sc, int count, int& blocks
         }
     }
     VJ_HIP(hipGetLastError());
+    return VJ_OK;
+}
+
+// A stage start (or the tuner's restored state): every feature's bound 0,
+// so the next round scans every feature; no seeds.
+int reset_pruning(vj_trainer* t, int nFeatures, hipStream_t s) {
+    hipLaunchKernelGGL(fill_f64_kernel, dim3((nFeatures + 255) / 256), dim3(256), 0, s, t->lb.get(), nFeatures, 0.0);
+    VJ_HIP(hipGetLastError());
+    t->lastBlocks = 0;
     return VJ_OK;
 }
 
@@ -1983,6 +2098,7 @@ int tune_scan(vj_trainer* t, hipStream_t s) {
         VJ_HIP(copy(t->strong.get(), strong0.get(), n * 4));
         VJ_HIP(copy(t->sw32.get(), sw0.get(), n * 4));
         VJ_HIP(copy(t->used.get(), used0.get(), nf));
+        VJ_TRY(reset_pruning(t, t->nFeat, s));
         VJ_HIP(hipStreamSynchronize(s));
         return VJ_OK;
     };
@@ -2028,6 +2144,11 @@ extern "C" int vj_trainer_stage_begin(vj_trainer* t, const uint8_t* dPatches, co
     VJ_TRY(t->inv.ensure(static_cast<size_t>(n)));
     VJ_TRY(t->ord.ensure(static_cast<size_t>(t->pitch) * nFeatures));
     VJ_TRY(t->label.ensure(static_cast<size_t>(n)));
+    VJ_TRY(t->lb.ensure(static_cast<size_t>(nFeatures)));
+    VJ_TRY(t->rho.ensure(1));
+    VJ_TRY(t->gBest.ensure(1));
+    VJ_TRY(t->scanCnt.ensure(1));
+    VJ_HIP(hipMemsetAsync(t->scanCnt.get(), 0, sizeof(unsigned long long), s));
     VJ_TRY(t->active.ensure(static_cast<size_t>(n)));
     VJ_TRY(t->w.ensure(static_cast<size_t>(n)));
     VJ_TRY(t->strong.ensure(static_cast<size_t>(n)));
@@ -2055,7 +2176,9 @@ extern "C" int vj_trainer_stage_begin(vj_trainer* t, const uint8_t* dPatches, co
                        t->active.get(), n, numPos, 0.5 / static_cast<double>(numPos), 0.5 / static_cast<double>(activeNeg));
     VJ_HIP(hipGetLastError());
     hipLaunchKernelGGL(normalize_kernel, dim3(1), dim3(1024), 0, s, t->w.get(), t->label.get(), t->active.get(), n,
-                       t->sw32.get());
+                       t->sw32.get(), t->rho.get(), t->gBest.get());
+    VJ_HIP(hipGetLastError());
+    VJ_TRY(reset_pruning(t, nFeatures, s));
     VJ_HIP(hipGetLastError());
     t->n = n;
     t->numPos = numPos;
@@ -2120,6 +2243,23 @@ extern "C" int vj_trainer_set_schedule(vj_trainer* t, const int* sched) {
     return VJ_OK;
 }
 
+extern "C" int vj_trainer_set_pruning(vj_trainer* t, int on) {
+    VJ_CHECK(t, "set_pruning: null trainer");
+    t->prune = on != 0;
+    t->lastBlocks = 0;
+    return VJ_OK;
+}
+
+extern "C" int vj_trainer_pruning_stats(vj_trainer* t, int64_t* scanned) {
+    VJ_CHECK(t && scanned, "pruning_stats: null argument");
+    *scanned = 0;
+    if (!t->scanCnt.get()) return VJ_OK;
+    unsigned long long v = 0;
+    VJ_HIP(hipMemcpy(&v, t->scanCnt.get(), sizeof v, hipMemcpyDeviceToHost));
+    *scanned = static_cast<int64_t>(v);
+    return VJ_OK;
+}
+
 extern "C" int vj_trainer_schedule(vj_trainer* t, int* sched) {
     VJ_CHECK(t && sched, "schedule: null argument");
     sched[0] = t->sched.wide;
@@ -2142,7 +2282,7 @@ int round_impl(vj_trainer* t, const vj_round_config* cfg, vj_weak* out, hipStrea
     std::memset(out, 0, sizeof *out);
     out->feature = -1;
     int blocks = 0;
-    VJ_TRY(launch_search(t, t->sched, t->nFeat, blocks, s));
+    VJ_TRY(launch_search(t, t->sched, t->nFeat, blocks, s, t->prune && !t->wide));
     hipLaunchKernelGGL(reduce_best_kernel, dim3(1), dim3(256), 0, s, t->blockBest.get(), blocks, t->ord.get(), t->pitch,
                        t->feats, t->sumT.get(), t->inv.get(), t->n, t->wide ? 0xFFFFu : 0x7FFFu,
                        t->roundBest.get());
@@ -2170,7 +2310,7 @@ int round_impl(vj_trainer* t, const vj_round_config* cfg, vj_weak* out, hipStrea
                        rb.parity, out->left, out->right, factor, t->used.get());
     VJ_HIP(hipGetLastError());
     hipLaunchKernelGGL(normalize_kernel, dim3(1), dim3(1024), 0, s, t->w.get(), t->label.get(), t->active.get(), t->n,
-                       t->sw32.get());
+                       t->sw32.get(), t->rho.get(), t->gBest.get());
     VJ_HIP(hipGetLastError());
     // calibration (src/main.cu:2172-2226)
     const int minPass = std::max(1, static_cast<int>(std::ceil(minHit * static_cast<float>(t->numPos))));

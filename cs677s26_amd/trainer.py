@@ -98,6 +98,16 @@ class AdaBoostTrainer:
         check(lib().vj_trainer_schedule(self.handle, out.ctypes.data), "schedule")
         return tuple(int(v) for v in out)
 
+    # -- exact pruning of the round's search (no reference counterpart) ------
+    def set_pruning(self, on=True):
+        check(lib().vj_trainer_set_pruning(self.handle, 1 if on else 0), "set_pruning")
+
+    def pruning_stats(self):
+        """Features scanned by the pruned search since the stage began."""
+        v = C.c_int64()
+        check(lib().vj_trainer_pruning_stats(self.handle, C.byref(v)), "pruning_stats")
+        return v.value
+
     def read_state(self):
         w = np.empty(self.n, np.float64)
         s = np.empty(self.n, np.float32)

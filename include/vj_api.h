@@ -388,6 +388,15 @@ int vj_trainer_read_splits(vj_trainer* t, int begin, int count, uint8_t* hOut);
  * and turns autotune off; vj_trainer_schedule reports the schedule of the
  * current stage. */
 int vj_trainer_set_autotune(vj_trainer* t, int on);
+/* No counterpart in the reference (it scans every feature every round): exact
+ * pruning of the fused round's search, on by default (LDS layout only).  A
+ * feature is skipped when a lower bound on its error -- its last exact error
+ * times the smallest sample-by-sample growth of the search weights since --
+ * exceeds the smallest error already found in the round, so the chosen stump
+ * is the full scan's.  pruning_stats gives the features scanned since the
+ * stage began (a full scan is nFeatures per round). */
+int vj_trainer_set_pruning(vj_trainer* t, int on);
+int vj_trainer_pruning_stats(vj_trainer* t, int64_t* scanned);
 int vj_trainer_set_schedule(vj_trainer* t, const int* sched);
 int vj_trainer_schedule(vj_trainer* t, int* sched);
 

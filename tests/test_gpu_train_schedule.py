@@ -101,3 +101,30 @@ def test_autotune_picks_a_candidate_and_reuses_it(case):
     tr2.set_schedule(None)
     tr2.stage_begin(d_p, labels, 1000, d_f, len(pool))
     assert tr2.schedule() == (0, 4, 2, 0)
+
+
+@pytest.mark.parametrize("prune", [False, True])
+def test_pruning_gives_the_oracle_rounds(case, prune):
+    """The round's search with and without its exact pruning (a feature is
+    skipped only when a lower bound on its error exceeds an error already
+    found, so it can neither win nor tie): the oracle's rounds bit for bit,
+    on the pool and on duplicated features that tie every round.  With the
+    pruning on, fewer features than a full scan's are scanned."""
+    patches, labels, pool, recs, w_or, orders, dup, recs_dup, w_dup = case
+    say(f"pruning {prune}: 8 rounds on 2000 samples, 5 rounds on duplicated features")
+    d_p = DeviceArray.from_host(patches)
+    tr = AdaBoostTrainer((24, 24), len(labels), 1024)
+    tr.set_schedule((0, 4, 2, 0))
+    tr.set_pruning(prune)
+    d_f = DeviceArray.from_host(pool)
+    tr.stage_begin(d_p, labels, 1000, d_f, len(pool))
+    _check_rounds(tr, recs, w_or, ("pruning", prune))
+    scanned = tr.pruning_stats()
+    if prune:
+        # the first round scans everything; later ones skip some features
+        assert len(pool) <= scanned < len(recs) * len(pool), scanned
+    else:
+        assert scanned == 0
+    d_dup = DeviceArray.from_host(dup)
+    tr.stage_begin(d_p, labels, 1000, d_dup, len(dup))
+    _check_rounds(tr, recs_dup, w_dup, ("pruning duplicates", prune))
