@@ -865,6 +865,7 @@ __global__ __launch_bounds__(kSearchThreads) __attribute__((amdgpu_waves_per_eu(
     const double grow = prune && !seeds ? rho[0] : 0.0;
     const int count = seeds ? nSeeds : nFeatures;
     unsigned long long done = 0;
+    unsigned long long g = prune ? __hip_atomic_load(gBest, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ~0ull;
     // this wave's features k0, k0 + T, k0 + 2T, ... (T waves in the grid),
     // 64 at a time: lane j loads the j-th one's index, used flag and bound
     for (int k0 = blockIdx.x * (blockDim.x >> 6) + wave; k0 < count; k0 += kWave * nWavesTotal) {
@@ -875,8 +876,6 @@ __global__ __launch_bounds__(kSearchThreads) __attribute__((amdgpu_waves_per_eu(
         const bool take = fl >= 0 && !(used && used[fl]);
         double bl = 0.0;
         if (take && prune && !seeds) bl = lb[fl] * grow * (1.0 - 0x1p-50);
-        unsigned long long g =
-            prune ? __hip_atomic_load(gBest, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ~0ull;
         uint64_t todo = __ballot(take);
         while (todo) {
             const int j = __ffsll(static_cast<long long>(todo)) - 1;
@@ -893,11 +892,16 @@ __global__ __launch_bounds__(kSearchThreads) __attribute__((amdgpu_waves_per_eu(
                     continue;
                 }
             }
+            // the round's smallest error so far, read again while this
+            // feature is scanned (for the next one's test)
+            const unsigned long long gNext =
+                prune ? __hip_atomic_load(gBest, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ~0ull;
             int64_t fE;
             int32_t fK, fP;
             scan_feature<false, false, kAhead>(ord + static_cast<int64_t>(f) * pitch, nullptr, n, lsw, totPos, totNeg,
                                                lane, fE, fK, fP);
             ++done;
+            g = gNext < g ? gNext : g;
             if (prune) {
                 if (!seeds && lane == 0) lb[f] = static_cast<double>(fE) * (1.0 - 0x1p-50);
                 if (fK >= 0 && static_cast<unsigned long long>(fE) < g) {
