@@ -1948,10 +1948,12 @@ int launch_search(vj_trainer* t, const TrainSchedule& sc, int count, int& blocks
         const bool i64 = !sc.packed && static_cast<size_t>(n) * 8 <= kSearchWideLds;
         const size_t lds = static_cast<size_t>(n) * (i64 ? 8 : 4);
         double* lb = prune ? t->lb.get() : nullptr;
-        auto go = [&](auto kern) {
-            // the seed pass: the previous round's per-workgroup winners
+        // the seed pass (the previous round's per-workgroup winners, one
+        // feature per wave, so it keeps eight loads in flight per lane)
+        // then the round's scan
+        auto go = [&](auto seedKern, auto kern) {
             if (prune && t->lastBlocks > 0)
-                hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((t->lastBlocks + 15) / 16)), b, lds, s,
+                hipLaunchKernelGGL(seedKern, dim3(static_cast<unsigned>((t->lastBlocks + 15) / 16)), b, lds, s,
                                    t->ord.get(), t->pitch, n, count, t->sw32.get(), t->used.get(), t->blockBest.get(),
                                    lb, t->rho.get(), t->gBest.get(), t->blockBest.get(), t->lastBlocks,
                                    t->scanCnt.get());
@@ -1959,16 +1961,16 @@ int launch_search(vj_trainer* t, const TrainSchedule& sc, int count, int& blocks
                                t->blockBest.get(), lb, t->rho.get(), t->gBest.get(), nullptr, 0,
                                prune ? t->scanCnt.get() : nullptr);
         };
-        if (prune) t->lastBlocks = blocks;
         if (i64) {
-            if (sc.ahead == 2) go(search_kernel<int64_t, 2>);
-            else if (sc.ahead == 8) go(search_kernel<int64_t, 8>);
-            else go(search_kernel<int64_t, 4>);
+            if (sc.ahead == 2) go(search_kernel<int64_t, 8>, search_kernel<int64_t, 2>);
+            else if (sc.ahead == 8) go(search_kernel<int64_t, 8>, search_kernel<int64_t, 8>);
+            else go(search_kernel<int64_t, 8>, search_kernel<int64_t, 4>);
         } else {
-            if (sc.ahead == 2) go(search_kernel<uint32_t, 2>);
-            else if (sc.ahead == 8) go(search_kernel<uint32_t, 8>);
-            else go(search_kernel<uint32_t, 4>);
+            if (sc.ahead == 2) go(search_kernel<uint32_t, 8>, search_kernel<uint32_t, 2>);
+            else if (sc.ahead == 8) go(search_kernel<uint32_t, 8>, search_kernel<uint32_t, 8>);
+            else go(search_kernel<uint32_t, 8>, search_kernel<uint32_t, 4>);
         }
+        if (prune) t->lastBlocks = blocks;
     }
     VJ_HIP(hipGetLastError());
     return VJ_OK;
