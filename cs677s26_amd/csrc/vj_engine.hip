@@ -665,6 +665,9 @@ constexpr int kTailPerDeep = kDeepSlice / kTailSlice;
 // Floats of LDS the deep and tail passes keep for pair-mode leaf values
 // (after the staged footprint): eight windows of a 181-stump stage fit.
 constexpr int kLeafCap = 2048;
+// The deep pass when it runs every remaining stage (no tail pass): a 512-float
+// leaf cache and at most 64 VGPRs, so that eight workgroups fit on a CU.
+constexpr int kLeafCapWide = 512;
 
 // LDS of one cascade workgroup besides the staged footprint.
 struct CascadeSmem {
@@ -1165,8 +1168,8 @@ __global__ __launch_bounds__(kListThreads) void deep_list_kernel(const int32_t* 
 // share words).  Otherwise the slice's survivors are written back over its
 // own entries of survIdx / survInv (no other item reads them) and appended
 // to the tail pass's work list in items of up to kTailSlice windows.
-template <bool kCount>
-__global__ __launch_bounds__(kCascadeBlock) void cascade_deep_kernel(
+template <bool kCount, int kWaves = 6>
+__global__ __launch_bounds__(kCascadeBlock) __attribute__((amdgpu_waves_per_eu(kWaves == 8 ? 8 : 1, 8))) void cascade_deep_kernel(
     const LevelDesc* __restrict__ L, const int32_t* __restrict__ lvlIdx, const int32_t* __restrict__ tileOffs,
     int nLv, const uint32_t* __restrict__ iiS, const KStump* __restrict__ stumpsAll, int nStumps,
     const KStage* __restrict__ stages, int nStages, int stageSplit, int stageTail, int winW, int winH, int leafOff,
@@ -1212,7 +1215,7 @@ __global__ __launch_bounds__(kCascadeBlock) void cascade_deep_kernel(
         __syncthreads();
         const KStump* __restrict__ stumps = stumpsAll + static_cast<size_t>(g.step - 1) * nStumps;
         const int slot = run_stages<kCount>(g, sm, lds, stumps, stages, stageSplit, stageTail, nStages, winW,
-                                            scores, enter, lds + leafOff, kLeafCap);
+                                            scores, enter, lds + leafOff, kWaves == 8 ? kLeafCapWide : kLeafCap);
         if (tail) {
             const int n = sm.listCnt[slot];
             const int nt = (n + kTailSlice - 1) / kTailSlice;
@@ -2778,6 +2781,9 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s, bool tablesCurre
     // deep and tail passes: the leaf values after the footprint, 16-byte aligned
     const int leafOff = static_cast<int>((ldsBytes / sizeof(float) + 3) & ~size_t(3));
     const size_t deepLds = (static_cast<size_t>(leafOff) + kLeafCap) * sizeof(float);
+    // without a tail pass the deep pass runs every remaining stage at eight
+    // workgroups per CU (smaller leaf cache, at most 64 VGPRs)
+    const size_t deepLdsWide = (static_cast<size_t>(leafOff) + kLeafCapWide) * sizeof(float);
     const int nBlocks = P.tab.nBlocks;
     const int deepGrid = deep ? std::min(P.nTiles * kDeepSlices, kDeepGrid) : 0;
     const bool tail = deep && e->stageTail < e->nStages;
@@ -2838,7 +2844,8 @@ int run_pipeline(vj_engine* e, Plan& P, int cap, hipStream_t s, bool tablesCurre
                 hipLaunchKernelGGL(deep_list_kernel, dim3(listGrid), dim3(kListThreads), 0, ls, e->tileCount.get(),
                                    P.nTiles, e->deepCtl.get(), e->deepItems.get());
                 VJ_HIP(hipGetLastError());
-                hipLaunchKernelGGL(enter ? cascade_deep_kernel<true> : cascade_deep_kernel<false>, dim3(deepGrid), dim3(kCascadeBlock), deepLds, ls, dL,
+                hipLaunchKernelGGL(enter ? cascade_deep_kernel<true> : tail ? cascade_deep_kernel<false> : (cascade_deep_kernel<false, 8>),
+                                   dim3(deepGrid), dim3(kCascadeBlock), enter || tail ? deepLds : deepLdsWide, ls, dL,
                                    m32 + oLvl, m32 + oTile, static_cast<int>(P.tileLvl.size()), e->iiS.get(),
                                    e->kst.get(), e->nStumps, e->kstage.get(), e->nStages, e->stageSplit, stageTail,
                                    e->winW, e->winH, leafOff, reinterpret_cast<unsigned long long*>(e->words.get()),
